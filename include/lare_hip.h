@@ -1,0 +1,187 @@
+/*
+ * lare_hip.h -- C-ABI of the MI355X LARE3D Lagrangian-remap step.
+ *
+ * The library (sams_amd/liblare_hip.so) owns one device-resident copy of a
+ * LARE3D grid per context and advances it with hand-written gfx950 kernels.
+ * All entry points return 0 on success and a negative code on failure; the
+ * message of the last failure on the calling thread is lare_last_error().
+ * A context is thread-compatible: one host thread drives it, on the HIP
+ * stream the context owns.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference tree).  Arrays cross the boundary in the reference's own
+ * layout: row-major with z fastest, with the reference's bounds
+ * (src/PhysicsPackages/LARE/setup.cpp:38-70, include/pp/array.h:72-95).
+ */
+#ifndef LARE_HIP_H
+#define LARE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Registered variables of the LARE3D package, setup.cpp:45-69. */
+enum lare_field {
+    LARE_RHO = 0,
+    LARE_ENERGY_ELECTRON = 1,
+    LARE_ENERGY_ION = 2,
+    LARE_VX = 3,
+    LARE_VY = 4,
+    LARE_VZ = 5,
+    LARE_BX = 6,
+    LARE_BY = 7,
+    LARE_BZ = 8,
+    LARE_VX1 = 9,
+    LARE_VY1 = 10,
+    LARE_VZ1 = 11,
+    LARE_DM = 12,
+    LARE_NFIELDS = 13
+};
+
+/* Built-in boundary conditions,
+ * include/boundaryConditions/builtInBoundaryConditions.h:54-300. */
+enum lare_bc_kind {
+    LARE_BC_NONE = 0,
+    LARE_BC_PERIODIC = 1,  /* simplePeriodicBC */
+    LARE_BC_ZERO_GRADIENT = 2, /* simpleZeroGradientBC */
+    LARE_BC_MIRROR = 3,    /* simpleMirrorBC */
+    LARE_BC_CLAMP = 4      /* simpleClamp(value) */
+};
+
+/* Storage precision of the state fields. */
+enum lare_precision {
+    LARE_FP64 = 0,
+    LARE_FP32_STORAGE = 1
+};
+
+/* Grid and physics scalars of LARE::simulationData
+ * (src/PhysicsPackages/LARE/shared_data.h:60-226) that the step reads.
+ * Geometry is Cartesian; the axes are the reference's linear axes built
+ * from (n, min, max) (include/harness/axisRegistry.h:973-984). */
+typedef struct lare_desc {
+    int64_t nx, ny, nz;
+    double x_min, x_max, y_min, y_max, z_min, z_max;
+    double gas_gamma;
+    double visc1, visc2;
+    double mu0;
+    double dt_multiplier;
+    int32_t resistive;      /* resistiveMHD */
+    int32_t precision;      /* enum lare_precision */
+    double eta_background, eta0, j_max;
+} lare_desc;
+
+typedef struct lare_ctx lare_ctx;
+
+/* Last error message of the calling thread ("" if none). */
+const char *lare_last_error(void);
+
+/* Allocate a context and its device arrays; fields start at zero and the
+ * previous dt at 0, as LARE3D::allocate leaves them (setup.cpp:76-255),
+ * with the metric of LARE3D::grid (setup.cpp:260-437).
+ * Replaces LARE3D::registerVariables / allocate / grid. */
+int lare_ctx_create(const lare_desc *desc, lare_ctx **out);
+int lare_ctx_destroy(lare_ctx *ctx);
+
+/* Bounds of a field in the reference's indexing (lb/ub inclusive).
+ * Replaces variableRegistry::fillPPArray (variableDef.h:611-624). */
+int lare_field_bounds(const lare_ctx *ctx, int field, int64_t lb[3], int64_t ub[3]);
+
+/* Copy a host array of the field's full reference extent (lare_field_bounds)
+ * to / from the device.  Replaces the host residency of the reference's
+ * arrays (initial conditions write them; diagnostics.cpp:36-47 reads them). */
+int lare_upload_field(lare_ctx *ctx, int field, const double *host);
+int lare_download_field(lare_ctx *ctx, int field, double *host);
+
+/* Attach a built-in boundary condition to one edge (0 lower, 1 upper) of one
+ * dimension of a field.  Replaces variableDef::addBoundaryCondition
+ * (variableDef.h:402-425) for the built-in classes. */
+int lare_set_bc(lare_ctx *ctx, int field, int dim, int edge, int kind, double value);
+
+/* dt of the previous step (the value the next step's shock viscosity reads,
+ * lagran.cpp:72,83-84); 0 for a fresh context. */
+int lare_get_dt(lare_ctx *ctx, double *dt);
+int lare_set_dt(lare_ctx *ctx, double dt);
+
+/* LARE3D::startOfTimestep (shared_data.h:397-399 -> lagran.cpp:95-173),
+ * including the CFL reduction of runner::calculateTimestep
+ * (runner.h:158-169 -> lagran.cpp:380-443).  If dt_out is not NULL the call
+ * waits for the step and returns the new dt there. */
+int lare_start_of_timestep(lare_ctx *ctx, double *dt_out);
+/* LARE3D::halfTimestep (shared_data.h:405-407 -> lagran.cpp:739-804). */
+int lare_half_timestep(lare_ctx *ctx);
+/* LARE3D::endOfTimestep (shared_data.h:413-419 -> remap.cpp:23-111). */
+int lare_end_of_timestep(lare_ctx *ctx);
+
+/* One runner iteration (runner.h:564-586): the three events in order.
+ * Enqueued on the context's stream; dt_out (optional) waits for it. */
+int lare_step(lare_ctx *ctx, double *dt_out);
+/* nsteps iterations; dt_seq (optional, nsteps doubles) receives every dt. */
+int lare_steps(lare_ctx *ctx, int32_t nsteps, double *dt_seq);
+
+/* Wait for all work enqueued on the context. */
+int lare_sync(lare_ctx *ctx);
+
+/* Per-kernel timing (no counterpart in the reference, whose only timers are
+ * per-event host clocks, runnerMacros.h:113-150): between begin and end
+ * every kernel launch on the context is bracketed by HIP events on its
+ * stream; end aggregates them by kernel name with the kernel's algorithmic
+ * HBM bytes (distinct arrays read and written, 8 bytes each, per cell of its
+ * range). */
+int lare_profile_begin(lare_ctx *ctx);
+int lare_profile_end(lare_ctx *ctx);
+int lare_profile_count(lare_ctx *ctx, int32_t *n);
+int lare_profile_get(lare_ctx *ctx, int32_t i, char *name, int32_t name_len, int32_t *calls,
+                     double *total_ms, double *bytes);
+
+
+/* ------------------------------------------------------------------------
+ * Problem set-up (the initial-condition packages' role: controlVariables,
+ * setDomain, initialConditions, setBoundaryConditions of
+ * src/InitialConditions/OrszagTang3D/OrszagTang3D.cpp:28-158 and
+ * src/InitialConditions/BrioAndWu/BrioAndWu.cpp:29-155).  Every array is
+ * filled over the centred extent rho.getRange(), as those packages do; the
+ * -2 layers of staggered arrays stay zero.
+ */
+enum lare_problem_kind {
+    /* Orszag-Tang on the unit cube, gamma 5/3, all boundaries periodic:
+     * rho = 25/(36 pi), p = 5/(12 pi) (or beta/(8 pi) when beta > 0),
+     * v = (-sin 2 pi yb, sin 2 pi xb, 0) [+ amp * N(0,1) per component],
+     * B = (-sin 2 pi yc, sin 4 pi xc, 0) / sqrt(4 pi),
+     * e_e = e_i = p / ((gamma-1) rho) / 2. */
+    LARE_PROBLEM_ORSZAG_TANG = 0,
+    /* x-directed Brio-Wu on x in [-1,1], y,z in [0, 2 ny/nx], [0, 2 nz/nx],
+     * gamma 2: rho,p,By = 1,1,1 for xc < 0 and 0.125,0.1,-1 otherwise,
+     * Bx = 0.75, rho *= 1 + noise*(2U-1); x boundaries bcx, y/z periodic. */
+    LARE_PROBLEM_BRIO_WU = 1
+};
+
+typedef struct lare_problem {
+    int32_t kind;       /* enum lare_problem_kind */
+    int32_t bcx;        /* Brio-Wu x boundary: LARE_BC_ZERO_GRADIENT (0 also means
+                           this), LARE_BC_MIRROR, LARE_BC_PERIODIC, or LARE_BC_CLAMP
+                           (walls: velocities clamped to 0, the rest zero-gradient) */
+    int64_t nx, ny, nz;
+    uint64_t seed;      /* counter-based streams: splitmix64 of (seed, stream, index) */
+    double amp;         /* Orszag-Tang: Gaussian velocity perturbation amplitude */
+    double beta;        /* Orszag-Tang: plasma beta; <= 0 keeps p = 5/(12 pi) */
+    double noise;       /* Brio-Wu: relative uniform density noise */
+    double eta;         /* Brio-Wu: background resistivity; > 0 enables resistive MHD */
+    int32_t precision;  /* enum lare_precision */
+    int32_t reserved;
+} lare_problem;
+
+/* Grid and physics scalars of the problem. */
+int lare_problem_desc(const lare_problem *p, lare_desc *out);
+/* Initial value of one field over its reference extent into host memory. */
+int lare_problem_fill(const lare_problem *p, int field, double *host);
+/* Attach the problem's boundary conditions to every registered variable. */
+int lare_problem_bcs(lare_ctx *ctx, const lare_problem *p);
+/* Fill, upload and attach: the context then holds the problem at t = 0. */
+int lare_problem_init(lare_ctx *ctx, const lare_problem *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LARE_HIP_H */

@@ -32,7 +32,20 @@ typedef struct { double *p; long l, n; } a1;
 /* Row-major, last index fastest, offsets from the array's own lower bounds
  * (include/pp/array.h:72-95).  No bounds check: an index one below a bound
  * addresses the previous row exactly as the reference's flat offset does. */
+#ifdef LO_BOUNDS_CHECK
+#include <stdio.h>
+static long lo_chk(const char *nm, long l, long n, long i, int line)
+{
+    if (i < l || i >= l + n) fprintf(stderr, "OOB %s idx %ld range [%ld,%ld] line %d\n", nm, i, l, l + n - 1, line);
+    return i;
+}
+#define A(a, i, j, k)                                                                                        \
+    ((a).p[((lo_chk(#a, (a).l0, (a).n0, (long)(i), __LINE__) - (a).l0) * (a).n1 +                              \
+            (lo_chk(#a, (a).l1, (a).n1, (long)(j), __LINE__) - (a).l1)) * (a).n2 +                             \
+           (lo_chk(#a, (a).l2, (a).n2, (long)(k), __LINE__) - (a).l2)])
+#else
 #define A(a, i, j, k) ((a).p[(((long)(i) - (a).l0) * (a).n1 + ((long)(j) - (a).l1)) * (a).n2 + ((long)(k) - (a).l2)])
+#endif
 #define L(a, i) ((a).p[(long)(i) - (a).l])
 /* Element s cells along the sweep direction (di,dj,dk) from (ix,iy,iz). */
 #define AP(a, s) A(a, ix + (s) * di, iy + (s) * dj, iz + (s) * dk)

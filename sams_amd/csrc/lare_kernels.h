@@ -1,0 +1,61 @@
+// Host-visible declarations of the LARE3D device step.
+#pragma once
+
+#include "lare_dev.h"
+
+#include <vector>
+
+namespace lare {
+
+// Built-in boundary condition of one registered variable (device view).
+struct BCVar {
+    double *p;
+    int stag[3];        // 1 = HALF_CELL in that dimension, 0 = CENTRED
+    int kind[3][2];     // enum lare_bc_kind per dimension and edge
+    double val[3][2];   // clamp values
+};
+// Up to four variables filled by one launch (blockIdx.z selects).
+struct BCSet {
+    BCVar v[4];
+};
+
+// Physics scalars of simulationData that the step reads.
+struct Params {
+    double gas_gamma, visc1, visc2, visc2_norm, mu0, dt_multiplier;
+    int resistive;
+    double eta_background, eta0, j_max;
+};
+
+// Optional per-kernel timing with HIP events on the grid's stream.
+struct Prof {
+    bool on = false;
+    struct Rec {
+        const char *name;
+        double bytes;  // algorithmic bytes of this launch
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+};
+
+// One device-resident grid.
+struct Grid {
+    Dev dev;
+    Params prm;
+    int bck[13][3][2];
+    double bcv[13][3][2];
+    hipStream_t stream;
+    size_t box;          // doubles per 3D array
+    void *pool;          // one allocation holding every device array
+    size_t pool_bytes;
+    Prof prof;
+};
+
+// Metric arrays from the 1D deltas (LARE3D::grid, setup.cpp:337-390).
+void build_metric(Grid &g);
+
+// The three LARE3D events (shared_data.h:397-419), enqueued on g.stream.
+void step_start(Grid &g);
+void step_half(Grid &g);
+void step_end(Grid &g);
+
+}  // namespace lare

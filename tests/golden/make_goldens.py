@@ -1,0 +1,67 @@
+"""Write tests/golden/goldens.json from the reference itself.
+
+Runs oracle/_ref/ref_driver -- the reference's own LARE sources compiled in
+place with -ffp-contract=off by oracle/Makefile, driven through the
+reference runner -- on small seeded problems, and records for each case the
+dt sequence (hex floats) and, at the dumped steps, the SHA-256 of every state
+field's raw little-endian float64 bytes over its full reference extent
+(ghosts included), with L-inf/L2 norms for reading.  Requires the reference
+tree; the JSON it writes is the fixture the tests use.
+
+    make -C oracle ref && python tests/golden/make_goldens.py
+"""
+import hashlib
+import json
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from oracle import oracle as O  # noqa: E402
+
+FIELDS = O.STATE_FIELDS
+
+# name, problem, nx, ny, nz, steps, dumps, extra
+CASES = [
+    ("ot_12", "ot", 12, 12, 12, 10, (0, 1, 2, 10), {}),
+    ("ot_pert_10x8x12", "ot", 10, 8, 12, 12, (0, 1, 12), {"amp": 0.05, "seed": 7}),
+    ("ot_pert_16", "ot", 16, 16, 16, 40, (0, 40), {"amp": 0.01, "seed": 5}),
+    ("bw_zerograd_24x4x6", "bw", 24, 4, 6, 15, (0, 1, 15), {"noise": 0.01, "seed": 3}),
+    ("bw_mirror_16x4x4", "bw", 16, 4, 4, 10, (0, 10), {"bcx": "mirror"}),
+    ("bw_walls_16x5x3", "bw", 16, 5, 3, 8, (0, 8), {"bcx": "clamp", "noise": 0.02, "seed": 11}),
+    ("bw_plain_32x3x3", "bw", 32, 3, 3, 20, (0, 20), {}),
+    ("ot_64_config0", "ot", 64, 64, 64, 100, (1, 100), {}),
+]
+
+
+def digest(path):
+    a = np.fromfile(path, dtype="<f8")
+    return {"sha256": hashlib.sha256(a.tobytes()).hexdigest(),
+            "linf": float(np.max(np.abs(a))), "l2": float(np.sqrt(np.sum(a * a)))}
+
+
+def main():
+    out = {"about": "Reference LARE3D (oracle/_ref/ref_driver) dt sequences and state-field "
+                    "SHA-256 digests; see tests/golden/make_goldens.py", "cases": {}}
+    for name, prob, nx, ny, nz, steps, dumps, extra in CASES:
+        with tempfile.TemporaryDirectory() as td:
+            meta = O.run_reference(td, problem=prob, nx=nx, ny=ny, nz=nz, steps=steps, dumps=dumps,
+                                   seed=extra.get("seed", 1), amp=extra.get("amp", 0.0),
+                                   noise=extra.get("noise", 0.0), bcx=extra.get("bcx"), threads=8)
+            snaps = {}
+            for s in dumps:
+                snaps[str(s)] = {f: digest(os.path.join(td, f"s{s:05d}_{f}.f64")) for f in FIELDS}
+        out["cases"][name] = {"problem": prob, "nx": nx, "ny": ny, "nz": nz, "steps": steps,
+                              "seed": extra.get("seed", 1), "amp": extra.get("amp", 0.0),
+                              "noise": extra.get("noise", 0.0), "bcx": extra.get("bcx", ""),
+                              "dt": meta["dt"], "snapshots": snaps}
+        print(name, "done", flush=True)
+    with open(os.path.join(ROOT, "tests", "golden", "goldens.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

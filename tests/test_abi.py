@@ -1,0 +1,49 @@
+"""The C-ABI library: it loads, exports every entry point include/lare_hip.h
+declares, and its problem set-up reproduces the reference's initial state
+(no GPU call is made here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import sams_amd.lare as L
+from lare_testlib import case_problem, goldens, sha
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    syms = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if fn.endswith(".h"):
+            with open(os.path.join(inc, fn)) as f:
+                text = f.read()
+            syms |= set(re.findall(r"^\s*(?:const\s+)?\w+\s*\**\s*(lare_\w+)\s*\(", text, re.M))
+    return sorted(syms)
+
+
+def test_library_exports_every_declared_symbol():
+    syms = declared_symbols()
+    assert "lare_step" in syms and "lare_ctx_create" in syms
+    lib = ctypes.CDLL(L.LIB_PATH)
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_errors_are_reported_not_raised():
+    d = L.LareDesc()
+    d.nx = d.ny = d.nz = 1
+    h = ctypes.c_void_p()
+    rc = L.lib().lare_ctx_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc != 0
+    assert b"at least 3" in L.lib().lare_last_error()
+
+
+@pytest.mark.parametrize("name", ["ot_12", "ot_pert_10x8x12", "bw_zerograd_24x4x6", "bw_walls_16x5x3"])
+def test_problem_initial_state_matches_reference(name):
+    c = goldens()[name]
+    p = case_problem(c)
+    for f in L.STATE_FIELDS:
+        assert sha(L.problem_fill(p, f)) == c["snapshots"]["0"][f]["sha256"], f
