@@ -123,6 +123,11 @@ int lare_steps(lare_ctx *ctx, int32_t nsteps, double *dt_seq);
 /* Wait for all work enqueued on the context. */
 int lare_sync(lare_ctx *ctx);
 
+/* Kernel set (no counterpart in the reference): 1 (default) runs the fused
+ * kernels, 0 one kernel per reference kernel over the reference's ranges.
+ * Both give bit-identical results; 0 exists to localise a parity failure. */
+int lare_set_kernel_set(lare_ctx *ctx, int32_t kernel_set);
+
 /* Per-kernel timing (no counterpart in the reference, whose only timers are
  * per-event host clocks, runnerMacros.h:113-150): between begin and end
  * every kernel launch on the context is bracketed by HIP events on its

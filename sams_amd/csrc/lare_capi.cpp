@@ -114,11 +114,12 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
     g.prm.eta_background = d->eta_background;
     g.prm.eta0 = d->eta0;
     g.prm.j_max = d->j_max;
+    g.kernel_set = 1;
     std::memset(g.bck, 0, sizeof(g.bck));
     std::memset(g.bcv, 0, sizeof(g.bcv));
 
     // one pool: 3D arrays, then the six 1D delta arrays, then the scalars
-    const int n3d = 13 + 28 + 6 + 10;
+    const int n3d = 13 + 28 + 6 + 10 + 5;
     const size_t lmax = (size_t)std::max(d->nx, std::max(d->ny, d->nz)) + 5;
     const size_t bytes3 = box * sizeof(double);
     g.pool_bytes = n3d * bytes3 + 6 * lmax * sizeof(double) + 256;
@@ -149,6 +150,7 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
     for (double **q : met) *q = take3();
     double **rem[] = {&a.rho1, &a.cv2, &a.cvc1, &a.db1, &a.rrho_v, &a.rho_v1, &a.rflux, &a.cv2v, &a.wsh, &a.dmv};
     for (double **q : rem) *q = take3();
+    for (int k = 0; k < 5; ++k) a.fl[k] = take3();
     double **ax1[] = {&a.dxb, &a.dyb, &a.dzb, &a.dxc, &a.dyc, &a.dzc};
     for (double **q : ax1) {
         *q = (double *)p;
@@ -406,5 +408,14 @@ extern "C" int lare_profile_get(lare_ctx *c, int32_t i, char *name, int32_t name
     if (calls) *calls = k.calls;
     if (total_ms) *total_ms = k.ms;
     if (bytes) *bytes = k.bytes;
+    return 0;
+}
+
+extern "C" int lare_set_kernel_set(lare_ctx *c, int32_t kernel_set)
+{
+    if (!c) return fail(-1, "lare_set_kernel_set: null context");
+    if (kernel_set < 0 || kernel_set > 1) return fail(-1, "lare_set_kernel_set: unknown kernel set");
+    HIPCHK(hipStreamSynchronize(c->g.stream));
+    c->g.kernel_set = kernel_set;
     return 0;
 }

@@ -49,6 +49,8 @@ struct Arr {
     double *dxab, *dyab, *dzab, *dxac, *dyac, *dzac;
     // remap work arrays (remap.cpp:30-35 and the per-sweep flux)
     double *rho1, *cv2, *cvc1, *db1, *rrho_v, *rho_v1, *rflux, *cv2v, *wsh, *dmv;
+    // fluxes of the fused sweep (two CT or energy fluxes, three momentum)
+    double *fl[5];
     // 1D axis deltas, index i stored at [i + 2]
     double *dxb, *dyb, *dzb, *dxc, *dyc, *dzc;
     Scal *s;

@@ -48,10 +48,25 @@ struct Grid {
     void *pool;          // one allocation holding every device array
     size_t pool_bytes;
     Prof prof;
+    int kernel_set;      // 0: one kernel per reference kernel, 1: fused
 };
 
 // Metric arrays from the 1D deltas (LARE3D::grid, setup.cpp:337-390).
 void build_metric(Grid &g);
+
+// Boundary fills used by the fused sweep (lare_kernels.hip).
+void dm_bcs(Grid &g, int dim);
+void sweep_bcs(Grid &g);
+void remap_v_fill(Grid &g);
+void velocity_fill(Grid &g);
+void energy_density_fill(Grid &g);
+void dt_final(Grid &g);
+void resistive_substeps(Grid &g);
+// Fused Lagrangian events (lare_fused.hip).
+void fused_start(Grid &g);
+void fused_half(Grid &g);
+// One fused directional remap sweep (lare_fused.hip).
+void fused_sweep(Grid &g, int d, double xpass, double ypass, double zpass);
 
 // The three LARE3D events (shared_data.h:397-419), enqueued on g.stream.
 void step_start(Grid &g);

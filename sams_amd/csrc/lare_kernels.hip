@@ -25,40 +25,7 @@ using lare::R3;
     if (iz > R.z1 || iy > R.y1) return;                                  \
     (void)ix;
 
-#define RHO A.f[0]
-#define EE A.f[1]
-#define EI A.f[2]
-#define VX A.f[3]
-#define VY A.f[4]
-#define VZ A.f[5]
-#define BX A.f[6]
-#define BY A.f[7]
-#define BZ A.f[8]
-#define VX1 A.f[9]
-#define VY1 A.f[10]
-#define VZ1 A.f[11]
-#define DM A.f[12]
-
-// Reference min/max semantics (include/pp/utils.h:277-335).
-__device__ __forceinline__ double pmin(double a, double b) { return a < b ? a : b; }
-__device__ __forceinline__ double pmax(double a, double b) { return a > b ? a : b; }
-__device__ __forceinline__ double pmin_list3(double a, double b, double c)
-{
-    double m = a;
-    if (b < m) m = b;
-    if (c < m) m = c;
-    return m;
-}
-__device__ __forceinline__ double pmin_list4(double a, double b, double c, double d)
-{
-    double m = a;
-    if (b < m) m = b;
-    if (c < m) m = c;
-    if (d < m) m = d;
-    return m;
-}
-
-static constexpr double kSixth = 1.0 / 6.0;  // constants.h:46
+#include "lare_math.h"
 
 // ------------------------------------------------------------- utilities
 
@@ -729,16 +696,6 @@ __global__ void k_cvc1(Dev D, R3 R)
     A.cvc1[c] = 0.125 * PLAIN8(A.cv1, c);
 }
 
-// The limited-slope core shared by the fifteen remap flux kernels
-// (xremap.cpp:304-316 and twins).
-__device__ __forceinline__ double vl_di(double sign_v, double dfi, double dfu, double phi, double dxci, double dxcu, double dxbu)
-{
-    double Da = (2.0 - phi) * fabs(dfi) / dxci + (1.0 + phi) * fabs(dfu) / dxcu;
-    Da = Da * kSixth;
-    double ss = 0.5 * ((dfi >= 0.0 ? 1.0 : -1.0) + (dfu >= 0.0 ? 1.0 : -1.0));
-    return sign_v * ss * pmin_list3(fabs(Da) * dxbu, fabs(dfi), fabs(dfu));
-}
-
 // Constrained-transport flux of a face magnetic flux B along direction d
 // (stride sd): v_advect = mean of w at c and c+ow, db1 averaged with c+tw.
 // vx_by_flux, vx_bz_flux (xremap.cpp:268-371), vy_bx_flux, vy_bz_flux
@@ -1196,9 +1153,51 @@ void remap_sweep(Grid &g, int d, double xpass, double ypass, double zpass)
 
 }  // namespace
 
+void dm_bcs(Grid &g, int dim) { bcs(g, kDm, 1, dim); }
+void sweep_bcs(Grid &g) { boundary_conditions(g); }
+void remap_v_fill(Grid &g) { remap_v_bcs(g); }
+void velocity_fill(Grid &g) { velocity_bcs(g); }
+void energy_density_fill(Grid &g)
+{
+    energy_bcs(g);
+    density_bcs(g);
+}
+void dt_final(Grid &g)
+{
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, g.stream, g.dev, g.prm.dt_multiplier);
+}
+
+// Two resistive substeps of dt/2 (lagran.cpp:157-170: int(dt/dtr)+1 with
+// dtr = dt), each eta_calc + resistive_effects (478-505).
+void resistive_substeps(Grid &g)
+{
+    const int nx = g.dev.g.nx, ny = g.dev.g.ny, nz = g.dev.g.nz;
+    const Params &p = g.prm;
+    const R3 C = rng(-1, nx + 2, -1, ny + 2, -1, nz + 2);
+    const R3 V = rng(-1, nx + 1, -1, ny + 1, -1, nz + 1);
+    for (int i = 0; i < 2; ++i) {
+        launch3(k_eta_calc, 32, g, V, p.j_max, p.eta_background, p.eta0);
+        launch3(k_res_copy, 48, g, C);
+        launch3(k_rkstep, 64, g, rng(0, nx, 0, ny, 0, nz));
+        launch3(k_bstep, 32, g, rng(0, nx, 1, ny, 1, nz), 0);
+        launch3(k_bstep, 32, g, rng(1, nx, 0, ny, 1, nz), 1);
+        launch3(k_bstep, 32, g, rng(1, nx, 1, ny, 0, nz), 2);
+        bfield_bcs(g);
+        launch3(k_res_energy, 32, g, rng(1, nx, 1, ny, 1, nz));
+        energy_bcs(g);
+        // the second rkstep of resistive_effects (lagran.cpp:504) only
+        // rewrites flux_x..z and curlb, which nothing reads before the
+        // next rkstep recomputes them
+    }
+}
+
 // LARE3D::lagrangian_step (lagran.cpp:95-173).
 void step_start(Grid &g)
 {
+    if (g.kernel_set == 1) {
+        fused_start(g);
+        return;
+    }
     const int nx = g.dev.g.nx, ny = g.dev.g.ny, nz = g.dev.g.nz;
     const Params &p = g.prm;
     const R3 C = rng(-1, nx + 2, -1, ny + 2, -1, nz + 2);
@@ -1216,23 +1215,7 @@ void step_start(Grid &g)
     // controlFunctions::calculateTimestep -> set_dt (lagran.cpp:156, 380-443)
     launch3(k_set_dt, 104, g, rng(0, nx, 0, ny, 0, nz), p.gas_gamma, p.mu0);
     hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, g.stream, g.dev, p.dt_multiplier);
-    if (p.resistive) {
-        // two substeps of dt/2 (int(dt/dtr)+1 with dtr = dt)
-        for (int i = 0; i < 2; ++i) {
-            launch3(k_eta_calc, 32, g, V, p.j_max, p.eta_background, p.eta0);
-            launch3(k_res_copy, 48, g, C);
-            launch3(k_rkstep, 64, g, rng(0, nx, 0, ny, 0, nz));
-            launch3(k_bstep, 32, g, rng(0, nx, 1, ny, 1, nz), 0);
-            launch3(k_bstep, 32, g, rng(1, nx, 0, ny, 1, nz), 1);
-            launch3(k_bstep, 32, g, rng(1, nx, 1, ny, 0, nz), 2);
-            bfield_bcs(g);
-            launch3(k_res_energy, 32, g, rng(1, nx, 1, ny, 1, nz));
-            energy_bcs(g);
-            // the second rkstep of resistive_effects (lagran.cpp:504) only
-            // rewrites flux_x..z and curlb, which nothing reads before the
-            // next rkstep recomputes them
-        }
-    }
+    if (p.resistive) resistive_substeps(g);
     // predictor_step (lagran.cpp:592-737)
     launch3(k_bfield_cv1, 112, g, C);
     launch3(k_energy_pred, 88, g, rng(0, nx + 1, 0, ny + 1, 0, nz + 1), p.gas_gamma);
@@ -1247,6 +1230,10 @@ void step_start(Grid &g)
 // and is idempotent for the built-in conditions, so it is issued once.
 void step_half(Grid &g)
 {
+    if (g.kernel_set == 1) {
+        fused_half(g);
+        return;
+    }
     const int nx = g.dev.g.nx, ny = g.dev.g.ny, nz = g.dev.g.nz;
     launch3(k_shock_heating, 88, g, rng(0, nx + 1, 0, ny + 1, 0, nz + 1));
     launch3(k_vel_update, 104, g, rng(0, nx, 0, ny, 0, nz));
@@ -1264,9 +1251,15 @@ void step_end(Grid &g)
     launch3(k_b_area, 24, g, rng(-2, nx + 2, -1, ny + 2, -1, nz + 2), 0, 0);
     launch3(k_b_area, 24, g, rng(-1, nx + 2, -2, ny + 2, -1, nz + 2), 1, 0);
     launch3(k_b_area, 24, g, rng(-1, nx + 2, -1, ny + 2, -2, nz + 2), 2, 0);
-    remap_sweep(g, 0, 1.0, 1.0, 1.0);
-    remap_sweep(g, 1, 0.0, 1.0, 1.0);
-    remap_sweep(g, 2, 0.0, 0.0, 1.0);
+    if (g.kernel_set == 0) {
+        remap_sweep(g, 0, 1.0, 1.0, 1.0);
+        remap_sweep(g, 1, 0.0, 1.0, 1.0);
+        remap_sweep(g, 2, 0.0, 0.0, 1.0);
+    } else {
+        fused_sweep(g, 0, 1.0, 1.0, 1.0);
+        fused_sweep(g, 1, 0.0, 1.0, 1.0);
+        fused_sweep(g, 2, 0.0, 0.0, 1.0);
+    }
     launch3(k_b_area, 24, g, rng(-2, nx + 2, -1, ny + 2, -1, nz + 2), 0, 1);
     launch3(k_b_area, 24, g, rng(-1, nx + 2, -2, ny + 2, -1, nz + 2), 1, 1);
     launch3(k_b_area, 24, g, rng(-1, nx + 2, -1, ny + 2, -2, nz + 2), 2, 1);

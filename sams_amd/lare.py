@@ -90,6 +90,7 @@ def lib():
             "lare_problem_fill": (ctypes.c_int, [P(LareProblem), ctypes.c_int, P(ctypes.c_double)]),
             "lare_problem_bcs": (ctypes.c_int, [vp, P(LareProblem)]),
             "lare_problem_init": (ctypes.c_int, [vp, P(LareProblem)]),
+            "lare_set_kernel_set": (ctypes.c_int, [vp, ctypes.c_int32]),
             "lare_profile_begin": (ctypes.c_int, [vp]),
             "lare_profile_end": (ctypes.c_int, [vp]),
             "lare_profile_count": (ctypes.c_int, [vp, P(ctypes.c_int32)]),
@@ -183,6 +184,10 @@ class Lare:
 
     def set_problem_bcs(self, p):
         _check(lib().lare_problem_bcs(self.h, ctypes.byref(p)), "lare_problem_bcs")
+
+    def set_kernel_set(self, k):
+        """1: fused kernels (default); 0: one kernel per reference kernel."""
+        _check(lib().lare_set_kernel_set(self.h, int(k)), "lare_set_kernel_set")
 
     def get_dt(self):
         d = ctypes.c_double()

@@ -26,12 +26,16 @@ def _state_diffs(dev, orc, fields=L.STATE_FIELDS):
     return out
 
 
-@pytest.mark.parametrize("name", ["ot_pert_10x8x12", "bw_zerograd_24x4x6", "bw_walls_16x5x3"])
-def test_events_of_first_two_steps_match_oracle(name):
+@pytest.mark.parametrize("kernel_set", [0, 1])
+@pytest.mark.parametrize("name", ["ot_pert_10x8x12", "bw_zerograd_24x4x6", "bw_walls_16x5x3",
+                                  "bw_mirror_16x4x4"])
+def test_events_of_first_two_steps_match_oracle(name, kernel_set):
     """Event by event (startOfTimestep, halfTimestep, endOfTimestep) for two
-    steps: dt, the half-step velocities and every state field."""
+    steps: dt, the half-step velocities and every state field; for the
+    one-kernel-per-reference-kernel set (0) and the fused set (1)."""
     p = case_problem(CASES[name])
     dev = L.Lare.from_problem(p)
+    dev.set_kernel_set(kernel_set)
     orc = make_oracle(p)
     for step in (1, 2):
         dt_d = dev.start_of_timestep()
@@ -49,11 +53,13 @@ def test_events_of_first_two_steps_match_oracle(name):
         assert not diffs, f"step {step} after endOfTimestep: {diffs}"
 
 
+@pytest.mark.parametrize("kernel_set", [0, 1])
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_reference_goldens(name):
+def test_reference_goldens(name, kernel_set):
     """Whole runs against the reference: the dt sequence and state digests."""
     c = CASES[name]
     dev = L.Lare.from_problem(case_problem(c))
+    dev.set_kernel_set(kernel_set)
     snaps = c["snapshots"]
     done = 0
     for s in sorted(int(k) for k in snaps):
@@ -77,10 +83,12 @@ def test_oracle_parity_random_perturbed_orszag_tang():
     assert not _state_diffs(dev, orc)
 
 
-def test_resistive_brio_wu_matches_oracle():
+@pytest.mark.parametrize("kernel_set", [0, 1])
+def test_resistive_brio_wu_matches_oracle(kernel_set):
     """Resistive MHD (eta = 1e-3): two half-dt substeps per step."""
     p = L.problem("bw", n=24, ny=5, nz=4, seed=4, noise=0.01, eta=1e-3)
     dev = L.Lare.from_problem(p)
+    dev.set_kernel_set(kernel_set)
     orc = make_oracle(p)
     for s in range(6):
         assert dev.step().hex() == orc.step().hex(), f"dt of step {s + 1}"
