@@ -37,7 +37,7 @@ __device__ __forceinline__ int sdir(const Geo &G)
 // R1: cell volumes before (cv1) and after (cv2) the remap of DIR and the
 // Lagrangian cell width db1 (xremap.cpp:26-70 and twins), centred extent.
 template <int DIR>
-__global__ void f_prep(Dev D, R3 R, double xpass, double ypass, double zpass)
+KERNEL void f_prep(Dev D, R3 R, double xpass, double ypass, double zpass)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -87,7 +87,7 @@ __global__ void f_prep(Dev D, R3 R, double xpass, double ypass, double zpass)
 }
 
 // R2: vertex volume before remap, cvc1 = 0.125 * (8 cells of cv1).
-__global__ void f_cvc1(Dev D, R3 R)
+KERNEL void f_cvc1(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -135,7 +135,7 @@ __device__ __forceinline__ double ct_flux(const Arr &A, const Geo &G, int c, con
 // The second flux reads the B component the first flux's updates leave
 // untouched, so both can be computed before any update.
 template <int DIR>
-__global__ void f_ct_fluxes(Dev D, R3 R)
+KERNEL void f_ct_fluxes(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -156,7 +156,7 @@ __global__ void f_ct_fluxes(Dev D, R3 R)
 // R4: the four CT updates of the sweep in their reference order, each over
 // its own range (xremap.cpp:85-114, yremap.cpp:93-127, zremap.cpp:97-117).
 template <int DIR>
-__global__ void f_ct_update(Dev D, R3 R)
+KERNEL void f_ct_update(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -214,7 +214,7 @@ __device__ __forceinline__ double face_area(const Arr &A, int ix, int iy, int iz
 // ghost faces, where dm is set to the zero of the reference's pw::assign(dm,
 // 0.0) before the single-dimension boundary fill overwrites them.
 template <int DIR>
-__global__ void f_mass_flux(Dev D, R3 R)
+KERNEL void f_mass_flux(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -273,7 +273,7 @@ __device__ __forceinline__ double energy_flux(const Arr &A, const Geo &G, int c,
 // (range -1..n+1 along DIR, 0..n across) and the vertex mass flux dmv
 // (range -1..n along DIR, 0..n across).  All read only pre-update values.
 template <int DIR>
-__global__ void f_energy_vertex(Dev D, R3 R)
+KERNEL void f_energy_vertex(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -334,7 +334,7 @@ __device__ __forceinline__ double mom_flux(const Arr &A, const Geo &G, int c, co
 
 // R7: the three momentum fluxes over (-1..n along DIR, 0..n across).
 template <int DIR>
-__global__ void f_mom_fluxes(Dev D, R3 R)
+KERNEL void f_mom_fluxes(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -361,7 +361,7 @@ __global__ void f_mom_fluxes(Dev D, R3 R)
 // 1..n (xremap.cpp:121-154), vertex density after remap and the three
 // velocities on vertices 0..n (xremap.cpp:180-262).
 template <int DIR>
-__global__ void f_update(Dev D, R3 R)
+KERNEL void f_update(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -469,7 +469,7 @@ void fused_sweep(Grid &g, int d, double xpass, double ypass, double zpass)
 // values of lagran.cpp:113-125; with RES, also stores p_e (the predictor
 // reads the prologue value after the resistive heating changed e_e).
 template <int RES>
-__global__ void g_cs(Dev D, R3 R, double gas_gamma)
+KERNEL void g_cs(Dev D, R3 R, double gas_gamma)
 {
     KBEGIN
     if (ix == R.x0 && iy == R.y0 && iz == R.z0) A.s->dtmin_bits = 0x7FEFFFFFFFFFFFFFull;
@@ -491,7 +491,7 @@ __global__ void g_cs(Dev D, R3 R, double gas_gamma)
 
 // L2 (-1..n+1)^3: vertex density rho_v (lagran.cpp:128-153) and the vertex
 // sound speed cs_v (lagran.cpp:199-216).
-__global__ void g_vertex(Dev D, R3 R)
+KERNEL void g_vertex(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -542,7 +542,7 @@ __device__ __forceinline__ double g_edge_visc(const Arr &A, double dt, double vi
 
 // L3: alpha1, alpha2, alpha3 (lagran.cpp:211-274), each over its own range
 // inside the union (-1..n+1, -1..n+2, 0..n+2).  dt is the previous step's.
-__global__ void g_alpha(Dev D, R3 R, double visc1, double visc2_norm)
+KERNEL void g_alpha(Dev D, R3 R, double visc1, double visc2_norm)
 {
     KBEGIN
     const int sx = G.sx, sy = G.sy;
@@ -575,7 +575,7 @@ __global__ void g_alpha(Dev D, R3 R, double visc1, double visc2_norm)
 // L4 over (0..n+1)^3: p_visc and visc_heat (lagran.cpp:279-323); on (0..n)^3
 // also the viscous forces (330-376), stored as fx_visc, and the CFL minimum
 // (set_dt, 380-443) reduced per wavefront, per block and by atomicMin.
-__global__ void g_visc_dt(Dev D, R3 R, double gas_gamma, double mu0)
+KERNEL void g_visc_dt(Dev D, R3 R, double gas_gamma, double mu0)
 {
     const Geo G = D.g;
     const Arr A = D.a;
@@ -669,7 +669,7 @@ __global__ void g_visc_dt(Dev D, R3 R, double gas_gamma, double mu0)
 // with bx1 *= cv1 (598-601); on (0..n+1)^3 also the energy/pressure
 // predictor (604-614), storing the predicted p_e and p_i.
 template <int RES>
-__global__ void g_predict(Dev D, R3 R, double gas_gamma)
+KERNEL void g_predict(Dev D, R3 R, double gas_gamma)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -740,7 +740,7 @@ __global__ void g_predict(Dev D, R3 R, double gas_gamma)
 
 // L7 over (0..n)^3: forces and half-step velocities (lagran.cpp:619-731);
 // fx_visc + fx is stored back into fx_visc for the corrector.
-__global__ void g_forces(Dev D, R3 R, double mu0)
+KERNEL void g_forces(Dev D, R3 R, double mu0)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -808,7 +808,7 @@ __global__ void g_forces(Dev D, R3 R, double mu0)
 
 // C1 over (0..n+1)^3: shock_heating (lagran.cpp:884-993) with the dy*dy
 // factors of a9 and a11 (lagran.cpp:987).
-__global__ void g_shock_heating(Dev D, R3 R)
+KERNEL void g_shock_heating(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -832,7 +832,7 @@ __global__ void g_shock_heating(Dev D, R3 R)
 
 // C2 over (0..n)^3: velocity correction (lagran.cpp:754-758) with the sum
 // fx_visc + fx stored by g_forces.
-__global__ void g_vel_update(Dev D, R3 R)
+KERNEL void g_vel_update(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -845,7 +845,7 @@ __global__ void g_vel_update(Dev D, R3 R)
 
 // C3 over (1..n)^3: density and energy correction (lagran.cpp:762-796).  The
 // cv1 it also writes in the reference is recomputed by the remap before use.
-__global__ void g_rho_e_update(Dev D, R3 R)
+KERNEL void g_rho_e_update(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);

@@ -7,16 +7,18 @@
 
 namespace lare {
 
-// Built-in boundary condition of one registered variable (device view).
+// Built-in boundary conditions of one registered variable along one
+// dimension (device view of one launch): the stagger along that dimension
+// and across it, and the kind/value of the lower and upper edge.
 struct BCVar {
     double *p;
-    int stag[3];        // 1 = HALF_CELL in that dimension, 0 = CENTRED
-    int kind[3][2];     // enum lare_bc_kind per dimension and edge
-    double val[3][2];   // clamp values
+    int sd, sa, sb;     // 1 = HALF_CELL: along dim, then the two other dims
+    int k0, k1;         // enum lare_bc_kind of the lower / upper edge
+    double v0, v1;      // clamp values
 };
-// Up to four variables filled by one launch (blockIdx.z selects).
+// Up to nine variables filled by one launch (blockIdx.z selects).
 struct BCSet {
-    BCVar v[4];
+    BCVar v[9];
 };
 
 // Physics scalars of simulationData that the step reads.

@@ -29,7 +29,7 @@ using lare::R3;
 
 // ------------------------------------------------------------- utilities
 
-__global__ void k_fill(double *p, long n, double v)
+KERNEL void k_fill(double *p, long n, double v)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -38,7 +38,7 @@ __global__ void k_fill(double *p, long n, double v)
 // ------------------------------------------------- Lagrangian: prologue
 
 // lagran.cpp:113-125 over the centred extent; also resets the CFL minimum.
-__global__ void k_prologue1(Dev D, R3 R, double gas_gamma)
+KERNEL void k_prologue1(Dev D, R3 R, double gas_gamma)
 {
     KBEGIN
     if (ix == R.x0 && iy == R.y0 && iz == R.z0) A.s->dtmin_bits = 0x7FEFFFFFFFFFFFFFull;  // DBL_MAX
@@ -61,7 +61,7 @@ __global__ void k_prologue1(Dev D, R3 R, double gas_gamma)
      (X)[c + G.sy + 1] + (X)[c + G.sx + G.sy + 1])
 
 // lagran.cpp:128-153 over (-1..n+1)^3.
-__global__ void k_prologue2(Dev D, R3 R)
+KERNEL void k_prologue2(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -74,7 +74,7 @@ __global__ void k_prologue2(Dev D, R3 R)
 // ------------------------------------------------- shock viscosity
 
 // lagran.cpp:190-197 over the centred extent.
-__global__ void k_cs(Dev D, R3 R, double gas_gamma)
+KERNEL void k_cs(Dev D, R3 R, double gas_gamma)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -85,7 +85,7 @@ __global__ void k_cs(Dev D, R3 R, double gas_gamma)
 }
 
 // lagran.cpp:199-216 over (-1..n+1)^3.
-__global__ void k_cs_v(Dev D, R3 R)
+KERNEL void k_cs_v(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -127,7 +127,7 @@ __device__ __forceinline__ double edge_viscosity(const Arr &A, double dt, double
 
 // alpha1 (lagran.cpp:211-232), alpha2 (235-253), alpha3 (256-274): dir
 // selects the edge direction.  dt is the previous step's dt.
-__global__ void k_alpha(Dev D, R3 R, int dir, double visc1, double visc2_norm)
+KERNEL void k_alpha(Dev D, R3 R, int dir, double visc1, double visc2_norm)
 {
     KBEGIN
     const double dt = A.s->dt;
@@ -159,7 +159,7 @@ __global__ void k_alpha(Dev D, R3 R, int dir, double visc1, double visc2_norm)
 
 // p_visc and visc_heat (lagran.cpp:279-323) over (0..n+1)^3; p_visc starts
 // from the zero fill of lagran.cpp:187.
-__global__ void k_pvisc(Dev D, R3 R)
+KERNEL void k_pvisc(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -204,7 +204,7 @@ __global__ void k_pvisc(Dev D, R3 R)
 }
 
 // Viscous forces (lagran.cpp:330-376) over (0..n)^3.
-__global__ void k_fvisc(Dev D, R3 R)
+KERNEL void k_fvisc(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -234,7 +234,7 @@ __global__ void k_fvisc(Dev D, R3 R)
 // reduced per wavefront with lane shuffles, per block through LDS, and
 // across blocks with one 64-bit atomicMin on the bit pattern (order-free
 // for the non-negative values the CFL produces).
-__global__ void k_set_dt(Dev D, R3 R, double gas_gamma, double mu0)
+KERNEL void k_set_dt(Dev D, R3 R, double gas_gamma, double mu0)
 {
     const Geo G = D.g;
     const Arr A = D.a;
@@ -296,7 +296,7 @@ __global__ void k_set_dt(Dev D, R3 R, double gas_gamma, double mu0)
 
 // dt = dt_multiplier * min; the runner keeps min(dt, DBL_MAX)
 // (runner.h:158-169, shared_data.h:428-440); dtr = dt (lagran.cpp:441-442).
-__global__ void k_dt_final(Dev D, double dt_multiplier)
+KERNEL void k_dt_final(Dev D, double dt_multiplier)
 {
     const Arr A = D.a;
     double red = __longlong_as_double((long long)A.s->dtmin_bits);
@@ -309,12 +309,12 @@ __global__ void k_dt_final(Dev D, double dt_multiplier)
 }
 
 // End of a step: count it.
-__global__ void k_step_done(Dev D) { D.a.s->nstep += 1; }
+KERNEL void k_step_done(Dev D) { D.a.s->nstep += 1; }
 
 // ------------------------------------------------- resistive path
 
 // eta_calc (lagran.cpp:445-476) over (-1..n+1)^3.
-__global__ void k_eta_calc(Dev D, R3 R, double j_max, double eta_background, double eta0)
+KERNEL void k_eta_calc(Dev D, R3 R, double j_max, double eta_background, double eta0)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -336,7 +336,7 @@ __global__ void k_eta_calc(Dev D, R3 R, double j_max, double eta_background, dou
 // Copy of B into bx1..bz1 over the centred extent: the reference's
 // pw::assign(data.bx1, data.bx) at lagran.cpp:482 throws on the extent
 // mismatch; this is its element-wise reading over bx1's extent.
-__global__ void k_res_copy(Dev D, R3 R)
+KERNEL void k_res_copy(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -346,7 +346,7 @@ __global__ void k_res_copy(Dev D, R3 R)
 }
 
 // rkstep (lagran.cpp:507-535) over (0..n)^3.
-__global__ void k_rkstep(Dev D, R3 R)
+KERNEL void k_rkstep(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -369,7 +369,7 @@ __global__ void k_rkstep(Dev D, R3 R)
 }
 
 // bstep (lagran.cpp:537-590): comp 0/1/2 updates bx/by/bz over its range.
-__global__ void k_bstep(Dev D, R3 R, int comp)
+KERNEL void k_bstep(Dev D, R3 R, int comp)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -389,7 +389,7 @@ __global__ void k_bstep(Dev D, R3 R, int comp)
 }
 
 // Ohmic heating of the electrons (lagran.cpp:490-500) over (1..n)^3.
-__global__ void k_res_energy(Dev D, R3 R)
+KERNEL void k_res_energy(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -403,7 +403,7 @@ __global__ void k_res_energy(Dev D, R3 R)
 // ------------------------------------------------- predictor
 
 // b_field_and_cv1_update (lagran.cpp:806-882) over (-1..n+2)^3.
-__global__ void k_bfield_cv1(Dev D, R3 R)
+KERNEL void k_bfield_cv1(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -461,7 +461,7 @@ __global__ void k_bfield_cv1(Dev D, R3 R)
 }
 
 // Energy/pressure predictor (lagran.cpp:604-614) over (0..n+1)^3.
-__global__ void k_energy_pred(Dev D, R3 R, double gas_gamma)
+KERNEL void k_energy_pred(Dev D, R3 R, double gas_gamma)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -479,7 +479,7 @@ __global__ void k_energy_pred(Dev D, R3 R, double gas_gamma)
 }
 
 // Forces and half-step velocities (lagran.cpp:619-731) over (0..n)^3.
-__global__ void k_forces(Dev D, R3 R, double mu0)
+KERNEL void k_forces(Dev D, R3 R, double mu0)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -551,7 +551,7 @@ __global__ void k_forces(Dev D, R3 R, double mu0)
 
 // shock_heating (lagran.cpp:884-993) over (0..n+1)^3, with the dy*dy
 // factors of the a9 and a11 terms (lagran.cpp:987).
-__global__ void k_shock_heating(Dev D, R3 R)
+KERNEL void k_shock_heating(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -585,7 +585,7 @@ __global__ void k_shock_heating(Dev D, R3 R)
 }
 
 // Velocity correction (lagran.cpp:754-758) over (0..n)^3.
-__global__ void k_vel_update(Dev D, R3 R)
+KERNEL void k_vel_update(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -597,7 +597,7 @@ __global__ void k_vel_update(Dev D, R3 R)
 }
 
 // Density and energy correction (lagran.cpp:762-796) over (1..n)^3.
-__global__ void k_rho_e_update(Dev D, R3 R)
+KERNEL void k_rho_e_update(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -626,11 +626,11 @@ __global__ void k_rho_e_update(Dev D, R3 R)
 
 // B *= area (remap.cpp:46-50) and B /= (area + none_zero) (91-95) over each
 // component's full extent.
-__global__ void k_b_area(Dev D, R3 R, int comp, int divide)
+KERNEL void k_b_area(Dev D, R3 R, int comp, int divide)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
-    double *B = A.f[6 + comp];
+    double *B = fsel(A, 6 + comp);
     const double *ar = comp == 0 ? A.dxab : comp == 1 ? A.dyab : A.dzab;
     if (divide)
         B[c] /= (ar[c] + DBL_EPSILON);
@@ -641,7 +641,7 @@ __global__ void k_b_area(Dev D, R3 R, int comp, int divide)
 // Cell volumes before/after the remap of direction d, db1, and the
 // per-sweep resets dm = 0 and rho1 = rho (xremap.cpp:24-70,
 // yremap.cpp:23-76, zremap.cpp:25-79) over the centred extent.
-__global__ void k_remap_prep(Dev D, R3 R, int d, double xpass, double ypass, double zpass)
+KERNEL void k_remap_prep(Dev D, R3 R, int d, double xpass, double ypass, double zpass)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -689,7 +689,7 @@ __global__ void k_remap_prep(Dev D, R3 R, int d, double xpass, double ypass, dou
 }
 
 // cvc1 (xremap.cpp:74-81 and twins) over (-1..n+1)^3.
-__global__ void k_cvc1(Dev D, R3 R)
+KERNEL void k_cvc1(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -700,13 +700,13 @@ __global__ void k_cvc1(Dev D, R3 R)
 // (stride sd): v_advect = mean of w at c and c+ow, db1 averaged with c+tw.
 // vx_by_flux, vx_bz_flux (xremap.cpp:268-371), vy_bx_flux, vy_bz_flux
 // (yremap.cpp:291-393), vz_bx_flux, vz_by_flux (zremap.cpp:250-346).
-__global__ void k_b_flux(Dev D, R3 R, int bcomp, int wcomp, int sd, int ow, int tw)
+KERNEL void k_b_flux(Dev D, R3 R, int bcomp, int wcomp, int sd, int ow, int tw)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
     const double dt = A.s->dt;
-    const double *B = A.f[6 + bcomp];
-    const double *w = A.f[9 + wcomp];
+    const double *B = fsel(A, 6 + bcomp);
+    const double *w = fsel(A, 9 + wcomp);
     const double *db1 = A.db1;
     double v_advect = (w[c] + w[c + ow]) * 0.5;
     double o_v = v_advect * dt;
@@ -736,11 +736,11 @@ __global__ void k_b_flux(Dev D, R3 R, int bcomp, int wcomp, int sd, int ow, int 
 // B[c] = B[c] + sgn*flux[c] - sgn*flux[c - so]: the CT updates that follow
 // each B flux (xremap.cpp:87-114 and twins).  sgn = -1 computes
 // B - flux + flux(m), sgn = +1 computes B + flux - flux(m).
-__global__ void k_b_ct(Dev D, R3 R, int bcomp, int so, int sgn)
+KERNEL void k_b_ct(Dev D, R3 R, int bcomp, int so, int sgn)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
-    double *B = A.f[6 + bcomp];
+    double *B = fsel(A, 6 + bcomp);
     const double *fl = A.rflux;
     if (sgn < 0)
         B[c] = B[c] - fl[c] + fl[c - so];
@@ -750,7 +750,7 @@ __global__ void k_b_ct(Dev D, R3 R, int bcomp, int so, int sgn)
 
 // x_mass_flux (xremap.cpp:373-422), y_mass_flux (yremap.cpp:395-444),
 // z_mass_flux (zremap.cpp:349-397).
-__global__ void k_mass_flux(Dev D, R3 R, int d)
+KERNEL void k_mass_flux(Dev D, R3 R, int d)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -785,7 +785,7 @@ __global__ void k_mass_flux(Dev D, R3 R, int d)
 }
 
 // rho = (rho1*cv1 + dm(m) - dm) / cv2 over (1..n)^3.
-__global__ void k_rho_remap(Dev D, R3 R, int sd)
+KERNEL void k_rho_remap(Dev D, R3 R, int sd)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -794,14 +794,14 @@ __global__ void k_rho_remap(Dev D, R3 R, int sd)
 
 // x/y/z_energy_flux (xremap.cpp:429-483, yremap.cpp:452-506,
 // zremap.cpp:405-459) for energy e (1 electron, 2 ion).
-__global__ void k_energy_flux(Dev D, R3 R, int d, int e)
+KERNEL void k_energy_flux(Dev D, R3 R, int d, int e)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     const int sd = d == 0 ? sx : d == 1 ? sy : 1;
     const double dt = A.s->dt;
-    const double *E = A.f[e];
+    const double *E = fsel(A, e);
     double area, v_advect;
     if (d == 0) {
         area = A.dxab[c];
@@ -832,18 +832,18 @@ __global__ void k_energy_flux(Dev D, R3 R, int d, int e)
 }
 
 // e = (e*cv1*rho1 + flux(m) - flux) / (cv2*rho) over (1..n)^3.
-__global__ void k_energy_remap(Dev D, R3 R, int sd, int e)
+KERNEL void k_energy_remap(Dev D, R3 R, int sd, int e)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
-    double *E = A.f[e];
+    double *E = fsel(A, e);
     E[c] = (E[c] * A.cv1[c] * A.rho1[c] + A.rflux[c - sd] - A.rflux[c]) / (A.cv2[c] * RHO[c]);
 }
 
 // Vertex density before remap (xremap.cpp:159-176, yremap.cpp:163-183,
 // zremap.cpp:150-168); x divides (s*0.125)/cvc1, y and z multiply by
 // (0.125/cvc1).  Also the vertex cv2 (into cv2v) where inside (0..n)^3.
-__global__ void k_vertex_rho(Dev D, R3 R, int d)
+KERNEL void k_vertex_rho(Dev D, R3 R, int d)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -855,7 +855,7 @@ __global__ void k_vertex_rho(Dev D, R3 R, int d)
 }
 
 // Vertex cv2 (xremap.cpp:180-190 and twins) over (0..n)^3 into cv2v.
-__global__ void k_vertex_cv2(Dev D, R3 R)
+KERNEL void k_vertex_cv2(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -863,16 +863,16 @@ __global__ void k_vertex_cv2(Dev D, R3 R)
 }
 
 // Face-centred sweep velocity (xremap.cpp:196-203 and twins) into wsh.
-__global__ void k_vshift(Dev D, R3 R, int wcomp, int sd)
+KERNEL void k_vshift(Dev D, R3 R, int wcomp, int sd)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
-    const double *w = A.f[9 + wcomp];
+    const double *w = fsel(A, 9 + wcomp);
     A.wsh[c] = (w[c] + w[c + sd]) * 0.5;
 }
 
 // Vertex mass flux (xremap.cpp:209-219 and twins) into dmv.
-__global__ void k_dm_shift(Dev D, R3 R)
+KERNEL void k_dm_shift(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -880,7 +880,7 @@ __global__ void k_dm_shift(Dev D, R3 R)
 }
 
 // Vertex density after remap (xremap.cpp:225-233 and twins) over (0..n)^3.
-__global__ void k_rho_v1(Dev D, R3 R, int sd)
+KERNEL void k_rho_v1(Dev D, R3 R, int sd)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -891,13 +891,13 @@ __global__ void k_rho_v1(Dev D, R3 R, int sd)
 // for velocity component comp, multiplied by dm as in their second kernel.
 // The z flux takes dyac as its area (zremap.cpp:475) and divides by the
 // product (dzbu*rhou) (zremap.cpp:505).
-__global__ void k_mom_flux(Dev D, R3 R, int d, int comp)
+KERNEL void k_mom_flux(Dev D, R3 R, int d, int comp)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
     const int sd = d == 0 ? G.sx : d == 1 ? G.sy : 1;
     const double dt = A.s->dt;
-    const double *f = A.f[3 + comp];
+    const double *f = fsel(A, 3 + comp);
     double area = d == 0 ? A.dxac[c] : A.dyac[c];
     double v_advect = A.wsh[c];
     double o_v = v_advect * dt * area;
@@ -921,71 +921,87 @@ __global__ void k_mom_flux(Dev D, R3 R, int d, int comp)
 }
 
 // v = (rho_v*v*cvc1 + flux(m) - flux) / (cv2*rho_v1) over (0..n)^3.
-__global__ void k_mom_remap(Dev D, R3 R, int sd, int comp)
+KERNEL void k_mom_remap(Dev D, R3 R, int sd, int comp)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
-    double *v = A.f[3 + comp];
+    double *v = fsel(A, 3 + comp);
     v[c] = (A.rrho_v[c] * v[c] * A.cvc1[c] + A.rflux[c - sd] - A.rflux[c]) / (A.cv2v[c] * A.rho_v1[c]);
 }
 
 // ------------------------------------------------- boundary conditions
 
-// One dimension of the built-in BCs of up to four variables (blockIdx.z):
+// One dimension of the built-in BCs of up to nine variables (blockIdx.z):
 // each thread owns one line along dim and applies the lower edge then the
 // upper edge to it, exactly as the reference's successive slice operations
 // (variableDef.h:539-605, builtInBoundaryConditions.h:54-300) leave it.
 // Lines span the other two dimensions' full extents including ghosts.
-__global__ void k_bc(Dev D, lare::BCSet S, int dim)
+__device__ __forceinline__ void bc_edge(double *p, int c0, int sd, int n, int s, int edge, int kind, double val)
+{
+    if (kind == 0) return;
+    const int LB = s ? 0 : 1, UB = n;
+    const int g0 = edge ? n + 1 - s : (s ? -2 : -1);
+    const int g1 = edge ? n + 2 : 0;
+    const int first = edge ? g0 : g1;
+    const int last = edge ? UB : LB;
+    for (int g = g0; g <= g1; ++g) {
+        double out;
+        if (kind == 1) {
+            out = p[c0 + (edge ? g - n : g + n) * sd];
+        } else if (kind == 2) {
+            out = p[c0 + last * sd];
+        } else if (kind == 3) {
+            const double bv = p[c0 + last * sd];
+            const double boundary = (bv + bv) / 2.0;
+            const double delta = p[c0 + (last - (g - first)) * sd] - boundary;
+            out = boundary - delta;
+        } else {
+            out = val;
+        }
+        p[c0 + g * sd] = out;
+    }
+}
+
+KERNEL void k_bc(Dev D, lare::BCSet S, int dim)
 {
     const Geo G = D.g;
-    const lare::BCVar V = S.v[blockIdx.z];
-    const int n3[3] = {G.nx, G.ny, G.nz};
-    // the two other dimensions, the faster one last
-    const int da = dim == 0 ? 1 : 0;
-    const int db = dim == 2 ? 1 : 2;
-    const int la = V.stag[da] ? -2 : -1, ua = n3[da] + 2;
-    const int lb = V.stag[db] ? -2 : -1, ub = n3[db] + 2;
-    const int nb = ub - lb + 1;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int total = (ua - la + 1) * nb;
-    if (t >= total) return;
-    const int ia = la + t / nb, ib = lb + t % nb;
-    int idx[3];
-    idx[da] = ia;
-    idx[db] = ib;
-    idx[dim] = 0;
-    const int sd = dim == 0 ? G.sx : dim == 1 ? G.sy : 1;
-    const int c0 = IX(idx[0], idx[1], idx[2]);  // line origin at index 0 along dim
-    double *p = V.p;
-    const int n = n3[dim];
-    const int s = V.stag[dim];
-    const int LB = s ? 0 : 1, UB = n;
-    for (int edge = 0; edge < 2; ++edge) {
-        const int kind = V.kind[dim][edge];
-        if (kind == 0) continue;
-        const int g0 = edge ? n + 1 - s : (s ? -2 : -1);
-        const int g1 = edge ? n + 2 : 0;
-        const int first = edge ? g0 : g1;
-        const int last = edge ? UB : LB;
-        const double val = V.val[dim][edge];
-        for (int g = g0; g <= g1; ++g) {
-            double out;
-            if (kind == 1) {
-                out = p[c0 + (edge ? g - n : g + n) * sd];
-            } else if (kind == 2) {
-                out = p[c0 + last * sd];
-            } else if (kind == 3) {
-                const double bv = p[c0 + last * sd];
-                const double boundary = (bv + bv) / 2.0;
-                const double delta = p[c0 + (last - (g - first)) * sd] - boundary;
-                out = boundary - delta;
-            } else {
-                out = val;
-            }
-            p[c0 + g * sd] = out;
-        }
+    lare::BCVar V;
+    switch (blockIdx.z) {
+    case 0: V = S.v[0]; break;
+    case 1: V = S.v[1]; break;
+    case 2: V = S.v[2]; break;
+    case 3: V = S.v[3]; break;
+    case 4: V = S.v[4]; break;
+    case 5: V = S.v[5]; break;
+    case 6: V = S.v[6]; break;
+    case 7: V = S.v[7]; break;
+    default: V = S.v[8]; break;
     }
+    // the two other dimensions, the faster (in memory) one second
+    const int na = dim == 0 ? G.ny : G.nx;
+    const int nb = dim == 2 ? G.ny : G.nz;
+    const int la = V.sa ? -2 : -1, ua = na + 2;
+    const int lb = V.sb ? -2 : -1, ub = nb + 2;
+    const int wb = ub - lb + 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (ua - la + 1) * wb) return;
+    const int ia = la + t / wb, ib = lb + t % wb;
+    int c0, sd, n;
+    if (dim == 0) {
+        c0 = IX(0, ia, ib);
+        sd = G.sx;
+        n = G.nx;
+    } else if (dim == 1) {
+        c0 = IX(ia, 0, ib);
+        sd = G.sy;
+        n = G.ny;
+    } else {
+        c0 = IX(ia, ib, 0);
+        sd = 1;
+        n = G.nz;
+    }
+    bc_edge(V.p, c0, sd, n, V.sd, 0, V.k0, V.v0);
+    bc_edge(V.p, c0, sd, n, V.sd, 1, V.k1, V.v1);
 }
 
 // =================================================================== host
@@ -1031,30 +1047,30 @@ const int kStag[13][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 1, 1}, {1, 1, 1},
 // dims 0, 1, 2 in turn (variableDef.h:601-605), or one dim (dim >= 0).
 void bcs(Grid &g, const int *vars, int nvar, int dim = -1)
 {
-    BCSet S{};
-    bool any = false;
-    for (int i = 0; i < nvar; ++i) {
-        const int v = vars[i];
-        S.v[i].p = g.dev.a.f[v];
-        for (int d = 0; d < 3; ++d) {
-            S.v[i].stag[d] = kStag[v][d];
-            for (int e = 0; e < 2; ++e) {
-                S.v[i].kind[d][e] = g.bck[v][d][e];
-                S.v[i].val[d][e] = g.bcv[v][d][e];
-                any = any || g.bck[v][d][e] != 0;
-            }
-        }
-    }
-    if (!any) return;
     const int n3[3] = {g.dev.g.nx, g.dev.g.ny, g.dev.g.nz};
     for (int d = 0; d < 3; ++d) {
         if (dim >= 0 && d != dim) continue;
+        const int da = d == 0 ? 1 : 0, db = d == 2 ? 1 : 2;
+        BCSet S{};
+        int nv = 0;
+        for (int i = 0; i < nvar; ++i) {
+            const int v = vars[i];
+            if (g.bck[v][d][0] == 0 && g.bck[v][d][1] == 0) continue;
+            BCVar &b = S.v[nv++];
+            b.p = g.dev.a.f[v];
+            b.sd = kStag[v][d];
+            b.sa = kStag[v][da];
+            b.sb = kStag[v][db];
+            b.k0 = g.bck[v][d][0];
+            b.k1 = g.bck[v][d][1];
+            b.v0 = g.bcv[v][d][0];
+            b.v1 = g.bcv[v][d][1];
+        }
+        if (nv == 0) continue;
         // lines: the product of the other two full extents (at most (n+5)^2)
-        int lines = 1;
-        for (int o = 0; o < 3; ++o)
-            if (o != d) lines *= n3[o] + 5;
+        const int lines = (n3[da] + 5) * (n3[db] + 5);
         dim3 blk(256, 1, 1);
-        dim3 grd((unsigned)((lines + 255) / 256), 1, (unsigned)nvar);
+        dim3 grd((unsigned)((lines + 255) / 256), 1, (unsigned)nv);
         Prof::Rec rec{"k_bc", 0.0, nullptr, nullptr};
         if (g.prof.on) {
             (void)hipEventCreate(&rec.a);
@@ -1081,13 +1097,10 @@ void energy_bcs(Grid &g) { bcs(g, kE, 2); }
 void density_bcs(Grid &g) { bcs(g, kRho, 1); }
 void velocity_bcs(Grid &g) { bcs(g, kV, 3); }
 void remap_v_bcs(Grid &g) { bcs(g, kV1, 3); }
-void boundary_conditions(Grid &g)
-{
-    bfield_bcs(g);
-    energy_bcs(g);
-    density_bcs(g);
-    velocity_bcs(g);
-}
+// bfield_bcs, energy_bcs, density_bcs, velocity_bcs (boundary.cpp:27-33):
+// the nine variables are independent, so their fills share launches.
+const int kAll[9] = {6, 7, 8, 1, 2, 0, 3, 4, 5};
+void boundary_conditions(Grid &g) { bcs(g, kAll, 9); }
 
 void remap_sweep(Grid &g, int d, double xpass, double ypass, double zpass)
 {
@@ -1157,11 +1170,8 @@ void dm_bcs(Grid &g, int dim) { bcs(g, kDm, 1, dim); }
 void sweep_bcs(Grid &g) { boundary_conditions(g); }
 void remap_v_fill(Grid &g) { remap_v_bcs(g); }
 void velocity_fill(Grid &g) { velocity_bcs(g); }
-void energy_density_fill(Grid &g)
-{
-    energy_bcs(g);
-    density_bcs(g);
-}
+const int kERho[3] = {1, 2, 0};
+void energy_density_fill(Grid &g) { bcs(g, kERho, 3); }
 void dt_final(Grid &g)
 {
     hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, g.stream, g.dev, g.prm.dt_multiplier);
@@ -1274,7 +1284,7 @@ void step_end(Grid &g)
 // Metric of LARE3D::grid (setup.cpp:337-390), Cartesian, over the extents
 // the reference fills: dxab/dyab/dzab over their staggered extents, cv and
 // the vertex areas over the centred extent.
-__global__ void k_metric(Dev D, R3 R)
+KERNEL void k_metric(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);

@@ -18,6 +18,32 @@
 #define VZ1 A.f[11]
 #define DM A.f[12]
 
+// Every kernel is launched with 256 threads per block; the bound lets the
+// compiler use up to 256 VGPRs per lane before spilling.
+#define KERNEL __global__ __launch_bounds__(256)
+
+// Field pointer by a launch-uniform index.  A switch over constant indices
+// keeps the selection in scalar registers; indexing the kernel-argument
+// array with a variable would copy the whole argument block to scratch.
+__device__ __forceinline__ double *fsel(const lare::Arr &A, int i)
+{
+    switch (i) {
+    case 0: return A.f[0];
+    case 1: return A.f[1];
+    case 2: return A.f[2];
+    case 3: return A.f[3];
+    case 4: return A.f[4];
+    case 5: return A.f[5];
+    case 6: return A.f[6];
+    case 7: return A.f[7];
+    case 8: return A.f[8];
+    case 9: return A.f[9];
+    case 10: return A.f[10];
+    case 11: return A.f[11];
+    default: return A.f[12];
+    }
+}
+
 // Reference min/max semantics (include/pp/utils.h:277-335).
 __device__ __forceinline__ double pmin(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double pmax(double a, double b) { return a > b ? a : b; }
