@@ -128,6 +128,11 @@ int lare_sync(lare_ctx *ctx);
  * Both give bit-identical results; 0 exists to localise a parity failure. */
 int lare_set_kernel_set(lare_ctx *ctx, int32_t kernel_set);
 
+/* Whether lare_steps replays one captured HIP graph of the step (default 1;
+ * no counterpart in the reference).  The graph is recaptured after any
+ * boundary-condition or kernel-set change. */
+int lare_set_graph(lare_ctx *ctx, int32_t on);
+
 /* Per-kernel timing (no counterpart in the reference, whose only timers are
  * per-event host clocks, runnerMacros.h:113-150): between begin and end
  * every kernel launch on the context is bracketed by HIP events on its

@@ -56,6 +56,36 @@ extern "C" const char *lare_last_error(void) { return g_err.c_str(); }
 
 namespace {
 
+// The captured step bakes in the boundary table and the kernel set; any
+// change to either drops it and the next lare_steps captures afresh.
+void drop_graph(lare::Grid &g)
+{
+    if (g.graph_exec) (void)hipGraphExecDestroy(g.graph_exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g.graph_exec = nullptr;
+    g.graph = nullptr;
+}
+
+// Capture one step (the kernels read dt and every other scalar from device
+// memory, so one capture serves every later step).  No memset or host copy
+// is recorded in it.
+hipError_t capture_step(lare::Grid &g)
+{
+    drop_graph(g);
+    hipError_t e = hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    lare::step_start(g);
+    lare::step_half(g);
+    lare::step_end(g);
+    e = hipStreamEndCapture(g.stream, &g.graph);
+    if (e != hipSuccess) return e;
+    return hipGraphInstantiate(&g.graph_exec, g.graph, nullptr, nullptr, 0);
+}
+
+}  // namespace
+
+namespace {
+
 // Linear axis and deltas of axisRegistry::setDomain (axisRegistry.h:973-984,
 // 393-431, 176-201): xb(i) = min + ((max-min)/n) * i on -2..n+2, centres
 // 0.5*(xb(i)+xb(i-1)) on -1..n+2, backward differences with the first
@@ -193,6 +223,7 @@ extern "C" int lare_ctx_destroy(lare_ctx *c)
 {
     if (!c) return 0;
     (void)hipStreamSynchronize(c->g.stream);
+    drop_graph(c->g);
     (void)hipFree(c->g.pool);
     (void)hipStreamDestroy(c->g.stream);
     delete c;
@@ -259,6 +290,7 @@ extern "C" int lare_set_bc(lare_ctx *c, int field, int dim, int edge, int kind, 
     if (kind < LARE_BC_NONE || kind > LARE_BC_CLAMP) return fail(-1, "lare_set_bc: unknown boundary kind");
     c->g.bck[field][dim][edge] = kind;
     c->g.bcv[field][dim][edge] = value;
+    drop_graph(c->g);
     return 0;
 }
 
@@ -328,23 +360,48 @@ extern "C" int lare_steps(lare_ctx *c, int32_t nsteps, double *dt_seq)
 {
     if (!c) return fail(-1, "lare_steps: null context");
     if (nsteps < 0) return fail(-1, "lare_steps: negative step count");
+    lare::Grid &g = c->g;
+    const bool graph = g.use_graph && !g.prof.on && nsteps > 0;
+    if (graph && !g.graph_exec) {
+        hipError_t e = capture_step(g);
+        if (e != hipSuccess) {
+            drop_graph(g);
+            return fail(-3, std::string("lare_steps: capturing the step graph: ") + hipGetErrorString(e));
+        }
+    }
     double *pinned = nullptr;
     if (dt_seq && nsteps > 0) HIPCHK(hipHostMalloc((void **)&pinned, nsteps * sizeof(double), hipHostMallocDefault));
-    for (int i = 0; i < nsteps; ++i) {
-        lare::step_start(c->g);
-        if (pinned)
-            (void)hipMemcpyAsync(pinned + i, &c->g.dev.a.s->dt, sizeof(double), hipMemcpyDeviceToHost, c->g.stream);
-        lare::step_half(c->g);
-        lare::step_end(c->g);
+    int rc = 0;
+    for (int i = 0; i < nsteps && !rc; ++i) {
+        if (graph) {
+            hipError_t e = hipGraphLaunch(g.graph_exec, g.stream);
+            if (e != hipSuccess) rc = fail(-3, std::string("lare_steps: hipGraphLaunch: ") + hipGetErrorString(e));
+        } else {
+            lare::step_start(g);
+            lare::step_half(g);
+            lare::step_end(g);
+        }
+        // the step's dt survives in device memory until the next step's
+        // start overwrites it; this copy is ordered before that on the stream
+        if (pinned && !rc)
+            (void)hipMemcpyAsync(pinned + i, &g.dev.a.s->dt, sizeof(double), hipMemcpyDeviceToHost, g.stream);
     }
-    int rc = after_launch(c, "lare_steps");
-    hipError_t e = hipStreamSynchronize(c->g.stream);
+    if (!rc) rc = after_launch(c, "lare_steps");
+    hipError_t e = hipStreamSynchronize(g.stream);
     if (!rc && e != hipSuccess) rc = fail(-3, std::string("lare_steps: ") + hipGetErrorString(e));
     if (pinned) {
         if (!rc) std::memcpy(dt_seq, pinned, nsteps * sizeof(double));
         (void)hipHostFree(pinned);
     }
     return rc;
+}
+
+extern "C" int lare_set_graph(lare_ctx *c, int32_t on)
+{
+    if (!c) return fail(-1, "lare_set_graph: null context");
+    c->g.use_graph = on != 0;
+    if (!on) drop_graph(c->g);
+    return 0;
 }
 
 extern "C" int lare_sync(lare_ctx *c)
@@ -417,5 +474,6 @@ extern "C" int lare_set_kernel_set(lare_ctx *c, int32_t kernel_set)
     if (kernel_set < 0 || kernel_set > 1) return fail(-1, "lare_set_kernel_set: unknown kernel set");
     HIPCHK(hipStreamSynchronize(c->g.stream));
     c->g.kernel_set = kernel_set;
+    drop_graph(c->g);
     return 0;
 }

@@ -51,6 +51,10 @@ struct Grid {
     size_t pool_bytes;
     Prof prof;
     int kernel_set;      // 0: one kernel per reference kernel, 1: fused
+    // one captured step (start + half + end), replayed by lare_steps
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool use_graph = true;
 };
 
 // Metric arrays from the 1D deltas (LARE3D::grid, setup.cpp:337-390).

@@ -91,6 +91,7 @@ def lib():
             "lare_problem_bcs": (ctypes.c_int, [vp, P(LareProblem)]),
             "lare_problem_init": (ctypes.c_int, [vp, P(LareProblem)]),
             "lare_set_kernel_set": (ctypes.c_int, [vp, ctypes.c_int32]),
+            "lare_set_graph": (ctypes.c_int, [vp, ctypes.c_int32]),
             "lare_profile_begin": (ctypes.c_int, [vp]),
             "lare_profile_end": (ctypes.c_int, [vp]),
             "lare_profile_count": (ctypes.c_int, [vp, P(ctypes.c_int32)]),
@@ -188,6 +189,10 @@ class Lare:
     def set_kernel_set(self, k):
         """1: fused kernels (default); 0: one kernel per reference kernel."""
         _check(lib().lare_set_kernel_set(self.h, int(k)), "lare_set_kernel_set")
+
+    def set_graph(self, on):
+        """Replay one captured HIP graph per step in steps() (default on)."""
+        _check(lib().lare_set_graph(self.h, int(bool(on))), "lare_set_graph")
 
     def get_dt(self):
         d = ctypes.c_double()
