@@ -109,6 +109,16 @@ int lare_set_dt(lare_ctx *ctx, double dt);
  * (runner.h:158-169 -> lagran.cpp:380-443).  If dt_out is not NULL the call
  * waits for the step and returns the new dt there. */
 int lare_start_of_timestep(lare_ctx *ctx, double *dt_out);
+/* The same event in the two halves the runner's timestep protocol needs
+ * (lagran.cpp:95-173): lare_viscosity_dt runs the prologue, the shock
+ * viscosity and the CFL reduction and returns dt (already times
+ * dt_multiplier) and dtr -- what LARE3D::set_dt hands to
+ * runner::calculateTimestep (runner.h:158-169); lare_lagrange then runs the
+ * resistive substeps (int(dt/dtr)+1 of them) and predictor_step with the dt
+ * the runner settled on. */
+int lare_viscosity_dt(lare_ctx *ctx, double *dt_out, double *dtr_out);
+int lare_lagrange(lare_ctx *ctx, double dt);
+
 /* LARE3D::halfTimestep (shared_data.h:405-407 -> lagran.cpp:739-804). */
 int lare_half_timestep(lare_ctx *ctx);
 /* LARE3D::endOfTimestep (shared_data.h:413-419 -> remap.cpp:23-111). */

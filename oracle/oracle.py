@@ -15,6 +15,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblare_oracle.so")
 REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
 REF_DRIVER_FAST = os.path.join(HERE, "_ref", "ref_driver_fast")
+# The same driver and reference runner with the MI355X LARE3D package in place
+# of the reference's step sources (needs a GPU to run).
+DROPIN_DRIVER = os.path.join(HERE, "_ref", "lare3d_dropin")
 
 FIELDS = ["rho", "energy_electron", "energy_ion", "vx", "vy", "vz", "bx", "by", "bz",
           "vx1", "vy1", "vz1", "dm"]
@@ -174,11 +177,12 @@ def problem_bcs(o, kind, bcx=0):
 
 
 def run_reference(out_dir, problem="ot", nx=16, ny=None, nz=None, steps=10, dumps=(), seed=1,
-                  amp=0.0, noise=0.0, bcx=None, threads=4, fast=False, beta=-1.0):
+                  amp=0.0, noise=0.0, bcx=None, threads=4, fast=False, beta=-1.0, exe=None):
     """Run the reference driver (oracle/_ref, built from the reference's own
-    sources) and return its meta dict.  Raises if it has not been built."""
+    sources; exe selects another build of it, such as DROPIN_DRIVER) and
+    return its meta dict.  Raises if it has not been built."""
     import json
-    exe = REF_DRIVER_FAST if fast else REF_DRIVER
+    exe = exe or (REF_DRIVER_FAST if fast else REF_DRIVER)
     if not os.path.exists(exe):
         raise FileNotFoundError(exe)
     os.makedirs(out_dir, exist_ok=True)

@@ -37,6 +37,16 @@
 #include "shared_data.h"
 #include "builtInBoundaryConditions.h"
 
+#if defined(LARE_DROPIN)
+// Built with the MI355X package (sams_amd/host/lare3d_hip.cpp) in place of
+// the reference's lagran/remap/x,y,zremap/diagnostics.cpp: the state lives on
+// the device and is downloaded into the harness arrays before a dump.
+namespace LARE
+{
+    void lare3d_hip_download(simulationData &data);
+}
+#endif
+
 namespace refdrv
 {
     namespace pw = portableWrapper;
@@ -127,6 +137,9 @@ namespace refdrv
 
     void dumpState(LARE::simulationData &d, long step)
     {
+#if defined(LARE_DROPIN)
+        LARE::lare3d_hip_download(d);
+#endif
         char pre[64];
         std::snprintf(pre, sizeof(pre), "/s%05ld_", step);
         std::string p = cfg().out + pre;
@@ -409,7 +422,13 @@ int main(int argc, char *argv[])
         else if (a == "--time")
             c.time = true;
         else if (a == "--phases")
+        {
+#if defined(LARE_DROPIN)
+            std::fprintf(stderr, "--phases: the Lagrangian arrays stay on the device in this build\n");
+            std::exit(2);
+#endif
             c.phases = true;
+        }
         else if (a == "--dump")
         {
             std::stringstream ss(next());

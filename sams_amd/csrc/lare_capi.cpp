@@ -477,3 +477,37 @@ extern "C" int lare_set_kernel_set(lare_ctx *c, int32_t kernel_set)
     drop_graph(c->g);
     return 0;
 }
+
+extern "C" int lare_viscosity_dt(lare_ctx *c, double *dt_out, double *dtr_out)
+{
+    if (!c) return fail(-1, "lare_viscosity_dt: null context");
+    lare::step_start_a(c->g);
+    int rc = after_launch(c, "lare_viscosity_dt");
+    if (rc) return rc;
+    lare::Scal s;
+    HIPCHK(hipMemcpyAsync(&s, c->g.dev.a.s, sizeof(s), hipMemcpyDeviceToHost, c->g.stream));
+    HIPCHK(hipStreamSynchronize(c->g.stream));
+    if (dt_out) *dt_out = s.dt;
+    if (dtr_out) *dtr_out = s.dtr;
+    return 0;
+}
+
+extern "C" int lare_lagrange(lare_ctx *c, double dt)
+{
+    if (!c) return fail(-1, "lare_lagrange: null context");
+    lare::Scal s;
+    HIPCHK(hipMemcpyAsync(&s, c->g.dev.a.s, sizeof(s), hipMemcpyDeviceToHost, c->g.stream));
+    HIPCHK(hipStreamSynchronize(c->g.stream));
+    // lagran.cpp:159-162: substeps = int(dt/dtr) + 1, dt_sub = dt/substeps
+    int substeps = 1;
+    if (c->g.prm.resistive) {
+        double q = dt / s.dtr;
+        if (!(q >= 0.0) || q > 1.0e6) return fail(-1, "lare_lagrange: dt/dtr out of range for the resistive substeps");
+        substeps = (int)q + 1;
+    }
+    s.dt = dt;
+    s.dt_sub = dt / (double)substeps;
+    HIPCHK(hipMemcpyAsync(c->g.dev.a.s, &s, sizeof(s), hipMemcpyHostToDevice, c->g.stream));
+    lare::step_start_b(c->g, substeps);
+    return after_launch(c, "lare_lagrange");
+}

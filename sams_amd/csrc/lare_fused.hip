@@ -873,7 +873,7 @@ KERNEL void g_rho_e_update(Dev D, R3 R)
 
 namespace lare {
 
-void fused_start(Grid &g)
+void fused_start_a(Grid &g)
 {
     const int nx = g.dev.g.nx, ny = g.dev.g.ny, nz = g.dev.g.nz;
     const Params &p = g.prm;
@@ -887,7 +887,14 @@ void fused_start(Grid &g)
     flaunch(g_alpha, 64, g, rng(-1, nx + 1, -1, ny + 2, 0, nz + 2), p.visc1, p.visc2_norm);
     flaunch(g_visc_dt, 128, g, rng(0, nx + 1, 0, ny + 1, 0, nz + 1), p.gas_gamma, p.mu0);
     dt_final(g);
-    if (p.resistive) resistive_substeps(g);
+}
+
+void fused_start_b(Grid &g, int substeps)
+{
+    const int nx = g.dev.g.nx, ny = g.dev.g.ny, nz = g.dev.g.nz;
+    const Params &p = g.prm;
+    const R3 C = rng(-1, nx + 2, -1, ny + 2, -1, nz + 2);
+    if (p.resistive) resistive_substeps(g, substeps);
     if (p.resistive)
         flaunch(g_predict<1>, 152, g, C, p.gas_gamma);
     else

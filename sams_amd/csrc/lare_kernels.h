@@ -67,15 +67,21 @@ void remap_v_fill(Grid &g);
 void velocity_fill(Grid &g);
 void energy_density_fill(Grid &g);
 void dt_final(Grid &g);
-void resistive_substeps(Grid &g);
-// Fused Lagrangian events (lare_fused.hip).
-void fused_start(Grid &g);
+void resistive_substeps(Grid &g, int substeps);
+// Fused Lagrangian events (lare_fused.hip): start_a runs up to the CFL
+// reduction, start_b the resistive substeps and the predictor.
+void fused_start_a(Grid &g);
+void fused_start_b(Grid &g, int substeps);
 void fused_half(Grid &g);
 // One fused directional remap sweep (lare_fused.hip).
 void fused_sweep(Grid &g, int d, double xpass, double ypass, double zpass);
 
 // The three LARE3D events (shared_data.h:397-419), enqueued on g.stream.
+// step_start = step_start_a + step_start_b(2): the CFL reduction of the
+// runner's calculateTimestep sits between them (lagran.cpp:156).
 void step_start(Grid &g);
+void step_start_a(Grid &g);
+void step_start_b(Grid &g, int substeps);
 void step_half(Grid &g);
 void step_end(Grid &g);
 

@@ -1232,9 +1232,10 @@ void dt_final(Grid &g)
     hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, g.stream, g.dev, g.prm.dt_multiplier);
 }
 
-// Two resistive substeps of dt/2 (lagran.cpp:157-170: int(dt/dtr)+1 with
-// dtr = dt), each eta_calc + resistive_effects (478-505).
-void resistive_substeps(Grid &g)
+// The resistive substeps of lagran.cpp:157-170 (int(dt/dtr)+1 of them, two
+// when the runner keeps set_dt's dt, with dt_sub = dt/substeps already in
+// the device scalars), each eta_calc + resistive_effects (478-505).
+void resistive_substeps(Grid &g, int substeps)
 {
     const int nx = g.dev.g.nx, ny = g.dev.g.ny, nz = g.dev.g.nz;
     const Params &p = g.prm;
@@ -1242,7 +1243,7 @@ void resistive_substeps(Grid &g)
     const R3 V = rng(-1, nx + 1, -1, ny + 1, -1, nz + 1);
     if (g.kernel_set == 1) {
         static const int kBE[5] = {6, 7, 8, 1, 2};
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < substeps; ++i) {
             launch3(k_res_eta_copy, 56, g, C, p.j_max, p.eta_background, p.eta0);
             launch3(k_rkstep, 64, g, rng(0, nx, 0, ny, 0, nz));
             launch3(k_res_b_energy, 112, g, rng(0, nx, 0, ny, 0, nz));
@@ -1250,7 +1251,7 @@ void resistive_substeps(Grid &g)
         }
         return;
     }
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < substeps; ++i) {
         launch3(k_eta_calc, 32, g, V, p.j_max, p.eta_background, p.eta0);
         launch3(k_res_copy, 48, g, C);
         launch3(k_rkstep, 64, g, rng(0, nx, 0, ny, 0, nz));
@@ -1269,8 +1270,15 @@ void resistive_substeps(Grid &g)
 // LARE3D::lagrangian_step (lagran.cpp:95-173).
 void step_start(Grid &g)
 {
+    step_start_a(g);
+    step_start_b(g, 2);
+}
+
+// Prologue, shock viscosity and the CFL reduction (lagran.cpp:113-156).
+void step_start_a(Grid &g)
+{
     if (g.kernel_set == 1) {
-        fused_start(g);
+        fused_start_a(g);
         return;
     }
     const int nx = g.dev.g.nx, ny = g.dev.g.ny, nz = g.dev.g.nz;
@@ -1290,8 +1298,19 @@ void step_start(Grid &g)
     // controlFunctions::calculateTimestep -> set_dt (lagran.cpp:156, 380-443)
     launch3(k_set_dt, 104, g, rng(0, nx, 0, ny, 0, nz), p.gas_gamma, p.mu0);
     hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, g.stream, g.dev, p.dt_multiplier);
-    if (p.resistive) resistive_substeps(g);
-    // predictor_step (lagran.cpp:592-737)
+}
+
+// Resistive substeps and predictor_step (lagran.cpp:157-173, 592-737).
+void step_start_b(Grid &g, int substeps)
+{
+    if (g.kernel_set == 1) {
+        fused_start_b(g, substeps);
+        return;
+    }
+    const int nx = g.dev.g.nx, ny = g.dev.g.ny, nz = g.dev.g.nz;
+    const Params &p = g.prm;
+    const R3 C = rng(-1, nx + 2, -1, ny + 2, -1, nz + 2);
+    if (p.resistive) resistive_substeps(g, substeps);
     launch3(k_bfield_cv1, 112, g, C);
     launch3(k_energy_pred, 88, g, rng(0, nx + 1, 0, ny + 1, 0, nz + 1), p.gas_gamma);
     launch3(k_forces, 144, g, rng(0, nx, 0, ny, 0, nz), p.mu0);

@@ -1,0 +1,364 @@
+// LARE::LARE3D on the MI355X: the package's step events implemented over
+// the C-ABI of include/lare_hip.h.
+//
+// This file replaces, in a SAMS build, the reference package's
+// src/PhysicsPackages/LARE/{lagran,remap,xremap,yremap,zremap,diagnostics}.cpp.
+// Everything else stays the reference's: the class and data declarations
+// (shared_data.h), setup.cpp (registration, allocation, grid), control.cpp,
+// boundary.cpp, the runner, the harness and every initial-condition package.
+// The harness arrays remain the host mirror: the state is uploaded at the
+// first step and downloaded before output.
+//
+// Event mapping (shared_data.h:397-419):
+//   startOfTimestep -> lagrangian_step: lare_viscosity_dt, then
+//       controlFunctions::calculateTimestep (runner.h:158-169, which calls
+//       back set_dt and getTimestep), then lare_lagrange with the runner's dt
+//   halfTimestep    -> corrector_step: lare_half_timestep
+//   endOfTimestep   -> eulerian_remap: lare_end_of_timestep; eta_calc is
+//       recomputed on the device inside the resistive substeps
+//
+// Boundary conditions: the built-in BC objects attached to each variable are
+// private to variableDef (variableDef.h:56).  They are identified once, at
+// the first step, by applying each variable's conditions along one dimension
+// (variableRegistry::applyBoundaryConditions(name, dim)) to a marked copy of
+// the array and reading which kind of fill each edge performed; the array is
+// restored afterwards.  Conditions that match none of the built-in fills
+// (simplePeriodicBC, simpleZeroGradientBC, simpleMirrorBC, simpleClamp) are
+// reported as an error.
+#include <cmath>
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "shared_data.h"
+#include "lare_hip.h"
+#if defined(USE_HDF5)
+#include "io/writerHDF5.h"
+#else
+#include "io/writerSimple.h"
+#endif
+
+namespace LARE
+{
+    namespace pw = portableWrapper;
+
+    namespace
+    {
+        // Device grid bound to one simulationData.
+        struct deviceGrid
+        {
+            lare_ctx *ctx = nullptr;
+            bool hostCurrent = true; // harness arrays hold the latest state
+            double dtNew = 0.0;      // dt of the last lare_viscosity_dt
+            double dtrNew = 0.0;
+            double devDt = 0.0;      // previous dt held by the device
+        };
+
+        std::map<const simulationData *, deviceGrid> &grids()
+        {
+            static std::map<const simulationData *, deviceGrid> g;
+            return g;
+        }
+
+        void check(int rc, const char *what)
+        {
+            if (rc != 0)
+                throw std::runtime_error(std::string("LARE3D (MI355X): ") + what + ": " + lare_last_error());
+        }
+
+        const char *kVarNames[LARE_NFIELDS] = {"rho", "energy_electron", "energy_ion", "vx", "vy", "vz",
+                                               "bx", "by", "bz", "LARE/vx1", "LARE/vy1", "LARE/vz1",
+                                               "LARE/dm"};
+
+        volumeArray *stateArray(simulationData &d, int f)
+        {
+            switch (f)
+            {
+            case LARE_RHO: return &d.rho;
+            case LARE_ENERGY_ELECTRON: return &d.energy_electron;
+            case LARE_ENERGY_ION: return &d.energy_ion;
+            case LARE_VX: return &d.vx;
+            case LARE_VY: return &d.vy;
+            case LARE_VZ: return &d.vz;
+            case LARE_BX: return &d.bx;
+            case LARE_BY: return &d.by;
+            case LARE_BZ: return &d.bz;
+            case LARE_VX1: return &d.vx1;
+            case LARE_VY1: return &d.vy1;
+            case LARE_VZ1: return &d.vz1;
+            default: return &d.dm;
+            }
+        }
+
+        // harness array <-> contiguous buffer in the reference layout
+        void toBuffer(volumeArray &a, std::vector<double> &buf)
+        {
+            auto r0 = a.getRange(0), r1 = a.getRange(1), r2 = a.getRange(2);
+            buf.resize((size_t)(r0.upper_bound - r0.lower_bound + 1) * (r1.upper_bound - r1.lower_bound + 1) *
+                       (r2.upper_bound - r2.lower_bound + 1));
+            size_t p = 0;
+            for (long i = r0.lower_bound; i <= r0.upper_bound; ++i)
+                for (long j = r1.lower_bound; j <= r1.upper_bound; ++j)
+                    for (long k = r2.lower_bound; k <= r2.upper_bound; ++k)
+                        buf[p++] = a(i, j, k);
+        }
+
+        void fromBuffer(volumeArray &a, const std::vector<double> &buf)
+        {
+            auto r0 = a.getRange(0), r1 = a.getRange(1), r2 = a.getRange(2);
+            size_t p = 0;
+            for (long i = r0.lower_bound; i <= r0.upper_bound; ++i)
+                for (long j = r1.lower_bound; j <= r1.upper_bound; ++j)
+                    for (long k = r2.lower_bound; k <= r2.upper_bound; ++k)
+                        a(i, j, k) = buf[p++];
+        }
+
+        // Identify the built-in boundary condition on one edge of dimension
+        // dim of a variable by its effect on a marked array.  Edges are
+        // applied one at a time: the upper periodic fill of a staggered
+        // variable reads index 0, which the lower fill writes.
+        void probeEdge(SAMS::harness &h, simulationData &d, lare_ctx *ctx, int f, int dim, int edge)
+        {
+            volumeArray &a = *stateArray(d, f);
+            const long n = dim == 0 ? d.nx : dim == 1 ? d.ny : d.nz;
+            const int s = a.getRange(dim).lower_bound == -2 ? 1 : 0; // HALF_CELL extent starts at -2
+            // marks: exact values that depend only on the index along dim
+            auto mark = [](long g) { return 1024.0 + 8.0 * static_cast<double>(g) + 0.5; };
+            auto r0 = a.getRange(0), r1 = a.getRange(1), r2 = a.getRange(2);
+            for (long i = r0.lower_bound; i <= r0.upper_bound; ++i)
+                for (long j = r1.lower_bound; j <= r1.upper_bound; ++j)
+                    for (long k = r2.lower_bound; k <= r2.upper_bound; ++k)
+                    {
+                        const long idx[3] = {i, j, k};
+                        a(i, j, k) = mark(idx[dim]);
+                    }
+            h.variableRegistry.applyBoundaryConditions(kVarNames[f], dim,
+                                                       edge ? SAMS::domain::edges::upper : SAMS::domain::edges::lower);
+            // one line through the middle of the other two dimensions
+            auto at = [&](long g) {
+                long idx[3] = {d.nx / 2, d.ny / 2, d.nz / 2};
+                idx[dim] = g;
+                return a(idx[0], idx[1], idx[2]);
+            };
+            // ghost layers of this edge (variableDef.h:539-605 with the
+            // dimension's ghost adjustment for HALF_CELL)
+            const long g0 = edge ? n + 1 - s : (s ? -2 : -1), g1 = edge ? n + 2 : 0;
+            const long first = edge ? g0 : g1;                // ghost nearest the domain
+            const long last = edge ? n : (s ? 0 : 1);         // domain point of the edge
+            auto all = [&](auto expect) {
+                for (long g = g0; g <= g1; ++g)
+                    if (at(g) != expect(g))
+                        return false;
+                return true;
+            };
+            int kind;
+            double value = 0.0;
+            if (all([&](long g) { return mark(g); }))
+                kind = LARE_BC_NONE;
+            else if (all([&](long g) { return mark(edge ? g - n : g + n); }))
+                kind = LARE_BC_PERIODIC;
+            else if (all([&](long) { return mark(last); }))
+                kind = LARE_BC_ZERO_GRADIENT;
+            else if (all([&](long g) { return mark(last) - (mark(last - (g - first)) - mark(last)); }))
+                kind = LARE_BC_MIRROR;
+            else if (all([&](long) { return at(g0); }))
+            {
+                kind = LARE_BC_CLAMP;
+                value = at(g0);
+            }
+            else
+                throw std::runtime_error(std::string("LARE3D (MI355X): the boundary condition of ") + kVarNames[f] +
+                                         " on the " + (edge ? "upper" : "lower") + " edge of dimension " +
+                                         std::to_string(dim) +
+                                         " is not a built-in periodic, zero-gradient, mirror or clamp condition");
+            check(lare_set_bc(ctx, f, dim, edge, kind, value), "lare_set_bc");
+        }
+
+        deviceGrid &bind(SAMS::harness &h, simulationData &d)
+        {
+            deviceGrid &g = grids()[&d];
+            if (g.ctx)
+                return g;
+            if (d.geometry != geometryType::Cartesian)
+                throw std::runtime_error("LARE3D (MI355X): only Cartesian geometry is implemented");
+            if (d.rke)
+                throw std::runtime_error("LARE3D (MI355X): the remap kinetic-energy correction (rke) is not implemented");
+            lare_desc desc{};
+            desc.nx = d.nx;
+            desc.ny = d.ny;
+            desc.nz = d.nz;
+            desc.x_min = d.x_min;
+            desc.x_max = d.x_max;
+            desc.y_min = d.y_min;
+            desc.y_max = d.y_max;
+            desc.z_min = d.z_min;
+            desc.z_max = d.z_max;
+            desc.gas_gamma = d.gas_gamma;
+            desc.visc1 = d.visc1;
+            desc.visc2 = d.visc2;
+            desc.mu0 = d.mu0;
+            desc.dt_multiplier = d.dt_multiplier;
+            desc.resistive = d.resistiveMHD ? 1 : 0;
+            desc.precision = LARE_FP64;
+            desc.eta_background = d.eta_background;
+            desc.eta0 = d.eta0;
+            desc.j_max = d.j_max;
+            check(lare_ctx_create(&desc, &g.ctx), "lare_ctx_create");
+            std::vector<double> buf;
+            for (int f = 0; f < LARE_NFIELDS; ++f)
+            {
+                toBuffer(*stateArray(d, f), buf);
+                check(lare_upload_field(g.ctx, f, buf.data()), "lare_upload_field");
+                for (int dim = 0; dim < 3; ++dim)
+                    for (int edge = 0; edge < 2; ++edge)
+                        probeEdge(h, d, g.ctx, f, dim, edge);
+                fromBuffer(*stateArray(d, f), buf); // the probes overwrote the host copy
+            }
+            check(lare_set_dt(g.ctx, d.dt), "lare_set_dt");
+            g.devDt = d.dt;
+            g.hostCurrent = true;
+            return g;
+        }
+
+        // Bring the harness arrays up to date (before output).
+        void download(simulationData &d)
+        {
+            auto it = grids().find(&d);
+            if (it == grids().end() || it->second.hostCurrent)
+                return;
+            std::vector<double> buf;
+            for (int f = 0; f < LARE_NFIELDS; ++f)
+            {
+                volumeArray &a = *stateArray(d, f);
+                toBuffer(a, buf); // sizes buf
+                check(lare_download_field(it->second.ctx, f, buf.data()), "lare_download_field");
+                fromBuffer(a, buf);
+            }
+            it->second.hostCurrent = true;
+        }
+    } // namespace
+
+    void LARE3D::lagrangian_step(simulationData &data, SAMS::controlFunctions &controlFns)
+    {
+        deviceGrid &g = bind(harness, data);
+        // the shock viscosity reads the previous dt (lagran.cpp:72): the
+        // runner's, which another package may have lowered
+        if (data.dt != g.devDt)
+            check(lare_set_dt(g.ctx, data.dt), "lare_set_dt");
+        check(lare_viscosity_dt(g.ctx, &g.dtNew, &g.dtrNew), "lare_viscosity_dt");
+        controlFns.calculateTimestep();
+        check(lare_lagrange(g.ctx, data.dt), "lare_lagrange");
+        g.devDt = data.dt;
+        g.hostCurrent = false;
+    }
+
+    void LARE3D::set_dt(simulationData &data)
+    {
+        auto it = grids().find(&data);
+        if (it == grids().end())
+            throw std::runtime_error("LARE3D (MI355X): set_dt outside lagrangian_step");
+        data.dt = it->second.dtNew;
+        data.dtr = it->second.dtrNew;
+    }
+
+    void LARE3D::predictor_step(simulationData &)
+    {
+        throw std::runtime_error("LARE3D (MI355X): predictor_step runs inside lagrangian_step");
+    }
+
+    void LARE3D::corrector_step(simulationData &data)
+    {
+        deviceGrid &g = bind(harness, data);
+        check(lare_half_timestep(g.ctx), "lare_half_timestep");
+        g.hostCurrent = false;
+    }
+
+    void LARE3D::eta_calc(simulationData &)
+    {
+        // eta is recomputed on the device at the start of every resistive
+        // substep, before any read (lagran.cpp:163-167)
+    }
+
+    void LARE3D::eulerian_remap(simulationData &data, remapData &)
+    {
+        deviceGrid &g = bind(harness, data);
+        check(lare_end_of_timestep(g.ctx), "lare_end_of_timestep");
+        g.hostCurrent = false;
+    }
+
+    void LARE3D::remap_x(simulationData &, remapData &)
+    {
+        throw std::runtime_error("LARE3D (MI355X): remap_x runs inside eulerian_remap");
+    }
+    void LARE3D::remap_y(simulationData &, remapData &)
+    {
+        throw std::runtime_error("LARE3D (MI355X): remap_y runs inside eulerian_remap");
+    }
+    void LARE3D::remap_z(simulationData &, remapData &)
+    {
+        throw std::runtime_error("LARE3D (MI355X): remap_z runs inside eulerian_remap");
+    }
+
+    void LARE3D::energy_correction(simulationData &)
+    {
+        throw std::runtime_error("LARE3D (MI355X): the remap kinetic-energy correction (rke) is not implemented");
+    }
+
+    // ---- output events (diagnostics.cpp:50-107): the same datasets, read
+    // from the harness arrays after the device state is downloaded
+
+    template <typename T_writer>
+    void LARE3D::registerOutputMeshes(writer<T_writer> &w, simulationData &data)
+    {
+        w.template registerRectilinearMesh<T_dataType>("MeshCC", data.nx, data.ny, data.nz);
+    }
+
+    template <typename T_writer>
+    void LARE3D::registerOutputVariables(writer<T_writer> &w, simulationData &)
+    {
+        for (const char *n : {"rho", "energy_electron", "energy_ion", "vx", "vy", "vz", "bx", "by", "bz"})
+            w.template registerData<T_dataType>(n, "MeshCC");
+    }
+
+    template <typename T_writer>
+    void LARE3D::writeOutputMeshes(writer<T_writer> &w, simulationData &data)
+    {
+        w.writeRectilinearMesh("MeshCC", &data.xc_host(1), &data.yc_host(1), &data.zc_host(1));
+    }
+
+    template <typename T_writer>
+    void LARE3D::writeOutputVariables(writer<T_writer> &w, simulationData &data)
+    {
+        download(data);
+        const char *names[9] = {"rho", "energy_electron", "energy_ion", "vx", "vy", "vz", "bx", "by", "bz"};
+        std::vector<double> interior((size_t)data.nx * data.ny * data.nz);
+        for (int f = 0; f < 9; ++f)
+        {
+            volumeArray &a = *stateArray(data, f);
+            size_t p = 0;
+            for (long i = 1; i <= data.nx; ++i)
+                for (long j = 1; j <= data.ny; ++j)
+                    for (long k = 1; k <= data.nz; ++k)
+                        interior[p++] = a(i, j, k);
+            w.writeData(names[f], interior.data());
+        }
+    }
+
+#if defined(USE_HDF5)
+    template void LARE3D::registerOutputMeshes<HDF5File>(writer<HDF5File> &, simulationData &);
+    template void LARE3D::registerOutputVariables<HDF5File>(writer<HDF5File> &, simulationData &);
+    template void LARE3D::writeOutputMeshes<HDF5File>(writer<HDF5File> &, simulationData &);
+    template void LARE3D::writeOutputVariables<HDF5File>(writer<HDF5File> &, simulationData &);
+#else
+    template void LARE3D::registerOutputMeshes<simpleFile>(writer<simpleFile> &, simulationData &);
+    template void LARE3D::registerOutputVariables<simpleFile>(writer<simpleFile> &, simulationData &);
+    template void LARE3D::writeOutputMeshes<simpleFile>(writer<simpleFile> &, simulationData &);
+    template void LARE3D::writeOutputVariables<simpleFile>(writer<simpleFile> &, simulationData &);
+#endif
+
+    // Access for drivers that need the harness arrays current outside an
+    // output event (e.g. to compare states).
+    void lare3d_hip_download(simulationData &data) { download(data); }
+} // namespace LARE

@@ -81,6 +81,8 @@ def lib():
             "lare_get_dt": (ctypes.c_int, [vp, P(ctypes.c_double)]),
             "lare_set_dt": (ctypes.c_int, [vp, ctypes.c_double]),
             "lare_start_of_timestep": (ctypes.c_int, [vp, P(ctypes.c_double)]),
+            "lare_viscosity_dt": (ctypes.c_int, [vp, P(ctypes.c_double), P(ctypes.c_double)]),
+            "lare_lagrange": (ctypes.c_int, [vp, ctypes.c_double]),
             "lare_half_timestep": (ctypes.c_int, [vp]),
             "lare_end_of_timestep": (ctypes.c_int, [vp]),
             "lare_step": (ctypes.c_int, [vp, P(ctypes.c_double)]),
@@ -206,6 +208,18 @@ class Lare:
         d = ctypes.c_double()
         _check(lib().lare_start_of_timestep(self.h, ctypes.byref(d)), "lare_start_of_timestep")
         return d.value
+
+    def viscosity_dt(self):
+        """Prologue, shock viscosity and CFL reduction of startOfTimestep;
+        returns (dt, dtr) as LARE3D::set_dt hands them to the runner."""
+        d, r = ctypes.c_double(), ctypes.c_double()
+        _check(lib().lare_viscosity_dt(self.h, ctypes.byref(d), ctypes.byref(r)), "lare_viscosity_dt")
+        return d.value, r.value
+
+    def lagrange(self, dt):
+        """Rest of startOfTimestep with the runner's dt: resistive substeps
+        and predictor."""
+        _check(lib().lare_lagrange(self.h, float(dt)), "lare_lagrange")
 
     def half_timestep(self):
         _check(lib().lare_half_timestep(self.h), "lare_half_timestep")

@@ -1,0 +1,61 @@
+"""The drop-in: the reference's own runner, set-up, control and boundary
+sources, with the MI355X LARE3D package (sams_amd/host/lare3d_hip.cpp over
+the C-ABI) in place of the reference's lagran/remap/x,y,zremap/diagnostics
+sources, reproduces the reference's dt sequences and state digests bit for
+bit (tests/golden/goldens.json).
+
+The binary (oracle/_ref/lare3d_dropin) is linked from the reference tree by
+oracle/Makefile; where it was not built the tests skip."""
+import os
+import tempfile
+
+import pytest
+
+from lare_testlib import goldens
+from oracle import oracle as O
+
+FIELDS = O.STATE_FIELDS
+CASES = goldens()
+
+needs_dropin = pytest.mark.skipif(not os.path.exists(O.DROPIN_DRIVER),
+                                  reason="oracle/_ref/lare3d_dropin not built (needs the reference tree)")
+
+
+def _sha_file(path):
+    import hashlib
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+@pytest.mark.gpu
+@needs_dropin
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_dropin_reproduces_reference(name):
+    c = CASES[name]
+    dumps = sorted(int(s) for s in c["snapshots"])
+    with tempfile.TemporaryDirectory() as td:
+        meta = O.run_reference(td, problem=c["problem"], nx=c["nx"], ny=c["ny"], nz=c["nz"],
+                               steps=c["steps"], dumps=dumps, seed=c["seed"], amp=c["amp"],
+                               noise=c["noise"], bcx=c["bcx"] or None, threads=2,
+                               exe=O.DROPIN_DRIVER)
+        assert meta["dt"] == c["dt"], "dt sequence"
+        for s in dumps:
+            for f in FIELDS:
+                got = _sha_file(os.path.join(td, f"s{s:05d}_{f}.f64"))
+                assert got == c["snapshots"][str(s)][f]["sha256"], f"step {s} {f}"
+
+
+def test_dropin_fails_loudly_without_gpu():
+    """Without a HIP device the package raises from its first step (no CPU
+    fallback): the process aborts with the package's message."""
+    import subprocess
+    import torch
+    if not os.path.exists(O.DROPIN_DRIVER):
+        pytest.skip("oracle/_ref/lare3d_dropin not built (needs the reference tree)")
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is present")
+    with tempfile.TemporaryDirectory() as td:
+        r = subprocess.run([O.DROPIN_DRIVER, "--problem", "ot", "--nx", "8", "--steps", "1", "--out", td],
+                           cwd=td, capture_output=True, text=True)
+    assert r.returncode != 0
+    assert "LARE3D (MI355X)" in r.stderr and "no HIP device" in r.stderr

@@ -132,3 +132,27 @@ def test_graph_replays_match_eager_steps():
         assert dg.hex() == de.hex() == do.hex(), f"dt of replay {s + 1}"
         assert not _state_diffs(g, e), f"replay {s + 1} vs eager"
         assert not _state_diffs(g, orc), f"replay {s + 1} vs oracle"
+
+
+@pytest.mark.parametrize("kernel_set", [0, 1])
+def test_split_start_of_timestep_matches_whole_event(kernel_set):
+    """lare_viscosity_dt followed by lare_lagrange with the dt it returned
+    (the runner's timestep protocol, runner.h:158-169) gives the same bits
+    as lare_start_of_timestep, on the resistive path where the substep count
+    int(dt/dtr)+1 is computed on the host."""
+    p = L.problem("bw", n=20, ny=4, nz=6, seed=5, noise=0.02, eta=1e-3)
+    split = L.Lare.from_problem(p)
+    whole = L.Lare.from_problem(p)
+    for d in (split, whole):
+        d.set_kernel_set(kernel_set)
+    for s in range(4):
+        dt, dtr = split.viscosity_dt()
+        assert dtr == dt
+        split.lagrange(dt)
+        dt_w = whole.start_of_timestep()
+        assert dt.hex() == dt_w.hex(), f"dt of step {s + 1}"
+        assert not _state_diffs(split, whole, L.STATE_FIELDS + ["vx1", "vy1", "vz1"])
+        for d in (split, whole):
+            d.half_timestep()
+            d.end_of_timestep()
+        assert not _state_diffs(split, whole), f"step {s + 1}"
