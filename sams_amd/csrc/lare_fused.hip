@@ -400,6 +400,10 @@ void flaunch_(void (*k)(Dev, R3, KA...), const char *name, double bpc, Grid &g, 
     dim3 blk(64, 4, 1);
     dim3 grd((unsigned)((nz + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)nx);
     Prof::Rec rec{name, bpc * (double)nz * ny * nx, nullptr, nullptr};
+    if (g.prof.on && g.prof.dir >= 0) {
+        const size_t at = rec.name.find("<DIR>");
+        if (at != std::string::npos) rec.name.replace(at, 5, "<" + std::to_string(g.prof.dir) + ">");
+    }
     if (g.prof.on) {
         (void)hipEventCreate(&rec.a);
         (void)hipEventCreate(&rec.b);
@@ -421,6 +425,7 @@ void fused_sweep_t(Grid &g, double xpass, double ypass, double zpass)
     const int nx = g.dev.g.nx, ny = g.dev.g.ny, nz = g.dev.g.nz;
     const R3 C = rng(-1, nx + 2, -1, ny + 2, -1, nz + 2);
     const R3 all0 = rng(0, nx, 0, ny, 0, nz);
+    g.prof.dir = DIR;
     // bytes per cell: distinct arrays read and written, 8 bytes each
     flaunch(f_prep<DIR>, 48, g, C, xpass, ypass, zpass);
     flaunch(f_cvc1, 16, g, rng(-1, nx + 1, -1, ny + 1, -1, nz + 1));
@@ -436,6 +441,7 @@ void fused_sweep_t(Grid &g, double xpass, double ypass, double zpass)
                 DIR == 2 ? -1 : 0, DIR == 2 ? nz + 1 : nz));
     flaunch(f_mom_fluxes<DIR>, 88, g, rng(DIR == 0 ? -1 : 0, nx, DIR == 1 ? -1 : 0, ny, DIR == 2 ? -1 : 0, nz));
     flaunch(f_update<DIR>, 184, g, all0);
+    g.prof.dir = -1;
     sweep_bcs(g);
 }
 

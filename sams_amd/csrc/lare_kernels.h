@@ -3,6 +3,7 @@
 
 #include "lare_dev.h"
 
+#include <string>
 #include <vector>
 
 namespace lare {
@@ -31,8 +32,9 @@ struct Params {
 // Optional per-kernel timing with HIP events on the grid's stream.
 struct Prof {
     bool on = false;
+    int dir = -1;  // sweep direction being launched; names "<DIR>" as "<0>".."<2>"
     struct Rec {
-        const char *name;
+        std::string name;
         double bytes;  // algorithmic bytes of this launch
         hipEvent_t a, b;
     };

@@ -27,6 +27,8 @@
 // reported as an error.
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -119,13 +121,25 @@ namespace LARE
         // dim of a variable by its effect on a marked array.  Edges are
         // applied one at a time: the upper periodic fill of a staggered
         // variable reads index 0, which the lower fill writes.
-        void probeEdge(SAMS::harness &h, simulationData &d, lare_ctx *ctx, int f, int dim, int edge)
+        struct edgeBC
+        {
+            int field, dim, edge, kind;
+            double value;
+        };
+
+        edgeBC probeEdge(SAMS::harness &h, simulationData &d, int f, int dim, int edge)
         {
             volumeArray &a = *stateArray(d, f);
             const long n = dim == 0 ? d.nx : dim == 1 ? d.ny : d.nz;
             const int s = a.getRange(dim).lower_bound == -2 ? 1 : 0; // HALF_CELL extent starts at -2
-            // marks: exact values that depend only on the index along dim
-            auto mark = [](long g) { return 1024.0 + 8.0 * static_cast<double>(g) + 0.5; };
+            // marks: exact values that depend only on the index along dim,
+            // quadratic so that no reflection, shift or copy of them equals
+            // the marks themselves (a linear mark is its own mirror image
+            // about a point)
+            auto mark = [](long g) {
+                const double x = static_cast<double>(g);
+                return 4096.0 + 16.0 * x + x * x + 0.5;
+            };
             auto r0 = a.getRange(0), r1 = a.getRange(1), r2 = a.getRange(2);
             for (long i = r0.lower_bound; i <= r0.upper_bound; ++i)
                 for (long j = r1.lower_bound; j <= r1.upper_bound; ++j)
@@ -173,7 +187,7 @@ namespace LARE
                                          " on the " + (edge ? "upper" : "lower") + " edge of dimension " +
                                          std::to_string(dim) +
                                          " is not a built-in periodic, zero-gradient, mirror or clamp condition");
-            check(lare_set_bc(ctx, f, dim, edge, kind, value), "lare_set_bc");
+            return edgeBC{f, dim, edge, kind, value};
         }
 
         deviceGrid &bind(SAMS::harness &h, simulationData &d)
@@ -205,16 +219,31 @@ namespace LARE
             desc.eta_background = d.eta_background;
             desc.eta0 = d.eta0;
             desc.j_max = d.j_max;
-            check(lare_ctx_create(&desc, &g.ctx), "lare_ctx_create");
+            // boundary conditions first (host only), so that a condition the
+            // device does not implement is reported before any device work
+            std::vector<edgeBC> bcs;
             std::vector<double> buf;
+            const bool report = std::getenv("LARE3D_HIP_REPORT_BCS") != nullptr;
+            for (int f = 0; f < LARE_NFIELDS; ++f)
+            {
+                toBuffer(*stateArray(d, f), buf);
+                for (int dim = 0; dim < 3; ++dim)
+                    for (int edge = 0; edge < 2; ++edge)
+                    {
+                        bcs.push_back(probeEdge(h, d, f, dim, edge));
+                        if (report)
+                            std::fprintf(stderr, "LARE3D (MI355X) bc %s %d %d %d %.17g\n", kVarNames[f], dim, edge,
+                                         bcs.back().kind, bcs.back().value);
+                    }
+                fromBuffer(*stateArray(d, f), buf); // the probes overwrote the host copy
+            }
+            check(lare_ctx_create(&desc, &g.ctx), "lare_ctx_create");
+            for (const edgeBC &b : bcs)
+                check(lare_set_bc(g.ctx, b.field, b.dim, b.edge, b.kind, b.value), "lare_set_bc");
             for (int f = 0; f < LARE_NFIELDS; ++f)
             {
                 toBuffer(*stateArray(d, f), buf);
                 check(lare_upload_field(g.ctx, f, buf.data()), "lare_upload_field");
-                for (int dim = 0; dim < 3; ++dim)
-                    for (int edge = 0; edge < 2; ++edge)
-                        probeEdge(h, d, g.ctx, f, dim, edge);
-                fromBuffer(*stateArray(d, f), buf); // the probes overwrote the host copy
             }
             check(lare_set_dt(g.ctx, d.dt), "lare_set_dt");
             g.devDt = d.dt;

@@ -59,3 +59,41 @@ def test_dropin_fails_loudly_without_gpu():
                            cwd=td, capture_output=True, text=True)
     assert r.returncode != 0
     assert "LARE3D (MI355X)" in r.stderr and "no HIP device" in r.stderr
+
+
+@pytest.mark.parametrize("problem,bcx", [("ot", None), ("bw", "zerograd"), ("bw", "mirror"),
+                                         ("bw", "clamp"), ("bw", "periodic")])
+def test_dropin_reads_the_attached_boundary_conditions(problem, bcx):
+    """The package identifies the built-in condition attached to every edge
+    of every variable (periodic, zero-gradient, mirror, clamp value) from the
+    harness alone.  The identification runs on the host before any device
+    call, so this runs without a GPU: the table is printed, then the context
+    creation fails (or, with a GPU, the one step runs)."""
+    import subprocess
+    import sams_amd.lare as L
+    if not os.path.exists(O.DROPIN_DRIVER):
+        pytest.skip("oracle/_ref/lare3d_dropin not built (needs the reference tree)")
+    args = [O.DROPIN_DRIVER, "--problem", problem, "--nx", "12", "--ny", "5", "--nz", "4", "--steps", "1"]
+    if bcx:
+        args += ["--bcx", bcx]
+    with tempfile.TemporaryDirectory() as td:
+        r = subprocess.run(args + ["--out", td], cwd=td, capture_output=True, text=True,
+                           env=dict(os.environ, LARE3D_HIP_REPORT_BCS="1", OMP_NUM_THREADS="2"))
+    got = {}
+    for line in r.stderr.splitlines():
+        if line.startswith("LARE3D (MI355X) bc "):
+            name, dim, edge, kind, value = line.split()[3:]
+            got[(name, int(dim), int(edge))] = (int(kind), float(value))
+    names = ["rho", "energy_electron", "energy_ion", "vx", "vy", "vz", "bx", "by", "bz",
+             "LARE/vx1", "LARE/vy1", "LARE/vz1", "LARE/dm"]
+    x = {None: L.BC_PERIODIC, "zerograd": L.BC_ZERO_GRADIENT, "mirror": L.BC_MIRROR,
+         "clamp": L.BC_CLAMP, "periodic": L.BC_PERIODIC}[bcx]
+    want = {}
+    for n in names:
+        vel = n in ("vx", "vy", "vz", "LARE/vx1", "LARE/vy1", "LARE/vz1")
+        kx = L.BC_ZERO_GRADIENT if (x == L.BC_CLAMP and not vel) else x
+        for e in (0, 1):
+            want[(n, 0, e)] = (kx, 0.0)
+            want[(n, 1, e)] = (L.BC_PERIODIC, 0.0)
+            want[(n, 2, e)] = (L.BC_PERIODIC, 0.0)
+    assert got == want
