@@ -155,6 +155,37 @@ int lare_profile_count(lare_ctx *ctx, int32_t *n);
 int lare_profile_get(lare_ctx *ctx, int32_t i, char *name, int32_t name_len, int32_t *calls,
                      double *total_ms, double *bytes);
 
+/* Total energy of the grid in FP64 (no counterpart in the reference): out[0]
+ * internal sum rho (e_e + e_i) cv over the domain cells, out[1] kinetic
+ * sum 1/2 rho_v |v|^2 cv_v over the vertices 1..n, out[2] magnetic
+ * sum |B_c|^2 / (2 mu0) cv with B_c the cell average of the face values,
+ * out[3] their sum.  Summed in a fixed order (reproducible). */
+int lare_energy(lare_ctx *ctx, double *out);
+
+/* ------------------------------------------------------------------------
+ * Batches (no counterpart in the reference, whose runner steps one
+ * simulationData at a time): contexts of one storage precision, of any
+ * sizes, boundary conditions and physics parameters, that advance together.
+ * Every kernel of a step is one launch over all members; each member keeps
+ * its own state, dt and step count, and its results are bit-identical to
+ * stepping it alone.  lare_batch_steps enqueues on the batch's own stream
+ * and returns; call lare_batch_sync before using a member on its own
+ * (download, lare_get_dt, lare_step ...).  A member must outlive its batch.
+ */
+typedef struct lare_batch lare_batch;
+int lare_batch_create(lare_ctx *const *members, int32_t n, lare_batch **out);
+int lare_batch_destroy(lare_batch *batch);
+/* nsteps full steps of every member (a captured HIP graph per step unless
+ * lare_batch_set_graph(batch, 0)). */
+int lare_batch_steps(lare_batch *batch, int32_t nsteps);
+int lare_batch_sync(lare_batch *batch);
+int lare_batch_set_graph(lare_batch *batch, int32_t on);
+/* lare_energy of every member: 4 doubles per member, in member order. */
+int lare_batch_energy(lare_batch *batch, double *out);
+/* HIP-event timing of every launch between begin and end (eager steps):
+ * the summed kernel time and the number of launches. */
+int lare_batch_profile_begin(lare_batch *batch);
+int lare_batch_profile_end(lare_batch *batch, double *total_ms, int32_t *launches);
 
 /* ------------------------------------------------------------------------
  * Problem set-up (the initial-condition packages' role: controlVariables,

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblare_hip.so")
 
-SOURCES = ["lare_kernels.hip", "lare_fused.hip", "lare_capi.cpp", "lare_problems.cpp"]
+SOURCES = ["lare_kernels.hip", "lare_engine.hip", "lare_capi.cpp", "lare_problems.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -13,7 +14,7 @@
 #include <string>
 #include <vector>
 
-#include "lare_kernels.h"
+#include "lare_engine.h"
 
 namespace {
 
@@ -43,6 +44,11 @@ struct lare_ctx {
     lare::Grid g;
     lare_desc desc;
     int device;
+    // the grid as a group of one for the engine (kernel set 1)
+    lare::Group solo;
+    bool solo_ready = false;
+    // arrays only the one-kernel-per-reference-kernel set reads
+    void *pool0 = nullptr;
     struct KStat {
         std::string name;
         int calls;
@@ -66,21 +72,97 @@ void drop_graph(lare::Grid &g)
     g.graph = nullptr;
 }
 
+void drop_graphs(lare_ctx *c)
+{
+    drop_graph(c->g);
+    lare::group_drop_graph(c->solo);
+}
+
+int ensure_solo(lare_ctx *c)
+{
+    if (c->solo_ready) return 0;
+    std::string err;
+    int rc = lare::group_init(c->solo, {&c->g}, c->g.stream, &c->g.prof, err);
+    if (rc) return fail(rc, "grid group: " + err);
+    c->solo_ready = true;
+    return 0;
+}
+
+// The three events on the selected kernel set.
+int ev_start_a(lare_ctx *c)
+{
+    if (c->g.kernel_set == 0) {
+        lare::step_start_a(c->g);
+        return 0;
+    }
+    int rc = ensure_solo(c);
+    if (!rc) lare::group_start_a(c->solo);
+    return rc;
+}
+int ev_start_b(lare_ctx *c, int substeps)
+{
+    if (c->g.kernel_set == 0) {
+        lare::step_start_b(c->g, substeps);
+        return 0;
+    }
+    int rc = ensure_solo(c);
+    if (!rc) lare::group_start_b(c->solo, substeps);
+    return rc;
+}
+int ev_half(lare_ctx *c)
+{
+    if (c->g.kernel_set == 0) {
+        lare::step_half(c->g);
+        return 0;
+    }
+    int rc = ensure_solo(c);
+    if (!rc) lare::group_half(c->solo);
+    return rc;
+}
+int ev_end(lare_ctx *c)
+{
+    if (c->g.kernel_set == 0) {
+        lare::step_end(c->g);
+        return 0;
+    }
+    int rc = ensure_solo(c);
+    if (!rc) lare::group_end(c->solo);
+    return rc;
+}
+int ev_step(lare_ctx *c)
+{
+    int rc = ev_start_a(c);
+    if (!rc) rc = ev_start_b(c, 2);
+    if (!rc) rc = ev_half(c);
+    if (!rc) rc = ev_end(c);
+    return rc;
+}
+
 // Capture one step (the kernels read dt and every other scalar from device
 // memory, so one capture serves every later step).  No memset or host copy
-// is recorded in it.
-hipError_t capture_step(lare::Grid &g)
+// is recorded in it; every table a launch needs exists before capture.
+hipError_t capture_step(lare_ctx *c, hipGraphExec_t *exec)
 {
-    drop_graph(g);
-    hipError_t e = hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal);
+    drop_graphs(c);
+    hipGraph_t *graph = &c->g.graph;
+    hipGraphExec_t *gexec = &c->g.graph_exec;
+    if (c->g.kernel_set == 1) {
+        if (ensure_solo(c)) return hipErrorInvalidValue;
+        lare::group_prepare(c->solo);
+        graph = &c->solo.graph;
+        gexec = &c->solo.graph_exec;
+    }
+    hipError_t e = hipStreamBeginCapture(c->g.stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
-    lare::step_start(g);
-    lare::step_half(g);
-    lare::step_end(g);
-    e = hipStreamEndCapture(g.stream, &g.graph);
+    (void)ev_step(c);
+    e = hipStreamEndCapture(c->g.stream, graph);
     if (e != hipSuccess) return e;
-    return hipGraphInstantiate(&g.graph_exec, g.graph, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(gexec, *graph, nullptr, nullptr, 0);
+    *exec = *gexec;
+    return e;
 }
+
+hipGraphExec_t captured(lare_ctx *c) { return c->g.kernel_set == 1 ? c->solo.graph_exec : c->g.graph_exec; }
 
 }  // namespace
 
@@ -106,14 +188,104 @@ void make_axis(long n, double mn, double mx, std::vector<double> &db, std::vecto
 
 }  // namespace
 
+namespace {
+
+// Engine arrays (lare::ArrE) of one grid in storage type T: 34 T arrays and
+// 10 FP64 flux arrays carved from the pool at p.
+template <class T>
+void carve_engine(lare::ArrE<T> &a, char *&p, size_t box)
+{
+    auto tk = [&]() {
+        lare::Fld<T> r{(T *)p};
+        p += box * sizeof(T);
+        return r;
+    };
+    auto dk = [&]() {
+        double *r = (double *)p;
+        p += box * sizeof(double);
+        return r;
+    };
+    lare::Fld<T> *ts[] = {&a.rho, &a.ee, &a.ei, &a.vx, &a.vy, &a.vz, &a.bx, &a.by, &a.bz, &a.vx1, &a.vy1,
+                          &a.vz1, &a.cs, &a.cs_v, &a.rho_v, &a.alpha1, &a.alpha2, &a.alpha3, &a.visc_heat,
+                          &a.fxv, &a.fyv, &a.fzv, &a.cv1, &a.bx1, &a.by1, &a.bz1, &a.p_e, &a.p_i, &a.eta,
+                          &a.curlb, &a.cv2, &a.cvc1, &a.db1, &a.rrho_v};
+    for (auto *q : ts) *q = tk();
+    double **ds[] = {&a.dm, &a.flux_x, &a.flux_y, &a.flux_z, &a.dmv, &a.fl[0], &a.fl[1], &a.fl[2], &a.fl[3], &a.fl[4]};
+    for (double **q : ds) *q = dk();
+}
+const int kEngineT = 34, kEngineD = 10;
+
+// The one-kernel-per-reference-kernel set's view of the FP64 engine arrays.
+void alias_set0(lare::Arr &a, const lare::ArrE<double> &e)
+{
+    const lare::Fld<double> f12[12] = {e.rho, e.ee, e.ei, e.vx, e.vy, e.vz, e.bx, e.by, e.bz, e.vx1, e.vy1, e.vz1};
+    for (int k = 0; k < 12; ++k) a.f[k] = f12[k].p;
+    a.f[12] = e.dm;
+    a.eta = e.eta.p;
+    a.cv1 = e.cv1.p;
+    a.bx1 = e.bx1.p;
+    a.by1 = e.by1.p;
+    a.bz1 = e.bz1.p;
+    a.alpha1 = e.alpha1.p;
+    a.alpha2 = e.alpha2.p;
+    a.alpha3 = e.alpha3.p;
+    a.visc_heat = e.visc_heat.p;
+    a.p_e = e.p_e.p;
+    a.p_i = e.p_i.p;
+    a.rho_v = e.rho_v.p;
+    a.fxv = e.fxv.p;
+    a.fyv = e.fyv.p;
+    a.fzv = e.fzv.p;
+    a.flux_x = e.flux_x;
+    a.flux_y = e.flux_y;
+    a.flux_z = e.flux_z;
+    a.curlb = e.curlb.p;
+    a.cs = e.cs.p;
+    a.cs_v = e.cs_v.p;
+    a.cv2 = e.cv2.p;
+    a.cvc1 = e.cvc1.p;
+    a.db1 = e.db1.p;
+    a.rrho_v = e.rrho_v.p;
+    a.dmv = e.dmv;
+    for (int k = 0; k < 5; ++k) a.fl[k] = e.fl[k];
+}
+
+// Arrays read only by the one-kernel-per-reference-kernel set (the stored
+// metric, cv, pressure, cv_v, the forces, p_visc and the remap copies),
+// allocated when that set is first selected.
+const int kSet0Arrays = 18;
+int ensure_set0(lare_ctx *c)
+{
+    if (c->pool0) return 0;
+    if (c->g.fp32) return fail(-2, "the one-kernel-per-reference-kernel set runs FP64 storage only");
+    const size_t bytes3 = c->g.box * sizeof(double);
+    HIPCHK(hipStreamSynchronize(c->g.stream));
+    HIPCHK(hipMalloc(&c->pool0, kSet0Arrays * bytes3));
+    char *p = (char *)c->pool0;
+    lare::Arr &a = c->g.dev.a;
+    double **q[kSet0Arrays] = {&a.p_visc, &a.cv, &a.pressure, &a.cv_v, &a.fx, &a.fy, &a.fz, &a.dxab, &a.dyab,
+                               &a.dzab, &a.dxac, &a.dyac, &a.dzac, &a.rho1, &a.rho_v1, &a.rflux, &a.cv2v, &a.wsh};
+    for (double **r : q) {
+        *r = (double *)p;
+        p += bytes3;
+    }
+    HIPCHK(hipMemsetAsync(c->pool0, 0, kSet0Arrays * bytes3, c->g.stream));
+    lare::build_metric(c->g);
+    HIPCHK(hipStreamSynchronize(c->g.stream));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
 extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
 {
     if (!d || !out) return fail(-1, "lare_ctx_create: null argument");
     *out = nullptr;
     if (d->nx < 3 || d->ny < 3 || d->nz < 3)
         return fail(-1, "lare_ctx_create: each of nx, ny, nz must be at least 3");
-    if (d->precision != LARE_FP64)
-        return fail(-2, "lare_ctx_create: only LARE_FP64 storage is implemented");
+    if (d->precision != LARE_FP64 && d->precision != LARE_FP32_STORAGE)
+        return fail(-1, "lare_ctx_create: unknown storage precision");
     const long pz = ((d->nz + 5) + 7) / 8 * 8;
     const long sx = (d->ny + 5) * pz;
     const long box = (d->nx + 5) * sx;
@@ -134,6 +306,7 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
     g.dev.g.sx = (int)sx;
     g.dev.g.base = (int)(2 * sx + 2 * pz + 2);
     g.box = (size_t)box;
+    g.fp32 = d->precision == LARE_FP32_STORAGE;
     g.prm.gas_gamma = d->gas_gamma;
     g.prm.visc1 = d->visc1;
     g.prm.visc2 = d->visc2;
@@ -148,11 +321,10 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
     std::memset(g.bck, 0, sizeof(g.bck));
     std::memset(g.bcv, 0, sizeof(g.bcv));
 
-    // one pool: 3D arrays, then the six 1D delta arrays, then the scalars
-    const int n3d = 13 + 28 + 6 + 10 + 5;
+    // one pool: the engine's 3D arrays, the six 1D delta arrays, the scalars
+    const size_t tsz = g.fp32 ? sizeof(float) : sizeof(double);
     const size_t lmax = (size_t)std::max(d->nx, std::max(d->ny, d->nz)) + 5;
-    const size_t bytes3 = box * sizeof(double);
-    g.pool_bytes = n3d * bytes3 + 6 * lmax * sizeof(double) + 256;
+    g.pool_bytes = box * (kEngineT * tsz + kEngineD * sizeof(double)) + 6 * lmax * sizeof(double) + 256;
     hipError_t e = hipStreamCreate(&g.stream);
     if (e != hipSuccess) {
         delete c;
@@ -165,28 +337,42 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
         return fail(-4, std::string("hipMalloc of the grid pool: ") + hipGetErrorString(e));
     }
     char *p = (char *)g.pool;
-    auto take3 = [&]() {
-        double *r = (double *)p;
-        p += bytes3;
-        return r;
-    };
     lare::Arr &a = g.dev.a;
-    for (int f = 0; f < 13; ++f) a.f[f] = take3();
-    double **lag[] = {&a.p_visc, &a.eta, &a.cv, &a.cv1, &a.bx1, &a.by1, &a.bz1, &a.alpha1, &a.alpha2, &a.alpha3,
-                      &a.visc_heat, &a.pressure, &a.p_e, &a.p_i, &a.rho_v, &a.cv_v, &a.fx, &a.fy, &a.fz, &a.fxv,
-                      &a.fyv, &a.fzv, &a.flux_x, &a.flux_y, &a.flux_z, &a.curlb, &a.cs, &a.cs_v};
-    for (double **q : lag) *q = take3();
-    double **met[] = {&a.dxab, &a.dyab, &a.dzab, &a.dxac, &a.dyac, &a.dzac};
-    for (double **q : met) *q = take3();
-    double **rem[] = {&a.rho1, &a.cv2, &a.cvc1, &a.db1, &a.rrho_v, &a.rho_v1, &a.rflux, &a.cv2v, &a.wsh, &a.dmv};
-    for (double **q : rem) *q = take3();
-    for (int k = 0; k < 5; ++k) a.fl[k] = take3();
-    double **ax1[] = {&a.dxb, &a.dyb, &a.dzb, &a.dxc, &a.dyc, &a.dzc};
-    for (double **q : ax1) {
-        *q = (double *)p;
+    double *d1[6];
+    if (g.fp32) {
+        carve_engine(g.e32.a, p, g.box);
+    } else {
+        carve_engine(g.e64.a, p, g.box);
+        alias_set0(a, g.e64.a);
+    }
+    for (int k = 0; k < 6; ++k) {
+        d1[k] = (double *)p;
         p += lmax * sizeof(double);
     }
+    a.dxb = d1[0];
+    a.dyb = d1[1];
+    a.dzb = d1[2];
+    a.dxc = d1[3];
+    a.dyc = d1[4];
+    a.dzc = d1[5];
     a.s = (lare::Scal *)p;
+    lare::Phys ph{g.prm.gas_gamma, g.prm.visc1, g.prm.visc2_norm, g.prm.mu0, g.prm.dt_multiplier,
+                  g.prm.eta_background, g.prm.eta0, g.prm.j_max, g.prm.resistive};
+    auto finish_view = [&](auto &ev) {
+        ev.g = g.dev.g;
+        ev.p = ph;
+        ev.a.dxb = a.dxb;
+        ev.a.dyb = a.dyb;
+        ev.a.dzb = a.dzb;
+        ev.a.dxc = a.dxc;
+        ev.a.dyc = a.dyc;
+        ev.a.dzc = a.dzc;
+        ev.a.s = a.s;
+    };
+    if (g.fp32)
+        finish_view(g.e32);
+    else
+        finish_view(g.e64);
 
     int rc = 0;
     do {
@@ -203,11 +389,6 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
             if (e == hipSuccess) e = hipMemcpy(dstc[k], dc[k].data(), dc[k].size() * sizeof(double), hipMemcpyHostToDevice);
             if (e != hipSuccess) rc = fail(-3, std::string("axis upload: ") + hipGetErrorString(e));
         }
-        if (rc) break;
-        lare::build_metric(g);
-        e = hipStreamSynchronize(g.stream);
-        if (e == hipSuccess) e = hipGetLastError();
-        if (e != hipSuccess) rc = fail(-3, std::string("grid set-up: ") + hipGetErrorString(e));
     } while (0);
     if (rc) {
         (void)hipFree(g.pool);
@@ -224,6 +405,8 @@ extern "C" int lare_ctx_destroy(lare_ctx *c)
     if (!c) return 0;
     (void)hipStreamSynchronize(c->g.stream);
     drop_graph(c->g);
+    lare::group_free(c->solo);
+    if (c->pool0) (void)hipFree(c->pool0);
     (void)hipFree(c->g.pool);
     (void)hipStreamDestroy(c->g.stream);
     delete c;
@@ -244,7 +427,18 @@ extern "C" int lare_field_bounds(const lare_ctx *c, int field, int64_t lb[3], in
 
 namespace {
 
-// host array (reference extent) <-> device box, one 2D copy per x plane
+void *field_dev_ptr(lare_ctx *c, int f)
+{
+    if (c->g.fp32) {
+        const lare::ArrE<float> &a = c->g.e32.a;
+        const lare::Fld<float> fs[12] = {a.rho, a.ee, a.ei, a.vx, a.vy, a.vz, a.bx, a.by, a.bz, a.vx1, a.vy1, a.vz1};
+        return f == LARE_DM ? (void *)a.dm : (void *)fs[f].p;
+    }
+    return c->g.dev.a.f[f];
+}
+
+// host array (reference extent) <-> device box, one 2D copy per x plane;
+// FP32-storage fields are rounded on upload and widened on download
 int copy_field(lare_ctx *c, int field, double *host, bool up)
 {
     int64_t lb[3], ub[3];
@@ -252,19 +446,29 @@ int copy_field(lare_ctx *c, int field, double *host, bool up)
     if (rc) return rc;
     if (!host) return fail(-1, "lare field copy: null host pointer");
     const lare::Geo &G = c->g.dev.g;
-    const size_t nyr = (size_t)(ub[1] - lb[1] + 1), nzr = (size_t)(ub[2] - lb[2] + 1);
-    double *dev = c->g.dev.a.f[field];
+    const size_t nxr = (size_t)(ub[0] - lb[0] + 1), nyr = (size_t)(ub[1] - lb[1] + 1), nzr = (size_t)(ub[2] - lb[2] + 1);
+    const bool f32 = c->g.fp32 && field != LARE_DM;
+    const size_t es = f32 ? sizeof(float) : sizeof(double);
+    char *dev = (char *)field_dev_ptr(c, field);
+    std::vector<float> stage;
+    char *hbase = (char *)host;
+    if (f32) {
+        stage.resize(nxr * nyr * nzr);
+        if (up)
+            for (size_t i = 0; i < stage.size(); ++i) stage[i] = (float)host[i];
+        hbase = (char *)stage.data();
+    }
     HIPCHK(hipStreamSynchronize(c->g.stream));
     for (int64_t ix = lb[0]; ix <= ub[0]; ++ix) {
-        double *d = dev + (G.base + ix * G.sx + lb[1] * G.sy + lb[2]);
-        double *h = host + (size_t)(ix - lb[0]) * nyr * nzr;
+        char *d = dev + (G.base + ix * G.sx + lb[1] * G.sy + lb[2]) * es;
+        char *h = hbase + (size_t)(ix - lb[0]) * nyr * nzr * es;
         if (up)
-            HIPCHK(hipMemcpy2D(d, G.sy * sizeof(double), h, nzr * sizeof(double), nzr * sizeof(double), nyr,
-                               hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy2D(d, G.sy * es, h, nzr * es, nzr * es, nyr, hipMemcpyHostToDevice));
         else
-            HIPCHK(hipMemcpy2D(h, nzr * sizeof(double), d, G.sy * sizeof(double), nzr * sizeof(double), nyr,
-                               hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy2D(h, nzr * es, d, G.sy * es, nzr * es, nyr, hipMemcpyDeviceToHost));
     }
+    if (f32 && !up)
+        for (size_t i = 0; i < stage.size(); ++i) host[i] = (double)stage[i];
     return 0;
 }
 
@@ -290,7 +494,8 @@ extern "C" int lare_set_bc(lare_ctx *c, int field, int dim, int edge, int kind, 
     if (kind < LARE_BC_NONE || kind > LARE_BC_CLAMP) return fail(-1, "lare_set_bc: unknown boundary kind");
     c->g.bck[field][dim][edge] = kind;
     c->g.bcv[field][dim][edge] = value;
-    drop_graph(c->g);
+    c->g.bc_version += 1;
+    drop_graphs(c);
     return 0;
 }
 
@@ -323,8 +528,10 @@ int after_launch(lare_ctx *c, const char *what)
 extern "C" int lare_start_of_timestep(lare_ctx *c, double *dt_out)
 {
     if (!c) return fail(-1, "lare_start_of_timestep: null context");
-    lare::step_start(c->g);
-    int rc = after_launch(c, "lare_start_of_timestep");
+    int rc = ev_start_a(c);
+    if (!rc) rc = ev_start_b(c, 2);
+    if (rc) return rc;
+    rc = after_launch(c, "lare_start_of_timestep");
     if (rc) return rc;
     if (dt_out) return lare_get_dt(c, dt_out);
     return 0;
@@ -333,24 +540,23 @@ extern "C" int lare_start_of_timestep(lare_ctx *c, double *dt_out)
 extern "C" int lare_half_timestep(lare_ctx *c)
 {
     if (!c) return fail(-1, "lare_half_timestep: null context");
-    lare::step_half(c->g);
-    return after_launch(c, "lare_half_timestep");
+    int rc = ev_half(c);
+    return rc ? rc : after_launch(c, "lare_half_timestep");
 }
 
 extern "C" int lare_end_of_timestep(lare_ctx *c)
 {
     if (!c) return fail(-1, "lare_end_of_timestep: null context");
-    lare::step_end(c->g);
-    return after_launch(c, "lare_end_of_timestep");
+    int rc = ev_end(c);
+    return rc ? rc : after_launch(c, "lare_end_of_timestep");
 }
 
 extern "C" int lare_step(lare_ctx *c, double *dt_out)
 {
     if (!c) return fail(-1, "lare_step: null context");
-    lare::step_start(c->g);
-    lare::step_half(c->g);
-    lare::step_end(c->g);
-    int rc = after_launch(c, "lare_step");
+    int rc = ev_step(c);
+    if (rc) return rc;
+    rc = after_launch(c, "lare_step");
     if (rc) return rc;
     if (dt_out) return lare_get_dt(c, dt_out);
     return 0;
@@ -362,10 +568,11 @@ extern "C" int lare_steps(lare_ctx *c, int32_t nsteps, double *dt_seq)
     if (nsteps < 0) return fail(-1, "lare_steps: negative step count");
     lare::Grid &g = c->g;
     const bool graph = g.use_graph && !g.prof.on && nsteps > 0;
-    if (graph && !g.graph_exec) {
-        hipError_t e = capture_step(g);
+    hipGraphExec_t exec = captured(c);
+    if (graph && !exec) {
+        hipError_t e = capture_step(c, &exec);
         if (e != hipSuccess) {
-            drop_graph(g);
+            drop_graphs(c);
             return fail(-3, std::string("lare_steps: capturing the step graph: ") + hipGetErrorString(e));
         }
     }
@@ -374,12 +581,10 @@ extern "C" int lare_steps(lare_ctx *c, int32_t nsteps, double *dt_seq)
     int rc = 0;
     for (int i = 0; i < nsteps && !rc; ++i) {
         if (graph) {
-            hipError_t e = hipGraphLaunch(g.graph_exec, g.stream);
+            hipError_t e = hipGraphLaunch(exec, g.stream);
             if (e != hipSuccess) rc = fail(-3, std::string("lare_steps: hipGraphLaunch: ") + hipGetErrorString(e));
         } else {
-            lare::step_start(g);
-            lare::step_half(g);
-            lare::step_end(g);
+            rc = ev_step(c);
         }
         // the step's dt survives in device memory until the next step's
         // start overwrites it; this copy is ordered before that on the stream
@@ -400,7 +605,7 @@ extern "C" int lare_set_graph(lare_ctx *c, int32_t on)
 {
     if (!c) return fail(-1, "lare_set_graph: null context");
     c->g.use_graph = on != 0;
-    if (!on) drop_graph(c->g);
+    if (!on) drop_graphs(c);
     return 0;
 }
 
@@ -473,16 +678,21 @@ extern "C" int lare_set_kernel_set(lare_ctx *c, int32_t kernel_set)
     if (!c) return fail(-1, "lare_set_kernel_set: null context");
     if (kernel_set < 0 || kernel_set > 1) return fail(-1, "lare_set_kernel_set: unknown kernel set");
     HIPCHK(hipStreamSynchronize(c->g.stream));
+    if (kernel_set == 0) {
+        int rc = ensure_set0(c);
+        if (rc) return rc;
+    }
     c->g.kernel_set = kernel_set;
-    drop_graph(c->g);
+    drop_graphs(c);
     return 0;
 }
 
 extern "C" int lare_viscosity_dt(lare_ctx *c, double *dt_out, double *dtr_out)
 {
     if (!c) return fail(-1, "lare_viscosity_dt: null context");
-    lare::step_start_a(c->g);
-    int rc = after_launch(c, "lare_viscosity_dt");
+    int rc = ev_start_a(c);
+    if (rc) return rc;
+    rc = after_launch(c, "lare_viscosity_dt");
     if (rc) return rc;
     lare::Scal s;
     HIPCHK(hipMemcpyAsync(&s, c->g.dev.a.s, sizeof(s), hipMemcpyDeviceToHost, c->g.stream));
@@ -508,6 +718,168 @@ extern "C" int lare_lagrange(lare_ctx *c, double dt)
     s.dt = dt;
     s.dt_sub = dt / (double)substeps;
     HIPCHK(hipMemcpyAsync(c->g.dev.a.s, &s, sizeof(s), hipMemcpyHostToDevice, c->g.stream));
-    lare::step_start_b(c->g, substeps);
-    return after_launch(c, "lare_lagrange");
+    int rc = ev_start_b(c, substeps);
+    return rc ? rc : after_launch(c, "lare_lagrange");
+}
+
+// ------------------------------------------------------------------ batches
+
+struct lare_batch {
+    std::vector<lare_ctx *> members;
+    lare::Group gp;
+    lare::Prof prof;
+    hipStream_t stream = nullptr;
+    bool use_graph = true;
+};
+
+extern "C" int lare_batch_create(lare_ctx *const *members, int32_t n, lare_batch **out)
+{
+    if (!members || !out || n <= 0) return fail(-1, "lare_batch_create: need at least one member");
+    *out = nullptr;
+    lare_batch *b = new (std::nothrow) lare_batch();
+    if (!b) return fail(-4, "lare_batch_create: out of host memory");
+    std::vector<lare::Grid *> grids;
+    for (int i = 0; i < n; ++i) {
+        if (!members[i]) {
+            delete b;
+            return fail(-1, "lare_batch_create: null member");
+        }
+        for (int j = 0; j < i; ++j)
+            if (members[j] == members[i]) {
+                delete b;
+                return fail(-1, "lare_batch_create: a context appears twice");
+            }
+        b->members.push_back(members[i]);
+        grids.push_back(&members[i]->g);
+    }
+    hipError_t e = hipStreamCreate(&b->stream);
+    if (e != hipSuccess) {
+        delete b;
+        return fail(-3, std::string("lare_batch_create: hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    std::string err;
+    int rc = lare::group_init(b->gp, grids, b->stream, &b->prof, err);
+    if (rc) {
+        (void)hipStreamDestroy(b->stream);
+        delete b;
+        return fail(rc, "lare_batch_create: " + err);
+    }
+    *out = b;
+    return 0;
+}
+
+extern "C" int lare_batch_destroy(lare_batch *b)
+{
+    if (!b) return 0;
+    (void)hipStreamSynchronize(b->stream);
+    lare::group_free(b->gp);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return 0;
+}
+
+extern "C" int lare_batch_set_graph(lare_batch *b, int32_t on)
+{
+    if (!b) return fail(-1, "lare_batch_set_graph: null batch");
+    b->use_graph = on != 0;
+    if (!on) lare::group_drop_graph(b->gp);
+    return 0;
+}
+
+extern "C" int lare_batch_steps(lare_batch *b, int32_t nsteps)
+{
+    if (!b) return fail(-1, "lare_batch_steps: null batch");
+    if (nsteps < 0) return fail(-1, "lare_batch_steps: negative step count");
+    if (nsteps == 0) return 0;
+    // work a caller enqueued on a member's own stream comes first
+    for (lare_ctx *c : b->members) HIPCHK(hipStreamSynchronize(c->g.stream));
+    lare::Group &gp = b->gp;
+    // a boundary-condition change since the capture invalidates it
+    bool stale = gp.bc_seen.size() != gp.mem.size();
+    for (size_t m = 0; !stale && m < gp.mem.size(); ++m) stale = gp.bc_seen[m] != gp.mem[m]->bc_version;
+    if (stale) lare::group_drop_graph(gp);
+    const bool graph = b->use_graph && !b->prof.on;
+    if (graph && !gp.graph_exec) {
+        lare::group_prepare(gp);
+        hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            lare::group_step(gp);
+            e = hipStreamEndCapture(b->stream, &gp.graph);
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&gp.graph_exec, gp.graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            lare::group_drop_graph(gp);
+            return fail(-3, std::string("lare_batch_steps: capturing the step graph: ") + hipGetErrorString(e));
+        }
+    }
+    for (int i = 0; i < nsteps; ++i) {
+        if (graph) {
+            hipError_t e = hipGraphLaunch(gp.graph_exec, b->stream);
+            if (e != hipSuccess) return fail(-3, std::string("lare_batch_steps: hipGraphLaunch: ") + hipGetErrorString(e));
+        } else {
+            lare::group_step(gp);
+        }
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(-3, std::string("lare_batch_steps: ") + hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int lare_batch_sync(lare_batch *b)
+{
+    if (!b) return fail(-1, "lare_batch_sync: null batch");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+extern "C" int lare_batch_energy(lare_batch *b, double *out)
+{
+    if (!b || !out) return fail(-1, "lare_batch_energy: null argument");
+    std::vector<double> e;
+    std::string err;
+    int rc = lare::group_energy(b->gp, e, err);
+    if (rc) return fail(rc, err);
+    std::memcpy(out, e.data(), e.size() * sizeof(double));
+    return 0;
+}
+
+extern "C" int lare_energy(lare_ctx *c, double *out)
+{
+    if (!c || !out) return fail(-1, "lare_energy: null argument");
+    int rc = ensure_solo(c);
+    if (rc) return rc;
+    std::vector<double> e;
+    std::string err;
+    rc = lare::group_energy(c->solo, e, err);
+    if (rc) return fail(rc, err);
+    std::memcpy(out, e.data(), 4 * sizeof(double));
+    return 0;
+}
+
+extern "C" int lare_batch_profile_begin(lare_batch *b)
+{
+    if (!b) return fail(-1, "lare_batch_profile_begin: null batch");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->prof.recs.clear();
+    b->prof.on = true;
+    return 0;
+}
+
+extern "C" int lare_batch_profile_end(lare_batch *b, double *total_ms, int32_t *launches)
+{
+    if (!b) return fail(-1, "lare_batch_profile_end: null batch");
+    b->prof.on = false;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    double ms_sum = 0.0;
+    for (auto &r : b->prof.recs) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+        ms_sum += ms;
+    }
+    if (total_ms) *total_ms = ms_sum;
+    if (launches) *launches = (int32_t)b->prof.recs.size();
+    b->prof.recs.clear();
+    return 0;
 }

@@ -61,6 +61,101 @@ struct Dev {
     Arr a;
 };
 
+// ------------------------------------------------------------------------
+// Grid-group engine (the product kernel set): every launch covers a group of
+// grids, a single grid being a group of one.  Arrays are typed by the
+// storage type T of the grid (double, or float for FP32 storage); loads
+// widen to double and stores round from double, so all arithmetic is FP64
+// in both modes.  Flux arrays (the remap fluxes, dm and the resistive EMFs)
+// are FP64 in both modes.
+
+// Inclusive range relative to the grid size: x0..nx+dx1, y0..ny+dy1,
+// z0..nz+dz1.
+struct RSpec {
+    int x0, dx1, y0, dy1, z0, dz1;
+};
+
+// Physics scalars of one grid (simulationData, shared_data.h:60-226).
+struct Phys {
+    double gas_gamma, visc1, visc2_norm, mu0, dt_multiplier;
+    double eta_background, eta0, j_max;
+    int resistive;
+};
+
+// Element of a T-typed array: reads as double, assignment rounds to T.
+template <class T>
+struct Ref {
+    T *q;
+    __device__ __forceinline__ operator double() const { return (double)*q; }
+    __device__ __forceinline__ const Ref &operator=(double v) const
+    {
+        *q = (T)v;
+        return *this;
+    }
+    __device__ __forceinline__ const Ref &operator=(const Ref &o) const { return *this = (double)o; }
+    __device__ __forceinline__ const Ref &operator+=(double v) const { return *this = (double)*q + v; }
+    __device__ __forceinline__ const Ref &operator-=(double v) const { return *this = (double)*q - v; }
+    __device__ __forceinline__ const Ref &operator*=(double v) const { return *this = (double)*q * v; }
+    __device__ __forceinline__ const Ref &operator/=(double v) const { return *this = (double)*q / v; }
+};
+
+template <class T>
+struct Fld {
+    T *p;
+    __device__ __forceinline__ Ref<T> operator[](int i) const { return Ref<T>{p + i}; }
+};
+
+template <class T>
+struct ArrE {
+    // registered variables (dm is a flux: FP64)
+    Fld<T> rho, ee, ei, vx, vy, vz, bx, by, bz, vx1, vy1, vz1;
+    double *dm;
+    // Lagrangian phase
+    Fld<T> cs, cs_v, rho_v, alpha1, alpha2, alpha3, visc_heat, fxv, fyv, fzv;
+    Fld<T> cv1, bx1, by1, bz1, p_e, p_i, eta, curlb;
+    double *flux_x, *flux_y, *flux_z;
+    // remap
+    Fld<T> cv2, cvc1, db1, rrho_v;
+    double *dmv;
+    double *fl[5];
+    // 1D axis deltas, index i stored at [i + 2]
+    double *dxb, *dyb, *dzb, *dxc, *dyc, *dzc;
+    Scal *s;
+};
+
+template <class T>
+struct EDev {
+    Geo g;
+    ArrE<T> a;
+    Phys p;
+};
+
+// One launch over a group: the members' device views, the first block of
+// each member (ngrid entries, ascending), the total block count and the
+// range.
+template <class T>
+struct Src {
+    const EDev<T> *devs;
+    const int *boff;
+    int ngrid;
+    int nblk;
+    RSpec r;
+};
+
+// Built-in boundary conditions of one variable along one dimension, for one
+// member of a group (see lare_kernels.h BCVar); f64 marks an FP64 array (dm)
+// in an FP32-storage grid.
+struct BCVarE {
+    void *p;
+    int f64;
+    int sd, sa, sb;
+    int k0, k1;
+    double v0, v1;
+};
+struct BCSetE {
+    BCVarE v[9];
+};
+
 }  // namespace lare
 
 // Linear index of (i,j,k) in the shared box.

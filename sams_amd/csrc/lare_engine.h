@@ -1,0 +1,81 @@
+// Grid-group engine: the product kernel set of the LARE3D step.
+//
+// A Group is an ordered set of grids of one storage type that advance
+// together: every kernel of the step is ONE launch covering all members
+// (uneven sizes included), each workgroup finding its grid by a binary
+// search over the members' first-block offsets.  A single grid is a group of
+// one.  Results per grid do not depend on the group it runs in: no value
+// crosses grids and the only reduction (the CFL minimum) is per grid and
+// exact.
+#pragma once
+
+#include "lare_kernels.h"
+
+#include <array>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace lare {
+
+struct Group {
+    bool fp32 = false;
+    std::vector<Grid *> mem;  // members, in order
+    std::vector<int> res;     // indices of resistive members
+    // device arrays of EDev<T>: [0] all members, [1] resistive members
+    void *devs[2] = {nullptr, nullptr};
+    int n[2] = {0, 0};
+    struct Tab {
+        int *boff = nullptr;
+        int nblk = 0;
+        double cells = 0.0;
+    };
+    std::map<std::pair<int, std::array<int, 6>>, Tab> tabs;  // (subset, range spec)
+    struct BTab {
+        BCSetE *sets = nullptr;
+        int *boff = nullptr;
+        int nblk = 0;
+        bool any = false;
+    };
+    std::map<std::pair<int, int>, BTab> btabs;  // (variable group, dim)
+    std::vector<long> bc_seen;                  // members' bc_version when btabs were built
+    hipStream_t stream = nullptr;
+    Prof *prof = nullptr;
+    bool planning = false;
+    // one captured step of the whole group
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+};
+
+// Build the group's device views (members must outlive it); the group
+// launches on `stream` and records into `prof` when profiling is on.
+int group_init(Group &gp, const std::vector<Grid *> &members, hipStream_t stream, Prof *prof, std::string &err);
+void group_free(Group &gp);
+// Drop the captured graph (after a boundary-condition change).
+void group_drop_graph(Group &gp);
+// Create every table a step needs (outside graph capture).
+void group_prepare(Group &gp);
+
+// The three events of the runner (runner.h:564-586) for every member:
+// start_a = prologue, shock viscosity and CFL reduction (dt on the device),
+// start_b = resistive substeps (substeps of them) and predictor,
+// half = corrector, end = remap with its boundary fills and the step count.
+void group_start_a(Group &gp);
+void group_start_b(Group &gp, int substeps);
+void group_half(Group &gp);
+void group_end(Group &gp);
+inline void group_step(Group &gp)
+{
+    group_start_a(gp);
+    group_start_b(gp, 2);
+    group_half(gp);
+    group_end(gp);
+}
+
+// Total energy of every member, FP64 (the energy-conservation diagnostic):
+// out[4*i + 0..3] = internal, kinetic, magnetic, total, summed over the
+// domain cells and vertices in a fixed order.
+int group_energy(Group &gp, std::vector<double> &out, std::string &err);
+
+}  // namespace lare

@@ -1,0 +1,1597 @@
+// Grid-group engine kernels: the LARE3D step (lagran.cpp, remap.cpp,
+// xremap.cpp, yremap.cpp, zremap.cpp) as 21 launch shapes, each covering
+// every grid of a group.
+//
+// Work decomposition.  A launch has one 256-lane workgroup per 256 cells of
+// one x-plane of one grid's range: lanes run along the flattened (y, z)
+// plane, z fastest, so a wavefront reads 64 consecutive doubles of a row
+// (or the tail of one row and the head of the next).  Workgroup b of the
+// launch belongs to the grid whose first block is the largest boff <= b.
+// The hardware hands workgroups to the 8 XCDs round-robin; xcd_swizzle
+// renumbers them so that each XCD walks one contiguous run of x-planes and
+// the planes a stencil shares stay in that XCD's L2.
+//
+// Arithmetic.  Every value is computed by the expression, association and
+// operand order of the reference (and of the one-kernel-per-reference-kernel
+// set in lare_kernels.hip); -ffp-contract=off.  In FP32 storage the same
+// expressions run in FP64 on values widened from FP32.
+//
+// Fusion (what differs from the reference's kernel boundaries; none changes
+// a value):
+//  * Lagrangian phase in 9 launches: quantities the reference stores and
+//    reads back on the same cell (bx1..bz1 face averages, p_e, p_i,
+//    pressure, cv, cv_v, the metric) are recomputed where read;
+//  * each remap sweep in 7 launches plus its boundary fills: rho, energies
+//    and velocities are updated in place by the last launch after every
+//    reader of their old values has run; the shifted sweep velocity and the
+//    vertex cv2 are recomputed where used;
+//  * the metric (dxab, cv, dxac, ...) is always the product of the 1D
+//    deltas, as LARE3D::grid stores it (setup.cpp:337-390, Cartesian h = 1).
+#include "lare_engine.h"
+#include "lare_num.h"
+
+#include <float.h>
+
+#include <algorithm>
+#include <cstring>
+
+using lare::ArrE;
+using lare::EDev;
+using lare::Fld;
+using lare::Geo;
+using lare::Phys;
+using lare::R3;
+using lare::Scal;
+using lare::Src;
+
+#define RHO A.rho
+#define EE A.ee
+#define EI A.ei
+#define VX A.vx
+#define VY A.vy
+#define VZ A.vz
+#define BX A.bx
+#define BY A.by
+#define BZ A.bz
+#define VX1 A.vx1
+#define VY1 A.vy1
+#define VZ1 A.vz1
+#define DM A.dm
+
+// Workgroup b in dispatch order -> logical workgroup: XCD b % 8 takes the
+// contiguous logical run [start(x), start(x) + count(x)).
+__device__ __forceinline__ int xcd_swizzle(int b, int nblk)
+{
+    const int per = nblk >> 3, rem = nblk & 7;
+    const int x = b & 7, k = b >> 3;
+    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + k;
+}
+
+// Largest g with boff[g] <= b (boff ascending, boff[0] = 0).
+__device__ __forceinline__ int grid_of(const int *boff, int n, int b)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (boff[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Cell of this lane: grid g_, (ix, iy, iz) in range R; ok_ is false for
+// the lanes past the end of the plane.
+#define EBEGIN_ALL                                                                                     \
+    const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);                                           \
+    const int g_ = grid_of(S.boff, S.ngrid, b_);                                                   \
+    const EDev<T> &D_ = S.devs[g_];                                                                \
+    const Geo G = D_.g;                                                                            \
+    const ArrE<T> A = D_.a;                                                                        \
+    const Phys P = D_.p;                                                                           \
+    const R3 R = {S.r.x0, G.nx + S.r.dx1, S.r.y0, G.ny + S.r.dy1, S.r.z0, G.nz + S.r.dz1};         \
+    const int nzr_ = R.z1 - R.z0 + 1;                                                              \
+    const int plane_ = (R.y1 - R.y0 + 1) * nzr_;                                                   \
+    const int bpp_ = (plane_ + 255) >> 8;                                                          \
+    const int lb_ = b_ - S.boff[g_];                                                               \
+    const int px_ = lb_ / bpp_;                                                                    \
+    const int t_ = (lb_ - px_ * bpp_) * 256 + (int)threadIdx.x;                                   \
+    const bool ok_ = t_ < plane_;                                                                  \
+    const int ix = R.x0 + px_;                                                                     \
+    const int iy = R.y0 + t_ / nzr_;                                                               \
+    const int iz = R.z0 + t_ % nzr_;                                                               \
+    (void)P;                                                                                       \
+    (void)ix;                                                                                      \
+    (void)iy;                                                                                      \
+    (void)iz;
+#define EBEGIN    \
+    EBEGIN_ALL    \
+    if (!ok_) return;
+
+template <int DIR>
+__device__ __forceinline__ int sdir(const Geo &G)
+{
+    return DIR == 0 ? G.sx : DIR == 1 ? G.sy : 1;
+}
+
+// =================================================================== remap
+
+// R1: cell volumes before (cv1) and after (cv2) the remap of DIR and the
+// Lagrangian cell width db1 (xremap.cpp:26-70 and twins), centred extent.
+template <class T, int DIR>
+KERNEL void f_prep(const Src<T> S, double xpass, double ypass, double zpass)
+{
+    EBEGIN
+    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    const double dt = A.s->dt;
+    const int ooo = c, moo = c - sx, omo = c - sy, oom = c - 1, mmo = c - sx - sy, mom = c - sx - 1,
+              omm = c - sy - 1, mmm = c - sx - sy - 1;
+    double vxb = (VX1[ooo] + VX1[omo] + VX1[oom] + VX1[omm]) * 0.25;
+    double vxbm = (VX1[moo] + VX1[mmo] + VX1[mom] + VX1[mmm]) * 0.25;
+    double vyb = (VY1[ooo] + VY1[moo] + VY1[oom] + VY1[mom]) * 0.25;
+    double vybm = (VY1[omo] + VY1[mmo] + VY1[omm] + VY1[mmm]) * 0.25;
+    double vzb = (VZ1[ooo] + VZ1[moo] + VZ1[omo] + VZ1[mmo]) * 0.25;
+    double vzbm = (VZ1[oom] + VZ1[mom] + VZ1[omm] + VZ1[mmm]) * 0.25;
+    const double vol = CV(ix, iy, iz);
+    const double axp = DXAB(iy, iz), axm = DXAB(iy, iz);
+    const double ayp = DYAB(ix, iz), aym = DYAB(ix, iz);
+    const double azp = DZAB(ix, iy), azm = DZAB(ix, iy);
+    double dv;
+    if (DIR == 0) {
+        double dvxdx = (vxb * axp - vxbm * axm) / vol;
+        double dvydy = ypass * (vyb * ayp - vybm * aym) / vol;
+        double dvzdz = zpass * (vzb * azp - vzbm * azm) / vol;
+        dv = (dvydy + dvzdz) * dt;
+        A.cv2[c] = vol * (1.0 + dv);
+        dv = dv + dvxdx * dt;
+        A.cv1[c] = vol * (1.0 + dv);
+        A.db1[c] = DXB(ix) + (vxb - vxbm) * dt;
+    } else if (DIR == 1) {
+        double dvxdx = xpass * (vxb * axp - vxbm * axm) / vol;
+        double dvydy = (vyb * ayp - vybm * aym) / vol;
+        double dvzdz = zpass * (vzb * azp - vzbm * azm) / vol;
+        dv = (dvxdx + dvzdz) * dt;
+        A.cv2[c] = vol * (1.0 + dv);
+        dv = dv + dvydy * dt;
+        A.cv1[c] = vol * (1.0 + dv);
+        A.db1[c] = DYB(iy) + (vyb - vybm) * dt;
+    } else {
+        double dvxdx = xpass * (vxb * axp - vxbm * axm) / vol;
+        double dvydy = ypass * (vyb * ayp - vybm * aym) / vol;
+        double dvzdz = (vzb * azp - vzbm * azm) / vol;
+        dv = (dvxdx + dvydy) * dt;
+        A.cv2[c] = vol * (1.0 + dv);
+        dv = dv + dvzdz * dt;
+        A.cv1[c] = vol * (1.0 + dv);
+        A.db1[c] = DZB(iz) + (vzb - vzbm) * dt;
+    }
+}
+
+// R2: vertex volume before remap, cvc1 = 0.125 * (8 cells of cv1).
+template <class T>
+KERNEL void f_cvc1(const Src<T> S)
+{
+    EBEGIN
+    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    const Fld<T> C1 = A.cv1;
+    A.cvc1[c] = 0.125 * (C1[c] + C1[c + sx] + C1[c + sy] + C1[c + sx + sy] + C1[c + 1] + C1[c + sx + 1] +
+                         C1[c + sy + 1] + C1[c + sx + sy + 1]);
+}
+
+// Constrained-transport flux of face magnetic flux B along DIR: v_advect is
+// the mean of w at c and c+ow, db1 is averaged with c+tw (vx_by_flux,
+// xremap.cpp:269-319, and its five twins).
+template <int DIR, class T>
+__device__ __forceinline__ double ct_flux(const ArrE<T> &A, const Geo &G, int c, const Fld<T> B, const Fld<T> w,
+                                          int ow, int tw, double dt)
+{
+    const int sd = sdir<DIR>(G);
+    const Fld<T> db1 = A.db1;
+    double v_advect = (w[c] + w[c + ow]) * 0.5;
+    double o_v = v_advect * dt;
+    double dbx = (db1[c] + db1[c + tw]) * 0.5;
+    double dbxp = (db1[c + sd] + db1[c + sd + tw]) * 0.5;
+    double dbxp2 = (db1[c + 2 * sd] + db1[c + 2 * sd + tw]) * 0.5;
+    double dbxm = (db1[c - sd] + db1[c - sd + tw]) * 0.5;
+    double fm = B[c - sd] / dbxm;
+    double fi = B[c] / dbx;
+    double fp = B[c + sd] / dbxp;
+    double fp2 = B[c + 2 * sd] / dbxp2;
+    double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
+    double sign_v = (v_advect >= 0.0) ? 1.0 : -1.0;
+    double vad_p = (sign_v + 1.0) * 0.5;
+    double vad_m = 1.0 - vad_p;
+    double fu = fi * vad_p + fp * vad_m;
+    double dfu = dfm * vad_p + dfp * vad_m;
+    double dxci = A.cvc1[c];
+    double dxcu = A.cvc1[c - sd] * vad_p + A.cvc1[c + sd] * vad_m;
+    double dxbu = A.cv1[c] * vad_p + A.cv1[c + sd] * vad_m;
+    double dxu = dbx * vad_p + dbxp * vad_m;
+    double phi = fabs(o_v) / dxu;
+    double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
+    return (fu + Di * (1.0 - phi)) * o_v;
+}
+
+// R3: both constrained-transport fluxes of the sweep, over (0..n)^3
+// (vx_by_flux + vx_bz_flux, vy_bx_flux + vy_bz_flux, vz_bx_flux + vz_by_flux).
+// The second flux reads the B component the first flux's updates leave
+// untouched, so both can be computed before any update.
+template <class T, int DIR>
+KERNEL void f_ct_fluxes(const Src<T> S)
+{
+    EBEGIN
+    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    const double dt = A.s->dt;
+    if (DIR == 0) {
+        A.fl[0][c] = ct_flux<0>(A, G, c, BY, VX1, -1, sy, dt);
+        A.fl[1][c] = ct_flux<0>(A, G, c, BZ, VX1, -sy, 1, dt);
+    } else if (DIR == 1) {
+        A.fl[0][c] = ct_flux<1>(A, G, c, BX, VY1, -1, sx, dt);
+        A.fl[1][c] = ct_flux<1>(A, G, c, BZ, VY1, -sx, 1, dt);
+    } else {
+        A.fl[0][c] = ct_flux<2>(A, G, c, BX, VZ1, -sy, sx, dt);
+        A.fl[1][c] = ct_flux<2>(A, G, c, BY, VZ1, -sx, sy, dt);
+    }
+}
+
+// R4: the four CT updates of the sweep in their reference order, each over
+// its own range (xremap.cpp:85-114, yremap.cpp:93-127, zremap.cpp:97-117).
+template <class T, int DIR>
+KERNEL void f_ct_update(const Src<T> S)
+{
+    EBEGIN
+    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const double *fa = A.fl[0], *fb = A.fl[1];
+    const bool x1 = ix >= 1 && ix <= nx, x0 = ix <= nx;
+    const bool y1 = iy >= 1 && iy <= ny, y0 = iy <= ny;
+    const bool z1 = iz >= 1 && iz <= nz, z0 = iz <= nz;
+    if (DIR == 0) {
+        if (x1 && y0 && z1) BY[c] = BY[c] - fa[c] + fa[c - sx];
+        double bx = BX[c];
+        const bool bxr = x0 && y1 && z1;
+        if (bxr) bx = bx + fa[c] - fa[c - sy];
+        if (x1 && y1 && z0) BZ[c] = BZ[c] - fb[c] + fb[c - sx];
+        if (bxr) BX[c] = bx + fb[c] - fb[c - 1];
+    } else if (DIR == 1) {
+        const bool bxr = x0 && y1 && z1;
+        if (bxr) BX[c] = BX[c] - fa[c] + fa[c - sy];
+        double by = BY[c];
+        const bool byr = x1 && y0 && z1;
+        if (byr) by = by + fa[c] - fa[c - sx];
+        if (x1 && y1 && z0) BZ[c] = BZ[c] - fb[c] + fb[c - sy];
+        if (byr) BY[c] = by + fb[c] - fb[c - 1];
+    } else {
+        if (x0 && y1 && z1) BX[c] = BX[c] - fa[c] + fa[c - 1];
+        double bz = BZ[c];
+        const bool bzr = x1 && y1 && z0;
+        if (bzr) bz = bz + fa[c] - fa[c - sx];
+        if (x1 && y0 && z1) BY[c] = BY[c] - fb[c] + fb[c - 1];
+        if (bzr) BZ[c] = bz + fb[c] - fb[c - sy];
+    }
+}
+
+// Face-averaged sweep velocity of the mass and energy fluxes.
+template <int DIR, class T>
+__device__ __forceinline__ double face_v(const ArrE<T> &A, const Geo &G, int c)
+{
+    const int sx = G.sx, sy = G.sy;
+    if (DIR == 0) return (VX1[c] + VX1[c - sy] + VX1[c - 1] + VX1[c - sy - 1]) * 0.25;
+    if (DIR == 1) return (VY1[c - sx] + VY1[c] + VY1[c - sx - 1] + VY1[c - 1]) * 0.25;
+    return (VZ1[c - sx] + VZ1[c] + VZ1[c - sx - sy] + VZ1[c - sy]) * 0.25;
+}
+
+template <int DIR, class T>
+__device__ __forceinline__ double face_area(const ArrE<T> &A, int ix, int iy, int iz)
+{
+    if (DIR == 0) return DXAB(iy, iz);
+    if (DIR == 1) return DYAB(ix, iz);
+    return DZAB(ix, iy);
+}
+
+// R5: mass flux dm on the faces of DIR (x_mass_flux, xremap.cpp:373-422 and
+// twins) over its reference range; along DIR the range is widened to the
+// ghost faces, where dm is set to the zero of the reference's pw::assign(dm,
+// 0.0) before the single-dimension boundary fill overwrites them.
+template <class T, int DIR>
+KERNEL void f_mass_flux(const Src<T> S)
+{
+    EBEGIN
+    const int c = IX(ix, iy, iz);
+    const int sd = sdir<DIR>(G);
+    const int id = DIR == 0 ? ix : DIR == 1 ? iy : iz;
+    const int nd = DIR == 0 ? G.nx : DIR == 1 ? G.ny : G.nz;
+    if (id < 0 || id > nd) {
+        DM[c] = 0.0;
+        return;
+    }
+    const double dt = A.s->dt;
+    double area = face_area<DIR>(A, ix, iy, iz);
+    double v_advect = face_v<DIR>(A, G, c);
+    double o_v = v_advect * dt * area;
+    double fm = RHO[c - sd], fi = RHO[c], fp = RHO[c + sd], fp2 = RHO[c + 2 * sd];
+    double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
+    double sign_v = (v_advect >= 0.0) ? 1.0 : -1.0;
+    double vad_p = (sign_v + 1.0) * 0.5;
+    double vad_m = 1.0 - vad_p;
+    double fu = fi * vad_p + fp * vad_m;
+    double dfu = dfm * vad_p + dfp * vad_m;
+    double dxci = A.cvc1[c];
+    double dxcu = A.cvc1[c - sd] * vad_p + A.cvc1[c + sd] * vad_m;
+    double dxbu = A.cv1[c] * vad_p + A.cv1[c + sd] * vad_m;
+    double phi = fabs(o_v) / dxbu;
+    double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
+    DM[c] = (fu + Di * (1.0 - phi)) * o_v;
+}
+
+// Energy flux of specific energy E on face c (x_energy_flux,
+// xremap.cpp:429-483 and twins); rho is still the pre-remap density.
+template <int DIR, class T>
+__device__ __forceinline__ double energy_flux(const ArrE<T> &A, const Geo &G, int c, const Fld<T> E,
+                                              double v_advect, double o_v)
+{
+    const int sd = sdir<DIR>(G);
+    double fm = E[c - sd], fi = E[c], fp = E[c + sd], fp2 = E[c + 2 * sd];
+    double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
+    double sign_v = (v_advect >= 0.0) ? 1.0 : -1.0;
+    double vad_p = (sign_v + 1.0) * 0.5;
+    double vad_m = 1.0 - vad_p;
+    double fu = fi * vad_p + fp * vad_m;
+    double dfu = dfm * vad_p + dfp * vad_m;
+    double dxci = A.cvc1[c];
+    double dxcu = A.cvc1[c - sd] * vad_p + A.cvc1[c + sd] * vad_m;
+    double dxbu = A.cv1[c] * vad_p + A.cv1[c + sd] * vad_m;
+    double phi = fabs(o_v) / dxbu;
+    double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
+    double rhou = RHO[c] * vad_p + RHO[c + sd] * vad_m;
+    double dm = DM[c];
+    double dmu = fabs(dm) / dxbu / rhou;
+    return (fu + Di * (1.0 - dmu)) * dm;
+}
+
+// R6: both energy fluxes on faces (0..n)^3, the vertex density before remap
+// (range -1..n+1 along DIR, 0..n across) and the vertex mass flux dmv
+// (range -1..n along DIR, 0..n across).  All read only pre-update values.
+template <class T, int DIR>
+KERNEL void f_energy_vertex(const Src<T> S)
+{
+    EBEGIN
+    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const int id = DIR == 0 ? ix : DIR == 1 ? iy : iz;
+    const int nd = DIR == 0 ? nx : DIR == 1 ? ny : nz;
+    const bool face = id >= 0 && id <= nd;
+    if (face) {
+        const double dt = A.s->dt;
+        double area = face_area<DIR>(A, ix, iy, iz);
+        double v_advect = face_v<DIR>(A, G, c);
+        double o_v = v_advect * dt * area;
+        A.fl[0][c] = energy_flux<DIR>(A, G, c, EE, v_advect, o_v);
+        A.fl[1][c] = energy_flux<DIR>(A, G, c, EI, v_advect, o_v);
+    }
+    // vertex density before remap: x divides (s*0.125)/cvc1, y and z
+    // multiply by (0.125/cvc1) (xremap.cpp:159-176, yremap.cpp:163-183)
+    const Fld<T> R1 = RHO, C1 = A.cv1;
+    double s = R1[c] * C1[c] + R1[c + sx] * C1[c + sx] + R1[c + sy] * C1[c + sy] +
+               R1[c + sx + sy] * C1[c + sx + sy] + R1[c + 1] * C1[c + 1] + R1[c + sx + 1] * C1[c + sx + 1] +
+               R1[c + sy + 1] * C1[c + sy + 1] + R1[c + sx + sy + 1] * C1[c + sx + sy + 1];
+    if (DIR == 0)
+        A.rrho_v[c] = s * 0.125 / A.cvc1[c];
+    else
+        A.rrho_v[c] = s * (0.125 / A.cvc1[c]);
+    if (id <= nd) {
+        const double *M = DM;
+        double q = M[c] + M[c + sx] + M[c + sy] + M[c + sx + sy] + M[c + 1] + M[c + sx + 1] + M[c + sy + 1] +
+                   M[c + sx + sy + 1];
+        A.dmv[c] = DIR == 0 ? q * 0.125 : 0.125 * q;
+    }
+}
+
+// Momentum flux of velocity component f on vertex face c (x_mom_flux,
+// xremap.cpp:485-566 and twins), multiplied by dm.  z takes dyac as its area
+// (zremap.cpp:475) and divides by (dzbu*rhou) (zremap.cpp:505).
+template <int DIR, class T>
+__device__ __forceinline__ double mom_flux(const Geo &G, int c, const Fld<T> f, double o_v, double sign_v,
+                                           double dxci, double dxcu, double dxbu, double rhou, double dm)
+{
+    const int sd = sdir<DIR>(G);
+    double fm = f[c - sd], fi = f[c], fp = f[c + sd], fp2 = f[c + 2 * sd];
+    double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
+    double vad_p = (sign_v + 1.0) * 0.5;
+    double vad_m = 1.0 - vad_p;
+    double fu = fi * vad_p + fp * vad_m;
+    double dfu = dfm * vad_p + dfp * vad_m;
+    double phi = fabs(o_v) / dxbu;
+    double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
+    double dmu = DIR == 2 ? fabs(dm) / (dxbu * rhou) : fabs(dm) / dxbu / rhou;
+    double fl = fu + Di * (1.0 - dmu);
+    return fl * dm;
+}
+
+// R7: the three momentum fluxes over (-1..n along DIR, 0..n across).
+template <class T, int DIR>
+KERNEL void f_mom_fluxes(const Src<T> S)
+{
+    EBEGIN
+    const int c = IX(ix, iy, iz);
+    const int sd = sdir<DIR>(G);
+    const double dt = A.s->dt;
+    const Fld<T> w = DIR == 0 ? VX1 : DIR == 1 ? VY1 : VZ1;
+    double area = DIR == 0 ? DXAC(iy, iz) : DYAC(ix, iz);
+    double v_advect = (w[c] + w[c + sd]) * 0.5;
+    double o_v = v_advect * dt * area;
+    double sign_v = (o_v >= 0.0) ? 1.0 : -1.0;
+    double vad_p = (sign_v + 1.0) * 0.5;
+    double vad_m = 1.0 - vad_p;
+    double dxci = A.cv1[c + sd];
+    double dxcu = A.cv1[c] * vad_p + A.cv1[c + 2 * sd] * vad_m;
+    double dxbu = A.cvc1[c] * vad_p + A.cvc1[c + sd] * vad_m;
+    double rhou = A.rrho_v[c] * vad_p + A.rrho_v[c + sd] * vad_m;
+    double dm = A.dmv[c];
+    A.fl[2][c] = mom_flux<DIR>(G, c, VX, o_v, sign_v, dxci, dxcu, dxbu, rhou, dm);
+    A.fl[3][c] = mom_flux<DIR>(G, c, VY, o_v, sign_v, dxci, dxcu, dxbu, rhou, dm);
+    A.fl[4][c] = mom_flux<DIR>(G, c, VZ, o_v, sign_v, dxci, dxcu, dxbu, rhou, dm);
+}
+
+// R8: the in-place updates over (0..n)^3: density and both energies on cells
+// 1..n (xremap.cpp:121-154), vertex density after remap and the three
+// velocities on vertices 0..n (xremap.cpp:180-262).
+template <class T, int DIR>
+KERNEL void f_update(const Src<T> S)
+{
+    EBEGIN
+    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    const int sd = sdir<DIR>(G);
+    const Fld<T> C2 = A.cv2;
+    // vertex cv2 (cv2 is not written here)
+    double s2 = C2[c] + C2[c + sx] + C2[c + sy] + C2[c + sx + sy] + C2[c + 1] + C2[c + sx + 1] + C2[c + sy + 1] +
+                C2[c + sx + sy + 1];
+    const double cv2v = DIR == 0 ? s2 * 0.125 : 0.125 * s2;
+    const double cvc1 = A.cvc1[c];
+    const double rv = A.rrho_v[c];
+    const double rho_v1 = (rv * cvc1 + A.dmv[c - sd] - A.dmv[c]) / cv2v;
+    const double den = cv2v * rho_v1;
+    VX[c] = (rv * VX[c] * cvc1 + A.fl[2][c - sd] - A.fl[2][c]) / den;
+    VY[c] = (rv * VY[c] * cvc1 + A.fl[3][c - sd] - A.fl[3][c]) / den;
+    VZ[c] = (rv * VZ[c] * cvc1 + A.fl[4][c - sd] - A.fl[4][c]) / den;
+    if (ix >= 1 && iy >= 1 && iz >= 1) {
+        const double rho1 = RHO[c], cv1 = A.cv1[c], cv2 = C2[c];
+        const double rho = (rho1 * cv1 + DM[c - sd] - DM[c]) / cv2;
+        RHO[c] = rho;
+        EE[c] = (EE[c] * cv1 * rho1 + A.fl[0][c - sd] - A.fl[0][c]) / (cv2 * rho);
+        EI[c] = (EI[c] * cv1 * rho1 + A.fl[1][c - sd] - A.fl[1][c]) / (cv2 * rho);
+    }
+}
+
+// B *= area (remap.cpp:46-50) and B /= (area + none_zero) (91-95) over each
+// component's full extent; the areas are dxab, dyab, dzab.
+template <class T, int COMP, int DIVIDE>
+KERNEL void f_b_area(const Src<T> S)
+{
+    EBEGIN
+    const int c = IX(ix, iy, iz);
+    const Fld<T> B = COMP == 0 ? BX : COMP == 1 ? BY : BZ;
+    const double ar = COMP == 0 ? DXAB(iy, iz) : COMP == 1 ? DYAB(ix, iz) : DZAB(ix, iy);
+    if (DIVIDE)
+        B[c] /= (ar + DBL_EPSILON);
+    else
+        B[c] *= ar;
+}
+
+// ============================================================= Lagrangian
+
+#define PE_OF(c) ((gas_gamma - 1.0) * RHO[c] * EE[c])
+#define PI_OF(c) ((gas_gamma - 1.0) * RHO[c] * EI[c])
+
+// L1 (centred extent): sound speed (lagran.cpp:190-197) from the prologue
+// values of lagran.cpp:113-125; resets the CFL minimum.  With resistive MHD
+// it also stores p_e: the predictor reads the prologue value after the
+// resistive heating changed e_e.
+template <class T>
+KERNEL void g_cs(const Src<T> S)
+{
+    EBEGIN
+    const double gas_gamma = P.gas_gamma;
+    if (ix == R.x0 && iy == R.y0 && iz == R.z0) A.s->dtmin_bits = 0x7FEFFFFFFFFFFFFFull;
+    const int c = IX(ix, iy, iz);
+    double bx1 = 0.5 * (BX[c] + BX[c - G.sx]);
+    double by1 = 0.5 * (BY[c] + BY[c - G.sy]);
+    double bz1 = 0.5 * (BZ[c] + BZ[c - 1]);
+    double pe = PE_OF(c), pi = PI_OF(c);
+    double p = pe + pi;
+    if (P.resistive) A.p_e[c] = pe;
+    double rmin = pmax(RHO[c], DBL_EPSILON);
+    double b2 = bx1 * bx1 + by1 * by1 + bz1 * bz1;
+    A.cs[c] = sqrt((gas_gamma * p + b2) / rmin);
+}
+
+#define CV8SUM(ix, iy, iz)                                                                               \
+    (CV(ix, iy, iz) + CV(ix + 1, iy, iz) + CV(ix, iy + 1, iz) + CV(ix + 1, iy + 1, iz) + CV(ix, iy, iz + 1) + \
+     CV(ix + 1, iy, iz + 1) + CV(ix, iy + 1, iz + 1) + CV(ix + 1, iy + 1, iz + 1))
+
+// L2 (-1..n+1)^3: vertex density rho_v (lagran.cpp:128-153) and the vertex
+// sound speed cs_v (lagran.cpp:199-216).
+template <class T>
+KERNEL void g_vertex(const Src<T> S)
+{
+    EBEGIN
+    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    const double c000 = CV(ix, iy, iz), c100 = CV(ix + 1, iy, iz), c010 = CV(ix, iy + 1, iz),
+                 c110 = CV(ix + 1, iy + 1, iz), c001 = CV(ix, iy, iz + 1), c101 = CV(ix + 1, iy, iz + 1),
+                 c011 = CV(ix, iy + 1, iz + 1), c111 = CV(ix + 1, iy + 1, iz + 1);
+    const Fld<T> r = RHO, s = A.cs;
+    double sum_rho_cv = r[c] * c000 + r[c + sx] * c100 + r[c + sy] * c010 + r[c + sx + sy] * c110 + r[c + 1] * c001 +
+                        r[c + sx + 1] * c101 + r[c + sy + 1] * c011 + r[c + sx + sy + 1] * c111;
+    double sum_cv = c000 + c100 + c010 + c110 + c001 + c101 + c011 + c111;
+    A.rho_v[c] = sum_rho_cv / sum_cv;
+    double cv_v = 0.125 * sum_cv;
+    double sum = s[c] * c000 + s[c + sx] * c100 + s[c + sy] * c010 + s[c + sx + sy] * c110 + s[c + 1] * c001 +
+                 s[c + sx + 1] * c101 + s[c + sy + 1] * c011 + s[c + sx + sy + 1] * c111;
+    A.cs_v[c] = 0.125 * sum / cv_v;
+}
+
+// edge_viscosity (lagran.cpp:56-93) between c1 and c1 + s.
+template <class T>
+__device__ __forceinline__ double g_edge_visc(const ArrE<T> &A, double dt, double visc1, double visc2_norm,
+                                              double dvdots, double dx, double dxm, double dxp, double cs_edge,
+                                              int c1, int s)
+{
+    const int c0 = c1 - s, c2 = c1 + s, c3 = c2 + s;
+    dvdots = pmin(0.0, dvdots);
+    double rho_edge = 2.0 * A.rho_v[c1] * A.rho_v[c2] / (A.rho_v[c1] + A.rho_v[c2]);
+    double dvx = VX[c1] - VX[c2];
+    double dvy = VY[c1] - VY[c2];
+    double dvz = VZ[c1] - VZ[c2];
+    double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
+    double dv = sqrt(dv2);
+    double psi = 0.0;
+    const bool small = dv * dt / dx < 1.e-14;
+    dvdots = small ? 0.0 : dvdots / dv;
+    double dvxm = VX[c0] - VX[c1];
+    double dvxp = VX[c2] - VX[c3];
+    double dvym = VY[c0] - VY[c1];
+    double dvyp = VY[c2] - VY[c3];
+    double dvzm = VZ[c0] - VZ[c1];
+    double dvzp = VZ[c2] - VZ[c3];
+    double rl = small ? 1.0 : (dvxp * dvx + dvyp * dvy + dvzp * dvz) * dx / (dxp * dv2);
+    double rr = small ? 1.0 : (dvxm * dvx + dvym * dvy + dvzm * dvz) * dx / (dxm * dv2);
+    psi = pmin_list4(0.5 * (rr + rl), 2.0 * rl, 2.0 * rr, 1.0);
+    psi = pmax(0.0, psi);
+    double q_k_bar = rho_edge * (visc2_norm * dv + sqrt(visc2_norm * visc2_norm * dv2 + (visc1 * cs_edge) * (visc1 * cs_edge)));
+    return q_k_bar * (1.0 - psi) * dvdots;
+}
+
+// L3: alpha1, alpha2, alpha3 (lagran.cpp:211-274), each over its own range
+// inside the union (-1..n+1, -1..n+2, 0..n+2).  dt is the previous step's.
+template <class T>
+KERNEL void g_alpha(const Src<T> S)
+{
+    EBEGIN
+    const double visc1 = P.visc1, visc2_norm = P.visc2_norm;
+    const int sx = G.sx, sy = G.sy;
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const double dt = A.s->dt;
+    const int c = IX(ix, iy, iz);
+    if (ix >= 0 && ix <= nx + 1 && iy >= 0 && iy <= ny + 2 && iz >= 0 && iz <= nz + 2) {
+        const int c1 = c - sx - sy - 1;
+        double dvdots = -(VX[c1] - VX[c1 + sx]);
+        double cs_edge = pmin(A.cs_v[c1], A.cs_v[c1 + sx]);
+        A.alpha1[c] = g_edge_visc(A, dt, visc1, visc2_norm, dvdots, DXB(ix), DXB(ix - 1), DXB(ix + 1), cs_edge, c1, sx);
+    }
+    if (ix <= nx + 1 && iy >= 0 && iy <= ny + 1 && iz >= 0 && iz <= nz + 2) {
+        const int c1 = c - sy - 1;
+        double dvdots = -(VY[c1] - VY[c1 + sy]);
+        double cs_edge = pmin(A.cs_v[c1], A.cs_v[c1 + sy]);
+        A.alpha2[c] = g_edge_visc(A, dt, visc1, visc2_norm, dvdots, DYB(iy), DYB(iy - 1), DYB(iy + 1), cs_edge, c1, sy);
+    }
+    if (ix <= nx + 1 && iy <= ny + 1 && iz >= 0 && iz <= nz + 1) {
+        const int c1 = c - 1;
+        double dvdots = -(VZ[c1] - VZ[c1 + 1]);
+        double cs_edge = pmin(A.cs_v[c1], A.cs_v[c1 + 1]);
+        A.alpha3[c] = g_edge_visc(A, dt, visc1, visc2_norm, dvdots, DZB(iz), DZB(iz - 1), DZB(iz + 1), cs_edge, c1, 1);
+    }
+}
+
+#define SQD(p, q) \
+    ((VX[p] - VX[q]) * (VX[p] - VX[q]) + (VY[p] - VY[q]) * (VY[p] - VY[q]) + (VZ[p] - VZ[q]) * (VZ[p] - VZ[q]))
+
+// L4 over (0..n+1)^3: p_visc and visc_heat (lagran.cpp:279-323); on (0..n)^3
+// also the viscous forces (330-376), stored as fx_visc, and the CFL minimum
+// (set_dt, 380-443) reduced per wavefront, per workgroup and by a 64-bit
+// atomicMin on the bit pattern (non-negative doubles order as integers; the
+// minimum is exact, so the result does not depend on the order).
+template <class T>
+KERNEL void g_visc_dt(const Src<T> S)
+{
+    EBEGIN_ALL
+    const double gas_gamma = P.gas_gamma, mu0 = P.mu0;
+    double m = DBL_MAX;
+    if (ok_) {
+        const int c = IX(ix, iy, iz);
+        const int sx = G.sx, sy = G.sy;
+        const int mmm = c - sx - sy - 1, omm = c - sy - 1, oom = c - 1, mom = c - sx - 1;
+        const int mmo = c - sx - sy, omo = c - sy, ooo = c, moo = c - sx;
+        double a1 = SQD(mmm, omm), a2 = SQD(omm, oom), a3 = SQD(oom, mom), a4 = SQD(mom, mmm);
+        double a5 = SQD(mmo, omo), a6 = SQD(omo, ooo), a7 = SQD(ooo, moo), a8 = SQD(moo, mmo);
+        double a9 = SQD(oom, ooo), a10 = SQD(mom, moo), a11 = SQD(mmm, mmo), a12 = SQD(omm, omo);
+        double pv = 0.0;
+        pv = pmax(pv, -A.alpha1[c] * sqrt(a1));
+        pv = pmax(pv, -A.alpha2[c] * sqrt(a2));
+        pv = pmax(pv, -A.alpha3[c] * sqrt(a9));
+        double dx = DXB(ix), dy = DYB(iy), dz = DZB(iz);
+        double vh =
+            -0.25 * dy * dz * A.alpha1[c] * a1
+            - 0.25 * dx * dz * A.alpha2[c] * a2
+            - 0.25 * dy * dz * A.alpha1[c + sy] * a3
+            - 0.25 * dx * dz * A.alpha2[c - sx] * a4
+            - 0.25 * dy * dz * A.alpha1[c + 1] * a5
+            - 0.25 * dx * dz * A.alpha2[c + 1] * a6
+            - 0.25 * dy * dz * A.alpha1[c + sy + 1] * a7
+            - 0.25 * dx * dz * A.alpha2[c - sx + 1] * a8
+            - 0.25 * dx * dy * A.alpha3[c] * a9
+            - 0.25 * dx * dy * A.alpha3[c - sx] * a10
+            - 0.25 * dx * dy * A.alpha3[c - sx - sy] * a11
+            - 0.25 * dx * dy * A.alpha3[c - sy] * a12;
+        const double cv = CV(ix, iy, iz);
+        A.visc_heat[c] = vh / cv;
+        if (ix <= G.nx && iy <= G.ny && iz <= G.nz) {
+            // viscous forces
+            double b1 = A.alpha1[c + sy + 1] * dy * dz;
+            double b2 = A.alpha1[c + sx + sy + 1] * dy * dz;
+            double b3 = A.alpha2[c + 1] * dx * dz;
+            double b4 = A.alpha2[c + sy + 1] * dx * dz;
+            double b5 = A.alpha3[c] * dx * dy;
+            double b6 = A.alpha3[c + 1] * dx * dy;
+            double cvv = 0.125 * CV8SUM(ix, iy, iz);
+#define FV(V)                                                                                   \
+    ((b1 * (V[c] - V[c - sx]) + b2 * (V[c] - V[c + sx]) + b3 * (V[c] - V[c - sy]) +            \
+      b4 * (V[c] - V[c + sy]) + b5 * (V[c] - V[c - 1]) + b6 * (V[c] - V[c + 1])) / cvv)
+            A.fxv[c] = FV(VX);
+            A.fyv[c] = FV(VY);
+            A.fzv[c] = FV(VZ);
+#undef FV
+            // set_dt
+            double rho0 = pmax(RHO[c], DBL_EPSILON);
+            double press = PE_OF(c) + PI_OF(c);
+            double cs2 = gas_gamma * press / rho0;
+            double w1 = (BX[c] * BX[c] + BY[c] * BY[c] + BZ[c] * BZ[c]) / mu0 / rho0;
+            double c_visc2 = pv / rho0;
+            double length = pmin(pmin(dx, dy), dz);
+            double dt1 = length / (sqrt(c_visc2) + sqrt(cs2 + w1 + c_visc2));
+            double ax = 0.25 * DXAB(iy, iz);
+            double ay = 0.25 * DYAB(ix, iz);
+            double az = 0.25 * DZAB(ix, iy);
+            double vxbm = (VX[c - sx] + VX[c - sx - sy] + VX[c - sx - 1] + VX[c - sx - sy - 1]) * ax;
+            double vxbp = (VX[c] + VX[c - sy] + VX[c - 1] + VX[c - sy - 1]) * ax;
+            double vybm = (VY[c - sy] + VY[c - sx - sy] + VY[c - sy - 1] + VY[c - sx - sy - 1]) * ay;
+            double vybp = (VY[c] + VY[c - sx] + VY[c - 1] + VY[c - sx - 1]) * ay;
+            double vzbm = (VZ[c - 1] + VZ[c - sx - 1] + VZ[c - sy - 1] + VZ[c - sx - sy - 1]) * az;
+            double vzbp = (VZ[c] + VZ[c - sx] + VZ[c - sy] + VZ[c - sx - sy]) * az;
+            double dvx = fabs(vxbp - vxbm), dvy = fabs(vybp - vybm), dvz = fabs(vzbp - vzbm);
+            double dt2 = cv / pmax(pmax(pmax(fabs(vxbm), fabs(vxbp)), dvx), 1.0e-10 * cv);
+            double dt3 = cv / pmax(pmax(pmax(fabs(vybm), fabs(vybp)), dvy), 1.0e-10 * cv);
+            double dt4 = cv / pmax(pmax(pmax(fabs(vzbm), fabs(vzbp)), dvz), 1.0e-10 * cv);
+            m = pmin(pmin(pmin(dt1, dt2), dt3), dt4);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) m = pmin(m, __shfl_xor(m, off, 64));
+    __shared__ double wmin[4];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    if (lane == 0) wmin[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = pmin(pmin(pmin(wmin[0], wmin[1]), wmin[2]), wmin[3]);
+        if (b >= 0.0 && b < DBL_MAX) atomicMin(&A.s->dtmin_bits, (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+// L5 per grid: dt = dt_multiplier * min (lagran.cpp:380-443), the runner's
+// minimum over packages (runner.h:158-169: one package here), dtr = dt and
+// the resistive substep dt = dt / (int(dt/dtr)+1) = dt / 2 (159-161).
+template <class T>
+KERNEL void g_dt_final(const EDev<T> *devs, int n)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    Scal *s = devs[i].a.s;
+    double red = __longlong_as_double((long long)s->dtmin_bits);
+    double dt = devs[i].p.dt_multiplier * red;
+    double tdt = DBL_MAX;
+    tdt = dt < tdt ? dt : tdt;
+    s->dt = tdt;
+    s->dtr = dt;
+    s->dt_sub = tdt / 2.0;
+}
+
+// End of a step: count it.
+template <class T>
+KERNEL void g_step_done(const EDev<T> *devs, int n)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) devs[i].a.s->nstep += 1;
+}
+
+// L6 over the centred extent: b_field_and_cv1_update (lagran.cpp:806-882)
+// with bx1 *= cv1 (598-601); on (0..n+1)^3 also the energy/pressure
+// predictor (604-614), storing the predicted p_e and p_i.  With resistive
+// MHD, bx1..bz1 hold the copy of B made by the resistive substeps
+// (lagran.cpp:482) and p_e the prologue value.
+template <class T>
+KERNEL void g_predict(const Src<T> S)
+{
+    EBEGIN
+    const double gas_gamma = P.gas_gamma;
+    const bool res = P.resistive != 0;
+    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    const double dt = A.s->dt;
+    const int ooo = c, moo = c - sx, omo = c - sy, oom = c - 1, mmo = c - sx - sy, mom = c - sx - 1,
+              omm = c - sy - 1, mmm = c - sx - sy - 1;
+    double vxb = 0.25 * (VX[ooo] + VX[omo] + VX[oom] + VX[omm]);
+    double vxbm = 0.25 * (VX[moo] + VX[mmo] + VX[mom] + VX[mmm]);
+    double vyb = 0.25 * (VY[ooo] + VY[moo] + VY[oom] + VY[mom]);
+    double vybm = 0.25 * (VY[omo] + VY[mmo] + VY[omm] + VY[mmm]);
+    double vzb = 0.25 * (VZ[ooo] + VZ[moo] + VZ[omo] + VZ[mmo]);
+    double vzbm = 0.25 * (VZ[oom] + VZ[mom] + VZ[omm] + VZ[mmm]);
+    const double vol = CV(ix, iy, iz);
+    const double axx = DXAB(iy, iz), ayy = DYAB(ix, iz), azz = DZAB(ix, iy);
+    double dvxdx = (vxb * axx - vxbm * axx) / vol;
+    double dvydy = (vyb * ayy - vybm * ayy) / vol;
+    double dvzdz = (vzb * azz - vzbm * azz) / vol;
+    double dv = (dvxdx + dvydy + dvzdz) * dt / 2.0;
+    const double cv1 = vol * (1.0 + dv);
+    A.cv1[c] = cv1;
+    vxb = 0.25 * (VX[ooo] + VX[moo] + VX[oom] + VX[mom]);
+    vxbm = 0.25 * (VX[omo] + VX[mmo] + VX[omm] + VX[mmm]);
+    vyb = 0.25 * (VY[ooo] + VY[omo] + VY[oom] + VY[omm]);
+    vybm = 0.25 * (VY[moo] + VY[mmo] + VY[mom] + VY[mmm]);
+    double dvxdy = (vxb * ayy - vxbm * ayy) / vol;
+    double dvydx = (vyb * axx - vybm * axx) / vol;
+    vxb = 0.25 * (VX[ooo] + VX[moo] + VX[omo] + VX[mmo]);
+    vxbm = 0.25 * (VX[oom] + VX[mom] + VX[omm] + VX[mmm]);
+    double vzb2 = 0.25 * (VZ[ooo] + VZ[omo] + VZ[oom] + VZ[omm]);
+    double vzbm2 = 0.25 * (VZ[moo] + VZ[mmo] + VZ[mom] + VZ[mmm]);
+    double dvxdz = (vxb * azz - vxbm * azz) / vol;
+    double dvzdx = (vzb2 * axx - vzbm2 * axx) / vol;
+    double vyb3 = 0.25 * (VY[ooo] + VY[moo] + VY[omo] + VY[mmo]);
+    double vybm3 = 0.25 * (VY[oom] + VY[mom] + VY[omm] + VY[mmm]);
+    double vzb3 = 0.25 * (VZ[ooo] + VZ[moo] + VZ[oom] + VZ[mom]);
+    double vzbm3 = 0.25 * (VZ[omo] + VZ[mmo] + VZ[omm] + VZ[mmm]);
+    double dvydz = (vyb3 * azz - vybm3 * azz) / vol;
+    double dvzdy = (vzb3 * ayy - vzbm3 * ayy) / vol;
+    double bx1, by1, bz1;
+    if (res) {
+        bx1 = A.bx1[c];
+        by1 = A.by1[c];
+        bz1 = A.bz1[c];
+    } else {
+        bx1 = 0.5 * (BX[c] + BX[c - sx]);
+        by1 = 0.5 * (BY[c] + BY[c - sy]);
+        bz1 = 0.5 * (BZ[c] + BZ[c - 1]);
+    }
+    double w3 = bx1 * dvxdx + by1 * dvxdy + bz1 * dvxdz;
+    double w4 = bx1 * dvydx + by1 * dvydy + bz1 * dvydz;
+    double w5 = bx1 * dvzdx + by1 * dvzdy + bz1 * dvzdz;
+    A.bx1[c] = (bx1 + w3 * dt / 2.0) / (1.0 + dv) * cv1;
+    A.by1[c] = (by1 + w4 * dt / 2.0) / (1.0 + dv) * cv1;
+    A.bz1[c] = (bz1 + w5 * dt / 2.0) / (1.0 + dv) * cv1;
+    if (ix >= 0 && iy >= 0 && iz >= 0 && ix <= G.nx + 1 && iy <= G.ny + 1 && iz <= G.nz + 1) {
+        double rho = RHO[c];
+        double dvp = cv1 / vol - 1.0;
+        double pe0 = res ? (double)A.p_e[c] : PE_OF(c);
+        double pi0 = PI_OF(c);
+        double e1_e = EE[c] - pe0 * dvp / rho;
+        double e1_i = EI[c] - pi0 * dvp / rho;
+        e1_i += A.visc_heat[c] * dt / 2.0 / rho;
+        A.p_e[c] = e1_e * (gas_gamma - 1.0) * rho * vol / cv1;
+        A.p_i[c] = e1_i * (gas_gamma - 1.0) * rho * vol / cv1;
+    }
+}
+
+// L7 over (0..n)^3: forces and half-step velocities (lagran.cpp:619-731);
+// fx_visc + fx is stored back into fx_visc for the corrector.
+template <class T>
+KERNEL void g_forces(const Src<T> S)
+{
+    EBEGIN
+    const double mu0 = P.mu0;
+    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    const double dt = A.s->dt;
+    const Fld<T> PE = A.p_e, PI = A.p_i, C1 = A.cv1, B1x = A.bx1, B1y = A.by1, B1z = A.bz1;
+    double dx = DXC(ix), dy = DYC(iy), dz = DZC(iz);
+    double dhy = dy, dhz = dz;
+    const int x_ = c + sx, y_ = c + sy, z_ = c + 1, xy = c + sx + sy, xz = c + sx + 1, yz = c + sy + 1,
+              xyz = c + sx + sy + 1;
+    double pp = PE[c] + PI[c], ppx = PE[x_] + PI[x_], ppy = PE[y_] + PI[y_], ppz = PE[z_] + PI[z_];
+    double ppxy = PE[xy] + PI[xy], ppxz = PE[xz] + PI[xz], ppyz = PE[yz] + PI[yz], ppxyz = PE[xyz] + PI[xyz];
+    double w1 = pp + ppy + ppz + ppyz;
+    double w2 = ppx + ppxy + ppxz + ppxyz;
+    double fx = (w1 - w2) * 0.25 / dx;
+    w1 = pp + ppx + ppz + ppxz;
+    w2 = ppy + ppxy + ppyz + ppxyz;
+    double fy = (w1 - w2) * 0.25 / dhy;
+    w1 = pp + ppx + ppy + ppxy;
+    w2 = ppz + ppxz + ppyz + ppxyz;
+    double fz = (w1 - w2) * 0.25 / dhz;
+    double cvx = C1[c] + C1[y_] + C1[z_] + C1[yz];
+    double cvxp = C1[x_] + C1[xy] + C1[xz] + C1[xyz];
+    double cvy = C1[c] + C1[x_] + C1[z_] + C1[xz];
+    double cvyp = C1[y_] + C1[xy] + C1[yz] + C1[xyz];
+    double cvz = C1[c] + C1[x_] + C1[y_] + C1[xy];
+    double cvzp = C1[z_] + C1[xz] + C1[yz] + C1[xyz];
+    w1 = (B1z[c] + B1z[x_] + B1z[z_] + B1z[xz]) / cvy;
+    w2 = (B1z[y_] + B1z[xy] + B1z[yz] + B1z[xyz]) / cvyp;
+    double jx = (w2 - w1) / dhy;
+    w1 = (B1y[c] + B1y[x_] + B1y[y_] + B1y[xy]) / cvz;
+    w2 = (B1y[z_] + B1y[xz] + B1y[yz] + B1y[xyz]) / cvzp;
+    jx -= (w2 - w1) / dhz;
+    w1 = (B1z[c] + B1z[y_] + B1z[z_] + B1z[yz]) / cvx;
+    w2 = (B1z[x_] + B1z[xy] + B1z[xz] + B1z[xyz]) / cvxp;
+    double jy = -(w2 - w1) / dx;
+    w1 = (B1x[c] + B1x[x_] + B1x[y_] + B1x[xy]) / cvz;
+    w2 = (B1x[z_] + B1x[xz] + B1x[yz] + B1x[xyz]) / cvzp;
+    jy += (w2 - w1) / dhz;
+    w1 = (B1y[c] + B1y[y_] + B1y[z_] + B1y[yz]) / cvx;
+    w2 = (B1y[x_] + B1y[xy] + B1y[xz] + B1y[xyz]) / cvxp;
+    double jz = (w2 - w1) / dx;
+    w1 = (B1x[c] + B1x[x_] + B1x[z_] + B1x[xz]) / cvy;
+    w2 = (B1x[y_] + B1x[xy] + B1x[yz] + B1x[xyz]) / cvyp;
+    jz -= (w2 - w1) / dhy;
+    double bxv = (B1x[c] + B1x[x_] + B1x[y_] + B1x[xy] + B1x[z_] + B1x[xz] + B1x[yz] + B1x[xyz]) / (cvx + cvxp);
+    double byv = (B1y[c] + B1y[x_] + B1y[y_] + B1y[xy] + B1y[z_] + B1y[xz] + B1y[yz] + B1y[xyz]) / (cvx + cvxp);
+    double bzv = (B1z[c] + B1z[x_] + B1z[y_] + B1z[xy] + B1z[z_] + B1z[xz] + B1z[yz] + B1z[xyz]) / (cvx + cvxp);
+    fx += (jy * bzv - jz * byv) / mu0;
+    fy += (jz * bxv - jx * bzv) / mu0;
+    fz += (jx * byv - jy * bxv) / mu0;
+    double rv = A.rho_v[c];
+    double sxv = A.fxv[c] + fx, syv = A.fyv[c] + fy, szv = A.fzv[c] + fz;
+    A.fxv[c] = sxv;
+    A.fyv[c] = syv;
+    A.fzv[c] = szv;
+    VX1[c] = VX[c] + dt / 2.0 * sxv / rv;
+    VY1[c] = VY[c] + dt / 2.0 * syv / rv;
+    VZ1[c] = VZ[c] + dt / 2.0 * szv / rv;
+}
+
+#define DOT(p, q)                                                                                   \
+    (1 * ((VX[p] - VX[q]) * (VX1[p] - VX1[q]) + (VY[p] - VY[q]) * (VY1[p] - VY1[q]) +               \
+          (VZ[p] - VZ[q]) * (VZ1[p] - VZ1[q])))
+
+// C1 over (0..n+1)^3: shock_heating (lagran.cpp:884-993) with the dy*dy
+// factors of a9 and a11 (lagran.cpp:987).
+template <class T>
+KERNEL void g_shock_heating(const Src<T> S)
+{
+    EBEGIN
+    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    const int ooo = c, moo = c - sx, omo = c - sy, mmo = c - sx - sy;
+    const int oop = c + 1, mop = c - sx + 1, omp = c - sy + 1, mmp = c - sx - sy + 1;
+    const int oom = c - 1, mom = c - sx - 1, omm = c - sy - 1, mmm = c - sx - sy - 1;
+    double a1 = DOT(mmo, omo), a2 = DOT(omo, ooo), a3 = DOT(ooo, moo), a4 = DOT(moo, mmo);
+    double a5 = DOT(mmp, omp), a6 = DOT(omp, oop), a7 = DOT(oop, mop), a8 = DOT(mop, mmp);
+    double a9 = DOT(oom, ooo), a10 = DOT(mom, moo), a11 = DOT(mmm, mmo), a12 = DOT(omm, omo);
+    double dx = DXB(ix), dy = DYB(iy), dz = DZB(iz);
+    double vh =
+        (-0.25 * dy * dz * A.alpha1[c] * a1 - 0.25 * dx * dz * A.alpha2[c] * a2 -
+         0.25 * dy * dz * A.alpha1[c + sy] * a3 - 0.25 * dx * dz * A.alpha2[c - sx] * a4 -
+         0.25 * dy * dz * A.alpha1[c + 1] * a5 - 0.25 * dx * dz * A.alpha2[c + 1] * a6 -
+         0.25 * dy * dz * A.alpha1[c + sy + 1] * a7 - 0.25 * dx * dz * A.alpha2[c - sx + 1] * a8 -
+         0.25 * dy * dy * A.alpha3[c] * a9 - 0.25 * dx * dy * A.alpha3[c - sx] * a10 -
+         0.25 * dy * dy * A.alpha3[c - sx - sy] * a11 - 0.25 * dx * dy * A.alpha3[c - sy] * a12);
+    A.visc_heat[c] = pmax(vh / CV(ix, iy, iz), 0.0);
+}
+
+// C2 over (0..n)^3: velocity correction (lagran.cpp:754-758) with the sum
+// fx_visc + fx stored by g_forces.
+template <class T>
+KERNEL void g_vel_update(const Src<T> S)
+{
+    EBEGIN
+    const int c = IX(ix, iy, iz);
+    const double dt = A.s->dt;
+    double rv = A.rho_v[c];
+    VX[c] += dt * A.fxv[c] / rv;
+    VY[c] += dt * A.fyv[c] / rv;
+    VZ[c] += dt * A.fzv[c] / rv;
+}
+
+// C3 over (1..n)^3: density and energy correction (lagran.cpp:762-796).  The
+// cv1 it also writes in the reference is recomputed by the remap before use.
+template <class T>
+KERNEL void g_rho_e_update(const Src<T> S)
+{
+    EBEGIN
+    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    const double dt = A.s->dt;
+    const int ooo = c, moo = c - sx, omo = c - sy, oom = c - 1, mmo = c - sx - sy, mom = c - sx - 1,
+              omm = c - sy - 1, mmm = c - sx - sy - 1;
+    double vxb = 0.25 * (VX1[ooo] + VX1[omo] + VX1[oom] + VX1[omm]);
+    double vxbm = 0.25 * (VX1[moo] + VX1[mmo] + VX1[mom] + VX1[mmm]);
+    double vyb = 0.25 * (VY1[ooo] + VY1[moo] + VY1[oom] + VY1[mom]);
+    double vybm = 0.25 * (VY1[omo] + VY1[mmo] + VY1[omm] + VY1[mmm]);
+    double vzb = 0.25 * (VZ1[ooo] + VZ1[moo] + VZ1[omo] + VZ1[mmo]);
+    double vzbm = 0.25 * (VZ1[oom] + VZ1[mom] + VZ1[omm] + VZ1[mmm]);
+    const double vol = CV(ix, iy, iz);
+    const double axx = DXAB(iy, iz), ayy = DYAB(ix, iz), azz = DZAB(ix, iy);
+    double dvxdx = (vxb * axx - vxbm * axx) / vol;
+    double dvydy = (vyb * ayy - vybm * ayy) / vol;
+    double dvzdz = (vzb * azz - vzbm * azz) / vol;
+    double dv = (dvxdx + dvydy + dvzdz) * dt;
+    double rho = RHO[c];
+    EE[c] -= dv * A.p_e[c] / rho;
+    EI[c] += (dt * A.visc_heat[c] - dv * A.p_i[c]) / rho;
+    RHO[c] = rho / (1.0 + dv);
+}
+
+// ------------------------------------------------- resistive substeps
+
+// eta_calc (lagran.cpp:445-476) over (-1..n+1)^3 with the copy of B into
+// bx1..bz1 over the centred extent: the reference's pw::assign(data.bx1,
+// data.bx) at lagran.cpp:482 throws on the extent mismatch; this is its
+// element-wise reading over bx1's extent.
+template <class T>
+KERNEL void r_eta_copy(const Src<T> S)
+{
+    EBEGIN
+    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    A.bx1[c] = BX[c];
+    A.by1[c] = BY[c];
+    A.bz1[c] = BZ[c];
+    if (ix > G.nx + 1 || iy > G.ny + 1 || iz > G.nz + 1) return;
+    double dxc = DXC(ix), dyc = DYC(iy), dzc = DZC(iz);
+    double jx = (BZ[c + sy] - BZ[c]) / dyc - (BY[c + 1] - BY[c]) / dzc;
+    double jxp = (BZ[c + sx + sy] - BZ[c + sx]) / dyc - (BY[c + sx + 1] - BY[c + sx]) / dzc;
+    double jy = (BX[c + 1] - BX[c]) / dzc - (BZ[c + sx] - BZ[c]) / dxc;
+    double jyp = (BX[c + sy + 1] - BX[c + sy]) / dzc - (BZ[c + sx + sy] - BZ[c + sy]) / dxc;
+    double jz = (BY[c + sx] - BY[c]) / dxc - (BX[c + sy] - BX[c]) / dyc;
+    double jzp = (BY[c + sx + 1] - BY[c + 1]) / dxc - (BX[c + sy + 1] - BX[c + 1]) / dyc;
+    jx = (jx + jxp) * 0.5;
+    jy = (jy + jyp) * 0.5;
+    jz = (jz + jzp) * 0.5;
+    double j_local = sqrt(jx * jx + jy * jy + jz * jz);
+    A.eta[c] = j_local > P.j_max ? P.eta_background + P.eta0 : P.eta_background;
+}
+
+// rkstep (lagran.cpp:507-535) over (0..n)^3.
+template <class T>
+KERNEL void r_rkstep(const Src<T> S)
+{
+    EBEGIN
+    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    double dxc = DXC(ix), dyc = DYC(iy), dzc = DZC(iz);
+    double jx1 = (BZ[c + sy] - BZ[c]) / dyc - (BY[c + 1] - BY[c]) / dzc;
+    double jx2 = (BZ[c + sx + sy] - BZ[c + sx]) / dyc - (BY[c + sx + 1] - BY[c + sx]) / dzc;
+    double jy1 = (BX[c + 1] - BX[c]) / dzc - (BZ[c + sx] - BZ[c]) / dxc;
+    double jy2 = (BX[c + sy + 1] - BX[c + sy]) / dzc - (BZ[c + sx + sy] - BZ[c + sy]) / dxc;
+    double jz1 = (BY[c + sx] - BY[c]) / dxc - (BX[c + sy] - BX[c]) / dyc;
+    double jz2 = (BY[c + sx + 1] - BY[c + 1]) / dxc - (BX[c + sy + 1] - BX[c + 1]) / dyc;
+    double jx = 0.5 * (jx1 + jx2);
+    double jy = 0.5 * (jy1 + jy2);
+    double jz = 0.5 * (jz1 + jz2);
+    double eta = A.eta[c];
+    A.flux_x[c] = -jx * eta * dxc * 0.5;
+    A.flux_y[c] = -jy * eta * dyc * 0.5;
+    A.flux_z[c] = -jz * eta * dzc * 0.5;
+    A.curlb[c] = eta * (jx * jx + jy * jy + jz * jz);
+}
+
+// bstep's three components (lagran.cpp:537-590) with the Ohmic heating of
+// lagran.cpp:490-500, over (0..n)^3: both read only rkstep's outputs; the B
+// fill between them in the reference touches neither curlb, rho nor
+// energy_electron.
+template <class T>
+KERNEL void r_b_energy(const Src<T> S)
+{
+    EBEGIN
+    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const double dt = A.s->dt_sub;
+    const double *fx = A.flux_x, *fy = A.flux_y, *fz = A.flux_z;
+    if (iy >= 1 && iz >= 1) {
+        double area = DYB(iy) * DZB(iz);
+        BX[c] = A.bx1[c] + (fz[c] - fz[c - sy] + fz[c - 1] - fz[c - sy - 1] - fy[c] + fy[c - 1] - fy[c - sy] + fy[c - sy - 1]) * dt / area;
+    }
+    if (ix >= 1 && iz >= 1) {
+        double area = DXB(ix) * DZB(iz);
+        BY[c] = A.by1[c] + (fx[c] - fx[c - 1] + fx[c - sx] - fx[c - sx - 1] - fz[c] + fz[c - sx] - fz[c - 1] + fz[c - sx - 1]) * dt / area;
+    }
+    if (ix >= 1 && iy >= 1) {
+        double area = DXB(ix) * DYB(iy);
+        BZ[c] = A.bz1[c] + (fy[c] - fy[c - sx] + fy[c - sy] - fy[c - sx - sy] - fx[c] + fx[c - sy] - fx[c - sx] + fx[c - sx - sy]) * dt / area;
+    }
+    if (ix >= 1 && iy >= 1 && iz >= 1 && ix <= nx && iy <= ny && iz <= nz) {
+        const Fld<T> cb = A.curlb;
+        double sum_curlb = cb[c] + cb[c - sx] + cb[c - sy] + cb[c - sx - sy] + cb[c - 1] + cb[c - sx - 1] + cb[c - sy - 1] + cb[c - sx - sy - 1];
+        EE[c] += sum_curlb * dt / (8.0 * RHO[c]);
+    }
+}
+
+// ------------------------------------------------- boundary conditions
+
+// One edge of one built-in boundary condition along a line of a variable
+// (variableDef.h:539-605, builtInBoundaryConditions.h:54-300): ghost layers
+// -1..0 (centred) or -2..0 (staggered, the boundary point included) at the
+// lower edge, n+1..n+2 or n..n+2 at the upper.
+template <class E>
+__device__ __forceinline__ void bc_edge_e(E *p, int c0, int sd, int n, int s, int edge, int kind, double val)
+{
+    if (kind == 0) return;
+    const int LB = s ? 0 : 1, UB = n;
+    const int g0 = edge ? n + 1 - s : (s ? -2 : -1);
+    const int g1 = edge ? n + 2 : 0;
+    const int first = edge ? g0 : g1;
+    const int last = edge ? UB : LB;
+    for (int g = g0; g <= g1; ++g) {
+        double out;
+        if (kind == 1) {
+            out = (double)p[c0 + (edge ? g - n : g + n) * sd];
+        } else if (kind == 2) {
+            out = (double)p[c0 + last * sd];
+        } else if (kind == 3) {
+            const double bv = (double)p[c0 + last * sd];
+            const double boundary = (bv + bv) / 2.0;
+            const double delta = (double)p[c0 + (last - (g - first)) * sd] - boundary;
+            out = boundary - delta;
+        } else {
+            out = val;
+        }
+        p[c0 + g * sd] = (E)out;
+    }
+}
+
+// Workgroups per grid: nvar variables x ceil(lines / 256); each lane owns
+// one line along dim (spanning the other dimensions' full extents) and
+// applies the lower edge, then the upper edge.
+template <class T>
+KERNEL void e_bc(const EDev<T> *devs, const lare::BCSetE *sets, const int *boff, int ngrid, int nblk, int dim,
+                 int nvar)
+{
+    const int b = xcd_swizzle((int)blockIdx.x, nblk);
+    const int g = grid_of(boff, ngrid, b);
+    const Geo G = devs[g].g;
+    const int na = dim == 0 ? G.ny : G.nx;
+    const int nb = dim == 2 ? G.ny : G.nz;
+    const int bpl = ((na + 5) * (nb + 5) + 255) >> 8;
+    const int lb = b - boff[g];
+    const int vi = lb / bpl;
+    if (vi >= nvar) return;
+    const lare::BCVarE V = sets[g].v[vi];
+    if (V.k0 == 0 && V.k1 == 0) return;
+    const int la = V.sa ? -2 : -1, ua = na + 2;
+    const int lbb = V.sb ? -2 : -1, ub = nb + 2;
+    const int wb = ub - lbb + 1;
+    const int t = (lb - vi * bpl) * 256 + (int)threadIdx.x;
+    if (t >= (ua - la + 1) * wb) return;
+    const int ia = la + t / wb, ib = lbb + t % wb;
+    int c0, sd, n;
+    if (dim == 0) {
+        c0 = IX(0, ia, ib);
+        sd = G.sx;
+        n = G.nx;
+    } else if (dim == 1) {
+        c0 = IX(ia, 0, ib);
+        sd = G.sy;
+        n = G.ny;
+    } else {
+        c0 = IX(ia, ib, 0);
+        sd = 1;
+        n = G.nz;
+    }
+    if (V.f64) {
+        bc_edge_e((double *)V.p, c0, sd, n, V.sd, 0, V.k0, V.v0);
+        bc_edge_e((double *)V.p, c0, sd, n, V.sd, 1, V.k1, V.v1);
+    } else {
+        bc_edge_e((T *)V.p, c0, sd, n, V.sd, 0, V.k0, V.v0);
+        bc_edge_e((T *)V.p, c0, sd, n, V.sd, 1, V.k1, V.v1);
+    }
+}
+
+// ------------------------------------------------- energy diagnostic
+
+// Per-workgroup partial sums over the domain cells (1..n)^3 of one grid:
+// internal rho (e_e + e_i) cv, kinetic 1/2 rho_v |v|^2 cv_v at the upper
+// vertex of the cell, magnetic |B_c|^2 / (2 mu0) cv with B_c the cell average
+// of the face values.  Fixed tree order: the result is reproducible.
+template <class T>
+KERNEL void e_energy_part(const Src<T> S, double *part)
+{
+    EBEGIN_ALL
+    double e[3] = {0.0, 0.0, 0.0};
+    if (ok_) {
+        const int c = IX(ix, iy, iz);
+        const int sx = G.sx, sy = G.sy;
+        const double cv = CV(ix, iy, iz);
+        e[0] = RHO[c] * (EE[c] + EI[c]) * cv;
+        const double c000 = cv, c100 = CV(ix + 1, iy, iz), c010 = CV(ix, iy + 1, iz),
+                     c110 = CV(ix + 1, iy + 1, iz), c001 = CV(ix, iy, iz + 1), c101 = CV(ix + 1, iy, iz + 1),
+                     c011 = CV(ix, iy + 1, iz + 1), c111 = CV(ix + 1, iy + 1, iz + 1);
+        const Fld<T> r = RHO;
+        double src = r[c] * c000 + r[c + sx] * c100 + r[c + sy] * c010 + r[c + sx + sy] * c110 + r[c + 1] * c001 +
+                     r[c + sx + 1] * c101 + r[c + sy + 1] * c011 + r[c + sx + sy + 1] * c111;
+        double scv = c000 + c100 + c010 + c110 + c001 + c101 + c011 + c111;
+        const double v2 = VX[c] * VX[c] + VY[c] * VY[c] + VZ[c] * VZ[c];
+        e[1] = 0.5 * (src / scv) * v2 * (0.125 * scv);
+        const double bx = 0.5 * (BX[c] + BX[c - sx]), by = 0.5 * (BY[c] + BY[c - sy]), bz = 0.5 * (BZ[c] + BZ[c - 1]);
+        e[2] = (bx * bx + by * by + bz * bz) / (2.0 * P.mu0) * cv;
+    }
+    __shared__ double sh[3][256];
+    for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] = e[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) part[3 * b_ + k] = sh[k][0];
+}
+
+// Sum of each grid's partials in block order (one workgroup per grid).
+KERNEL void e_energy_sum(const double *part, const int *boff, int ngrid, int nblk, double *out)
+{
+    const int g = (int)blockIdx.x;
+    if (g >= ngrid || threadIdx.x != 0) return;
+    const int b1 = g + 1 < ngrid ? boff[g + 1] : nblk;
+    double e[3] = {0.0, 0.0, 0.0};
+    for (int b = boff[g]; b < b1; ++b)
+        for (int k = 0; k < 3; ++k) e[k] += part[3 * b + k];
+    out[4 * g + 0] = e[0];
+    out[4 * g + 1] = e[1];
+    out[4 * g + 2] = e[2];
+    out[4 * g + 3] = e[0] + e[1] + e[2];
+}
+
+// ================================================================== host
+
+namespace lare {
+
+namespace {
+
+// Stagger per dimension of each registered variable (setup.cpp:45-69).
+const int kStagE[13][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 1, 1}, {1, 1, 1}, {1, 1, 1}, {1, 0, 0},
+                           {0, 1, 0}, {0, 0, 1}, {1, 1, 1}, {1, 1, 1}, {1, 1, 1}, {0, 0, 0}};
+
+inline RSpec rs(int x0, int dx1, int y0, int dy1, int z0, int dz1) { return RSpec{x0, dx1, y0, dy1, z0, dz1}; }
+
+void *field_ptr(const Grid &g, int f)
+{
+    if (g.fp32) {
+        const ArrE<float> &a = g.e32.a;
+        const Fld<float> fs[12] = {a.rho, a.ee, a.ei, a.vx, a.vy, a.vz, a.bx, a.by, a.bz, a.vx1, a.vy1, a.vz1};
+        return f == 12 ? (void *)a.dm : (void *)fs[f].p;
+    }
+    const ArrE<double> &a = g.e64.a;
+    const Fld<double> fs[12] = {a.rho, a.ee, a.ei, a.vx, a.vy, a.vz, a.bx, a.by, a.bz, a.vx1, a.vy1, a.vz1};
+    return f == 12 ? (void *)a.dm : (void *)fs[f].p;
+}
+
+// Block offsets of a launch over `subset` with range r (built outside graph
+// capture, by group_prepare or on first use).
+Group::Tab &table(Group &gp, int subset, const RSpec &r)
+{
+    const std::array<int, 6> key{r.x0, r.dx1, r.y0, r.dy1, r.z0, r.dz1};
+    auto it = gp.tabs.find({subset, key});
+    if (it != gp.tabs.end()) return it->second;
+    Group::Tab t;
+    std::vector<int> boff;
+    long total = 0;
+    for (size_t m = 0; m < gp.mem.size(); ++m) {
+        if (subset == 1 && std::find(gp.res.begin(), gp.res.end(), (int)m) == gp.res.end()) continue;
+        const Geo &G = gp.mem[m]->dev.g;
+        const long nxr = (long)G.nx + r.dx1 - r.x0 + 1, nyr = (long)G.ny + r.dy1 - r.y0 + 1,
+                   nzr = (long)G.nz + r.dz1 - r.z0 + 1;
+        boff.push_back((int)total);
+        if (nxr > 0 && nyr > 0 && nzr > 0) {
+            total += nxr * ((nyr * nzr + 255) / 256);
+            t.cells += (double)(nxr * nyr * nzr);
+        }
+    }
+    t.nblk = (int)total;
+    if (!boff.empty()) {
+        (void)hipMalloc(&t.boff, boff.size() * sizeof(int));
+        (void)hipMemcpy(t.boff, boff.data(), boff.size() * sizeof(int), hipMemcpyHostToDevice);
+    }
+    return gp.tabs.emplace(std::make_pair(subset, key), t).first->second;
+}
+
+// "(f_prep<T, DIR>)" -> "f_prep<DIR>", "g_cs<T>" -> "g_cs"
+std::string kname(const char *k)
+{
+    std::string n;
+    for (const char *p = k; *p; ++p)
+        if (*p != '(' && *p != ')') n += *p;
+    for (const char *pat : {"<T>", "T, "}) {
+        size_t at;
+        while ((at = n.find(pat)) != std::string::npos) n.erase(at, std::strlen(pat));
+    }
+    return n;
+}
+
+template <class T, typename... KA, typename... Args>
+void elaunch_(void (*k)(Src<T>, KA...), const char *name, double bpc, Group &gp, int subset, RSpec r,
+              Args... args)
+{
+    Group::Tab &t = table(gp, subset, r);
+    if (gp.planning || t.nblk == 0) return;
+    Src<T> s{(const EDev<T> *)gp.devs[subset], t.boff, gp.n[subset], t.nblk, r};
+    Prof *pf = gp.prof;
+    Prof::Rec rec{kname(name), bpc * t.cells, nullptr, nullptr};
+    if (pf && pf->on) {
+        if (pf->dir >= 0) {
+            const size_t at = rec.name.find("<DIR>");
+            if (at != std::string::npos) rec.name.replace(at, 5, "<" + std::to_string(pf->dir) + ">");
+        }
+        (void)hipEventCreate(&rec.a);
+        (void)hipEventCreate(&rec.b);
+        (void)hipEventRecord(rec.a, gp.stream);
+    }
+    hipLaunchKernelGGL(k, dim3((unsigned)t.nblk), dim3(256), 0, gp.stream, s, static_cast<KA>(args)...);
+    if (pf && pf->on) {
+        (void)hipEventRecord(rec.b, gp.stream);
+        pf->recs.push_back(rec);
+    }
+}
+#define EL(k, bpc, gp, subset, r, ...) elaunch_(k, #k, bpc, gp, subset, r, ##__VA_ARGS__)
+
+// Per-grid kernels (one lane per member).
+template <class T>
+void per_grid(void (*k)(const EDev<T> *, int), const char *name, Group &gp)
+{
+    if (gp.planning || gp.n[0] == 0) return;
+    Prof *pf = gp.prof;
+    Prof::Rec rec{name, 0.0, nullptr, nullptr};
+    if (pf && pf->on) {
+        (void)hipEventCreate(&rec.a);
+        (void)hipEventCreate(&rec.b);
+        (void)hipEventRecord(rec.a, gp.stream);
+    }
+    hipLaunchKernelGGL(k, dim3((unsigned)((gp.n[0] + 255) / 256)), dim3(256), 0, gp.stream,
+                       (const EDev<T> *)gp.devs[0], gp.n[0]);
+    if (pf && pf->on) {
+        (void)hipEventRecord(rec.b, gp.stream);
+        pf->recs.push_back(rec);
+    }
+}
+
+// Variable groups of the boundary fills (boundary.cpp:27-96).
+enum { VB, VE, VRHO, VV, VV1, VDM, VALL, VERHO, VBE, NVG };
+const int kVars[NVG][9] = {{6, 7, 8},       {1, 2},          {0},    {3, 4, 5}, {9, 10, 11},
+                           {12},            {6, 7, 8, 1, 2, 0, 3, 4, 5},     {1, 2, 0}, {6, 7, 8, 1, 2}};
+const int kNVars[NVG] = {3, 2, 1, 3, 3, 1, 9, 3, 5};
+
+void drop_btabs(Group &gp)
+{
+    for (auto &kv : gp.btabs) {
+        if (kv.second.sets) (void)hipFree(kv.second.sets);
+        if (kv.second.boff) (void)hipFree(kv.second.boff);
+    }
+    gp.btabs.clear();
+}
+
+// Boundary-fill table of variable group vg along dim over a member subset
+// (0 all, 1 resistive): each member's BC descriptors and block offsets.
+Group::BTab &btable(Group &gp, int vg, int dim, int subset)
+{
+    const int key = (vg * 3 + dim) * 2 + subset;
+    auto it = gp.btabs.find({key, 0});
+    if (it != gp.btabs.end()) return it->second;
+    Group::BTab t;
+    std::vector<const Grid *> mem;
+    for (size_t m = 0; m < gp.mem.size(); ++m)
+        if (subset == 0 || gp.mem[m]->prm.resistive) mem.push_back(gp.mem[m]);
+    std::vector<BCSetE> sets(mem.size());
+    std::vector<int> boff(mem.size());
+    long total = 0;
+    const int da = dim == 0 ? 1 : 0, db = dim == 2 ? 1 : 2;
+    for (size_t m = 0; m < mem.size(); ++m) {
+        const Grid &g = *mem[m];
+        BCSetE &S = sets[m];
+        std::memset(&S, 0, sizeof(S));
+        for (int i = 0; i < kNVars[vg]; ++i) {
+            const int v = kVars[vg][i];
+            BCVarE &b = S.v[i];
+            b.p = field_ptr(g, v);
+            b.f64 = (v == 12 || !g.fp32) ? 1 : 0;
+            b.sd = kStagE[v][dim];
+            b.sa = kStagE[v][da];
+            b.sb = kStagE[v][db];
+            b.k0 = g.bck[v][dim][0];
+            b.k1 = g.bck[v][dim][1];
+            b.v0 = g.bcv[v][dim][0];
+            b.v1 = g.bcv[v][dim][1];
+            if (b.k0 || b.k1) t.any = true;
+        }
+        const int n3[3] = {g.dev.g.nx, g.dev.g.ny, g.dev.g.nz};
+        boff[m] = (int)total;
+        total += (long)kNVars[vg] * (((n3[da] + 5) * (n3[db] + 5) + 255) / 256);
+    }
+    t.nblk = (int)total;
+    if (!mem.empty()) {
+        (void)hipMalloc(&t.sets, sets.size() * sizeof(BCSetE));
+        (void)hipMemcpy(t.sets, sets.data(), sets.size() * sizeof(BCSetE), hipMemcpyHostToDevice);
+        (void)hipMalloc(&t.boff, boff.size() * sizeof(int));
+        (void)hipMemcpy(t.boff, boff.data(), boff.size() * sizeof(int), hipMemcpyHostToDevice);
+    }
+    return gp.btabs.emplace(std::make_pair(key, 0), t).first->second;
+}
+
+// variableRegistry::applyBoundaryConditions for a variable group: dims 0, 1,
+// 2 in turn (variableDef.h:601-605), or one dim.
+template <class T>
+void ebcs(Group &gp, int vg, int only_dim = -1, int subset = 0)
+{
+    for (int d = 0; d < 3; ++d) {
+        if (only_dim >= 0 && d != only_dim) continue;
+        Group::BTab &t = btable(gp, vg, d, subset);
+        if (gp.planning || !t.any || t.nblk == 0) continue;
+        Prof *pf = gp.prof;
+        Prof::Rec rec{"e_bc", 0.0, nullptr, nullptr};
+        if (pf && pf->on) {
+            (void)hipEventCreate(&rec.a);
+            (void)hipEventCreate(&rec.b);
+            (void)hipEventRecord(rec.a, gp.stream);
+        }
+        hipLaunchKernelGGL(e_bc<T>, dim3((unsigned)t.nblk), dim3(256), 0, gp.stream,
+                           (const EDev<T> *)gp.devs[subset], (const BCSetE *)t.sets, (const int *)t.boff,
+                           gp.n[subset], t.nblk, d, kNVars[vg]);
+        if (pf && pf->on) {
+            (void)hipEventRecord(rec.b, gp.stream);
+            pf->recs.push_back(rec);
+        }
+    }
+}
+
+// Rebuild the boundary tables if a member's conditions changed.
+void check_bcs(Group &gp)
+{
+    bool changed = gp.bc_seen.size() != gp.mem.size();
+    for (size_t m = 0; !changed && m < gp.mem.size(); ++m) changed = gp.bc_seen[m] != gp.mem[m]->bc_version;
+    if (!changed) return;
+    drop_btabs(gp);
+    gp.bc_seen.resize(gp.mem.size());
+    for (size_t m = 0; m < gp.mem.size(); ++m) gp.bc_seen[m] = gp.mem[m]->bc_version;
+}
+
+// ------------------------------------------------------------ the events
+
+template <class T>
+void start_a_t(Group &gp)
+{
+    // bytes per cell: distinct arrays read and written, sizeof(T) each
+    // (FP64 flux arrays 8)
+    const double w = (double)sizeof(T);
+    const RSpec C = rs(-1, 2, -1, 2, -1, 2);
+    const RSpec V = rs(-1, 1, -1, 1, -1, 1);
+    EL(g_cs<T>, 7 * w, gp, 0, C);
+    EL(g_vertex<T>, 4 * w, gp, 0, V);
+    EL(g_alpha<T>, 8 * w, gp, 0, rs(-1, 1, -1, 2, 0, 2));
+    EL(g_visc_dt<T>, 16 * w, gp, 0, rs(0, 1, 0, 1, 0, 1));
+    per_grid<T>(g_dt_final<T>, "g_dt_final", gp);
+}
+
+template <class T>
+void start_b_t(Group &gp, int substeps)
+{
+    const double w = (double)sizeof(T);
+    const RSpec C = rs(-1, 2, -1, 2, -1, 2);
+    if (gp.n[1] > 0) {
+        for (int i = 0; i < substeps; ++i) {
+            EL(r_eta_copy<T>, 7 * w, gp, 1, C);
+            EL(r_rkstep<T>, 4 * w + 4 * 8, gp, 1, rs(0, 0, 0, 0, 0, 0));
+            EL(r_b_energy<T>, 11 * w + 3 * 8, gp, 1, rs(0, 0, 0, 0, 0, 0));
+            ebcs<T>(gp, VBE, -1, 1);
+        }
+    }
+    EL(g_predict<T>, 16 * w, gp, 0, C);
+    EL(g_forces<T>, 19 * w, gp, 0, rs(0, 0, 0, 0, 0, 0));
+    ebcs<T>(gp, VV1);
+}
+
+template <class T>
+void half_t(Group &gp)
+{
+    const double w = (double)sizeof(T);
+    EL(g_shock_heating<T>, 10 * w, gp, 0, rs(0, 1, 0, 1, 0, 1));
+    EL(g_vel_update<T>, 8 * w, gp, 0, rs(0, 0, 0, 0, 0, 0));
+    ebcs<T>(gp, VV);
+    EL(g_rho_e_update<T>, 12 * w, gp, 0, rs(1, 0, 1, 0, 1, 0));
+    ebcs<T>(gp, VERHO);
+}
+
+template <class T, int DIR>
+void sweep_t(Group &gp, double xpass, double ypass, double zpass)
+{
+    const double w = (double)sizeof(T);
+    const RSpec all0 = rs(0, 0, 0, 0, 0, 0);
+    if (gp.prof) gp.prof->dir = DIR;
+    EL((f_prep<T, DIR>), 6 * w, gp, 0, rs(-1, 2, -1, 2, -1, 2), xpass, ypass, zpass);
+    EL(f_cvc1<T>, 2 * w, gp, 0, rs(-1, 1, -1, 1, -1, 1));
+    EL((f_ct_fluxes<T, DIR>), 7 * w + 2 * 8, gp, 0, all0);
+    EL((f_ct_update<T, DIR>), 6 * w + 2 * 8, gp, 0, all0);
+    // mass flux: its reference range, widened to the ghost faces along DIR
+    EL((f_mass_flux<T, DIR>), 4 * w + 8, gp, 0,
+       rs(DIR == 0 ? -1 : 0, DIR == 0 ? 2 : 1, DIR == 1 ? -1 : 0, DIR == 1 ? 2 : 1, DIR == 2 ? -1 : 0,
+          DIR == 2 ? 2 : 1));
+    ebcs<T>(gp, VDM, DIR);
+    EL((f_energy_vertex<T, DIR>), 7 * w + 4 * 8, gp, 0,
+       rs(DIR == 0 ? -1 : 0, DIR == 0 ? 1 : 0, DIR == 1 ? -1 : 0, DIR == 1 ? 1 : 0, DIR == 2 ? -1 : 0,
+          DIR == 2 ? 1 : 0));
+    EL((f_mom_fluxes<T, DIR>), 7 * w + 4 * 8, gp, 0, rs(DIR == 0 ? -1 : 0, 0, DIR == 1 ? -1 : 0, 0, DIR == 2 ? -1 : 0, 0));
+    EL((f_update<T, DIR>), 13 * w + 9 * 8, gp, 0, all0);
+    if (gp.prof) gp.prof->dir = -1;
+    ebcs<T>(gp, VALL);
+}
+
+template <class T>
+void end_t(Group &gp)
+{
+    const double w = (double)sizeof(T);
+    EL((f_b_area<T, 0, 0>), 2 * w, gp, 0, rs(-2, 2, -1, 2, -1, 2));
+    EL((f_b_area<T, 1, 0>), 2 * w, gp, 0, rs(-1, 2, -2, 2, -1, 2));
+    EL((f_b_area<T, 2, 0>), 2 * w, gp, 0, rs(-1, 2, -1, 2, -2, 2));
+    sweep_t<T, 0>(gp, 1.0, 1.0, 1.0);
+    sweep_t<T, 1>(gp, 0.0, 1.0, 1.0);
+    sweep_t<T, 2>(gp, 0.0, 0.0, 1.0);
+    EL((f_b_area<T, 0, 1>), 2 * w, gp, 0, rs(-2, 2, -1, 2, -1, 2));
+    EL((f_b_area<T, 1, 1>), 2 * w, gp, 0, rs(-1, 2, -2, 2, -1, 2));
+    EL((f_b_area<T, 2, 1>), 2 * w, gp, 0, rs(-1, 2, -1, 2, -2, 2));
+    ebcs<T>(gp, VB);
+    per_grid<T>(g_step_done<T>, "g_step_done", gp);
+}
+
+}  // namespace
+
+int group_init(Group &gp, const std::vector<Grid *> &members, hipStream_t stream, Prof *prof, std::string &err)
+{
+    group_free(gp);
+    gp.mem = members;
+    gp.stream = stream;
+    gp.prof = prof;
+    if (members.empty()) return 0;
+    gp.fp32 = members[0]->fp32;
+    for (size_t m = 0; m < members.size(); ++m) {
+        if (members[m]->fp32 != gp.fp32) {
+            err = "a group holds grids of one storage precision";
+            return -1;
+        }
+        if (members[m]->prm.resistive) gp.res.push_back((int)m);
+    }
+    const size_t es = gp.fp32 ? sizeof(EDev<float>) : sizeof(EDev<double>);
+    for (int sub = 0; sub < 2; ++sub) {
+        std::vector<char> host;
+        int n = 0;
+        for (size_t m = 0; m < members.size(); ++m) {
+            if (sub == 1 && !members[m]->prm.resistive) continue;
+            const char *src = gp.fp32 ? (const char *)&members[m]->e32 : (const char *)&members[m]->e64;
+            host.insert(host.end(), src, src + es);
+            ++n;
+        }
+        gp.n[sub] = n;
+        if (n == 0) continue;
+        hipError_t e = hipMalloc(&gp.devs[sub], host.size());
+        if (e == hipSuccess) e = hipMemcpy(gp.devs[sub], host.data(), host.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            err = std::string("group device views: ") + hipGetErrorString(e);
+            group_free(gp);
+            return -3;
+        }
+    }
+    return 0;
+}
+
+void group_drop_graph(Group &gp)
+{
+    if (gp.graph_exec) (void)hipGraphExecDestroy(gp.graph_exec);
+    if (gp.graph) (void)hipGraphDestroy(gp.graph);
+    gp.graph_exec = nullptr;
+    gp.graph = nullptr;
+}
+
+void group_free(Group &gp)
+{
+    group_drop_graph(gp);
+    for (auto &kv : gp.tabs)
+        if (kv.second.boff) (void)hipFree(kv.second.boff);
+    gp.tabs.clear();
+    drop_btabs(gp);
+    gp.bc_seen.clear();
+    for (int s = 0; s < 2; ++s) {
+        if (gp.devs[s]) (void)hipFree(gp.devs[s]);
+        gp.devs[s] = nullptr;
+        gp.n[s] = 0;
+    }
+    gp.mem.clear();
+    gp.res.clear();
+}
+
+void group_start_a(Group &gp)
+{
+    check_bcs(gp);
+    if (gp.fp32)
+        start_a_t<float>(gp);
+    else
+        start_a_t<double>(gp);
+}
+
+void group_start_b(Group &gp, int substeps)
+{
+    check_bcs(gp);
+    if (gp.fp32)
+        start_b_t<float>(gp, substeps);
+    else
+        start_b_t<double>(gp, substeps);
+}
+
+void group_half(Group &gp)
+{
+    check_bcs(gp);
+    if (gp.fp32)
+        half_t<float>(gp);
+    else
+        half_t<double>(gp);
+}
+
+void group_end(Group &gp)
+{
+    check_bcs(gp);
+    if (gp.fp32)
+        end_t<float>(gp);
+    else
+        end_t<double>(gp);
+}
+
+void group_prepare(Group &gp)
+{
+    gp.planning = true;
+    group_step(gp);
+    gp.planning = false;
+}
+
+int group_energy(Group &gp, std::vector<double> &out, std::string &err)
+{
+    out.assign(4 * gp.mem.size(), 0.0);
+    if (gp.mem.empty()) return 0;
+    const RSpec in1 = rs(1, 0, 1, 0, 1, 0);
+    Group::Tab &t = table(gp, 0, in1);
+    double *part = nullptr, *res = nullptr;
+    hipError_t e = hipMalloc(&part, (size_t)std::max(t.nblk, 1) * 3 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&res, out.size() * sizeof(double));
+    if (e == hipSuccess) {
+        if (gp.fp32) {
+            Src<float> s{(const EDev<float> *)gp.devs[0], t.boff, gp.n[0], t.nblk, in1};
+            hipLaunchKernelGGL(e_energy_part<float>, dim3((unsigned)t.nblk), dim3(256), 0, gp.stream, s, part);
+        } else {
+            Src<double> s{(const EDev<double> *)gp.devs[0], t.boff, gp.n[0], t.nblk, in1};
+            hipLaunchKernelGGL(e_energy_part<double>, dim3((unsigned)t.nblk), dim3(256), 0, gp.stream, s, part);
+        }
+        hipLaunchKernelGGL(e_energy_sum, dim3((unsigned)gp.n[0]), dim3(64), 0, gp.stream, (const double *)part,
+                           (const int *)t.boff, gp.n[0], t.nblk, res);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out.data(), res, out.size() * sizeof(double), hipMemcpyDeviceToHost, gp.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(gp.stream);
+    if (part) (void)hipFree(part);
+    if (res) (void)hipFree(res);
+    if (e != hipSuccess) {
+        err = std::string("energy diagnostic: ") + hipGetErrorString(e);
+        return -3;
+    }
+    return 0;
+}
+
+}  // namespace lare

@@ -52,31 +52,23 @@ struct Grid {
     void *pool;          // one allocation holding every device array
     size_t pool_bytes;
     Prof prof;
-    int kernel_set;      // 0: one kernel per reference kernel, 1: fused
+    int kernel_set;      // 0: one kernel per reference kernel, 1: the grid-group engine
     // one captured step (start + half + end), replayed by lare_steps
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool use_graph = true;
+    // storage of the engine's arrays (kernel set 1): FP64, or FP32 with the
+    // fluxes in FP64; the engine's device view in that storage type
+    bool fp32 = false;
+    EDev<double> e64{};
+    EDev<float> e32{};
+    long bc_version = 0;  // bumped by every boundary-condition change
 };
 
 // Metric arrays from the 1D deltas (LARE3D::grid, setup.cpp:337-390).
 void build_metric(Grid &g);
 
-// Boundary fills used by the fused sweep (lare_kernels.hip).
-void dm_bcs(Grid &g, int dim);
-void sweep_bcs(Grid &g);
-void remap_v_fill(Grid &g);
-void velocity_fill(Grid &g);
-void energy_density_fill(Grid &g);
-void dt_final(Grid &g);
-void resistive_substeps(Grid &g, int substeps);
-// Fused Lagrangian events (lare_fused.hip): start_a runs up to the CFL
-// reduction, start_b the resistive substeps and the predictor.
-void fused_start_a(Grid &g);
-void fused_start_b(Grid &g, int substeps);
-void fused_half(Grid &g);
-// One fused directional remap sweep (lare_fused.hip).
-void fused_sweep(Grid &g, int d, double xpass, double ypass, double zpass);
+
 
 // The three LARE3D events (shared_data.h:397-419), enqueued on g.stream.
 // step_start = step_start_a + step_start_b(2): the CFL reduction of the

@@ -400,61 +400,6 @@ KERNEL void k_res_energy(Dev D, R3 R)
     EE[c] += sum_curlb * dt / (8.0 * RHO[c]);
 }
 
-// Merged forms of the resistive substep for the fused kernel set: eta_calc
-// (-1..n+1) with the B copy into bx1..bz1 (centred extent), and bstep's three
-// components with the Ohmic heating of lagran.cpp:490-500 (both read only
-// rkstep's outputs; the B fill between them in the reference touches
-// neither curlb, rho nor energy_electron).
-KERNEL void k_res_eta_copy(Dev D, R3 R, double j_max, double eta_background, double eta0)
-{
-    KBEGIN
-    const int c = IX(ix, iy, iz);
-    const int sx = G.sx, sy = G.sy;
-    A.bx1[c] = BX[c];
-    A.by1[c] = BY[c];
-    A.bz1[c] = BZ[c];
-    if (ix > G.nx + 1 || iy > G.ny + 1 || iz > G.nz + 1) return;
-    double dxc = DXC(ix), dyc = DYC(iy), dzc = DZC(iz);
-    double jx = (BZ[c + sy] - BZ[c]) / dyc - (BY[c + 1] - BY[c]) / dzc;
-    double jxp = (BZ[c + sx + sy] - BZ[c + sx]) / dyc - (BY[c + sx + 1] - BY[c + sx]) / dzc;
-    double jy = (BX[c + 1] - BX[c]) / dzc - (BZ[c + sx] - BZ[c]) / dxc;
-    double jyp = (BX[c + sy + 1] - BX[c + sy]) / dzc - (BZ[c + sx + sy] - BZ[c + sy]) / dxc;
-    double jz = (BY[c + sx] - BY[c]) / dxc - (BX[c + sy] - BX[c]) / dyc;
-    double jzp = (BY[c + sx + 1] - BY[c + 1]) / dxc - (BX[c + sy + 1] - BX[c + 1]) / dyc;
-    jx = (jx + jxp) * 0.5;
-    jy = (jy + jyp) * 0.5;
-    jz = (jz + jzp) * 0.5;
-    double j_local = sqrt(jx * jx + jy * jy + jz * jz);
-    A.eta[c] = j_local > j_max ? eta_background + eta0 : eta_background;
-}
-
-KERNEL void k_res_b_energy(Dev D, R3 R)
-{
-    KBEGIN
-    const int c = IX(ix, iy, iz);
-    const int sx = G.sx, sy = G.sy;
-    const int nx = G.nx, ny = G.ny, nz = G.nz;
-    const double dt = A.s->dt_sub;
-    const double *fx = A.flux_x, *fy = A.flux_y, *fz = A.flux_z;
-    if (iy >= 1 && iz >= 1) {
-        double area = DYB(iy) * DZB(iz);
-        BX[c] = A.bx1[c] + (fz[c] - fz[c - sy] + fz[c - 1] - fz[c - sy - 1] - fy[c] + fy[c - 1] - fy[c - sy] + fy[c - sy - 1]) * dt / area;
-    }
-    if (ix >= 1 && iz >= 1) {
-        double area = DXB(ix) * DZB(iz);
-        BY[c] = A.by1[c] + (fx[c] - fx[c - 1] + fx[c - sx] - fx[c - sx - 1] - fz[c] + fz[c - sx] - fz[c - 1] + fz[c - sx - 1]) * dt / area;
-    }
-    if (ix >= 1 && iy >= 1) {
-        double area = DXB(ix) * DYB(iy);
-        BZ[c] = A.bz1[c] + (fy[c] - fy[c - sx] + fy[c - sy] - fy[c - sx - sy] - fx[c] + fx[c - sy] - fx[c - sx] + fx[c - sx - sy]) * dt / area;
-    }
-    if (ix >= 1 && iy >= 1 && iz >= 1 && ix <= nx && iy <= ny && iz <= nz) {
-        const double *cb = A.curlb;
-        double sum_curlb = cb[c] + cb[c - sx] + cb[c - sy] + cb[c - sx - sy] + cb[c - 1] + cb[c - sx - 1] + cb[c - sy - 1] + cb[c - sx - sy - 1];
-        EE[c] += sum_curlb * dt / (8.0 * RHO[c]);
-    }
-}
-
 // ------------------------------------------------- predictor
 
 // b_field_and_cv1_update (lagran.cpp:806-882) over (-1..n+2)^3.
@@ -1221,17 +1166,6 @@ void remap_sweep(Grid &g, int d, double xpass, double ypass, double zpass)
 
 }  // namespace
 
-void dm_bcs(Grid &g, int dim) { bcs(g, kDm, 1, dim); }
-void sweep_bcs(Grid &g) { boundary_conditions(g); }
-void remap_v_fill(Grid &g) { remap_v_bcs(g); }
-void velocity_fill(Grid &g) { velocity_bcs(g); }
-const int kERho[3] = {1, 2, 0};
-void energy_density_fill(Grid &g) { bcs(g, kERho, 3); }
-void dt_final(Grid &g)
-{
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, g.stream, g.dev, g.prm.dt_multiplier);
-}
-
 // The resistive substeps of lagran.cpp:157-170 (int(dt/dtr)+1 of them, two
 // when the runner keeps set_dt's dt, with dt_sub = dt/substeps already in
 // the device scalars), each eta_calc + resistive_effects (478-505).
@@ -1241,16 +1175,6 @@ void resistive_substeps(Grid &g, int substeps)
     const Params &p = g.prm;
     const R3 C = rng(-1, nx + 2, -1, ny + 2, -1, nz + 2);
     const R3 V = rng(-1, nx + 1, -1, ny + 1, -1, nz + 1);
-    if (g.kernel_set == 1) {
-        static const int kBE[5] = {6, 7, 8, 1, 2};
-        for (int i = 0; i < substeps; ++i) {
-            launch3(k_res_eta_copy, 56, g, C, p.j_max, p.eta_background, p.eta0);
-            launch3(k_rkstep, 64, g, rng(0, nx, 0, ny, 0, nz));
-            launch3(k_res_b_energy, 112, g, rng(0, nx, 0, ny, 0, nz));
-            bcs(g, kBE, 5);
-        }
-        return;
-    }
     for (int i = 0; i < substeps; ++i) {
         launch3(k_eta_calc, 32, g, V, p.j_max, p.eta_background, p.eta0);
         launch3(k_res_copy, 48, g, C);
@@ -1277,10 +1201,6 @@ void step_start(Grid &g)
 // Prologue, shock viscosity and the CFL reduction (lagran.cpp:113-156).
 void step_start_a(Grid &g)
 {
-    if (g.kernel_set == 1) {
-        fused_start_a(g);
-        return;
-    }
     const int nx = g.dev.g.nx, ny = g.dev.g.ny, nz = g.dev.g.nz;
     const Params &p = g.prm;
     const R3 C = rng(-1, nx + 2, -1, ny + 2, -1, nz + 2);
@@ -1303,10 +1223,6 @@ void step_start_a(Grid &g)
 // Resistive substeps and predictor_step (lagran.cpp:157-173, 592-737).
 void step_start_b(Grid &g, int substeps)
 {
-    if (g.kernel_set == 1) {
-        fused_start_b(g, substeps);
-        return;
-    }
     const int nx = g.dev.g.nx, ny = g.dev.g.ny, nz = g.dev.g.nz;
     const Params &p = g.prm;
     const R3 C = rng(-1, nx + 2, -1, ny + 2, -1, nz + 2);
@@ -1324,10 +1240,6 @@ void step_start_b(Grid &g, int substeps)
 // and is idempotent for the built-in conditions, so it is issued once.
 void step_half(Grid &g)
 {
-    if (g.kernel_set == 1) {
-        fused_half(g);
-        return;
-    }
     const int nx = g.dev.g.nx, ny = g.dev.g.ny, nz = g.dev.g.nz;
     launch3(k_shock_heating, 88, g, rng(0, nx + 1, 0, ny + 1, 0, nz + 1));
     launch3(k_vel_update, 104, g, rng(0, nx, 0, ny, 0, nz));
@@ -1345,15 +1257,9 @@ void step_end(Grid &g)
     launch3(k_b_area, 24, g, rng(-2, nx + 2, -1, ny + 2, -1, nz + 2), 0, 0);
     launch3(k_b_area, 24, g, rng(-1, nx + 2, -2, ny + 2, -1, nz + 2), 1, 0);
     launch3(k_b_area, 24, g, rng(-1, nx + 2, -1, ny + 2, -2, nz + 2), 2, 0);
-    if (g.kernel_set == 0) {
-        remap_sweep(g, 0, 1.0, 1.0, 1.0);
-        remap_sweep(g, 1, 0.0, 1.0, 1.0);
-        remap_sweep(g, 2, 0.0, 0.0, 1.0);
-    } else {
-        fused_sweep(g, 0, 1.0, 1.0, 1.0);
-        fused_sweep(g, 1, 0.0, 1.0, 1.0);
-        fused_sweep(g, 2, 0.0, 0.0, 1.0);
-    }
+    remap_sweep(g, 0, 1.0, 1.0, 1.0);
+    remap_sweep(g, 1, 0.0, 1.0, 1.0);
+    remap_sweep(g, 2, 0.0, 0.0, 1.0);
     launch3(k_b_area, 24, g, rng(-2, nx + 2, -1, ny + 2, -1, nz + 2), 0, 1);
     launch3(k_b_area, 24, g, rng(-1, nx + 2, -2, ny + 2, -1, nz + 2), 1, 1);
     launch3(k_b_area, 24, g, rng(-1, nx + 2, -1, ny + 2, -2, nz + 2), 2, 1);

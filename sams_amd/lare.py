@@ -99,6 +99,15 @@ def lib():
             "lare_profile_count": (ctypes.c_int, [vp, P(ctypes.c_int32)]),
             "lare_profile_get": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32,
                                                 P(ctypes.c_int32), P(ctypes.c_double), P(ctypes.c_double)]),
+            "lare_energy": (ctypes.c_int, [vp, P(ctypes.c_double)]),
+            "lare_batch_create": (ctypes.c_int, [P(vp), ctypes.c_int32, P(vp)]),
+            "lare_batch_destroy": (ctypes.c_int, [vp]),
+            "lare_batch_steps": (ctypes.c_int, [vp, ctypes.c_int32]),
+            "lare_batch_sync": (ctypes.c_int, [vp]),
+            "lare_batch_set_graph": (ctypes.c_int, [vp, ctypes.c_int32]),
+            "lare_batch_energy": (ctypes.c_int, [vp, P(ctypes.c_double)]),
+            "lare_batch_profile_begin": (ctypes.c_int, [vp]),
+            "lare_batch_profile_end": (ctypes.c_int, [vp, P(ctypes.c_double), P(ctypes.c_int32)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -263,3 +272,58 @@ class Lare:
                                           ctypes.byref(by)), "lare_profile_get")
             out[buf.value.decode()] = (calls.value, ms.value, by.value)
         return out
+
+
+    def energy(self):
+        """FP64 energy diagnostic: (internal, kinetic, magnetic, total)."""
+        out = (ctypes.c_double * 4)()
+        _check(lib().lare_energy(self.h, out), "lare_energy")
+        return tuple(out)
+
+
+class Batch:
+    """Grids of one storage precision that advance together: every kernel of
+    a step is one launch over all members.  Members keep their own state,
+    dt and boundary conditions."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = ctypes.c_void_p()
+        _check(lib().lare_batch_create(arr, len(self.members), ctypes.byref(h)), "lare_batch_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lare_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def steps(self, n):
+        """Enqueue n steps of every member (returns at once)."""
+        _check(lib().lare_batch_steps(self.h, int(n)), "lare_batch_steps")
+
+    def sync(self):
+        _check(lib().lare_batch_sync(self.h), "lare_batch_sync")
+
+    def set_graph(self, on):
+        _check(lib().lare_batch_set_graph(self.h, int(bool(on))), "lare_batch_set_graph")
+
+    def energy(self):
+        out = (ctypes.c_double * (4 * len(self.members)))()
+        _check(lib().lare_batch_energy(self.h, out), "lare_batch_energy")
+        return np.array(out).reshape(len(self.members), 4)
+
+    def profile(self, nsteps):
+        """Eager steps with HIP events around every launch: (ms of kernel time
+        per step, launches per step)."""
+        _check(lib().lare_batch_profile_begin(self.h), "lare_batch_profile_begin")
+        self.steps(nsteps)
+        ms, n = ctypes.c_double(), ctypes.c_int32()
+        _check(lib().lare_batch_profile_end(self.h, ctypes.byref(ms), ctypes.byref(n)), "lare_batch_profile_end")
+        return ms.value / nsteps, n.value / nsteps
