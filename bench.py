@@ -1,10 +1,23 @@
 """LARE3D Lagrangian-remap step on one MI355X: cell-updates/s.
 
-Workload: the 256^3 resistive Brio-Wu problem (x-directed shock tube,
-rho/p/By = 1/1/1 left and 0.125/0.1/-1 right, Bx = 0.75, gamma 2, 1% seeded
-density noise, eta = 1e-3, zero-gradient x and periodic y/z boundaries), FP64,
-inputs resident in HBM.  A step is one full LARE3D timestep
-(startOfTimestep + halfTimestep + endOfTimestep).  Prints one JSON line.
+Headline workload (`value`): the 256^3 resistive Brio-Wu problem
+(x-directed shock tube, rho/p/By = 1/1/1 left and 0.125/0.1/-1 right,
+Bx = 0.75, gamma 2, 1% seeded density noise, eta = 1e-3, zero-gradient x and
+periodic y/z boundaries), FP64, inputs resident in HBM.  A step is one full
+LARE3D timestep (startOfTimestep + halfTimestep + endOfTimestep).
+
+Also measured, in the same JSON line:
+  "batched": 256 Orszag-Tang grids, sizes log-uniform in [32^3, 128^3],
+             plasma beta U[0.1, 10], perturbation amplitude U[0, 0.05], FP64,
+             stepped as one batch (one launch per kernel, graph-captured
+             step) against the same grids solved one after another;
+  "serving": a seeded Poisson stream of solves (sizes log-uniform in
+             [32^3, 192^3], 50-500 steps, Orszag-Tang or Brio-Wu, FP64 or
+             FP32 storage) served with continuous batching under a 200 GB
+             HBM budget: throughput and p50/p99 completion latency;
+  "fp32":    FP32 storage against FP64 on Orszag-Tang with 1e-3 Gaussian
+             velocity perturbations: time per step and per-field error.
+Prints one JSON line.
 """
 import argparse
 import json
@@ -60,6 +73,101 @@ def cpu_baseline(n, steps, threads):
             "sample": f"C restatement of the reference (oracle/), Brio-Wu {n}^3 resistive, {steps} steps"}
 
 
+def bench_batched(steps, seq_steps, ngrids=256, seed=2024):
+    """Config: 256 Orszag-Tang grids as one batch vs solved one after another."""
+    import math
+    import numpy as np
+    import sams_amd.lare as L
+    rng = np.random.default_rng(seed)
+    sizes = [int(round(math.exp(rng.uniform(math.log(32), math.log(128))))) for _ in range(ngrids)]
+    betas = rng.uniform(0.1, 10.0, ngrids)
+    amps = rng.uniform(0.0, 0.05, ngrids)
+    t0 = time.perf_counter()
+    grids = [L.Lare.from_problem(L.problem("ot", n=sizes[i], seed=1 + i, amp=float(amps[i]), beta=float(betas[i])))
+             for i in range(ngrids)]
+    setup = time.perf_counter() - t0
+    cells = float(sum(n ** 3 for n in sizes))
+    b = L.Batch(grids)
+    b.steps(2)  # capture and warm
+    b.sync()
+    t0 = time.perf_counter()
+    b.steps(steps)
+    b.sync()
+    tb = time.perf_counter() - t0
+    prof_ms, launches = b.profile(1)
+    b.close()
+    for g in grids:  # capture each grid's own step graph
+        g.steps(1, record_dt=False)
+    t0 = time.perf_counter()
+    for g in grids:
+        g.steps(seq_steps, record_dt=False)
+    ts = time.perf_counter() - t0
+    for g in grids:
+        g.close()
+    rb, rs = cells * steps / tb, cells * seq_steps / ts
+    log(f"bench: batched {ngrids} grids ({cells / 1e6:.1f} M cells): {rb / 1e9:.3f} G cell-updates/s batched, "
+        f"{rs / 1e9:.3f} sequential, x{rb / rs:.2f} (set-up {setup:.1f} s)")
+    return {"workload": f"{ngrids} Orszag-Tang grids, n log-uniform in [32, 128], beta U[0.1, 10], "
+                        f"amp U[0, 0.05], FP64, seed {seed}",
+            "cells": cells, "value": rb, "unit": "cell-updates/s", "steps": steps,
+            "ms_per_step": 1e3 * tb / steps, "sequential_value": rs, "sequential_steps_per_grid": seq_steps,
+            "speedup_vs_sequential": rb / rs, "launches_per_step": launches,
+            "kernel_ms_per_step_eager": prof_ms}
+
+
+def bench_serving(qps, duration, seed, budget, k):
+    from sams_amd import serve as S
+    reqs = S.poisson_requests(qps, duration, seed)
+    srv = S.Server(budget_bytes=budget, steps_per_tick=k)
+    t0 = time.perf_counter()
+    done = srv.run(reqs)
+    wall = time.perf_counter() - t0
+    out = S.summarize(done, wall)
+    out.update({"workload": f"Poisson {qps} req/s for {duration} s (seed {seed}): n log-uniform in [32, 192], "
+                           f"50-500 steps, Orszag-Tang or Brio-Wu, FP64 or FP32 storage 50/50",
+                "hbm_budget_bytes": budget, "steps_per_tick": k, "ticks": srv.ticks,
+                "peak_resident_grids": srv.peak_active, "peak_resident_bytes": srv.peak_used,
+                "unit": "cell-updates/s"})
+    log(f"bench: serving {out['requests']} requests in {wall:.1f} s: {out['throughput'] / 1e9:.3f} G cell-updates/s, "
+        f"p50 {out['p50_latency_s']:.2f} s, p99 {out['p99_latency_s']:.2f} s")
+    return out
+
+
+def bench_fp32(n, steps):
+    """FP32 storage vs FP64 on Orszag-Tang n^3 with 1e-3 velocity noise."""
+    import numpy as np
+    import sams_amd.lare as L
+    res = {}
+    runs = {}
+    for name, prec in (("fp64", L.FP64), ("fp32", L.FP32_STORAGE)):
+        g = L.Lare.from_problem(L.problem("ot", n=n, seed=7, amp=1e-3, precision=prec))
+        g.steps(1, record_dt=False)
+        g.sync()
+        t0 = time.perf_counter()
+        g.steps(steps - 1, record_dt=False)
+        g.sync()
+        el = time.perf_counter() - t0
+        res[name + "_ms_per_step"] = 1e3 * el / (steps - 1)
+        res[name + "_energy"] = list(g.energy())
+        runs[name] = g
+    err = {}
+    for f in L.STATE_FIELDS:
+        x = runs["fp64"].download(f)[2:-2, 2:-2, 2:-2]
+        y = runs["fp32"].download(f)[2:-2, 2:-2, 2:-2]
+        d = x - y
+        nx2, nxi = float(np.sqrt(np.sum(x * x))), float(np.max(np.abs(x)))
+        e2, ei = float(np.sqrt(np.sum(d * d))), float(np.max(np.abs(d)))
+        err[f] = {"rel_l2": e2 / nx2 if nx2 > 0 else e2, "rel_linf": ei / nxi if nxi > 0 else ei}
+        del x, y, d
+    for g in runs.values():
+        g.close()
+    res.update({"workload": f"Orszag-Tang {n}^3 + N(0, 1e-3^2) velocity perturbation, {steps} steps",
+                "error_vs_fp64": err, "speedup": res["fp64_ms_per_step"] / res["fp32_ms_per_step"]})
+    log(f"bench: fp32 storage {n}^3: {res['fp32_ms_per_step']:.2f} ms/step vs fp64 {res['fp64_ms_per_step']:.2f}; "
+        f"max rel L2 error {max(e['rel_l2'] for e in err.values()):.2e}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +177,16 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=100)
+    ap.add_argument("--batched-seq-steps", type=int, default=10)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--serve-qps", type=float, default=2.0)
+    ap.add_argument("--serve-duration", type=float, default=20.0)
+    ap.add_argument("--serve-steps-per-tick", type=int, default=8)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--fp32-n", type=int, default=256)
+    ap.add_argument("--fp32-steps", type=int, default=100)
+    ap.add_argument("--no-fp32", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
@@ -117,12 +235,21 @@ def main():
         except Exception as e:  # report, never fake
             log(f"bench: cpu baseline failed: {e}")
 
+    extra = {}
+    if not a.no_batched:
+        extra["batched"] = bench_batched(a.batched_steps, a.batched_seq_steps)
+    if not a.no_serving:
+        extra["serving"] = bench_serving(a.serve_qps, a.serve_duration, 7, 200e9, a.serve_steps_per_tick)
+    if not a.no_fp32:
+        extra["fp32"] = bench_fp32(a.fp32_n, a.fp32_steps)
+
     out = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"brio-wu {n}^3 resistive (eta=1e-3, 1% density noise), FP64",
                       "nx": n, "ny": n, "nz": n, "parallelism": "single-gpu"},
            "roofline": roofline, "cpu_baseline": cpu}
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 

@@ -49,6 +49,9 @@ struct lare_ctx {
     bool solo_ready = false;
     // arrays only the one-kernel-per-reference-kernel set reads
     void *pool0 = nullptr;
+    // device staging of field transfers (allocated on first use)
+    void *stage = nullptr;
+    size_t stage_bytes = 0;
     struct KStat {
         std::string name;
         int calls;
@@ -407,6 +410,7 @@ extern "C" int lare_ctx_destroy(lare_ctx *c)
     drop_graph(c->g);
     lare::group_free(c->solo);
     if (c->pool0) (void)hipFree(c->pool0);
+    if (c->stage) (void)hipFree(c->stage);
     (void)hipFree(c->g.pool);
     (void)hipStreamDestroy(c->g.stream);
     delete c;
@@ -437,38 +441,36 @@ void *field_dev_ptr(lare_ctx *c, int f)
     return c->g.dev.a.f[f];
 }
 
-// host array (reference extent) <-> device box, one 2D copy per x plane;
-// FP32-storage fields are rounded on upload and widened on download
+// host array (reference extent) <-> device box: one contiguous copy through
+// the context's staging buffer and one scatter / gather kernel (FP32-storage
+// fields are rounded on the way in and widened on the way out)
 int copy_field(lare_ctx *c, int field, double *host, bool up)
 {
     int64_t lb[3], ub[3];
     int rc = lare_field_bounds(c, field, lb, ub);
     if (rc) return rc;
     if (!host) return fail(-1, "lare field copy: null host pointer");
-    const lare::Geo &G = c->g.dev.g;
-    const size_t nxr = (size_t)(ub[0] - lb[0] + 1), nyr = (size_t)(ub[1] - lb[1] + 1), nzr = (size_t)(ub[2] - lb[2] + 1);
+    const size_t count = (size_t)(ub[0] - lb[0] + 1) * (ub[1] - lb[1] + 1) * (ub[2] - lb[2] + 1);
+    if (!c->stage) {
+        // the largest field extent, (n+5)^3 doubles
+        c->stage_bytes = (size_t)(c->desc.nx + 5) * (c->desc.ny + 5) * (c->desc.nz + 5) * sizeof(double);
+        HIPCHK(hipMalloc(&c->stage, c->stage_bytes));
+    }
+    if (count * sizeof(double) > c->stage_bytes) return fail(-1, "lare field copy: field larger than staging");
+    const int l3[3] = {(int)lb[0], (int)lb[1], (int)lb[2]}, u3[3] = {(int)ub[0], (int)ub[1], (int)ub[2]};
     const bool f32 = c->g.fp32 && field != LARE_DM;
-    const size_t es = f32 ? sizeof(float) : sizeof(double);
-    char *dev = (char *)field_dev_ptr(c, field);
-    std::vector<float> stage;
-    char *hbase = (char *)host;
-    if (f32) {
-        stage.resize(nxr * nyr * nzr);
-        if (up)
-            for (size_t i = 0; i < stage.size(); ++i) stage[i] = (float)host[i];
-        hbase = (char *)stage.data();
+    double *st = (double *)c->stage;
+    if (up) {
+        HIPCHK(hipMemcpyAsync(st, host, count * sizeof(double), hipMemcpyHostToDevice, c->g.stream));
+        lare::box_copy(c->g.stream, field_dev_ptr(c, field), f32, st, c->g.dev.g, l3, u3, true);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->g.stream));
+    } else {
+        lare::box_copy(c->g.stream, field_dev_ptr(c, field), f32, st, c->g.dev.g, l3, u3, false);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(host, st, count * sizeof(double), hipMemcpyDeviceToHost, c->g.stream));
+        HIPCHK(hipStreamSynchronize(c->g.stream));
     }
-    HIPCHK(hipStreamSynchronize(c->g.stream));
-    for (int64_t ix = lb[0]; ix <= ub[0]; ++ix) {
-        char *d = dev + (G.base + ix * G.sx + lb[1] * G.sy + lb[2]) * es;
-        char *h = hbase + (size_t)(ix - lb[0]) * nyr * nzr * es;
-        if (up)
-            HIPCHK(hipMemcpy2D(d, G.sy * es, h, nzr * es, nzr * es, nyr, hipMemcpyHostToDevice));
-        else
-            HIPCHK(hipMemcpy2D(h, nzr * es, d, G.sy * es, nzr * es, nyr, hipMemcpyDeviceToHost));
-    }
-    if (f32 && !up)
-        for (size_t i = 0; i < stage.size(); ++i) host[i] = (double)stage[i];
     return 0;
 }
 

@@ -68,6 +68,12 @@ struct Grid {
 // Metric arrays from the 1D deltas (LARE3D::grid, setup.cpp:337-390).
 void build_metric(Grid &g);
 
+// A field over its reference extent lb..ub (inclusive), held contiguous in
+// device memory `lin` (z fastest), into (to_box) or out of its padded box;
+// f32 marks an FP32-storage box.  Enqueued on st.
+void box_copy(hipStream_t st, void *box, bool f32, double *lin, const Geo &G, const int lb[3], const int ub[3],
+              bool to_box);
+
 
 
 // The three LARE3D events (shared_data.h:397-419), enqueued on g.stream.

@@ -35,6 +35,26 @@ KERNEL void k_fill(double *p, long n, double v)
     if (i < n) p[i] = v;
 }
 
+// A field over its reference extent x0..x0+nx-1 (likewise y, z), contiguous
+// with z fastest, to / from its place in the padded box; E is the box's
+// element type (FP32 storage rounds on the way in and widens on the way out).
+template <class E>
+KERNEL void k_scatter(E *box, const double *src, Geo G, int x0, int y0, int z0, int nx, int ny, int nz)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)nx * ny * nz) return;
+    const int iz = (int)(i % nz), iy = (int)((i / nz) % ny), ix = (int)(i / ((long)ny * nz));
+    box[IX(x0 + ix, y0 + iy, z0 + iz)] = (E)src[i];
+}
+template <class E>
+KERNEL void k_gather(double *dst, const E *box, Geo G, int x0, int y0, int z0, int nx, int ny, int nz)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)nx * ny * nz) return;
+    const int iz = (int)(i % nz), iy = (int)((i / nz) % ny), ix = (int)(i / ((long)ny * nz));
+    dst[i] = (double)box[IX(x0 + ix, y0 + iy, z0 + iz)];
+}
+
 // ------------------------------------------------- Lagrangian: prologue
 
 // lagran.cpp:113-125 over the centred extent; also resets the CFL minimum.
@@ -1265,6 +1285,29 @@ void step_end(Grid &g)
     launch3(k_b_area, 24, g, rng(-1, nx + 2, -1, ny + 2, -2, nz + 2), 2, 1);
     bfield_bcs(g);
     hipLaunchKernelGGL(k_step_done, dim3(1), dim3(1), 0, g.stream, g.dev);
+}
+
+void box_copy(hipStream_t st, void *box, bool f32, double *lin, const Geo &G, const int lb[3], const int ub[3],
+              bool to_box)
+{
+    const int nx = ub[0] - lb[0] + 1, ny = ub[1] - lb[1] + 1, nz = ub[2] - lb[2] + 1;
+    const long total = (long)nx * ny * nz;
+    const unsigned blocks = (unsigned)((total + 255) / 256);
+    if (to_box) {
+        if (f32)
+            hipLaunchKernelGGL(k_scatter<float>, dim3(blocks), dim3(256), 0, st, (float *)box, (const double *)lin, G,
+                               lb[0], lb[1], lb[2], nx, ny, nz);
+        else
+            hipLaunchKernelGGL(k_scatter<double>, dim3(blocks), dim3(256), 0, st, (double *)box, (const double *)lin,
+                               G, lb[0], lb[1], lb[2], nx, ny, nz);
+    } else {
+        if (f32)
+            hipLaunchKernelGGL(k_gather<float>, dim3(blocks), dim3(256), 0, st, lin, (const float *)box, G, lb[0],
+                               lb[1], lb[2], nx, ny, nz);
+        else
+            hipLaunchKernelGGL(k_gather<double>, dim3(blocks), dim3(256), 0, st, lin, (const double *)box, G, lb[0],
+                               lb[1], lb[2], nx, ny, nz);
+    }
 }
 
 }  // namespace lare

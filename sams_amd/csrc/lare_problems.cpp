@@ -1,9 +1,11 @@
 // Initial conditions and boundary conditions of the benchmark problems
 // (include/lare_hip.h, lare_problem_*): host code that plays the role of
 // the reference's initial-condition packages.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "lare_hip.h"
@@ -108,7 +110,10 @@ extern "C" int lare_problem_fill(const lare_problem *p, int field, double *host)
     axes(nz, d.z_min, d.z_max, zb, zc);
     const double gam = d.gas_gamma;
     const long ny4 = ny + 4, nz4 = nz + 4;
-    for (long ix = -1; ix <= nx + 2; ++ix)
+    // x planes split over host threads (every value depends on its own
+    // indices only, so the result does not depend on the split)
+    auto planes = [&](long x0, long x1) {
+    for (long ix = x0; ix <= x1; ++ix)
         for (long iy = -1; iy <= ny + 2; ++iy)
             for (long iz = -1; iz <= nz + 2; ++iz) {
                 const uint64_t lin = (uint64_t)(((ix + 1) * ny4 + (iy + 1)) * nz4 + (iz + 1));
@@ -161,6 +166,21 @@ extern "C" int lare_problem_fill(const lare_problem *p, int field, double *host)
                 }
                 host[((ix - lb[0]) * nyr + (iy - lb[1])) * nzr + (iz - lb[2])] = v;
             }
+    };
+    const long np = nx + 4;
+    const long work = np * (ny + 4) * (nz + 4);
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const long nt = std::min<long>((long)hw, std::max<long>(1, work / 65536));
+    if (nt <= 1) {
+        planes(-1, nx + 2);
+        return 0;
+    }
+    std::vector<std::thread> th;
+    for (long t = 0; t < nt; ++t) {
+        const long a = -1 + np * t / nt, b = -1 + np * (t + 1) / nt - 1;
+        if (a <= b) th.emplace_back(planes, a, b);
+    }
+    for (auto &t : th) t.join();
     return 0;
 }
 
