@@ -168,50 +168,10 @@ def bench_fp32(n, steps):
     return res
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--profile-steps", type=int, default=2)
-    ap.add_argument("--cpu-steps", type=int, default=3)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--batched-steps", type=int, default=100)
-    ap.add_argument("--batched-seq-steps", type=int, default=10)
-    ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--serve-qps", type=float, default=2.0)
-    ap.add_argument("--serve-duration", type=float, default=20.0)
-    ap.add_argument("--serve-steps-per-tick", type=int, default=8)
-    ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--fp32-n", type=int, default=256)
-    ap.add_argument("--fp32-steps", type=int, default=100)
-    ap.add_argument("--no-fp32", action="store_true")
-    a = ap.parse_args()
-    if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
-              "the LARE3D step is not sharded across GPUs here", file=sys.stderr)
-        sys.exit(2)
-
-    import sams_amd.lare as L
-    n = a.n
-    p = L.problem("bw", n=n, seed=1, noise=0.01, eta=1e-3)
-    log(f"bench: Brio-Wu {n}^3 resistive FP64, warmup {a.warmup}, steps {a.steps}")
-    dev = L.Lare.from_problem(p)
-    if a.warmup:
-        dev.steps(a.warmup, record_dt=False)
-    dev.sync()
-    t0 = time.perf_counter()
-    dev.steps(a.steps, record_dt=False)
-    dev.sync()
-    el = time.perf_counter() - t0
-    cells = n ** 3
-    value = cells * a.steps / el
-    ms_step = 1e3 * el / a.steps
-    log(f"bench: {ms_step:.3f} ms/step, {value / 1e9:.3f} G cell-updates/s")
-
-    # per-kernel HIP-event timing on the grid's stream (after the timed region)
-    prof = dev.profile(a.profile_steps)
+def kernel_roofline(dev, nsteps, cells, ms_step):
+    """Per-kernel HIP-event timing on the grid's stream (after the timed
+    region); the dominant kernel's algorithmic GB/s against the HBM peak."""
+    prof = dev.profile(nsteps)
     dom = max(prof.items(), key=lambda kv: kv[1][1])
     name, (calls, tot_ms, tot_bytes) = dom
     avg_ms = tot_ms / calls
@@ -225,6 +185,71 @@ def main():
                 "avg_launch_us": round(avg_ms * 1e3, 2),
                 "step_algorithmic_GBs": round(step_bytes / (ms_step * 1e-3) / 1e9, 1),
                 "step_frac": round(step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    return roofline
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=100)
+    ap.add_argument("--batched-seq-steps", type=int, default=10)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--serve-qps", type=float, default=1.5)
+    ap.add_argument("--serve-duration", type=float, default=30.0)
+    ap.add_argument("--serve-steps-per-tick", type=int, default=8)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--fp32-n", type=int, default=256)
+    ap.add_argument("--fp32-steps", type=int, default=100)
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every kernel from the host instead of replaying the captured step graph")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="first run one plain device copy of this many MiB (a known-traffic kernel for "
+                         "scaling hardware byte counters)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
+              "the LARE3D step is not sharded across GPUs here", file=sys.stderr)
+        sys.exit(2)
+
+    import sams_amd.lare as L
+    if a.copy_probe_mb:
+        import torch
+        src = torch.ones(a.copy_probe_mb << 17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: copy probe {a.copy_probe_mb} MiB read + {a.copy_probe_mb} MiB written")
+        del src, dst
+    n = a.n
+    p = L.problem("bw", n=n, seed=1, noise=0.01, eta=1e-3)
+    log(f"bench: Brio-Wu {n}^3 resistive FP64, warmup {a.warmup}, steps {a.steps}")
+    dev = L.Lare.from_problem(p)
+    if a.eager:
+        dev.set_graph(False)
+    if a.warmup:
+        dev.steps(a.warmup, record_dt=False)
+    dev.sync()
+    t0 = time.perf_counter()
+    dev.steps(a.steps, record_dt=False)
+    dev.sync()
+    el = time.perf_counter() - t0
+    cells = n ** 3
+    value = cells * a.steps / el
+    ms_step = 1e3 * el / a.steps
+    log(f"bench: {ms_step:.3f} ms/step, {value / 1e9:.3f} G cell-updates/s")
+
+    roofline = None
+    if a.profile_steps > 0:
+        roofline = kernel_roofline(dev, a.profile_steps, cells, ms_step)
 
     cpu = None
     if not a.no_cpu_baseline:

@@ -328,7 +328,7 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
     const size_t tsz = g.fp32 ? sizeof(float) : sizeof(double);
     const size_t lmax = (size_t)std::max(d->nx, std::max(d->ny, d->nz)) + 5;
     g.pool_bytes = box * (kEngineT * tsz + kEngineD * sizeof(double)) + 6 * lmax * sizeof(double) + 256;
-    hipError_t e = hipStreamCreate(&g.stream);
+    hipError_t e = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
         return fail(-3, std::string("hipStreamCreate: ") + hipGetErrorString(e));
@@ -388,8 +388,10 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
         make_axis(d->nz, d->z_min, d->z_max, db[2], dc[2]);
         double *dstb[3] = {a.dxb, a.dyb, a.dzb}, *dstc[3] = {a.dxc, a.dyc, a.dzc};
         for (int k = 0; k < 3 && rc == 0; ++k) {
-            e = hipMemcpy(dstb[k], db[k].data(), db[k].size() * sizeof(double), hipMemcpyHostToDevice);
-            if (e == hipSuccess) e = hipMemcpy(dstc[k], dc[k].data(), dc[k].size() * sizeof(double), hipMemcpyHostToDevice);
+            e = hipMemcpyAsync(dstb[k], db[k].data(), db[k].size() * sizeof(double), hipMemcpyHostToDevice, g.stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(dstc[k], dc[k].data(), dc[k].size() * sizeof(double), hipMemcpyHostToDevice, g.stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
             if (e != hipSuccess) rc = fail(-3, std::string("axis upload: ") + hipGetErrorString(e));
         }
     } while (0);
@@ -754,7 +756,7 @@ extern "C" int lare_batch_create(lare_ctx *const *members, int32_t n, lare_batch
         b->members.push_back(members[i]);
         grids.push_back(&members[i]->g);
     }
-    hipError_t e = hipStreamCreate(&b->stream);
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete b;
         return fail(-3, std::string("lare_batch_create: hipStreamCreate: ") + hipGetErrorString(e));

@@ -1204,7 +1204,8 @@ Group::Tab &table(Group &gp, int subset, const RSpec &r)
     t.nblk = (int)total;
     if (!boff.empty()) {
         (void)hipMalloc(&t.boff, boff.size() * sizeof(int));
-        (void)hipMemcpy(t.boff, boff.data(), boff.size() * sizeof(int), hipMemcpyHostToDevice);
+        (void)hipMemcpyAsync(t.boff, boff.data(), boff.size() * sizeof(int), hipMemcpyHostToDevice, gp.stream);
+        (void)hipStreamSynchronize(gp.stream);
     }
     return gp.tabs.emplace(std::make_pair(subset, key), t).first->second;
 }
@@ -1323,9 +1324,10 @@ Group::BTab &btable(Group &gp, int vg, int dim, int subset)
     t.nblk = (int)total;
     if (!mem.empty()) {
         (void)hipMalloc(&t.sets, sets.size() * sizeof(BCSetE));
-        (void)hipMemcpy(t.sets, sets.data(), sets.size() * sizeof(BCSetE), hipMemcpyHostToDevice);
+        (void)hipMemcpyAsync(t.sets, sets.data(), sets.size() * sizeof(BCSetE), hipMemcpyHostToDevice, gp.stream);
         (void)hipMalloc(&t.boff, boff.size() * sizeof(int));
-        (void)hipMemcpy(t.boff, boff.data(), boff.size() * sizeof(int), hipMemcpyHostToDevice);
+        (void)hipMemcpyAsync(t.boff, boff.data(), boff.size() * sizeof(int), hipMemcpyHostToDevice, gp.stream);
+        (void)hipStreamSynchronize(gp.stream);
     }
     return gp.btabs.emplace(std::make_pair(key, 0), t).first->second;
 }
@@ -1484,7 +1486,8 @@ int group_init(Group &gp, const std::vector<Grid *> &members, hipStream_t stream
         gp.n[sub] = n;
         if (n == 0) continue;
         hipError_t e = hipMalloc(&gp.devs[sub], host.size());
-        if (e == hipSuccess) e = hipMemcpy(gp.devs[sub], host.data(), host.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpyAsync(gp.devs[sub], host.data(), host.size(), hipMemcpyHostToDevice, gp.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(gp.stream);
         if (e != hipSuccess) {
             err = std::string("group device views: ") + hipGetErrorString(e);
             group_free(gp);

@@ -15,6 +15,7 @@ The reference has no serving layer; this is the request loop around the
 step, in the shape of the runner's time loop (runner.h:564-586) applied to
 many independent simulations.
 """
+import concurrent.futures
 import dataclasses
 import math
 import time
@@ -36,6 +37,7 @@ class Request:
     grid: object = None
     done_steps: int = 0
     dt_last: float = math.nan
+    prefill: object = None   # future of the host initial conditions
 
     @property
     def cells(self):
@@ -88,7 +90,8 @@ def poisson_requests(qps, duration, seed, nmin=32, nmax=192, smin=50, smax=500):
 class Server:
     """Admission by HBM budget, continuous batching, per-request latency."""
 
-    def __init__(self, budget_bytes=200e9, steps_per_tick=8, clock=time.perf_counter, keep_results=False):
+    def __init__(self, budget_bytes=200e9, steps_per_tick=8, clock=time.perf_counter, keep_results=False,
+                 prefill_ahead=4):
         self.budget = float(budget_bytes)
         self.k = int(steps_per_tick)
         self.clock = clock
@@ -103,6 +106,19 @@ class Server:
         self.peak_used = 0.0
         self._batches = {}
         self._members = {}
+        # host initial conditions of the next waiting requests are computed
+        # on worker threads while the GPU steps (the fill releases the GIL)
+        self.ahead = int(prefill_ahead)
+        self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=2)
+
+    @staticmethod
+    def _fill(p):
+        return {f: L.problem_fill(p, f) for f in L.STATE_FIELDS}
+
+    def _prefetch(self):
+        for r in self.waiting[:self.ahead]:
+            if r.prefill is None:
+                r.prefill = self._pool.submit(self._fill, r.problem)
 
     def _admit(self, now):
         while self.waiting:
@@ -113,7 +129,15 @@ class Server:
             if self.used + need > self.budget:
                 break
             self.waiting.pop(0)
-            r.grid = L.Lare.from_problem(r.problem)
+            if r.prefill is None:
+                r.prefill = self._pool.submit(self._fill, r.problem)
+            host = r.prefill.result()
+            r.prefill = None
+            g = L.Lare(L.problem_desc(r.problem))
+            g.set_problem_bcs(r.problem)
+            for f, a in host.items():
+                g.upload(f, a)
+            r.grid = g
             r.admitted = now
             self.used += need
             self.active.append(r)
@@ -146,6 +170,7 @@ class Server:
             while i < len(pending) and pending[i].arrival * time_scale <= now:
                 self.waiting.append(pending[i])
                 i += 1
+            self._prefetch()
             return now
 
         while i < len(pending) or self.waiting or self.active:
@@ -193,6 +218,7 @@ class Server:
             b.close()
         self._batches.clear()
         self._members.clear()
+        self._pool.shutdown(wait=True)
         return self.done
 
 

@@ -15,7 +15,7 @@ def _requests():
     specs = [
         (L.problem("ot", n=12, seed=3, amp=0.02), 17, 0.00),
         (L.problem("bw", n=24, ny=4, nz=4, seed=5, noise=0.01, eta=1e-3), 9, 0.00),
-        (L.problem("ot", n=10, ny=14, nz=8, seed=8, amp=0.04, precision=L.FP32_STORAGE), 13, 0.02),
+        (L.problem("ot", n=10, ny=14, nz=8, seed=8, amp=0.04, precision=L.FP32_STORAGE), 13, 0.00),
         (L.problem("bw", n=20, ny=5, nz=4, seed=2, noise=0.02, bcx=L.BC_MIRROR), 21, 0.05),
         (L.problem("bw", n=16, ny=4, nz=4, seed=6, eta=1e-3, precision=L.FP32_STORAGE), 7, 0.05),
         (L.problem("ot", n=14, seed=4, beta=0.5), 11, 0.10),
@@ -36,7 +36,7 @@ def test_served_results_equal_solo_runs():
     srv = S.Server(budget_bytes=50e9, steps_per_tick=4, keep_results=True)
     done = srv.run(_requests())
     assert len(done) == 6
-    assert srv.peak_active >= 3  # grids did share the device
+    assert srv.peak_active >= 3  # the first three arrive together and share the device
     _check_alone(done)
     summ = S.summarize(done, max(r.finished for r in done))
     assert summ["p99_latency_s"] >= summ["p50_latency_s"] > 0
