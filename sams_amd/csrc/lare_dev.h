@@ -131,8 +131,8 @@ struct EDev {
 };
 
 // One launch over a group: the members' device views, the first block of
-// each member (ngrid entries, ascending), the total block count and the
-// range.
+// each member (ngrid entries, ascending), the total block count, the range
+// and the workgroup order.
 template <class T>
 struct Src {
     const EDev<T> *devs;
@@ -140,6 +140,7 @@ struct Src {
     int ngrid;
     int nblk;
     RSpec r;
+    int order;  // 0: chunk fastest, then plane; 1: chunks in 8 bands, band-major
 };
 
 // Built-in boundary conditions of one variable along one dimension, for one

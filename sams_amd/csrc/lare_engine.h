@@ -29,9 +29,11 @@ struct Group {
     struct Tab {
         int *boff = nullptr;
         int nblk = 0;
+        int order = 0;
         double cells = 0.0;
     };
-    std::map<std::pair<int, std::array<int, 6>>, Tab> tabs;  // (subset, range spec)
+    // (subset, range spec, order)
+    std::map<std::pair<int, std::array<int, 7>>, Tab> tabs;
     struct BTab {
         BCSetE *sets = nullptr;
         int *boff = nullptr;
@@ -43,6 +45,7 @@ struct Group {
     hipStream_t stream = nullptr;
     Prof *prof = nullptr;
     bool planning = false;
+    int order = 1;  // workgroup order within a grid (Src::order)
     // one captured step of the whole group
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;

@@ -33,6 +33,7 @@
 #include <float.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 using lare::ArrE;
@@ -43,6 +44,11 @@ using lare::Phys;
 using lare::R3;
 using lare::Scal;
 using lare::Src;
+
+// Lower bound on resident waves per SIMD for the register-heavy kernels (the
+// compiler then keeps them within the matching register budget; the bounds
+// are the highest that compile without scratch spills).
+#define OCC(n) __attribute__((amdgpu_waves_per_eu(n, 8)))
 
 #define RHO A.rho
 #define EE A.ee
@@ -81,9 +87,14 @@ __device__ __forceinline__ int grid_of(const int *boff, int n, int b)
     return lo;
 }
 
-// Cell of this lane: grid g_, (ix, iy, iz) in range R; ok_ is false for
-// the lanes past the end of the plane.
-#define EBEGIN_ALL                                                                                     \
+// Work of this workgroup: grid g_ (found from the first-block offsets) and
+// one chunk of 256 cells of the flattened (y, z) plane px_ of range R (lane
+// -> (iy_, iz_), z fastest; ok_ is false past the end of the plane).  Order
+// within a grid (S.order): 0 chunk fastest, then plane; 1 the chunks of a
+// plane split into 8 bands, band-major, so that with the XCD renumbering
+// each XCD walks one band of every plane and the x- and y-neighbours a
+// stencil reads stay in its L2.
+#define EGRID                                                                                        \
     const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);                                           \
     const int g_ = grid_of(S.boff, S.ngrid, b_);                                                   \
     const EDev<T> &D_ = S.devs[g_];                                                                \
@@ -93,21 +104,29 @@ __device__ __forceinline__ int grid_of(const int *boff, int n, int b)
     const R3 R = {S.r.x0, G.nx + S.r.dx1, S.r.y0, G.ny + S.r.dy1, S.r.z0, G.nz + S.r.dz1};         \
     const int nzr_ = R.z1 - R.z0 + 1;                                                              \
     const int plane_ = (R.y1 - R.y0 + 1) * nzr_;                                                   \
-    const int bpp_ = (plane_ + 255) >> 8;                                                          \
+    const int nxr_ = R.x1 - R.x0 + 1;                                                              \
     const int lb_ = b_ - S.boff[g_];                                                               \
-    const int px_ = lb_ / bpp_;                                                                    \
-    const int t_ = (lb_ - px_ * bpp_) * 256 + (int)threadIdx.x;                                   \
-    const bool ok_ = t_ < plane_;                                                                  \
-    const int ix = R.x0 + px_;                                                                     \
-    const int iy = R.y0 + t_ / nzr_;                                                               \
-    const int iz = R.z0 + t_ % nzr_;                                                               \
-    (void)P;                                                                                       \
-    (void)ix;                                                                                      \
-    (void)iy;                                                                                      \
-    (void)iz;
-#define EBEGIN    \
-    EBEGIN_ALL    \
-    if (!ok_) return;
+    const int bpp_ = (plane_ + 255) >> 8;                                                          \
+    int px_, ch_;                                                                                  \
+    if (S.order) {                                                                                 \
+        const int bs_ = (bpp_ + 7) >> 3;                                                           \
+        const int band_ = lb_ / (nxr_ * bs_);                                                      \
+        const int rem_ = lb_ - band_ * (nxr_ * bs_);                                               \
+        px_ = rem_ / bs_;                                                                          \
+        ch_ = band_ * bs_ + (rem_ - px_ * bs_);                                                     \
+    } else {                                                                                       \
+        px_ = lb_ / bpp_;                                                                          \
+        ch_ = lb_ - px_ * bpp_;                                                                    \
+    }                                                                                              \
+    const int t_ = ch_ * 256 + (int)threadIdx.x;                                                   \
+    const bool ok_ = ch_ < bpp_ && t_ < plane_;                                                    \
+    const int ix_ = R.x0 + px_;                                                                    \
+    const int iy_ = R.y0 + t_ / nzr_;                                                              \
+    const int iz_ = R.z0 + t_ % nzr_;                                                              \
+    (void)P;
+// Run cell(ix, iy, iz) for the cell of this lane.
+#define ELOOP(cell) \
+    if (ok_) cell(ix_, iy_, iz_);
 
 template <int DIR>
 __device__ __forceinline__ int sdir(const Geo &G)
@@ -122,7 +141,8 @@ __device__ __forceinline__ int sdir(const Geo &G)
 template <class T, int DIR>
 KERNEL void f_prep(const Src<T> S, double xpass, double ypass, double zpass)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     const double dt = A.s->dt;
@@ -167,18 +187,23 @@ KERNEL void f_prep(const Src<T> S, double xpass, double ypass, double zpass)
         A.cv1[c] = vol * (1.0 + dv);
         A.db1[c] = DZB(iz) + (vzb - vzbm) * dt;
     }
+    };
+    ELOOP(cell)
 }
 
 // R2: vertex volume before remap, cvc1 = 0.125 * (8 cells of cv1).
 template <class T>
 KERNEL void f_cvc1(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     const Fld<T> C1 = A.cv1;
     A.cvc1[c] = 0.125 * (C1[c] + C1[c + sx] + C1[c + sy] + C1[c + sx + sy] + C1[c + 1] + C1[c + sx + 1] +
                          C1[c + sy + 1] + C1[c + sx + sy + 1]);
+    };
+    ELOOP(cell)
 }
 
 // Constrained-transport flux of face magnetic flux B along DIR: v_advect is
@@ -222,7 +247,8 @@ __device__ __forceinline__ double ct_flux(const ArrE<T> &A, const Geo &G, int c,
 template <class T, int DIR>
 KERNEL void f_ct_fluxes(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     const double dt = A.s->dt;
@@ -236,6 +262,8 @@ KERNEL void f_ct_fluxes(const Src<T> S)
         A.fl[0][c] = ct_flux<2>(A, G, c, BX, VZ1, -sy, sx, dt);
         A.fl[1][c] = ct_flux<2>(A, G, c, BY, VZ1, -sx, sy, dt);
     }
+    };
+    ELOOP(cell)
 }
 
 // R4: the four CT updates of the sweep in their reference order, each over
@@ -243,7 +271,8 @@ KERNEL void f_ct_fluxes(const Src<T> S)
 template <class T, int DIR>
 KERNEL void f_ct_update(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     const int nx = G.nx, ny = G.ny, nz = G.nz;
@@ -274,6 +303,8 @@ KERNEL void f_ct_update(const Src<T> S)
         if (x1 && y0 && z1) BY[c] = BY[c] - fb[c] + fb[c - 1];
         if (bzr) BZ[c] = bz + fb[c] - fb[c - sy];
     }
+    };
+    ELOOP(cell)
 }
 
 // Face-averaged sweep velocity of the mass and energy fluxes.
@@ -301,7 +332,8 @@ __device__ __forceinline__ double face_area(const ArrE<T> &A, int ix, int iy, in
 template <class T, int DIR>
 KERNEL void f_mass_flux(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sd = sdir<DIR>(G);
     const int id = DIR == 0 ? ix : DIR == 1 ? iy : iz;
@@ -327,6 +359,8 @@ KERNEL void f_mass_flux(const Src<T> S)
     double phi = fabs(o_v) / dxbu;
     double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
     DM[c] = (fu + Di * (1.0 - phi)) * o_v;
+    };
+    ELOOP(cell)
 }
 
 // Energy flux of specific energy E on face c (x_energy_flux,
@@ -360,7 +394,8 @@ __device__ __forceinline__ double energy_flux(const ArrE<T> &A, const Geo &G, in
 template <class T, int DIR>
 KERNEL void f_energy_vertex(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     const int nx = G.nx, ny = G.ny, nz = G.nz;
@@ -391,6 +426,8 @@ KERNEL void f_energy_vertex(const Src<T> S)
                    M[c + sx + sy + 1];
         A.dmv[c] = DIR == 0 ? q * 0.125 : 0.125 * q;
     }
+    };
+    ELOOP(cell)
 }
 
 // Momentum flux of velocity component f on vertex face c (x_mom_flux,
@@ -418,7 +455,8 @@ __device__ __forceinline__ double mom_flux(const Geo &G, int c, const Fld<T> f, 
 template <class T, int DIR>
 KERNEL void f_mom_fluxes(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sd = sdir<DIR>(G);
     const double dt = A.s->dt;
@@ -437,6 +475,8 @@ KERNEL void f_mom_fluxes(const Src<T> S)
     A.fl[2][c] = mom_flux<DIR>(G, c, VX, o_v, sign_v, dxci, dxcu, dxbu, rhou, dm);
     A.fl[3][c] = mom_flux<DIR>(G, c, VY, o_v, sign_v, dxci, dxcu, dxbu, rhou, dm);
     A.fl[4][c] = mom_flux<DIR>(G, c, VZ, o_v, sign_v, dxci, dxcu, dxbu, rhou, dm);
+    };
+    ELOOP(cell)
 }
 
 // R8: the in-place updates over (0..n)^3: density and both energies on cells
@@ -445,7 +485,8 @@ KERNEL void f_mom_fluxes(const Src<T> S)
 template <class T, int DIR>
 KERNEL void f_update(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     const int sd = sdir<DIR>(G);
@@ -468,6 +509,8 @@ KERNEL void f_update(const Src<T> S)
         EE[c] = (EE[c] * cv1 * rho1 + A.fl[0][c - sd] - A.fl[0][c]) / (cv2 * rho);
         EI[c] = (EI[c] * cv1 * rho1 + A.fl[1][c - sd] - A.fl[1][c]) / (cv2 * rho);
     }
+    };
+    ELOOP(cell)
 }
 
 // B *= area (remap.cpp:46-50) and B /= (area + none_zero) (91-95) over each
@@ -475,7 +518,8 @@ KERNEL void f_update(const Src<T> S)
 template <class T, int COMP, int DIVIDE>
 KERNEL void f_b_area(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const Fld<T> B = COMP == 0 ? BX : COMP == 1 ? BY : BZ;
     const double ar = COMP == 0 ? DXAB(iy, iz) : COMP == 1 ? DYAB(ix, iz) : DZAB(ix, iy);
@@ -483,6 +527,8 @@ KERNEL void f_b_area(const Src<T> S)
         B[c] /= (ar + DBL_EPSILON);
     else
         B[c] *= ar;
+    };
+    ELOOP(cell)
 }
 
 // ============================================================= Lagrangian
@@ -497,7 +543,8 @@ KERNEL void f_b_area(const Src<T> S)
 template <class T>
 KERNEL void g_cs(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const double gas_gamma = P.gas_gamma;
     if (ix == R.x0 && iy == R.y0 && iz == R.z0) A.s->dtmin_bits = 0x7FEFFFFFFFFFFFFFull;
     const int c = IX(ix, iy, iz);
@@ -510,6 +557,8 @@ KERNEL void g_cs(const Src<T> S)
     double rmin = pmax(RHO[c], DBL_EPSILON);
     double b2 = bx1 * bx1 + by1 * by1 + bz1 * bz1;
     A.cs[c] = sqrt((gas_gamma * p + b2) / rmin);
+    };
+    ELOOP(cell)
 }
 
 #define CV8SUM(ix, iy, iz)                                                                               \
@@ -521,7 +570,8 @@ KERNEL void g_cs(const Src<T> S)
 template <class T>
 KERNEL void g_vertex(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     const double c000 = CV(ix, iy, iz), c100 = CV(ix + 1, iy, iz), c010 = CV(ix, iy + 1, iz),
@@ -536,6 +586,8 @@ KERNEL void g_vertex(const Src<T> S)
     double sum = s[c] * c000 + s[c + sx] * c100 + s[c + sy] * c010 + s[c + sx + sy] * c110 + s[c + 1] * c001 +
                  s[c + sx + 1] * c101 + s[c + sy + 1] * c011 + s[c + sx + sy + 1] * c111;
     A.cs_v[c] = 0.125 * sum / cv_v;
+    };
+    ELOOP(cell)
 }
 
 // edge_viscosity (lagran.cpp:56-93) between c1 and c1 + s.
@@ -574,7 +626,8 @@ __device__ __forceinline__ double g_edge_visc(const ArrE<T> &A, double dt, doubl
 template <class T>
 KERNEL void g_alpha(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const double visc1 = P.visc1, visc2_norm = P.visc2_norm;
     const int sx = G.sx, sy = G.sy;
     const int nx = G.nx, ny = G.ny, nz = G.nz;
@@ -598,6 +651,8 @@ KERNEL void g_alpha(const Src<T> S)
         double cs_edge = pmin(A.cs_v[c1], A.cs_v[c1 + 1]);
         A.alpha3[c] = g_edge_visc(A, dt, visc1, visc2_norm, dvdots, DZB(iz), DZB(iz - 1), DZB(iz + 1), cs_edge, c1, 1);
     }
+    };
+    ELOOP(cell)
 }
 
 #define SQD(p, q) \
@@ -609,12 +664,12 @@ KERNEL void g_alpha(const Src<T> S)
 // atomicMin on the bit pattern (non-negative doubles order as integers; the
 // minimum is exact, so the result does not depend on the order).
 template <class T>
-KERNEL void g_visc_dt(const Src<T> S)
+KERNEL OCC(4) void g_visc_dt(const Src<T> S)
 {
-    EBEGIN_ALL
+    EGRID
     const double gas_gamma = P.gas_gamma, mu0 = P.mu0;
-    double m = DBL_MAX;
-    if (ok_) {
+    auto cell = [&](const int ix, const int iy, const int iz) -> double {
+        double m = DBL_MAX;
         const int c = IX(ix, iy, iz);
         const int sx = G.sx, sy = G.sy;
         const int mmm = c - sx - sy - 1, omm = c - sy - 1, oom = c - 1, mom = c - sx - 1;
@@ -681,7 +736,10 @@ KERNEL void g_visc_dt(const Src<T> S)
             double dt4 = cv / pmax(pmax(pmax(fabs(vzbm), fabs(vzbp)), dvz), 1.0e-10 * cv);
             m = pmin(pmin(pmin(dt1, dt2), dt3), dt4);
         }
-    }
+        return m;
+    };
+    double m = DBL_MAX;
+    if (ok_) m = cell(ix_, iy_, iz_);
     for (int off = 32; off > 0; off >>= 1) m = pmin(m, __shfl_xor(m, off, 64));
     __shared__ double wmin[4];
     const int lane = threadIdx.x & 63;
@@ -728,7 +786,8 @@ KERNEL void g_step_done(const EDev<T> *devs, int n)
 template <class T>
 KERNEL void g_predict(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const double gas_gamma = P.gas_gamma;
     const bool res = P.resistive != 0;
     const int c = IX(ix, iy, iz);
@@ -795,6 +854,8 @@ KERNEL void g_predict(const Src<T> S)
         A.p_e[c] = e1_e * (gas_gamma - 1.0) * rho * vol / cv1;
         A.p_i[c] = e1_i * (gas_gamma - 1.0) * rho * vol / cv1;
     }
+    };
+    ELOOP(cell)
 }
 
 // L7 over (0..n)^3: forces and half-step velocities (lagran.cpp:619-731);
@@ -802,7 +863,8 @@ KERNEL void g_predict(const Src<T> S)
 template <class T>
 KERNEL void g_forces(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const double mu0 = P.mu0;
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
@@ -861,6 +923,8 @@ KERNEL void g_forces(const Src<T> S)
     VX1[c] = VX[c] + dt / 2.0 * sxv / rv;
     VY1[c] = VY[c] + dt / 2.0 * syv / rv;
     VZ1[c] = VZ[c] + dt / 2.0 * szv / rv;
+    };
+    ELOOP(cell)
 }
 
 #define DOT(p, q)                                                                                   \
@@ -870,9 +934,10 @@ KERNEL void g_forces(const Src<T> S)
 // C1 over (0..n+1)^3: shock_heating (lagran.cpp:884-993) with the dy*dy
 // factors of a9 and a11 (lagran.cpp:987).
 template <class T>
-KERNEL void g_shock_heating(const Src<T> S)
+KERNEL OCC(3) void g_shock_heating(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     const int ooo = c, moo = c - sx, omo = c - sy, mmo = c - sx - sy;
@@ -890,6 +955,8 @@ KERNEL void g_shock_heating(const Src<T> S)
          0.25 * dy * dy * A.alpha3[c] * a9 - 0.25 * dx * dy * A.alpha3[c - sx] * a10 -
          0.25 * dy * dy * A.alpha3[c - sx - sy] * a11 - 0.25 * dx * dy * A.alpha3[c - sy] * a12);
     A.visc_heat[c] = pmax(vh / CV(ix, iy, iz), 0.0);
+    };
+    ELOOP(cell)
 }
 
 // C2 over (0..n)^3: velocity correction (lagran.cpp:754-758) with the sum
@@ -897,13 +964,16 @@ KERNEL void g_shock_heating(const Src<T> S)
 template <class T>
 KERNEL void g_vel_update(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const double dt = A.s->dt;
     double rv = A.rho_v[c];
     VX[c] += dt * A.fxv[c] / rv;
     VY[c] += dt * A.fyv[c] / rv;
     VZ[c] += dt * A.fzv[c] / rv;
+    };
+    ELOOP(cell)
 }
 
 // C3 over (1..n)^3: density and energy correction (lagran.cpp:762-796).  The
@@ -911,7 +981,8 @@ KERNEL void g_vel_update(const Src<T> S)
 template <class T>
 KERNEL void g_rho_e_update(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     const double dt = A.s->dt;
@@ -933,6 +1004,8 @@ KERNEL void g_rho_e_update(const Src<T> S)
     EE[c] -= dv * A.p_e[c] / rho;
     EI[c] += (dt * A.visc_heat[c] - dv * A.p_i[c]) / rho;
     RHO[c] = rho / (1.0 + dv);
+    };
+    ELOOP(cell)
 }
 
 // ------------------------------------------------- resistive substeps
@@ -944,7 +1017,8 @@ KERNEL void g_rho_e_update(const Src<T> S)
 template <class T>
 KERNEL void r_eta_copy(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     A.bx1[c] = BX[c];
@@ -963,13 +1037,16 @@ KERNEL void r_eta_copy(const Src<T> S)
     jz = (jz + jzp) * 0.5;
     double j_local = sqrt(jx * jx + jy * jy + jz * jz);
     A.eta[c] = j_local > P.j_max ? P.eta_background + P.eta0 : P.eta_background;
+    };
+    ELOOP(cell)
 }
 
 // rkstep (lagran.cpp:507-535) over (0..n)^3.
 template <class T>
 KERNEL void r_rkstep(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     double dxc = DXC(ix), dyc = DYC(iy), dzc = DZC(iz);
@@ -987,6 +1064,8 @@ KERNEL void r_rkstep(const Src<T> S)
     A.flux_y[c] = -jy * eta * dyc * 0.5;
     A.flux_z[c] = -jz * eta * dzc * 0.5;
     A.curlb[c] = eta * (jx * jx + jy * jy + jz * jz);
+    };
+    ELOOP(cell)
 }
 
 // bstep's three components (lagran.cpp:537-590) with the Ohmic heating of
@@ -996,7 +1075,8 @@ KERNEL void r_rkstep(const Src<T> S)
 template <class T>
 KERNEL void r_b_energy(const Src<T> S)
 {
-    EBEGIN
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     const int nx = G.nx, ny = G.ny, nz = G.nz;
@@ -1019,6 +1099,8 @@ KERNEL void r_b_energy(const Src<T> S)
         double sum_curlb = cb[c] + cb[c - sx] + cb[c - sy] + cb[c - sx - sy] + cb[c - 1] + cb[c - sx - 1] + cb[c - sy - 1] + cb[c - sx - sy - 1];
         EE[c] += sum_curlb * dt / (8.0 * RHO[c]);
     }
+    };
+    ELOOP(cell)
 }
 
 // ------------------------------------------------- boundary conditions
@@ -1110,13 +1192,13 @@ KERNEL void e_bc(const EDev<T> *devs, const lare::BCSetE *sets, const int *boff,
 template <class T>
 KERNEL void e_energy_part(const Src<T> S, double *part)
 {
-    EBEGIN_ALL
+    EGRID
     double e[3] = {0.0, 0.0, 0.0};
-    if (ok_) {
+    auto cell = [&](const int ix, const int iy, const int iz) {
         const int c = IX(ix, iy, iz);
         const int sx = G.sx, sy = G.sy;
         const double cv = CV(ix, iy, iz);
-        e[0] = RHO[c] * (EE[c] + EI[c]) * cv;
+        e[0] += RHO[c] * (EE[c] + EI[c]) * cv;
         const double c000 = cv, c100 = CV(ix + 1, iy, iz), c010 = CV(ix, iy + 1, iz),
                      c110 = CV(ix + 1, iy + 1, iz), c001 = CV(ix, iy, iz + 1), c101 = CV(ix + 1, iy, iz + 1),
                      c011 = CV(ix, iy + 1, iz + 1), c111 = CV(ix + 1, iy + 1, iz + 1);
@@ -1125,10 +1207,11 @@ KERNEL void e_energy_part(const Src<T> S, double *part)
                      r[c + sx + 1] * c101 + r[c + sy + 1] * c011 + r[c + sx + sy + 1] * c111;
         double scv = c000 + c100 + c010 + c110 + c001 + c101 + c011 + c111;
         const double v2 = VX[c] * VX[c] + VY[c] * VY[c] + VZ[c] * VZ[c];
-        e[1] = 0.5 * (src / scv) * v2 * (0.125 * scv);
+        e[1] += 0.5 * (src / scv) * v2 * (0.125 * scv);
         const double bx = 0.5 * (BX[c] + BX[c - sx]), by = 0.5 * (BY[c] + BY[c - sy]), bz = 0.5 * (BZ[c] + BZ[c - 1]);
-        e[2] = (bx * bx + by * by + bz * bz) / (2.0 * P.mu0) * cv;
-    }
+        e[2] += (bx * bx + by * by + bz * bz) / (2.0 * P.mu0) * cv;
+    };
+    ELOOP(cell)
     __shared__ double sh[3][256];
     for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] = e[k];
     __syncthreads();
@@ -1182,9 +1265,9 @@ void *field_ptr(const Grid &g, int f)
 
 // Block offsets of a launch over `subset` with range r (built outside graph
 // capture, by group_prepare or on first use).
-Group::Tab &table(Group &gp, int subset, const RSpec &r)
+Group::Tab &table(Group &gp, int subset, const RSpec &r, int order)
 {
-    const std::array<int, 6> key{r.x0, r.dx1, r.y0, r.dy1, r.z0, r.dz1};
+    const std::array<int, 7> key{r.x0, r.dx1, r.y0, r.dy1, r.z0, r.dz1, order};
     auto it = gp.tabs.find({subset, key});
     if (it != gp.tabs.end()) return it->second;
     Group::Tab t;
@@ -1197,11 +1280,13 @@ Group::Tab &table(Group &gp, int subset, const RSpec &r)
                    nzr = (long)G.nz + r.dz1 - r.z0 + 1;
         boff.push_back((int)total);
         if (nxr > 0 && nyr > 0 && nzr > 0) {
-            total += nxr * ((nyr * nzr + 255) / 256);
+            const long bpp = (nyr * nzr + 255) / 256;
+            total += nxr * (order ? 8 * ((bpp + 7) / 8) : bpp);
             t.cells += (double)(nxr * nyr * nzr);
         }
     }
     t.nblk = (int)total;
+    t.order = order;
     if (!boff.empty()) {
         (void)hipMalloc(&t.boff, boff.size() * sizeof(int));
         (void)hipMemcpyAsync(t.boff, boff.data(), boff.size() * sizeof(int), hipMemcpyHostToDevice, gp.stream);
@@ -1227,9 +1312,9 @@ template <class T, typename... KA, typename... Args>
 void elaunch_(void (*k)(Src<T>, KA...), const char *name, double bpc, Group &gp, int subset, RSpec r,
               Args... args)
 {
-    Group::Tab &t = table(gp, subset, r);
+    Group::Tab &t = table(gp, subset, r, gp.order);
     if (gp.planning || t.nblk == 0) return;
-    Src<T> s{(const EDev<T> *)gp.devs[subset], t.boff, gp.n[subset], t.nblk, r};
+    Src<T> s{(const EDev<T> *)gp.devs[subset], t.boff, gp.n[subset], t.nblk, r, t.order};
     Prof *pf = gp.prof;
     Prof::Rec rec{kname(name), bpc * t.cells, nullptr, nullptr};
     if (pf && pf->on) {
@@ -1464,6 +1549,7 @@ int group_init(Group &gp, const std::vector<Grid *> &members, hipStream_t stream
     gp.mem = members;
     gp.stream = stream;
     gp.prof = prof;
+    if (const char *o = std::getenv("LARE_ORDER")) gp.order = std::atoi(o) != 0;
     if (members.empty()) return 0;
     gp.fp32 = members[0]->fp32;
     for (size_t m = 0; m < members.size(); ++m) {
@@ -1570,16 +1656,16 @@ int group_energy(Group &gp, std::vector<double> &out, std::string &err)
     out.assign(4 * gp.mem.size(), 0.0);
     if (gp.mem.empty()) return 0;
     const RSpec in1 = rs(1, 0, 1, 0, 1, 0);
-    Group::Tab &t = table(gp, 0, in1);
+    Group::Tab &t = table(gp, 0, in1, 0);
     double *part = nullptr, *res = nullptr;
     hipError_t e = hipMalloc(&part, (size_t)std::max(t.nblk, 1) * 3 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&res, out.size() * sizeof(double));
     if (e == hipSuccess) {
         if (gp.fp32) {
-            Src<float> s{(const EDev<float> *)gp.devs[0], t.boff, gp.n[0], t.nblk, in1};
+            Src<float> s{(const EDev<float> *)gp.devs[0], t.boff, gp.n[0], t.nblk, in1, 0};
             hipLaunchKernelGGL(e_energy_part<float>, dim3((unsigned)t.nblk), dim3(256), 0, gp.stream, s, part);
         } else {
-            Src<double> s{(const EDev<double> *)gp.devs[0], t.boff, gp.n[0], t.nblk, in1};
+            Src<double> s{(const EDev<double> *)gp.devs[0], t.boff, gp.n[0], t.nblk, in1, 0};
             hipLaunchKernelGGL(e_energy_part<double>, dim3((unsigned)t.nblk), dim3(256), 0, gp.stream, s, part);
         }
         hipLaunchKernelGGL(e_energy_sum, dim3((unsigned)gp.n[0]), dim3(64), 0, gp.stream, (const double *)part,
