@@ -176,7 +176,7 @@ def kernel_roofline(dev, nsteps, cells, ms_step):
     name, (calls, tot_ms, tot_bytes) = dom
     avg_ms = tot_ms / calls
     achieved = (tot_bytes / calls) / (avg_ms * 1e-3) / 1e9
-    for k, (c, t, b) in sorted(prof.items(), key=lambda kv: -kv[1][1])[:12]:
+    for k, (c, t, b) in sorted(prof.items(), key=lambda kv: -kv[1][1])[:60]:
         log(f"  {k:20s} calls {c:4d} avg {t / c * 1e3:9.1f} us  share {t / sum(v[1] for v in prof.values()):6.1%}"
             f"  {b / c / (t / c * 1e-3) / 1e9 if t > 0 else 0:8.1f} GB/s")
     step_bytes = STEP_BYTES_RESISTIVE * cells

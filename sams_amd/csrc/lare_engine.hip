@@ -1010,38 +1010,30 @@ KERNEL void g_rho_e_update(const Src<T> S)
 
 // ------------------------------------------------- resistive substeps
 
-// eta_calc (lagran.cpp:445-476) over (-1..n+1)^3 with the copy of B into
-// bx1..bz1 over the centred extent: the reference's pw::assign(data.bx1,
-// data.bx) at lagran.cpp:482 throws on the extent mismatch; this is its
-// element-wise reading over bx1's extent.
+// The copy of B into bx1..bz1 over the centred extent at the start of a
+// resistive substep: the reference's pw::assign(data.bx1, data.bx) at
+// lagran.cpp:482 throws on the extent mismatch; this is its element-wise
+// reading over bx1's extent.  Only the last substep's copy is read later (by
+// the predictor): within a substep bstep reads bx1 only at the cell it
+// writes, where it still equals B, so earlier substeps skip the copy.
 template <class T>
-KERNEL void r_eta_copy(const Src<T> S)
+KERNEL void r_copy(const Src<T> S)
 {
     EGRID
     auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
-    const int sx = G.sx, sy = G.sy;
     A.bx1[c] = BX[c];
     A.by1[c] = BY[c];
     A.bz1[c] = BZ[c];
-    if (ix > G.nx + 1 || iy > G.ny + 1 || iz > G.nz + 1) return;
-    double dxc = DXC(ix), dyc = DYC(iy), dzc = DZC(iz);
-    double jx = (BZ[c + sy] - BZ[c]) / dyc - (BY[c + 1] - BY[c]) / dzc;
-    double jxp = (BZ[c + sx + sy] - BZ[c + sx]) / dyc - (BY[c + sx + 1] - BY[c + sx]) / dzc;
-    double jy = (BX[c + 1] - BX[c]) / dzc - (BZ[c + sx] - BZ[c]) / dxc;
-    double jyp = (BX[c + sy + 1] - BX[c + sy]) / dzc - (BZ[c + sx + sy] - BZ[c + sy]) / dxc;
-    double jz = (BY[c + sx] - BY[c]) / dxc - (BX[c + sy] - BX[c]) / dyc;
-    double jzp = (BY[c + sx + 1] - BY[c + 1]) / dxc - (BX[c + sy + 1] - BX[c + 1]) / dyc;
-    jx = (jx + jxp) * 0.5;
-    jy = (jy + jyp) * 0.5;
-    jz = (jz + jzp) * 0.5;
-    double j_local = sqrt(jx * jx + jy * jy + jz * jz);
-    A.eta[c] = j_local > P.j_max ? P.eta_background + P.eta0 : P.eta_background;
     };
     ELOOP(cell)
 }
 
-// rkstep (lagran.cpp:507-535) over (0..n)^3.
+// eta_calc (lagran.cpp:445-476) and rkstep (lagran.cpp:507-535) over
+// (0..n)^3.  rkstep reads eta only at its own cell, and eta_calc's current
+// there is rkstep's own (the same differences; (j + jp) * 0.5 and
+// 0.5 * (j1 + j2) are the same product), so eta is computed in place rather
+// than stored over eta_calc's wider range (-1..n+1), which nothing else reads.
 template <class T>
 KERNEL void r_rkstep(const Src<T> S)
 {
@@ -1059,7 +1051,8 @@ KERNEL void r_rkstep(const Src<T> S)
     double jx = 0.5 * (jx1 + jx2);
     double jy = 0.5 * (jy1 + jy2);
     double jz = 0.5 * (jz1 + jz2);
-    double eta = A.eta[c];
+    double j_local = sqrt(jx * jx + jy * jy + jz * jz);
+    double eta = j_local > P.j_max ? P.eta_background + P.eta0 : P.eta_background;
     A.flux_x[c] = -jx * eta * dxc * 0.5;
     A.flux_y[c] = -jy * eta * dyc * 0.5;
     A.flux_z[c] = -jz * eta * dzc * 0.5;
@@ -1071,7 +1064,8 @@ KERNEL void r_rkstep(const Src<T> S)
 // bstep's three components (lagran.cpp:537-590) with the Ohmic heating of
 // lagran.cpp:490-500, over (0..n)^3: both read only rkstep's outputs; the B
 // fill between them in the reference touches neither curlb, rho nor
-// energy_electron.
+// energy_electron.  bstep's bx1 (B at the substep's start) is read from B at
+// the same cell before it is overwritten.
 template <class T>
 KERNEL void r_b_energy(const Src<T> S)
 {
@@ -1084,15 +1078,15 @@ KERNEL void r_b_energy(const Src<T> S)
     const double *fx = A.flux_x, *fy = A.flux_y, *fz = A.flux_z;
     if (iy >= 1 && iz >= 1) {
         double area = DYB(iy) * DZB(iz);
-        BX[c] = A.bx1[c] + (fz[c] - fz[c - sy] + fz[c - 1] - fz[c - sy - 1] - fy[c] + fy[c - 1] - fy[c - sy] + fy[c - sy - 1]) * dt / area;
+        BX[c] = BX[c] + (fz[c] - fz[c - sy] + fz[c - 1] - fz[c - sy - 1] - fy[c] + fy[c - 1] - fy[c - sy] + fy[c - sy - 1]) * dt / area;
     }
     if (ix >= 1 && iz >= 1) {
         double area = DXB(ix) * DZB(iz);
-        BY[c] = A.by1[c] + (fx[c] - fx[c - 1] + fx[c - sx] - fx[c - sx - 1] - fz[c] + fz[c - sx] - fz[c - 1] + fz[c - sx - 1]) * dt / area;
+        BY[c] = BY[c] + (fx[c] - fx[c - 1] + fx[c - sx] - fx[c - sx - 1] - fz[c] + fz[c - sx] - fz[c - 1] + fz[c - sx - 1]) * dt / area;
     }
     if (ix >= 1 && iy >= 1) {
         double area = DXB(ix) * DYB(iy);
-        BZ[c] = A.bz1[c] + (fy[c] - fy[c - sx] + fy[c - sy] - fy[c - sx - sy] - fx[c] + fx[c - sy] - fx[c - sx] + fx[c - sx - sy]) * dt / area;
+        BZ[c] = BZ[c] + (fy[c] - fy[c - sx] + fy[c - sy] - fy[c - sx - sy] - fx[c] + fx[c - sy] - fx[c - sx] + fx[c - sx - sy]) * dt / area;
     }
     if (ix >= 1 && iy >= 1 && iz >= 1 && ix <= nx && iy <= ny && iz <= nz) {
         const Fld<T> cb = A.curlb;
@@ -1478,9 +1472,9 @@ void start_b_t(Group &gp, int substeps)
     const RSpec C = rs(-1, 2, -1, 2, -1, 2);
     if (gp.n[1] > 0) {
         for (int i = 0; i < substeps; ++i) {
-            EL(r_eta_copy<T>, 7 * w, gp, 1, C);
-            EL(r_rkstep<T>, 4 * w + 4 * 8, gp, 1, rs(0, 0, 0, 0, 0, 0));
-            EL(r_b_energy<T>, 11 * w + 3 * 8, gp, 1, rs(0, 0, 0, 0, 0, 0));
+            if (i == substeps - 1) EL(r_copy<T>, 6 * w, gp, 1, C);
+            EL(r_rkstep<T>, 4 * w + 3 * 8, gp, 1, rs(0, 0, 0, 0, 0, 0));
+            EL(r_b_energy<T>, 10 * w + 3 * 8, gp, 1, rs(0, 0, 0, 0, 0, 0));
             ebcs<T>(gp, VBE, -1, 1);
         }
     }
