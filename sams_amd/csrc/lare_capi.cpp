@@ -193,8 +193,8 @@ void make_axis(long n, double mn, double mx, std::vector<double> &db, std::vecto
 
 namespace {
 
-// Engine arrays (lare::ArrE) of one grid in storage type T: 34 T arrays and
-// 10 FP64 flux arrays carved from the pool at p.
+// Engine arrays (lare::ArrE) of one grid in storage type T: 39 T arrays and
+// 6 FP64 flux arrays carved from the pool at p.
 template <class T>
 void carve_engine(lare::ArrE<T> &a, char *&p, size_t box)
 {
@@ -211,12 +211,12 @@ void carve_engine(lare::ArrE<T> &a, char *&p, size_t box)
     lare::Fld<T> *ts[] = {&a.rho, &a.ee, &a.ei, &a.vx, &a.vy, &a.vz, &a.bx, &a.by, &a.bz, &a.vx1, &a.vy1,
                           &a.vz1, &a.cs, &a.cs_v, &a.rho_v, &a.alpha1, &a.alpha2, &a.alpha3, &a.visc_heat,
                           &a.fxv, &a.fyv, &a.fzv, &a.cv1, &a.bx1, &a.by1, &a.bz1, &a.p_e, &a.p_i, &a.eta,
-                          &a.curlb, &a.cv2, &a.cvc1, &a.db1, &a.rrho_v};
+                          &a.curlb, &a.cv2, &a.cvc1, &a.db1, &a.rho2, &a.ee2, &a.ei2, &a.vx2, &a.vy2, &a.vz2};
     for (auto *q : ts) *q = tk();
-    double **ds[] = {&a.dm, &a.flux_x, &a.flux_y, &a.flux_z, &a.dmv, &a.fl[0], &a.fl[1], &a.fl[2], &a.fl[3], &a.fl[4]};
+    double **ds[] = {&a.dm, &a.flux_x, &a.flux_y, &a.flux_z, &a.fl[0], &a.fl[1]};
     for (double **q : ds) *q = dk();
 }
-const int kEngineT = 34, kEngineD = 10;
+const int kEngineT = 39, kEngineD = 6;
 
 // The one-kernel-per-reference-kernel set's view of the FP64 engine arrays.
 void alias_set0(lare::Arr &a, const lare::ArrE<double> &e)
@@ -248,15 +248,13 @@ void alias_set0(lare::Arr &a, const lare::ArrE<double> &e)
     a.cv2 = e.cv2.p;
     a.cvc1 = e.cvc1.p;
     a.db1 = e.db1.p;
-    a.rrho_v = e.rrho_v.p;
-    a.dmv = e.dmv;
-    for (int k = 0; k < 5; ++k) a.fl[k] = e.fl[k];
+    for (int k = 0; k < 2; ++k) a.fl[k] = e.fl[k];
 }
 
 // Arrays read only by the one-kernel-per-reference-kernel set (the stored
-// metric, cv, pressure, cv_v, the forces, p_visc and the remap copies),
-// allocated when that set is first selected.
-const int kSet0Arrays = 18;
+// metric, cv, pressure, cv_v, the forces, p_visc and the remap
+// intermediates), allocated when that set is first selected.
+const int kSet0Arrays = 20;
 int ensure_set0(lare_ctx *c)
 {
     if (c->pool0) return 0;
@@ -266,8 +264,9 @@ int ensure_set0(lare_ctx *c)
     HIPCHK(hipMalloc(&c->pool0, kSet0Arrays * bytes3));
     char *p = (char *)c->pool0;
     lare::Arr &a = c->g.dev.a;
-    double **q[kSet0Arrays] = {&a.p_visc, &a.cv, &a.pressure, &a.cv_v, &a.fx, &a.fy, &a.fz, &a.dxab, &a.dyab,
-                               &a.dzab, &a.dxac, &a.dyac, &a.dzac, &a.rho1, &a.rho_v1, &a.rflux, &a.cv2v, &a.wsh};
+    double **q[kSet0Arrays] = {&a.p_visc, &a.cv,   &a.pressure, &a.cv_v,  &a.fx,   &a.fy,     &a.fz,
+                               &a.dxab,   &a.dyab, &a.dzab,     &a.dxac,  &a.dyac, &a.dzac,   &a.rho1,
+                               &a.rho_v1, &a.rflux, &a.cv2v,    &a.wsh,   &a.rrho_v, &a.dmv};
     for (double **r : q) {
         *r = (double *)p;
         p += bytes3;
@@ -433,14 +432,20 @@ extern "C" int lare_field_bounds(const lare_ctx *c, int field, int64_t lb[3], in
 
 namespace {
 
+// Device array of field f where it currently lives (rho, e_e, e_i and v are
+// in the engine's second state copy between the corrector and the remap).
+template <class T>
+void *field_dev_ptr_t(const lare::ArrE<T> &a, int f, int st)
+{
+    const lare::Fld<T> fs[12] = {a.rho, a.ee, a.ei, a.vx, a.vy, a.vz, a.bx, a.by, a.bz, a.vx1, a.vy1, a.vz1};
+    const lare::Fld<T> s2[6] = {a.rho2, a.ee2, a.ei2, a.vx2, a.vy2, a.vz2};
+    if (f == LARE_DM) return (void *)a.dm;
+    return (st && f < 6) ? (void *)s2[f].p : (void *)fs[f].p;
+}
 void *field_dev_ptr(lare_ctx *c, int f)
 {
-    if (c->g.fp32) {
-        const lare::ArrE<float> &a = c->g.e32.a;
-        const lare::Fld<float> fs[12] = {a.rho, a.ee, a.ei, a.vx, a.vy, a.vz, a.bx, a.by, a.bz, a.vx1, a.vy1, a.vz1};
-        return f == LARE_DM ? (void *)a.dm : (void *)fs[f].p;
-    }
-    return c->g.dev.a.f[f];
+    if (c->g.fp32) return field_dev_ptr_t(c->g.e32.a, f, c->g.st);
+    return field_dev_ptr_t(c->g.e64.a, f, c->g.st);
 }
 
 // host array (reference extent) <-> device box: one contiguous copy through
@@ -682,6 +687,7 @@ extern "C" int lare_set_kernel_set(lare_ctx *c, int32_t kernel_set)
     if (!c) return fail(-1, "lare_set_kernel_set: null context");
     if (kernel_set < 0 || kernel_set > 1) return fail(-1, "lare_set_kernel_set: unknown kernel set");
     HIPCHK(hipStreamSynchronize(c->g.stream));
+    if (c->g.st != 0) return fail(-1, "lare_set_kernel_set: only between steps");
     if (kernel_set == 0) {
         int rc = ensure_set0(c);
         if (rc) return rc;

@@ -110,14 +110,19 @@ struct ArrE {
     // registered variables (dm is a flux: FP64)
     Fld<T> rho, ee, ei, vx, vy, vz, bx, by, bz, vx1, vy1, vz1;
     double *dm;
+    // second copy of rho, e_e, e_i and v: the corrector and each remap sweep
+    // read one copy and write the other (corrector A -> B, x B -> A,
+    // y A -> B, z B -> A), so a kernel may read neighbours' old values
+    // while it writes new ones
+    Fld<T> rho2, ee2, ei2, vx2, vy2, vz2;
     // Lagrangian phase
     Fld<T> cs, cs_v, rho_v, alpha1, alpha2, alpha3, visc_heat, fxv, fyv, fzv;
     Fld<T> cv1, bx1, by1, bz1, p_e, p_i, eta, curlb;
     double *flux_x, *flux_y, *flux_z;
-    // remap
-    Fld<T> cv2, cvc1, db1, rrho_v;
-    double *dmv;
-    double *fl[5];
+    // remap: cell volumes, vertex volume, Lagrangian width, the two
+    // constrained-transport fluxes
+    Fld<T> cv2, cvc1, db1;
+    double *fl[2];
     // 1D axis deltas, index i stored at [i + 2]
     double *dxb, *dyb, *dzb, *dxc, *dyc, *dzc;
     Scal *s;
@@ -141,6 +146,7 @@ struct Src {
     int nblk;
     RSpec r;
     int order;  // 0: chunk fastest, then plane; 1: chunks in 8 bands, band-major
+    int st;     // 1: the state is in the second copy (rho2 ...); kernels see it as rho ...
 };
 
 // Built-in boundary conditions of one variable along one dimension, for one
@@ -148,6 +154,7 @@ struct Src {
 // in an FP32-storage grid.
 struct BCVarE {
     void *p;
+    void *src;  // the other state copy, for a variable just written into p's copy
     int f64;
     int sd, sa, sb;
     int k0, k1;

@@ -73,6 +73,30 @@ __device__ __forceinline__ int xcd_swizzle(int b, int nblk)
     return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + k;
 }
 
+// The arrays with the state copies named for this launch: rho .. vz the
+// copy holding the current state (the second copy when st is 1), rho2 ..
+// vz2 the other one.
+template <class T>
+__device__ __forceinline__ ArrE<T> state_view(const ArrE<T> &a, int st)
+{
+    ArrE<T> v = a;
+    if (st) {
+        v.rho = a.rho2;
+        v.ee = a.ee2;
+        v.ei = a.ei2;
+        v.vx = a.vx2;
+        v.vy = a.vy2;
+        v.vz = a.vz2;
+        v.rho2 = a.rho;
+        v.ee2 = a.ee;
+        v.ei2 = a.ei;
+        v.vx2 = a.vx;
+        v.vy2 = a.vy;
+        v.vz2 = a.vz;
+    }
+    return v;
+}
+
 // Largest g with boff[g] <= b (boff ascending, boff[0] = 0).
 __device__ __forceinline__ int grid_of(const int *boff, int n, int b)
 {
@@ -99,7 +123,7 @@ __device__ __forceinline__ int grid_of(const int *boff, int n, int b)
     const int g_ = grid_of(S.boff, S.ngrid, b_);                                                   \
     const EDev<T> &D_ = S.devs[g_];                                                                \
     const Geo G = D_.g;                                                                            \
-    const ArrE<T> A = D_.a;                                                                        \
+    const ArrE<T> A = state_view(D_.a, S.st);                                                      \
     const Phys P = D_.p;                                                                           \
     const R3 R = {S.r.x0, G.nx + S.r.dx1, S.r.y0, G.ny + S.r.dy1, S.r.z0, G.nz + S.r.dz1};         \
     const int nzr_ = R.z1 - R.z0 + 1;                                                              \
@@ -388,46 +412,17 @@ __device__ __forceinline__ double energy_flux(const ArrE<T> &A, const Geo &G, in
     return (fu + Di * (1.0 - dmu)) * dm;
 }
 
-// R6: both energy fluxes on faces (0..n)^3, the vertex density before remap
-// (range -1..n+1 along DIR, 0..n across) and the vertex mass flux dmv
-// (range -1..n along DIR, 0..n across).  All read only pre-update values.
-template <class T, int DIR>
-KERNEL void f_energy_vertex(const Src<T> S)
+// Energy flux of E through cell face f = (fx, fy, fz) along DIR with the
+// face's own sweep velocity and area (f_energy: the same v_advect and o_v as
+// the mass flux of that face).
+template <int DIR, class T>
+__device__ __forceinline__ double energy_flux_face(const ArrE<T> &A, const Geo &G, int f, int fx, int fy, int fz,
+                                                   const Fld<T> E, double dt)
 {
-    EGRID
-    auto cell = [&](const int ix, const int iy, const int iz) {
-    const int c = IX(ix, iy, iz);
-    const int sx = G.sx, sy = G.sy;
-    const int nx = G.nx, ny = G.ny, nz = G.nz;
-    const int id = DIR == 0 ? ix : DIR == 1 ? iy : iz;
-    const int nd = DIR == 0 ? nx : DIR == 1 ? ny : nz;
-    const bool face = id >= 0 && id <= nd;
-    if (face) {
-        const double dt = A.s->dt;
-        double area = face_area<DIR>(A, ix, iy, iz);
-        double v_advect = face_v<DIR>(A, G, c);
-        double o_v = v_advect * dt * area;
-        A.fl[0][c] = energy_flux<DIR>(A, G, c, EE, v_advect, o_v);
-        A.fl[1][c] = energy_flux<DIR>(A, G, c, EI, v_advect, o_v);
-    }
-    // vertex density before remap: x divides (s*0.125)/cvc1, y and z
-    // multiply by (0.125/cvc1) (xremap.cpp:159-176, yremap.cpp:163-183)
-    const Fld<T> R1 = RHO, C1 = A.cv1;
-    double s = R1[c] * C1[c] + R1[c + sx] * C1[c + sx] + R1[c + sy] * C1[c + sy] +
-               R1[c + sx + sy] * C1[c + sx + sy] + R1[c + 1] * C1[c + 1] + R1[c + sx + 1] * C1[c + sx + 1] +
-               R1[c + sy + 1] * C1[c + sy + 1] + R1[c + sx + sy + 1] * C1[c + sx + sy + 1];
-    if (DIR == 0)
-        A.rrho_v[c] = s * 0.125 / A.cvc1[c];
-    else
-        A.rrho_v[c] = s * (0.125 / A.cvc1[c]);
-    if (id <= nd) {
-        const double *M = DM;
-        double q = M[c] + M[c + sx] + M[c + sy] + M[c + sx + sy] + M[c + 1] + M[c + sx + 1] + M[c + sy + 1] +
-                   M[c + sx + sy + 1];
-        A.dmv[c] = DIR == 0 ? q * 0.125 : 0.125 * q;
-    }
-    };
-    ELOOP(cell)
+    double area = face_area<DIR>(A, fx, fy, fz);
+    double v_advect = face_v<DIR>(A, G, f);
+    double o_v = v_advect * dt * area;
+    return energy_flux<DIR>(A, G, f, E, v_advect, o_v);
 }
 
 // Momentum flux of velocity component f on vertex face c (x_mom_flux,
@@ -451,64 +446,127 @@ __device__ __forceinline__ double mom_flux(const Geo &G, int c, const Fld<T> f, 
     return fl * dm;
 }
 
-// R7: the three momentum fluxes over (-1..n along DIR, 0..n across).
-template <class T, int DIR>
-KERNEL void f_mom_fluxes(const Src<T> S)
+// Vertex density before remap at vertex c (xremap.cpp:159-176,
+// yremap.cpp:163-183, zremap.cpp:160-177): x divides (s*0.125)/cvc1, y and z
+// multiply by (0.125/cvc1).
+template <int DIR, class T>
+__device__ __forceinline__ double rrho_v_at(const ArrE<T> &A, const Geo &G, int c)
 {
-    EGRID
-    auto cell = [&](const int ix, const int iy, const int iz) {
-    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    const Fld<T> R1 = RHO, C1 = A.cv1;
+    double s = R1[c] * C1[c] + R1[c + sx] * C1[c + sx] + R1[c + sy] * C1[c + sy] +
+               R1[c + sx + sy] * C1[c + sx + sy] + R1[c + 1] * C1[c + 1] + R1[c + sx + 1] * C1[c + sx + 1] +
+               R1[c + sy + 1] * C1[c + sy + 1] + R1[c + sx + sy + 1] * C1[c + sx + sy + 1];
+    if (DIR == 0) return s * 0.125 / A.cvc1[c];
+    return s * (0.125 / A.cvc1[c]);
+}
+
+// Vertex mass flux at vertex c (xremap.cpp:209-219 and twins).
+template <int DIR, class T>
+__device__ __forceinline__ double dmv_at(const ArrE<T> &A, const Geo &G, int c)
+{
+    const int sx = G.sx, sy = G.sy;
+    const double *M = DM;
+    double q = M[c] + M[c + sx] + M[c + sy] + M[c + sx + sy] + M[c + 1] + M[c + sx + 1] + M[c + sy + 1] +
+               M[c + sx + sy + 1];
+    return DIR == 0 ? q * 0.125 : 0.125 * q;
+}
+
+// The three momentum fluxes through vertex face f = (fx, fy, fz) along DIR
+// (x_mom_flux, xremap.cpp:485-566 and twins), given the vertex densities
+// at f and f + sd.  z takes dyac as its area (zremap.cpp:475).
+template <int DIR, class T>
+__device__ __forceinline__ void mom_fluxes_at(const ArrE<T> &A, const Geo &G, int f, int fx, int fy, int fz,
+                                              double rv0, double rv1, double dm, double dt, double out[3])
+{
     const int sd = sdir<DIR>(G);
-    const double dt = A.s->dt;
     const Fld<T> w = DIR == 0 ? VX1 : DIR == 1 ? VY1 : VZ1;
-    double area = DIR == 0 ? DXAC(iy, iz) : DYAC(ix, iz);
-    double v_advect = (w[c] + w[c + sd]) * 0.5;
+    double area = DIR == 0 ? DXAC(fy, fz) : DYAC(fx, fz);
+    double v_advect = (w[f] + w[f + sd]) * 0.5;
     double o_v = v_advect * dt * area;
     double sign_v = (o_v >= 0.0) ? 1.0 : -1.0;
     double vad_p = (sign_v + 1.0) * 0.5;
     double vad_m = 1.0 - vad_p;
-    double dxci = A.cv1[c + sd];
-    double dxcu = A.cv1[c] * vad_p + A.cv1[c + 2 * sd] * vad_m;
-    double dxbu = A.cvc1[c] * vad_p + A.cvc1[c + sd] * vad_m;
-    double rhou = A.rrho_v[c] * vad_p + A.rrho_v[c + sd] * vad_m;
-    double dm = A.dmv[c];
-    A.fl[2][c] = mom_flux<DIR>(G, c, VX, o_v, sign_v, dxci, dxcu, dxbu, rhou, dm);
-    A.fl[3][c] = mom_flux<DIR>(G, c, VY, o_v, sign_v, dxci, dxcu, dxbu, rhou, dm);
-    A.fl[4][c] = mom_flux<DIR>(G, c, VZ, o_v, sign_v, dxci, dxcu, dxbu, rhou, dm);
-    };
-    ELOOP(cell)
+    double dxci = A.cv1[f + sd];
+    double dxcu = A.cv1[f] * vad_p + A.cv1[f + 2 * sd] * vad_m;
+    double dxbu = A.cvc1[f] * vad_p + A.cvc1[f + sd] * vad_m;
+    double rhou = rv0 * vad_p + rv1 * vad_m;
+    out[0] = mom_flux<DIR>(G, f, VX, o_v, sign_v, dxci, dxcu, dxbu, rhou, dm);
+    out[1] = mom_flux<DIR>(G, f, VY, o_v, sign_v, dxci, dxcu, dxbu, rhou, dm);
+    out[2] = mom_flux<DIR>(G, f, VZ, o_v, sign_v, dxci, dxcu, dxbu, rhou, dm);
 }
 
-// R8: the in-place updates over (0..n)^3: density and both energies on cells
-// 1..n (xremap.cpp:121-154), vertex density after remap and the three
-// velocities on vertices 0..n (xremap.cpp:180-262).
+// R6 over (0..n)^3: the whole transport of the sweep from the current state
+// copy into the other.  Each lane computes the fluxes of both faces of its
+// cell along DIR itself (the reference stores them and the next kernel
+// reads two of them; the values are the same expressions on the same
+// inputs): on cells 1..n the mass (dm, stored), energy fluxes and the
+// density and energy update (xremap.cpp:121-154 and x_energy_flux
+// 429-483); on vertices 0..n the vertex densities, vertex mass fluxes,
+// momentum fluxes and the velocity update (xremap.cpp:159-262, x_mom_flux
+// 485-566).  Nothing here is written where anything is read, so no lane
+// sees another's new values.
 template <class T, int DIR>
-KERNEL void f_update(const Src<T> S)
+KERNEL void f_update_all(const Src<T> S)
 {
     EGRID
     auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     const int sd = sdir<DIR>(G);
+    const double dt = A.s->dt;
+    const int mx = DIR == 0 ? ix - 1 : ix, my = DIR == 1 ? iy - 1 : iy, mz = DIR == 2 ? iz - 1 : iz;
+    // vertex quantities at c - sd, c, c + sd
+    const double rvm = rrho_v_at<DIR>(A, G, c - sd), rv = rrho_v_at<DIR>(A, G, c), rvp = rrho_v_at<DIR>(A, G, c + sd);
+    const double dmvm = dmv_at<DIR>(A, G, c - sd), dmv = dmv_at<DIR>(A, G, c);
+    double flm[3], flc[3];
+    mom_fluxes_at<DIR>(A, G, c - sd, mx, my, mz, rvm, rv, dmvm, dt, flm);
+    mom_fluxes_at<DIR>(A, G, c, ix, iy, iz, rv, rvp, dmv, dt, flc);
     const Fld<T> C2 = A.cv2;
-    // vertex cv2 (cv2 is not written here)
+    // vertex cv2
     double s2 = C2[c] + C2[c + sx] + C2[c + sy] + C2[c + sx + sy] + C2[c + 1] + C2[c + sx + 1] + C2[c + sy + 1] +
                 C2[c + sx + sy + 1];
     const double cv2v = DIR == 0 ? s2 * 0.125 : 0.125 * s2;
     const double cvc1 = A.cvc1[c];
-    const double rv = A.rrho_v[c];
-    const double rho_v1 = (rv * cvc1 + A.dmv[c - sd] - A.dmv[c]) / cv2v;
+    const double rho_v1 = (rv * cvc1 + dmvm - dmv) / cv2v;
     const double den = cv2v * rho_v1;
-    VX[c] = (rv * VX[c] * cvc1 + A.fl[2][c - sd] - A.fl[2][c]) / den;
-    VY[c] = (rv * VY[c] * cvc1 + A.fl[3][c - sd] - A.fl[3][c]) / den;
-    VZ[c] = (rv * VZ[c] * cvc1 + A.fl[4][c - sd] - A.fl[4][c]) / den;
+    A.vx2[c] = (rv * VX[c] * cvc1 + flm[0] - flc[0]) / den;
+    A.vy2[c] = (rv * VY[c] * cvc1 + flm[1] - flc[1]) / den;
+    A.vz2[c] = (rv * VZ[c] * cvc1 + flm[2] - flc[2]) / den;
     if (ix >= 1 && iy >= 1 && iz >= 1) {
+        const double em[2] = {energy_flux_face<DIR>(A, G, c - sd, mx, my, mz, EE, dt),
+                              energy_flux_face<DIR>(A, G, c - sd, mx, my, mz, EI, dt)};
+        const double ec[2] = {energy_flux_face<DIR>(A, G, c, ix, iy, iz, EE, dt),
+                              energy_flux_face<DIR>(A, G, c, ix, iy, iz, EI, dt)};
         const double rho1 = RHO[c], cv1 = A.cv1[c], cv2 = C2[c];
         const double rho = (rho1 * cv1 + DM[c - sd] - DM[c]) / cv2;
-        RHO[c] = rho;
-        EE[c] = (EE[c] * cv1 * rho1 + A.fl[0][c - sd] - A.fl[0][c]) / (cv2 * rho);
-        EI[c] = (EI[c] * cv1 * rho1 + A.fl[1][c - sd] - A.fl[1][c]) / (cv2 * rho);
+        A.rho2[c] = rho;
+        A.ee2[c] = (EE[c] * cv1 * rho1 + em[0] - ec[0]) / (cv2 * rho);
+        A.ei2[c] = (EI[c] * cv1 * rho1 + em[1] - ec[1]) / (cv2 * rho);
     }
+    };
+    ELOOP(cell)
+}
+
+// The shifted sweep velocity the reference writes back into the registered
+// v1 component at the end of the sweep's vertex stage (xremap.cpp:193-203,
+// yremap.cpp:197-207, zremap.cpp:194-204), over (-2..n+1 along DIR, 0..n
+// across): BACK = 0 stages (w[c] + w[c+sd]) * 0.5 in fl[0], BACK = 1 copies
+// it into w.  Needed only where a v1 edge along DIR has no boundary
+// condition: otherwise every shifted cell is rewritten (by the next
+// predictor or by remap_v_bcs) before anything but a zero-weighted term of
+// a later sweep reads it.
+template <class T, int DIR, int BACK>
+KERNEL void f_wshift(const Src<T> S)
+{
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
+    const int c = IX(ix, iy, iz);
+    const Fld<T> w = DIR == 0 ? VX1 : DIR == 1 ? VY1 : VZ1;
+    if (BACK)
+        w[c] = A.fl[0][c];
+    else
+        A.fl[0][c] = (w[c] + w[c + sdir<DIR>(G)]) * 0.5;
     };
     ELOOP(cell)
 }
@@ -960,7 +1018,7 @@ KERNEL OCC(3) void g_shock_heating(const Src<T> S)
 }
 
 // C2 over (0..n)^3: velocity correction (lagran.cpp:754-758) with the sum
-// fx_visc + fx stored by g_forces.
+// fx_visc + fx stored by g_forces, into the second state copy.
 template <class T>
 KERNEL void g_vel_update(const Src<T> S)
 {
@@ -969,15 +1027,16 @@ KERNEL void g_vel_update(const Src<T> S)
     const int c = IX(ix, iy, iz);
     const double dt = A.s->dt;
     double rv = A.rho_v[c];
-    VX[c] += dt * A.fxv[c] / rv;
-    VY[c] += dt * A.fyv[c] / rv;
-    VZ[c] += dt * A.fzv[c] / rv;
+    A.vx2[c] = VX[c] + dt * A.fxv[c] / rv;
+    A.vy2[c] = VY[c] + dt * A.fyv[c] / rv;
+    A.vz2[c] = VZ[c] + dt * A.fzv[c] / rv;
     };
     ELOOP(cell)
 }
 
-// C3 over (1..n)^3: density and energy correction (lagran.cpp:762-796).  The
-// cv1 it also writes in the reference is recomputed by the remap before use.
+// C3 over (1..n)^3: density and energy correction (lagran.cpp:762-796), into
+// the second state copy.  The cv1 it also writes in the reference is
+// recomputed by the remap before use.
 template <class T>
 KERNEL void g_rho_e_update(const Src<T> S)
 {
@@ -1001,9 +1060,9 @@ KERNEL void g_rho_e_update(const Src<T> S)
     double dvzdz = (vzb * azz - vzbm * azz) / vol;
     double dv = (dvxdx + dvydy + dvzdz) * dt;
     double rho = RHO[c];
-    EE[c] -= dv * A.p_e[c] / rho;
-    EI[c] += (dt * A.visc_heat[c] - dv * A.p_i[c]) / rho;
-    RHO[c] = rho / (1.0 + dv);
+    A.ee2[c] = EE[c] - dv * A.p_e[c] / rho;
+    A.ei2[c] = EI[c] + (dt * A.visc_heat[c] - dv * A.p_i[c]) / rho;
+    A.rho2[c] = rho / (1.0 + dv);
     };
     ELOOP(cell)
 }
@@ -1147,7 +1206,7 @@ KERNEL void e_bc(const EDev<T> *devs, const lare::BCSetE *sets, const int *boff,
     const int vi = lb / bpl;
     if (vi >= nvar) return;
     const lare::BCVarE V = sets[g].v[vi];
-    if (V.k0 == 0 && V.k1 == 0) return;
+    if (V.k0 == 0 && V.k1 == 0 && !V.src) return;
     const int la = V.sa ? -2 : -1, ua = na + 2;
     const int lbb = V.sb ? -2 : -1, ub = nb + 2;
     const int wb = ub - lbb + 1;
@@ -1167,6 +1226,22 @@ KERNEL void e_bc(const EDev<T> *devs, const lare::BCSetE *sets, const int *boff,
         c0 = IX(ia, ib, 0);
         sd = 1;
         n = G.nz;
+    }
+    if (V.src) {
+        // The variable was just written into the other state copy over its
+        // update range (1..n centred, 0..n staggered, per dimension).  Its
+        // remaining cells take the values of the copy it was read from, so
+        // that the fills below see what an in-place update would have left.
+        const int loa = V.sa ? 0 : 1, lob = V.sb ? 0 : 1, lo = V.sd ? 0 : 1;
+        const bool shell = ia < loa || ia > na || ib < lob || ib > nb;
+        T *q = (T *)V.p;
+        const T *r = (const T *)V.src;
+        if (shell) {
+            for (int g = V.sd ? -2 : -1; g <= n + 2; ++g) q[c0 + g * sd] = r[c0 + g * sd];
+        } else {
+            for (int g = V.sd ? -2 : -1; g < lo; ++g) q[c0 + g * sd] = r[c0 + g * sd];
+            for (int g = n + 1; g <= n + 2; ++g) q[c0 + g * sd] = r[c0 + g * sd];
+        }
     }
     if (V.f64) {
         bc_edge_e((double *)V.p, c0, sd, n, V.sd, 0, V.k0, V.v0);
@@ -1245,17 +1320,19 @@ const int kStagE[13][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 1, 1}, {1, 1, 1}
 
 inline RSpec rs(int x0, int dx1, int y0, int dy1, int z0, int dz1) { return RSpec{x0, dx1, y0, dy1, z0, dz1}; }
 
-void *field_ptr(const Grid &g, int f)
+// Variable ids of the boundary fills: 0..12 the registered variables (state
+// in the first copy), 13..18 the second copy of rho, e_e, e_i, vx, vy, vz.
+const int kSecond = 13;
+inline int base_var(int v) { return v >= kSecond ? v - kSecond : v; }
+
+template <class T>
+void *field_ptr_t(const ArrE<T> &a, int f)
 {
-    if (g.fp32) {
-        const ArrE<float> &a = g.e32.a;
-        const Fld<float> fs[12] = {a.rho, a.ee, a.ei, a.vx, a.vy, a.vz, a.bx, a.by, a.bz, a.vx1, a.vy1, a.vz1};
-        return f == 12 ? (void *)a.dm : (void *)fs[f].p;
-    }
-    const ArrE<double> &a = g.e64.a;
-    const Fld<double> fs[12] = {a.rho, a.ee, a.ei, a.vx, a.vy, a.vz, a.bx, a.by, a.bz, a.vx1, a.vy1, a.vz1};
+    const Fld<T> fs[19] = {a.rho, a.ee, a.ei, a.vx, a.vy, a.vz, a.bx, a.by, a.bz, a.vx1, a.vy1, a.vz1, {nullptr},
+                           a.rho2, a.ee2, a.ei2, a.vx2, a.vy2, a.vz2};
     return f == 12 ? (void *)a.dm : (void *)fs[f].p;
 }
+void *field_ptr(const Grid &g, int f) { return g.fp32 ? field_ptr_t(g.e32.a, f) : field_ptr_t(g.e64.a, f); }
 
 // Block offsets of a launch over `subset` with range r (built outside graph
 // capture, by group_prepare or on first use).
@@ -1303,12 +1380,12 @@ std::string kname(const char *k)
 }
 
 template <class T, typename... KA, typename... Args>
-void elaunch_(void (*k)(Src<T>, KA...), const char *name, double bpc, Group &gp, int subset, RSpec r,
+void elaunch_(void (*k)(Src<T>, KA...), const char *name, double bpc, Group &gp, int subset, RSpec r, int st,
               Args... args)
 {
     Group::Tab &t = table(gp, subset, r, gp.order);
     if (gp.planning || t.nblk == 0) return;
-    Src<T> s{(const EDev<T> *)gp.devs[subset], t.boff, gp.n[subset], t.nblk, r, t.order};
+    Src<T> s{(const EDev<T> *)gp.devs[subset], t.boff, gp.n[subset], t.nblk, r, t.order, st};
     Prof *pf = gp.prof;
     Prof::Rec rec{kname(name), bpc * t.cells, nullptr, nullptr};
     if (pf && pf->on) {
@@ -1326,7 +1403,9 @@ void elaunch_(void (*k)(Src<T>, KA...), const char *name, double bpc, Group &gp,
         pf->recs.push_back(rec);
     }
 }
-#define EL(k, bpc, gp, subset, r, ...) elaunch_(k, #k, bpc, gp, subset, r, ##__VA_ARGS__)
+// launch on the state copy 0 / on state copy st
+#define EL(k, bpc, gp, subset, r, ...) elaunch_(k, #k, bpc, gp, subset, r, 0, ##__VA_ARGS__)
+#define ELS(k, bpc, gp, subset, r, st, ...) elaunch_(k, #k, bpc, gp, subset, r, st, ##__VA_ARGS__)
 
 // Per-grid kernels (one lane per member).
 template <class T>
@@ -1348,11 +1427,34 @@ void per_grid(void (*k)(const EDev<T> *, int), const char *name, Group &gp)
     }
 }
 
-// Variable groups of the boundary fills (boundary.cpp:27-96).
-enum { VB, VE, VRHO, VV, VV1, VDM, VALL, VERHO, VBE, NVG };
-const int kVars[NVG][9] = {{6, 7, 8},       {1, 2},          {0},    {3, 4, 5}, {9, 10, 11},
-                           {12},            {6, 7, 8, 1, 2, 0, 3, 4, 5},     {1, 2, 0}, {6, 7, 8, 1, 2}};
-const int kNVars[NVG] = {3, 2, 1, 3, 3, 1, 9, 3, 5};
+// Variable groups of the boundary fills (boundary.cpp:27-96): the variables
+// filled and, for a variable just written into the other state copy, the
+// copy it was read from (-1: updated in place).
+enum { VB, VE, VRHO, VV, VV1, VDM, VALL, VERHO, VBE, VV_2, VERHO_2, VALL_2, VALL_1, NVG };
+struct VarGroup {
+    int n;
+    int dst[9];
+    int src[9];
+};
+const VarGroup kGroups[NVG] = {
+    {3, {6, 7, 8}, {-1, -1, -1}},
+    {2, {1, 2}, {-1, -1}},
+    {1, {0}, {-1}},
+    {3, {3, 4, 5}, {-1, -1, -1}},
+    {3, {9, 10, 11}, {-1, -1, -1}},
+    {1, {12}, {-1}},
+    {9, {6, 7, 8, 1, 2, 0, 3, 4, 5}, {-1, -1, -1, -1, -1, -1, -1, -1, -1}},
+    {3, {1, 2, 0}, {-1, -1, -1}},
+    {5, {6, 7, 8, 1, 2}, {-1, -1, -1, -1, -1}},
+    // velocity_bcs after the corrector wrote the second copy
+    {3, {16, 17, 18}, {3, 4, 5}},
+    // energy_bcs + density_bcs after the corrector
+    {3, {14, 15, 13}, {1, 2, 0}},
+    // boundary_conditions after a sweep that wrote the second copy
+    {9, {6, 7, 8, 14, 15, 13, 16, 17, 18}, {-1, -1, -1, 1, 2, 0, 3, 4, 5}},
+    // boundary_conditions after a sweep that wrote the first copy
+    {9, {6, 7, 8, 1, 2, 0, 3, 4, 5}, {-1, -1, -1, 14, 15, 13, 16, 17, 18}},
+};
 
 void drop_btabs(Group &gp)
 {
@@ -1382,23 +1484,32 @@ Group::BTab &btable(Group &gp, int vg, int dim, int subset)
         const Grid &g = *mem[m];
         BCSetE &S = sets[m];
         std::memset(&S, 0, sizeof(S));
-        for (int i = 0; i < kNVars[vg]; ++i) {
-            const int v = kVars[vg][i];
+        for (int i = 0; i < kGroups[vg].n; ++i) {
+            const int v = kGroups[vg].dst[i], u = base_var(v), src = kGroups[vg].src[i];
             BCVarE &b = S.v[i];
             b.p = field_ptr(g, v);
-            b.f64 = (v == 12 || !g.fp32) ? 1 : 0;
-            b.sd = kStagE[v][dim];
-            b.sa = kStagE[v][da];
-            b.sb = kStagE[v][db];
-            b.k0 = g.bck[v][dim][0];
-            b.k1 = g.bck[v][dim][1];
-            b.v0 = g.bcv[v][dim][0];
-            b.v1 = g.bcv[v][dim][1];
-            if (b.k0 || b.k1) t.any = true;
+            // The carry-over of the other copy (first pass only) matters only
+            // for a variable with an edge without a condition: when every
+            // edge has one, each cell outside the update range is rewritten
+            // by the fill of the last dimension in whose ghost layers it
+            // lies, from values that derive from the updated cells alone.
+            bool none_edge = false;
+            for (int d = 0; d < 3; ++d)
+                for (int e = 0; e < 2; ++e) none_edge = none_edge || g.bck[u][d][e] == 0;
+            b.src = (src >= 0 && dim == 0 && none_edge) ? field_ptr(g, src) : nullptr;
+            b.f64 = (u == 12 || !g.fp32) ? 1 : 0;
+            b.sd = kStagE[u][dim];
+            b.sa = kStagE[u][da];
+            b.sb = kStagE[u][db];
+            b.k0 = g.bck[u][dim][0];
+            b.k1 = g.bck[u][dim][1];
+            b.v0 = g.bcv[u][dim][0];
+            b.v1 = g.bcv[u][dim][1];
+            if (b.k0 || b.k1 || b.src) t.any = true;
         }
         const int n3[3] = {g.dev.g.nx, g.dev.g.ny, g.dev.g.nz};
         boff[m] = (int)total;
-        total += (long)kNVars[vg] * (((n3[da] + 5) * (n3[db] + 5) + 255) / 256);
+        total += (long)kGroups[vg].n * (((n3[da] + 5) * (n3[db] + 5) + 255) / 256);
     }
     t.nblk = (int)total;
     if (!mem.empty()) {
@@ -1429,7 +1540,7 @@ void ebcs(Group &gp, int vg, int only_dim = -1, int subset = 0)
         }
         hipLaunchKernelGGL(e_bc<T>, dim3((unsigned)t.nblk), dim3(256), 0, gp.stream,
                            (const EDev<T> *)gp.devs[subset], (const BCSetE *)t.sets, (const int *)t.boff,
-                           gp.n[subset], t.nblk, d, kNVars[vg]);
+                           gp.n[subset], t.nblk, d, kGroups[vg].n);
         if (pf && pf->on) {
             (void)hipEventRecord(rec.b, gp.stream);
             pf->recs.push_back(rec);
@@ -1483,19 +1594,22 @@ void start_b_t(Group &gp, int substeps)
     ebcs<T>(gp, VV1);
 }
 
+// The corrector writes rho, e_e, e_i, v into the second state copy; the
+// x sweep moves them back to the first, y to the second, z to the first.
 template <class T>
 void half_t(Group &gp)
 {
     const double w = (double)sizeof(T);
     EL(g_shock_heating<T>, 10 * w, gp, 0, rs(0, 1, 0, 1, 0, 1));
     EL(g_vel_update<T>, 8 * w, gp, 0, rs(0, 0, 0, 0, 0, 0));
-    ebcs<T>(gp, VV);
+    ebcs<T>(gp, VV_2);
     EL(g_rho_e_update<T>, 12 * w, gp, 0, rs(1, 0, 1, 0, 1, 0));
-    ebcs<T>(gp, VERHO);
+    ebcs<T>(gp, VERHO_2);
+    for (Grid *g : gp.mem) g->st = 1;
 }
 
 template <class T, int DIR>
-void sweep_t(Group &gp, double xpass, double ypass, double zpass)
+void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
 {
     const double w = (double)sizeof(T);
     const RSpec all0 = rs(0, 0, 0, 0, 0, 0);
@@ -1505,17 +1619,25 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass)
     EL((f_ct_fluxes<T, DIR>), 7 * w + 2 * 8, gp, 0, all0);
     EL((f_ct_update<T, DIR>), 6 * w + 2 * 8, gp, 0, all0);
     // mass flux: its reference range, widened to the ghost faces along DIR
-    EL((f_mass_flux<T, DIR>), 4 * w + 8, gp, 0,
-       rs(DIR == 0 ? -1 : 0, DIR == 0 ? 2 : 1, DIR == 1 ? -1 : 0, DIR == 1 ? 2 : 1, DIR == 2 ? -1 : 0,
-          DIR == 2 ? 2 : 1));
+    ELS((f_mass_flux<T, DIR>), 4 * w + 8, gp, 0,
+        rs(DIR == 0 ? -1 : 0, DIR == 0 ? 2 : 1, DIR == 1 ? -1 : 0, DIR == 1 ? 2 : 1, DIR == 2 ? -1 : 0,
+           DIR == 2 ? 2 : 1),
+        st);
     ebcs<T>(gp, VDM, DIR);
-    EL((f_energy_vertex<T, DIR>), 7 * w + 4 * 8, gp, 0,
-       rs(DIR == 0 ? -1 : 0, DIR == 0 ? 1 : 0, DIR == 1 ? -1 : 0, DIR == 1 ? 1 : 0, DIR == 2 ? -1 : 0,
-          DIR == 2 ? 1 : 0));
-    EL((f_mom_fluxes<T, DIR>), 7 * w + 4 * 8, gp, 0, rs(DIR == 0 ? -1 : 0, 0, DIR == 1 ? -1 : 0, 0, DIR == 2 ? -1 : 0, 0));
-    EL((f_update<T, DIR>), 13 * w + 9 * 8, gp, 0, all0);
+    // reads rho e_e e_i v cv1 cv2 cvc1 VX1..VZ1 (sweep component) dm;
+    // writes rho e_e e_i v
+    ELS((f_update_all<T, DIR>), 16 * w + 8, gp, 0, all0, st);
+    bool wshift = false;
+    for (const Grid *g : gp.mem) wshift = wshift || g->bck[9 + DIR][DIR][0] == 0 || g->bck[9 + DIR][DIR][1] == 0;
+    if (wshift) {
+        const RSpec sh = rs(DIR == 0 ? -2 : 0, DIR == 0 ? 1 : 0, DIR == 1 ? -2 : 0, DIR == 1 ? 1 : 0,
+                            DIR == 2 ? -2 : 0, DIR == 2 ? 1 : 0);
+        EL((f_wshift<T, DIR, 0>), w + 8, gp, 0, sh);
+        EL((f_wshift<T, DIR, 1>), w + 8, gp, 0, sh);
+    }
     if (gp.prof) gp.prof->dir = -1;
-    ebcs<T>(gp, VALL);
+    ebcs<T>(gp, st ? VALL_1 : VALL_2);
+    for (Grid *g : gp.mem) g->st = 1 - st;
 }
 
 template <class T>
@@ -1525,9 +1647,9 @@ void end_t(Group &gp)
     EL((f_b_area<T, 0, 0>), 2 * w, gp, 0, rs(-2, 2, -1, 2, -1, 2));
     EL((f_b_area<T, 1, 0>), 2 * w, gp, 0, rs(-1, 2, -2, 2, -1, 2));
     EL((f_b_area<T, 2, 0>), 2 * w, gp, 0, rs(-1, 2, -1, 2, -2, 2));
-    sweep_t<T, 0>(gp, 1.0, 1.0, 1.0);
-    sweep_t<T, 1>(gp, 0.0, 1.0, 1.0);
-    sweep_t<T, 2>(gp, 0.0, 0.0, 1.0);
+    sweep_t<T, 0>(gp, 1.0, 1.0, 1.0, 1);
+    sweep_t<T, 1>(gp, 0.0, 1.0, 1.0, 0);
+    sweep_t<T, 2>(gp, 0.0, 0.0, 1.0, 1);
     EL((f_b_area<T, 0, 1>), 2 * w, gp, 0, rs(-2, 2, -1, 2, -1, 2));
     EL((f_b_area<T, 1, 1>), 2 * w, gp, 0, rs(-1, 2, -2, 2, -1, 2));
     EL((f_b_area<T, 2, 1>), 2 * w, gp, 0, rs(-1, 2, -1, 2, -2, 2));
@@ -1656,10 +1778,10 @@ int group_energy(Group &gp, std::vector<double> &out, std::string &err)
     if (e == hipSuccess) e = hipMalloc(&res, out.size() * sizeof(double));
     if (e == hipSuccess) {
         if (gp.fp32) {
-            Src<float> s{(const EDev<float> *)gp.devs[0], t.boff, gp.n[0], t.nblk, in1, 0};
+            Src<float> s{(const EDev<float> *)gp.devs[0], t.boff, gp.n[0], t.nblk, in1, 0, 0};
             hipLaunchKernelGGL(e_energy_part<float>, dim3((unsigned)t.nblk), dim3(256), 0, gp.stream, s, part);
         } else {
-            Src<double> s{(const EDev<double> *)gp.devs[0], t.boff, gp.n[0], t.nblk, in1, 0};
+            Src<double> s{(const EDev<double> *)gp.devs[0], t.boff, gp.n[0], t.nblk, in1, 0, 0};
             hipLaunchKernelGGL(e_energy_part<double>, dim3((unsigned)t.nblk), dim3(256), 0, gp.stream, s, part);
         }
         hipLaunchKernelGGL(e_energy_sum, dim3((unsigned)gp.n[0]), dim3(64), 0, gp.stream, (const double *)part,

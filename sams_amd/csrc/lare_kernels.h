@@ -63,6 +63,9 @@ struct Grid {
     EDev<double> e64{};
     EDev<float> e32{};
     long bc_version = 0;  // bumped by every boundary-condition change
+    // which copy holds rho, e_e, e_i, v between the engine's events: 0 the
+    // first (between steps), 1 the second (after the corrector)
+    int st = 0;
 };
 
 // Metric arrays from the 1D deltas (LARE3D::grid, setup.cpp:337-390).

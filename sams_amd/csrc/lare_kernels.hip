@@ -891,6 +891,16 @@ KERNEL void k_vshift(Dev D, R3 R, int wcomp, int sd)
     A.wsh[c] = (w[c] + w[c + sd]) * 0.5;
 }
 
+// The shifted sweep velocity written back into v1 (xremap.cpp:203 and
+// twins), over k_vshift's range, after every reader of the unshifted value.
+KERNEL void k_wback(Dev D, R3 R, int wcomp)
+{
+    KBEGIN
+    const int c = IX(ix, iy, iz);
+    double *w = fsel(A, 9 + wcomp);
+    w[c] = A.wsh[c];
+}
+
 // Vertex mass flux (xremap.cpp:209-219 and twins) into dmv.
 KERNEL void k_dm_shift(Dev D, R3 R)
 {
@@ -1181,6 +1191,10 @@ void remap_sweep(Grid &g, int d, double xpass, double ypass, double zpass)
         launch3(k_mom_flux, 64, g, mom, d, comp);
         launch3(k_mom_remap, 56, g, all0, sd, comp);
     }
+    launch3(k_wback, 16, g,
+            rng(d == 0 ? -2 : 0, d == 0 ? nx + 1 : nx, d == 1 ? -2 : 0, d == 1 ? ny + 1 : ny, d == 2 ? -2 : 0,
+                d == 2 ? nz + 1 : nz),
+            d);
     boundary_conditions(g);
 }
 

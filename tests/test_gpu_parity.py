@@ -156,3 +156,23 @@ def test_split_start_of_timestep_matches_whole_event(kernel_set):
             d.half_timestep()
             d.end_of_timestep()
         assert not _state_diffs(split, whole), f"step {s + 1}"
+
+
+@pytest.mark.parametrize("kernel_set", [0, 1])
+def test_edges_without_conditions_match_oracle(kernel_set):
+    """Edges with no boundary condition keep their ghost values (the
+    reference fills nothing there): all z edges, and the upper x edge of rho
+    and energy_ion only."""
+    p = L.problem("ot", n=10, ny=8, nz=9, seed=6, amp=0.03)
+    dev = L.Lare.from_problem(p)
+    dev.set_kernel_set(kernel_set)
+    orc = make_oracle(p)
+    for g in (dev, orc):
+        for f in L.FIELDS:
+            for e in (0, 1):
+                g.set_bc(f, 2, e, L.BC_NONE)
+        for f in ("rho", "energy_ion"):
+            g.set_bc(f, 0, 1, L.BC_NONE)
+    for s in range(3):
+        assert dev.step().hex() == orc.step().hex(), f"dt of step {s + 1}"
+        assert not _state_diffs(dev, orc), f"step {s + 1}"
