@@ -989,64 +989,46 @@ KERNEL void g_forces(const Src<T> S)
     (1 * ((VX[p] - VX[q]) * (VX1[p] - VX1[q]) + (VY[p] - VY[q]) * (VY1[p] - VY1[q]) +               \
           (VZ[p] - VZ[q]) * (VZ1[p] - VZ1[q])))
 
-// C1 over (0..n+1)^3: shock_heating (lagran.cpp:884-993) with the dy*dy
-// factors of a9 and a11 (lagran.cpp:987).
+// C over (0..n)^3: corrector_step (lagran.cpp:739-804) into the second state
+// copy: the velocity correction (754-758, with the sum fx_visc + fx stored by
+// g_forces) on vertices 0..n; on cells 1..n shock_heating (884-993, with the
+// dy*dy factors of a9 and a11 at 987) and the density and energy correction
+// (762-796).  The corrector's visc_heat is read only by the energy
+// correction of the same cell, so it stays in a register; over the rest of
+// shock_heating's range (0..n+1) nothing reads it before the next step's
+// viscosity recomputes it.  The cv1 the reference also writes here is
+// recomputed by the remap before use.
 template <class T>
-KERNEL OCC(3) void g_shock_heating(const Src<T> S)
+KERNEL OCC(3) void g_correct(const Src<T> S)
 {
     EGRID
     auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
-    const int ooo = c, moo = c - sx, omo = c - sy, mmo = c - sx - sy;
-    const int oop = c + 1, mop = c - sx + 1, omp = c - sy + 1, mmp = c - sx - sy + 1;
-    const int oom = c - 1, mom = c - sx - 1, omm = c - sy - 1, mmm = c - sx - sy - 1;
-    double a1 = DOT(mmo, omo), a2 = DOT(omo, ooo), a3 = DOT(ooo, moo), a4 = DOT(moo, mmo);
-    double a5 = DOT(mmp, omp), a6 = DOT(omp, oop), a7 = DOT(oop, mop), a8 = DOT(mop, mmp);
-    double a9 = DOT(oom, ooo), a10 = DOT(mom, moo), a11 = DOT(mmm, mmo), a12 = DOT(omm, omo);
-    double dx = DXB(ix), dy = DYB(iy), dz = DZB(iz);
-    double vh =
-        (-0.25 * dy * dz * A.alpha1[c] * a1 - 0.25 * dx * dz * A.alpha2[c] * a2 -
-         0.25 * dy * dz * A.alpha1[c + sy] * a3 - 0.25 * dx * dz * A.alpha2[c - sx] * a4 -
-         0.25 * dy * dz * A.alpha1[c + 1] * a5 - 0.25 * dx * dz * A.alpha2[c + 1] * a6 -
-         0.25 * dy * dz * A.alpha1[c + sy + 1] * a7 - 0.25 * dx * dz * A.alpha2[c - sx + 1] * a8 -
-         0.25 * dy * dy * A.alpha3[c] * a9 - 0.25 * dx * dy * A.alpha3[c - sx] * a10 -
-         0.25 * dy * dy * A.alpha3[c - sx - sy] * a11 - 0.25 * dx * dy * A.alpha3[c - sy] * a12);
-    A.visc_heat[c] = pmax(vh / CV(ix, iy, iz), 0.0);
-    };
-    ELOOP(cell)
-}
-
-// C2 over (0..n)^3: velocity correction (lagran.cpp:754-758) with the sum
-// fx_visc + fx stored by g_forces, into the second state copy.
-template <class T>
-KERNEL void g_vel_update(const Src<T> S)
-{
-    EGRID
-    auto cell = [&](const int ix, const int iy, const int iz) {
-    const int c = IX(ix, iy, iz);
     const double dt = A.s->dt;
     double rv = A.rho_v[c];
     A.vx2[c] = VX[c] + dt * A.fxv[c] / rv;
     A.vy2[c] = VY[c] + dt * A.fyv[c] / rv;
     A.vz2[c] = VZ[c] + dt * A.fzv[c] / rv;
-    };
-    ELOOP(cell)
-}
-
-// C3 over (1..n)^3: density and energy correction (lagran.cpp:762-796), into
-// the second state copy.  The cv1 it also writes in the reference is
-// recomputed by the remap before use.
-template <class T>
-KERNEL void g_rho_e_update(const Src<T> S)
-{
-    EGRID
-    auto cell = [&](const int ix, const int iy, const int iz) {
-    const int c = IX(ix, iy, iz);
-    const int sx = G.sx, sy = G.sy;
-    const double dt = A.s->dt;
-    const int ooo = c, moo = c - sx, omo = c - sy, oom = c - 1, mmo = c - sx - sy, mom = c - sx - 1,
-              omm = c - sy - 1, mmm = c - sx - sy - 1;
+    if (ix < 1 || iy < 1 || iz < 1) return;
+    const int ooo = c, moo = c - sx, omo = c - sy, mmo = c - sx - sy;
+    const int oop = c + 1, mop = c - sx + 1, omp = c - sy + 1, mmp = c - sx - sy + 1;
+    const int oom = c - 1, mom = c - sx - 1, omm = c - sy - 1, mmm = c - sx - sy - 1;
+    double visc_heat;
+    {
+        double a1 = DOT(mmo, omo), a2 = DOT(omo, ooo), a3 = DOT(ooo, moo), a4 = DOT(moo, mmo);
+        double a5 = DOT(mmp, omp), a6 = DOT(omp, oop), a7 = DOT(oop, mop), a8 = DOT(mop, mmp);
+        double a9 = DOT(oom, ooo), a10 = DOT(mom, moo), a11 = DOT(mmm, mmo), a12 = DOT(omm, omo);
+        double dx = DXB(ix), dy = DYB(iy), dz = DZB(iz);
+        double vh =
+            (-0.25 * dy * dz * A.alpha1[c] * a1 - 0.25 * dx * dz * A.alpha2[c] * a2 -
+             0.25 * dy * dz * A.alpha1[c + sy] * a3 - 0.25 * dx * dz * A.alpha2[c - sx] * a4 -
+             0.25 * dy * dz * A.alpha1[c + 1] * a5 - 0.25 * dx * dz * A.alpha2[c + 1] * a6 -
+             0.25 * dy * dz * A.alpha1[c + sy + 1] * a7 - 0.25 * dx * dz * A.alpha2[c - sx + 1] * a8 -
+             0.25 * dy * dy * A.alpha3[c] * a9 - 0.25 * dx * dy * A.alpha3[c - sx] * a10 -
+             0.25 * dy * dy * A.alpha3[c - sx - sy] * a11 - 0.25 * dx * dy * A.alpha3[c - sy] * a12);
+        visc_heat = pmax(vh / CV(ix, iy, iz), 0.0);
+    }
     double vxb = 0.25 * (VX1[ooo] + VX1[omo] + VX1[oom] + VX1[omm]);
     double vxbm = 0.25 * (VX1[moo] + VX1[mmo] + VX1[mom] + VX1[mmm]);
     double vyb = 0.25 * (VY1[ooo] + VY1[moo] + VY1[oom] + VY1[mom]);
@@ -1061,7 +1043,7 @@ KERNEL void g_rho_e_update(const Src<T> S)
     double dv = (dvxdx + dvydy + dvzdz) * dt;
     double rho = RHO[c];
     A.ee2[c] = EE[c] - dv * A.p_e[c] / rho;
-    A.ei2[c] = EI[c] + (dt * A.visc_heat[c] - dv * A.p_i[c]) / rho;
+    A.ei2[c] = EI[c] + (dt * visc_heat - dv * A.p_i[c]) / rho;
     A.rho2[c] = rho / (1.0 + dv);
     };
     ELOOP(cell)
@@ -1430,7 +1412,7 @@ void per_grid(void (*k)(const EDev<T> *, int), const char *name, Group &gp)
 // Variable groups of the boundary fills (boundary.cpp:27-96): the variables
 // filled and, for a variable just written into the other state copy, the
 // copy it was read from (-1: updated in place).
-enum { VB, VE, VRHO, VV, VV1, VDM, VALL, VERHO, VBE, VV_2, VERHO_2, VALL_2, VALL_1, NVG };
+enum { VB, VE, VRHO, VV, VV1, VDM, VALL, VERHO, VBE, VCORR_2, VALL_2, VALL_1, NVG };
 struct VarGroup {
     int n;
     int dst[9];
@@ -1446,10 +1428,9 @@ const VarGroup kGroups[NVG] = {
     {9, {6, 7, 8, 1, 2, 0, 3, 4, 5}, {-1, -1, -1, -1, -1, -1, -1, -1, -1}},
     {3, {1, 2, 0}, {-1, -1, -1}},
     {5, {6, 7, 8, 1, 2}, {-1, -1, -1, -1, -1}},
-    // velocity_bcs after the corrector wrote the second copy
-    {3, {16, 17, 18}, {3, 4, 5}},
-    // energy_bcs + density_bcs after the corrector
-    {3, {14, 15, 13}, {1, 2, 0}},
+    // velocity_bcs, energy_bcs and density_bcs after the corrector wrote the
+    // second copy
+    {6, {16, 17, 18, 14, 15, 13}, {3, 4, 5, 1, 2, 0}},
     // boundary_conditions after a sweep that wrote the second copy
     {9, {6, 7, 8, 14, 15, 13, 16, 17, 18}, {-1, -1, -1, 1, 2, 0, 3, 4, 5}},
     // boundary_conditions after a sweep that wrote the first copy
@@ -1600,11 +1581,11 @@ template <class T>
 void half_t(Group &gp)
 {
     const double w = (double)sizeof(T);
-    EL(g_shock_heating<T>, 10 * w, gp, 0, rs(0, 1, 0, 1, 0, 1));
-    EL(g_vel_update<T>, 8 * w, gp, 0, rs(0, 0, 0, 0, 0, 0));
-    ebcs<T>(gp, VV_2);
-    EL(g_rho_e_update<T>, 12 * w, gp, 0, rs(1, 0, 1, 0, 1, 0));
-    ebcs<T>(gp, VERHO_2);
+    // reads v v1 alpha1..3 rho_v fx_visc..fz_visc rho e_e e_i p_e p_i;
+    // writes v rho e_e e_i
+    EL(g_correct<T>, 22 * w, gp, 0, rs(0, 0, 0, 0, 0, 0));
+    // velocity_bcs, energy_bcs, density_bcs (independent variables)
+    ebcs<T>(gp, VCORR_2);
     for (Grid *g : gp.mem) g->st = 1;
 }
 
