@@ -82,10 +82,21 @@ struct Phys {
     int resistive;
 };
 
+// Device arrays are global memory: accessing them through address-space-1
+// pointers lets the compiler emit global (not flat) loads and stores, which
+// do not count against the scalar-load wait counter.
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gptr<T> gbl(T *p)
+{
+    return (gptr<T>)p;
+}
+
 // Element of a T-typed array: reads as double, assignment rounds to T.
 template <class T>
 struct Ref {
-    T *q;
+    gptr<T> q;
     __device__ __forceinline__ operator double() const { return (double)*q; }
     __device__ __forceinline__ const Ref &operator=(double v) const
     {
@@ -102,7 +113,7 @@ struct Ref {
 template <class T>
 struct Fld {
     T *p;
-    __device__ __forceinline__ Ref<T> operator[](int i) const { return Ref<T>{p + i}; }
+    __device__ __forceinline__ Ref<T> operator[](int i) const { return Ref<T>{gbl(p) + i}; }
 };
 
 template <class T>
@@ -169,9 +180,9 @@ struct BCSetE {
 // Linear index of (i,j,k) in the shared box.
 #define IX(i, j, k) (G.base + (i) * G.sx + (j) * G.sy + (k))
 // 1D deltas
-#define DXB(i) (A.dxb[(i) + 2])
-#define DYB(i) (A.dyb[(i) + 2])
-#define DZB(i) (A.dzb[(i) + 2])
-#define DXC(i) (A.dxc[(i) + 2])
-#define DYC(i) (A.dyc[(i) + 2])
-#define DZC(i) (A.dzc[(i) + 2])
+#define DXB(i) (lare::gbl(A.dxb)[(i) + 2])
+#define DYB(i) (lare::gbl(A.dyb)[(i) + 2])
+#define DZB(i) (lare::gbl(A.dzb)[(i) + 2])
+#define DXC(i) (lare::gbl(A.dxc)[(i) + 2])
+#define DYC(i) (lare::gbl(A.dyc)[(i) + 2])
+#define DZC(i) (lare::gbl(A.dzc)[(i) + 2])

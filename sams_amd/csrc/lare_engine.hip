@@ -62,7 +62,7 @@ using lare::Src;
 #define VX1 A.vx1
 #define VY1 A.vy1
 #define VZ1 A.vz1
-#define DM A.dm
+#define DM lare::gbl(A.dm)
 
 // Workgroup b in dispatch order -> logical workgroup: XCD b % 8 takes the
 // contiguous logical run [start(x), start(x) + count(x)).
@@ -169,7 +169,7 @@ KERNEL void f_prep(const Src<T> S, double xpass, double ypass, double zpass)
     auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
-    const double dt = A.s->dt;
+    const double dt = lare::gbl(A.s)->dt;
     const int ooo = c, moo = c - sx, omo = c - sy, oom = c - 1, mmo = c - sx - sy, mom = c - sx - 1,
               omm = c - sy - 1, mmm = c - sx - sy - 1;
     double vxb = (VX1[ooo] + VX1[omo] + VX1[oom] + VX1[omm]) * 0.25;
@@ -275,16 +275,16 @@ KERNEL void f_ct_fluxes(const Src<T> S)
     auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
-    const double dt = A.s->dt;
+    const double dt = lare::gbl(A.s)->dt;
     if (DIR == 0) {
-        A.fl[0][c] = ct_flux<0>(A, G, c, BY, VX1, -1, sy, dt);
-        A.fl[1][c] = ct_flux<0>(A, G, c, BZ, VX1, -sy, 1, dt);
+        lare::gbl(A.fl[0])[c] = ct_flux<0>(A, G, c, BY, VX1, -1, sy, dt);
+        lare::gbl(A.fl[1])[c] = ct_flux<0>(A, G, c, BZ, VX1, -sy, 1, dt);
     } else if (DIR == 1) {
-        A.fl[0][c] = ct_flux<1>(A, G, c, BX, VY1, -1, sx, dt);
-        A.fl[1][c] = ct_flux<1>(A, G, c, BZ, VY1, -sx, 1, dt);
+        lare::gbl(A.fl[0])[c] = ct_flux<1>(A, G, c, BX, VY1, -1, sx, dt);
+        lare::gbl(A.fl[1])[c] = ct_flux<1>(A, G, c, BZ, VY1, -sx, 1, dt);
     } else {
-        A.fl[0][c] = ct_flux<2>(A, G, c, BX, VZ1, -sy, sx, dt);
-        A.fl[1][c] = ct_flux<2>(A, G, c, BY, VZ1, -sx, sy, dt);
+        lare::gbl(A.fl[0])[c] = ct_flux<2>(A, G, c, BX, VZ1, -sy, sx, dt);
+        lare::gbl(A.fl[1])[c] = ct_flux<2>(A, G, c, BY, VZ1, -sx, sy, dt);
     }
     };
     ELOOP(cell)
@@ -300,7 +300,7 @@ KERNEL void f_ct_update(const Src<T> S)
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     const int nx = G.nx, ny = G.ny, nz = G.nz;
-    const double *fa = A.fl[0], *fb = A.fl[1];
+    const auto fa = lare::gbl(A.fl[0]), fb = lare::gbl(A.fl[1]);
     const bool x1 = ix >= 1 && ix <= nx, x0 = ix <= nx;
     const bool y1 = iy >= 1 && iy <= ny, y0 = iy <= ny;
     const bool z1 = iz >= 1 && iz <= nz, z0 = iz <= nz;
@@ -366,7 +366,7 @@ KERNEL void f_mass_flux(const Src<T> S)
         DM[c] = 0.0;
         return;
     }
-    const double dt = A.s->dt;
+    const double dt = lare::gbl(A.s)->dt;
     double area = face_area<DIR>(A, ix, iy, iz);
     double v_advect = face_v<DIR>(A, G, c);
     double o_v = v_advect * dt * area;
@@ -466,7 +466,7 @@ template <int DIR, class T>
 __device__ __forceinline__ double dmv_at(const ArrE<T> &A, const Geo &G, int c)
 {
     const int sx = G.sx, sy = G.sy;
-    const double *M = DM;
+    const auto M = DM;
     double q = M[c] + M[c + sx] + M[c + sy] + M[c + sx + sy] + M[c + 1] + M[c + sx + 1] + M[c + sy + 1] +
                M[c + sx + sy + 1];
     return DIR == 0 ? q * 0.125 : 0.125 * q;
@@ -514,7 +514,7 @@ KERNEL void f_update_all(const Src<T> S)
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     const int sd = sdir<DIR>(G);
-    const double dt = A.s->dt;
+    const double dt = lare::gbl(A.s)->dt;
     const int mx = DIR == 0 ? ix - 1 : ix, my = DIR == 1 ? iy - 1 : iy, mz = DIR == 2 ? iz - 1 : iz;
     // vertex quantities at c - sd, c, c + sd
     const double rvm = rrho_v_at<DIR>(A, G, c - sd), rv = rrho_v_at<DIR>(A, G, c), rvp = rrho_v_at<DIR>(A, G, c + sd);
@@ -564,9 +564,9 @@ KERNEL void f_wshift(const Src<T> S)
     const int c = IX(ix, iy, iz);
     const Fld<T> w = DIR == 0 ? VX1 : DIR == 1 ? VY1 : VZ1;
     if (BACK)
-        w[c] = A.fl[0][c];
+        w[c] = lare::gbl(A.fl[0])[c];
     else
-        A.fl[0][c] = (w[c] + w[c + sdir<DIR>(G)]) * 0.5;
+        lare::gbl(A.fl[0])[c] = (w[c] + w[c + sdir<DIR>(G)]) * 0.5;
     };
     ELOOP(cell)
 }
@@ -604,7 +604,7 @@ KERNEL void g_cs(const Src<T> S)
     EGRID
     auto cell = [&](const int ix, const int iy, const int iz) {
     const double gas_gamma = P.gas_gamma;
-    if (ix == R.x0 && iy == R.y0 && iz == R.z0) A.s->dtmin_bits = 0x7FEFFFFFFFFFFFFFull;
+    if (ix == R.x0 && iy == R.y0 && iz == R.z0) lare::gbl(A.s)->dtmin_bits = 0x7FEFFFFFFFFFFFFFull;
     const int c = IX(ix, iy, iz);
     double bx1 = 0.5 * (BX[c] + BX[c - G.sx]);
     double by1 = 0.5 * (BY[c] + BY[c - G.sy]);
@@ -689,7 +689,7 @@ KERNEL void g_alpha(const Src<T> S)
     const double visc1 = P.visc1, visc2_norm = P.visc2_norm;
     const int sx = G.sx, sy = G.sy;
     const int nx = G.nx, ny = G.ny, nz = G.nz;
-    const double dt = A.s->dt;
+    const double dt = lare::gbl(A.s)->dt;
     const int c = IX(ix, iy, iz);
     if (ix >= 0 && ix <= nx + 1 && iy >= 0 && iy <= ny + 2 && iz >= 0 && iz <= nz + 2) {
         const int c1 = c - sx - sy - 1;
@@ -850,7 +850,7 @@ KERNEL void g_predict(const Src<T> S)
     const bool res = P.resistive != 0;
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
-    const double dt = A.s->dt;
+    const double dt = lare::gbl(A.s)->dt;
     const int ooo = c, moo = c - sx, omo = c - sy, oom = c - 1, mmo = c - sx - sy, mom = c - sx - 1,
               omm = c - sy - 1, mmm = c - sx - sy - 1;
     double vxb = 0.25 * (VX[ooo] + VX[omo] + VX[oom] + VX[omm]);
@@ -926,7 +926,7 @@ KERNEL void g_forces(const Src<T> S)
     const double mu0 = P.mu0;
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
-    const double dt = A.s->dt;
+    const double dt = lare::gbl(A.s)->dt;
     const Fld<T> PE = A.p_e, PI = A.p_i, C1 = A.cv1, B1x = A.bx1, B1y = A.by1, B1z = A.bz1;
     double dx = DXC(ix), dy = DYC(iy), dz = DZC(iz);
     double dhy = dy, dhz = dz;
@@ -999,13 +999,13 @@ KERNEL void g_forces(const Src<T> S)
 // viscosity recomputes it.  The cv1 the reference also writes here is
 // recomputed by the remap before use.
 template <class T>
-KERNEL OCC(3) void g_correct(const Src<T> S)
+KERNEL OCC(2) void g_correct(const Src<T> S)
 {
     EGRID
     auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
-    const double dt = A.s->dt;
+    const double dt = lare::gbl(A.s)->dt;
     double rv = A.rho_v[c];
     A.vx2[c] = VX[c] + dt * A.fxv[c] / rv;
     A.vy2[c] = VY[c] + dt * A.fyv[c] / rv;
@@ -1094,9 +1094,9 @@ KERNEL void r_rkstep(const Src<T> S)
     double jz = 0.5 * (jz1 + jz2);
     double j_local = sqrt(jx * jx + jy * jy + jz * jz);
     double eta = j_local > P.j_max ? P.eta_background + P.eta0 : P.eta_background;
-    A.flux_x[c] = -jx * eta * dxc * 0.5;
-    A.flux_y[c] = -jy * eta * dyc * 0.5;
-    A.flux_z[c] = -jz * eta * dzc * 0.5;
+    lare::gbl(A.flux_x)[c] = -jx * eta * dxc * 0.5;
+    lare::gbl(A.flux_y)[c] = -jy * eta * dyc * 0.5;
+    lare::gbl(A.flux_z)[c] = -jz * eta * dzc * 0.5;
     A.curlb[c] = eta * (jx * jx + jy * jy + jz * jz);
     };
     ELOOP(cell)
@@ -1115,8 +1115,8 @@ KERNEL void r_b_energy(const Src<T> S)
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     const int nx = G.nx, ny = G.ny, nz = G.nz;
-    const double dt = A.s->dt_sub;
-    const double *fx = A.flux_x, *fy = A.flux_y, *fz = A.flux_z;
+    const double dt = lare::gbl(A.s)->dt_sub;
+    const auto fx = lare::gbl(A.flux_x), fy = lare::gbl(A.flux_y), fz = lare::gbl(A.flux_z);
     if (iy >= 1 && iz >= 1) {
         double area = DYB(iy) * DZB(iz);
         BX[c] = BX[c] + (fz[c] - fz[c - sy] + fz[c - 1] - fz[c - sy - 1] - fy[c] + fy[c - 1] - fy[c - sy] + fy[c - sy - 1]) * dt / area;
