@@ -168,6 +168,21 @@ def bench_fp32(n, steps):
     return res
 
 
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by a
+    plain copy's reading) from the rocprofv3 counter runs summarised in
+    profiles/ for this workload; None where no measurement matches."""
+    path = os.path.join(ROOT, "profiles", f"bw{n}_resistive_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and not line.startswith("#") and parts[0] == kernel:
+                return int(parts[1]) + int(parts[2])
+    return None
+
+
 def kernel_roofline(dev, nsteps, cells, ms_step):
     """Per-kernel HIP-event timing on the grid's stream (after the timed
     region); the dominant kernel's algorithmic GB/s against the HBM peak."""
@@ -180,8 +195,10 @@ def kernel_roofline(dev, nsteps, cells, ms_step):
         log(f"  {k:20s} calls {c:4d} avg {t / c * 1e3:9.1f} us  share {t / sum(v[1] for v in prof.values()):6.1%}"
             f"  {b / c / (t / c * 1e-3) / 1e9 if t > 0 else 0:8.1f} GB/s")
     step_bytes = STEP_BYTES_RESISTIVE * cells
+    traffic = measured_traffic(name, round(cells ** (1 / 3)))
     roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "algorithmic_bytes_per_launch": round(tot_bytes / calls),
                 "avg_launch_us": round(avg_ms * 1e3, 2),
                 "step_algorithmic_GBs": round(step_bytes / (ms_step * 1e-3) / 1e9, 1),
                 "step_frac": round(step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
