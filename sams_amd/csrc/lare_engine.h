@@ -45,7 +45,9 @@ struct Group {
     hipStream_t stream = nullptr;
     Prof *prof = nullptr;
     bool planning = false;
-    int order = 1;  // workgroup order within a grid (Src::order)
+    int order = 1;    // workgroup order within a grid (Src::order)
+    int march_k = 16;   // cells per lane of the x and y transport kernels (0: one)
+    bool zline = true;  // z transport with neighbours exchanged between lanes
     // one captured step of the whole group
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;

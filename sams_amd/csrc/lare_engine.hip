@@ -230,25 +230,40 @@ KERNEL void f_cvc1(const Src<T> S)
     ELOOP(cell)
 }
 
+// Face area of B component SC (remap.cpp:46-50): dxab, dyab, dzab.
+template <int SC, class T>
+__device__ __forceinline__ double b_area(const ArrE<T> &A, int ix, int iy, int iz)
+{
+    return SC == 0 ? DXAB(iy, iz) : SC == 1 ? DYAB(ix, iz) : DZAB(ix, iy);
+}
+
 // Constrained-transport flux of face magnetic flux B along DIR: v_advect is
 // the mean of w at c and c+ow, db1 is averaged with c+tw (vx_by_flux,
 // xremap.cpp:269-319, and its five twins).
-template <int DIR, class T>
-__device__ __forceinline__ double ct_flux(const ArrE<T> &A, const Geo &G, int c, const Fld<T> B, const Fld<T> w,
-                                          int ow, int tw, double dt)
+// SC >= 0: B is read unscaled and multiplied here by the area of
+// component SC at each position, as remap.cpp:46-50 stores it (the x sweep's
+// flux kernel runs before that product is stored).
+template <int DIR, int SC = -1, class T>
+__device__ __forceinline__ double ct_flux(const ArrE<T> &A, const Geo &G, int c, const Fld<T> Bf, const Fld<T> w,
+                                          int ow, int tw, double dt, int ix = 0, int iy = 0, int iz = 0)
 {
     const int sd = sdir<DIR>(G);
     const Fld<T> db1 = A.db1;
+    auto B = [&](int k) -> double {
+        double b = Bf[c + k * sd];
+        if (SC >= 0) b = b * b_area<SC>(A, ix + (DIR == 0 ? k : 0), iy + (DIR == 1 ? k : 0), iz + (DIR == 2 ? k : 0));
+        return b;
+    };
     double v_advect = (w[c] + w[c + ow]) * 0.5;
     double o_v = v_advect * dt;
     double dbx = (db1[c] + db1[c + tw]) * 0.5;
     double dbxp = (db1[c + sd] + db1[c + sd + tw]) * 0.5;
     double dbxp2 = (db1[c + 2 * sd] + db1[c + 2 * sd + tw]) * 0.5;
     double dbxm = (db1[c - sd] + db1[c - sd + tw]) * 0.5;
-    double fm = B[c - sd] / dbxm;
-    double fi = B[c] / dbx;
-    double fp = B[c + sd] / dbxp;
-    double fp2 = B[c + 2 * sd] / dbxp2;
+    double fm = B(-1) / dbxm;
+    double fi = B(0) / dbx;
+    double fp = B(1) / dbxp;
+    double fp2 = B(2) / dbxp2;
     double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
     double sign_v = (v_advect >= 0.0) ? 1.0 : -1.0;
     double vad_p = (sign_v + 1.0) * 0.5;
@@ -277,8 +292,8 @@ KERNEL void f_ct_fluxes(const Src<T> S)
     const int sx = G.sx, sy = G.sy;
     const double dt = lare::gbl(A.s)->dt;
     if (DIR == 0) {
-        lare::gbl(A.fl[0])[c] = ct_flux<0>(A, G, c, BY, VX1, -1, sy, dt);
-        lare::gbl(A.fl[1])[c] = ct_flux<0>(A, G, c, BZ, VX1, -sy, 1, dt);
+        lare::gbl(A.fl[0])[c] = ct_flux<0, 1>(A, G, c, BY, VX1, -1, sy, dt, ix, iy, iz);
+        lare::gbl(A.fl[1])[c] = ct_flux<0, 2>(A, G, c, BZ, VX1, -sy, 1, dt, ix, iy, iz);
     } else if (DIR == 1) {
         lare::gbl(A.fl[0])[c] = ct_flux<1>(A, G, c, BX, VY1, -1, sx, dt);
         lare::gbl(A.fl[1])[c] = ct_flux<1>(A, G, c, BZ, VY1, -sx, 1, dt);
@@ -292,6 +307,10 @@ KERNEL void f_ct_fluxes(const Src<T> S)
 
 // R4: the four CT updates of the sweep in their reference order, each over
 // its own range (xremap.cpp:85-114, yremap.cpp:93-127, zremap.cpp:97-117).
+// The x sweep's kernel runs over the union of the B extents and also stores
+// B *= area (remap.cpp:46-50) on every cell of each component's extent; the
+// z sweep's stores B /= (area + none_zero) (remap.cpp:91-95) likewise.  No
+// kernel reads B between those products and these stores.
 template <class T, int DIR>
 KERNEL void f_ct_update(const Src<T> S)
 {
@@ -301,16 +320,23 @@ KERNEL void f_ct_update(const Src<T> S)
     const int sx = G.sx, sy = G.sy;
     const int nx = G.nx, ny = G.ny, nz = G.nz;
     const auto fa = lare::gbl(A.fl[0]), fb = lare::gbl(A.fl[1]);
-    const bool x1 = ix >= 1 && ix <= nx, x0 = ix <= nx;
-    const bool y1 = iy >= 1 && iy <= ny, y0 = iy <= ny;
-    const bool z1 = iz >= 1 && iz <= nz, z0 = iz <= nz;
+    const bool x1 = ix >= 1 && ix <= nx, x0 = ix >= 0 && ix <= nx;
+    const bool y1 = iy >= 1 && iy <= ny, y0 = iy >= 0 && iy <= ny;
+    const bool z1 = iz >= 1 && iz <= nz, z0 = iz >= 0 && iz <= nz;
+    // extents: bx (-2.., -1.., -1..), by (-1.., -2.., -1..), bz (-1.., -1.., -2..)
+    const bool ebx = iy >= -1 && iz >= -1, eby = ix >= -1 && iz >= -1, ebz = ix >= -1 && iy >= -1;
     if (DIR == 0) {
-        if (x1 && y0 && z1) BY[c] = BY[c] - fa[c] + fa[c - sx];
-        double bx = BX[c];
+        double bx = ebx ? BX[c] * b_area<0>(A, ix, iy, iz) : 0.0;
+        double by = eby ? BY[c] * b_area<1>(A, ix, iy, iz) : 0.0;
+        double bz = ebz ? BZ[c] * b_area<2>(A, ix, iy, iz) : 0.0;
+        if (x1 && y0 && z1) by = by - fa[c] + fa[c - sx];
         const bool bxr = x0 && y1 && z1;
         if (bxr) bx = bx + fa[c] - fa[c - sy];
-        if (x1 && y1 && z0) BZ[c] = BZ[c] - fb[c] + fb[c - sx];
-        if (bxr) BX[c] = bx + fb[c] - fb[c - 1];
+        if (x1 && y1 && z0) bz = bz - fb[c] + fb[c - sx];
+        if (bxr) bx = bx + fb[c] - fb[c - 1];
+        if (ebx) BX[c] = bx;
+        if (eby) BY[c] = by;
+        if (ebz) BZ[c] = bz;
     } else if (DIR == 1) {
         const bool bxr = x0 && y1 && z1;
         if (bxr) BX[c] = BX[c] - fa[c] + fa[c - sy];
@@ -320,12 +346,17 @@ KERNEL void f_ct_update(const Src<T> S)
         if (x1 && y1 && z0) BZ[c] = BZ[c] - fb[c] + fb[c - sy];
         if (byr) BY[c] = by + fb[c] - fb[c - 1];
     } else {
-        if (x0 && y1 && z1) BX[c] = BX[c] - fa[c] + fa[c - 1];
-        double bz = BZ[c];
+        double bx = ebx ? (double)BX[c] : 0.0;
+        double by = eby ? (double)BY[c] : 0.0;
+        double bz = ebz ? (double)BZ[c] : 0.0;
+        if (x0 && y1 && z1) bx = bx - fa[c] + fa[c - 1];
         const bool bzr = x1 && y1 && z0;
         if (bzr) bz = bz + fa[c] - fa[c - sx];
-        if (x1 && y0 && z1) BY[c] = BY[c] - fb[c] + fb[c - 1];
-        if (bzr) BZ[c] = bz + fb[c] - fb[c - sy];
+        if (x1 && y0 && z1) by = by - fb[c] + fb[c - 1];
+        if (bzr) bz = bz + fb[c] - fb[c - sy];
+        if (ebx) BX[c] = bx / (b_area<0>(A, ix, iy, iz) + DBL_EPSILON);
+        if (eby) BY[c] = by / (b_area<1>(A, ix, iy, iz) + DBL_EPSILON);
+        if (ebz) BZ[c] = bz / (b_area<2>(A, ix, iy, iz) + DBL_EPSILON);
     }
     };
     ELOOP(cell)
@@ -389,27 +420,35 @@ KERNEL void f_mass_flux(const Src<T> S)
 
 // Energy flux of specific energy E on face c (x_energy_flux,
 // xremap.cpp:429-483 and twins); rho is still the pre-remap density.
-template <int DIR, class T>
-__device__ __forceinline__ double energy_flux(const ArrE<T> &A, const Geo &G, int c, const Fld<T> E,
-                                              double v_advect, double o_v)
+// The same from the values along DIR: E at c-sd..c+2sd, cvc1 at c-sd..c+sd,
+// cv1 and rho at c, c+sd, and the face's mass flux dm.
+__device__ __forceinline__ double energy_flux_v(double fm, double fi, double fp, double fp2, double v_advect,
+                                                double o_v, double cvc_m, double cvc_0, double cvc_p, double cv1_0,
+                                                double cv1_p, double rho_0, double rho_p, double dm)
 {
-    const int sd = sdir<DIR>(G);
-    double fm = E[c - sd], fi = E[c], fp = E[c + sd], fp2 = E[c + 2 * sd];
     double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
     double sign_v = (v_advect >= 0.0) ? 1.0 : -1.0;
     double vad_p = (sign_v + 1.0) * 0.5;
     double vad_m = 1.0 - vad_p;
     double fu = fi * vad_p + fp * vad_m;
     double dfu = dfm * vad_p + dfp * vad_m;
-    double dxci = A.cvc1[c];
-    double dxcu = A.cvc1[c - sd] * vad_p + A.cvc1[c + sd] * vad_m;
-    double dxbu = A.cv1[c] * vad_p + A.cv1[c + sd] * vad_m;
+    double dxci = cvc_0;
+    double dxcu = cvc_m * vad_p + cvc_p * vad_m;
+    double dxbu = cv1_0 * vad_p + cv1_p * vad_m;
     double phi = fabs(o_v) / dxbu;
     double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
-    double rhou = RHO[c] * vad_p + RHO[c + sd] * vad_m;
-    double dm = DM[c];
+    double rhou = rho_0 * vad_p + rho_p * vad_m;
     double dmu = fabs(dm) / dxbu / rhou;
     return (fu + Di * (1.0 - dmu)) * dm;
+}
+
+template <int DIR, class T>
+__device__ __forceinline__ double energy_flux(const ArrE<T> &A, const Geo &G, int c, const Fld<T> E,
+                                              double v_advect, double o_v)
+{
+    const int sd = sdir<DIR>(G);
+    return energy_flux_v(E[c - sd], E[c], E[c + sd], E[c + 2 * sd], v_advect, o_v, A.cvc1[c - sd], A.cvc1[c],
+                         A.cvc1[c + sd], A.cv1[c], A.cv1[c + sd], RHO[c], RHO[c + sd], DM[c]);
 }
 
 // Energy flux of E through cell face f = (fx, fy, fz) along DIR with the
@@ -428,12 +467,10 @@ __device__ __forceinline__ double energy_flux_face(const ArrE<T> &A, const Geo &
 // Momentum flux of velocity component f on vertex face c (x_mom_flux,
 // xremap.cpp:485-566 and twins), multiplied by dm.  z takes dyac as its area
 // (zremap.cpp:475) and divides by (dzbu*rhou) (zremap.cpp:505).
-template <int DIR, class T>
-__device__ __forceinline__ double mom_flux(const Geo &G, int c, const Fld<T> f, double o_v, double sign_v,
-                                           double dxci, double dxcu, double dxbu, double rhou, double dm)
+template <int DIR>
+__device__ __forceinline__ double mom_flux_v(double fm, double fi, double fp, double fp2, double o_v, double sign_v,
+                                             double dxci, double dxcu, double dxbu, double rhou, double dm)
 {
-    const int sd = sdir<DIR>(G);
-    double fm = f[c - sd], fi = f[c], fp = f[c + sd], fp2 = f[c + 2 * sd];
     double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
     double vad_p = (sign_v + 1.0) * 0.5;
     double vad_m = 1.0 - vad_p;
@@ -444,6 +481,14 @@ __device__ __forceinline__ double mom_flux(const Geo &G, int c, const Fld<T> f, 
     double dmu = DIR == 2 ? fabs(dm) / (dxbu * rhou) : fabs(dm) / dxbu / rhou;
     double fl = fu + Di * (1.0 - dmu);
     return fl * dm;
+}
+
+template <int DIR, class T>
+__device__ __forceinline__ double mom_flux(const Geo &G, int c, const Fld<T> f, double o_v, double sign_v,
+                                           double dxci, double dxcu, double dxbu, double rhou, double dm)
+{
+    const int sd = sdir<DIR>(G);
+    return mom_flux_v<DIR>(f[c - sd], f[c], f[c + sd], f[c + 2 * sd], o_v, sign_v, dxci, dxcu, dxbu, rhou, dm);
 }
 
 // Vertex density before remap at vertex c (xremap.cpp:159-176,
@@ -548,6 +593,315 @@ KERNEL void f_update_all(const Src<T> S)
     ELOOP(cell)
 }
 
+// Sum of the eight corner values of vertex c in the reference's order (c,
+// c+sx, c+sy, c+sx+sy, c+1, c+sx+1, c+sy+1, c+sx+sy+1), given the four
+// corners on the lower plane along DIR (lo) and on the upper one (hi), each
+// indexed j = (offset across the other non-z axis) + 2 * (offset in z).
+template <int DIR>
+__device__ __forceinline__ double sum8(const double lo[4], const double hi[4])
+{
+    if (DIR == 0)
+        return lo[0] + hi[0] + lo[1] + hi[1] + lo[2] + hi[2] + lo[3] + hi[3];
+    return lo[0] + lo[1] + hi[0] + hi[1] + lo[2] + lo[3] + hi[2] + hi[3];
+}
+
+// R6' for the x and y sweeps: the transport of f_update_all with each lane
+// walking K consecutive cells along DIR (one warm-up position before them
+// writes nothing).  Every value that consecutive cells share is loaded once
+// and carried in registers: the line values along DIR, the corner planes of
+// the vertex sums, the face and vertex fluxes and vertex densities.  The
+// arithmetic of each flux and update is the same expression on the same
+// operands as in f_update_all, so the results are identical.  The plane
+// across DIR is flattened z fastest: (y, z) for x, (x, z) for y.
+template <class T, int DIR, int K>
+KERNEL void f_update_march(const Src<T> S)
+{
+    const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
+    const int g_ = grid_of(S.boff, S.ngrid, b_);
+    const EDev<T> &D_ = S.devs[g_];
+    const Geo G = D_.g;
+    const ArrE<T> A = state_view(D_.a, S.st);
+    const R3 R = {S.r.x0, G.nx + S.r.dx1, S.r.y0, G.ny + S.r.dy1, S.r.z0, G.nz + S.r.dz1};
+    const int ps0 = DIR == 0 ? R.y0 : R.x0, ps1 = DIR == 0 ? R.y1 : R.x1;
+    const int nzr = R.z1 - R.z0 + 1;
+    const int plane = (ps1 - ps0 + 1) * nzr;
+    const int bpp = (plane + 255) >> 8;
+    const int lb = b_ - S.boff[g_];
+    const int mb = lb / bpp;
+    const int t = (lb - mb * bpp) * 256 + (int)threadIdx.x;
+    if (t >= plane) return;
+    const int ps = ps0 + t / nzr, iz = R.z0 + t % nzr;
+    const int dlo = DIR == 0 ? R.x0 : R.y0, dhi = DIR == 0 ? R.x1 : R.y1;
+    const int d0 = dlo + mb * K;
+    const int sd = sdir<DIR>(G);
+    const int so = DIR == 0 ? G.sy : G.sx;  // stride of the other non-z axis
+    const int tj[4] = {0, so, 1, so + 1};   // corner offsets within a plane
+    const double dt = lare::gbl(A.s)->dt;
+    const int ix0 = DIR == 0 ? d0 - 1 : ps, iy0 = DIR == 0 ? ps : d0 - 1;
+    const bool tr1 = ps >= 1 && iz >= 1;  // transverse indices >= 1: density/energy cells
+    // face areas are constant along DIR
+    const double area_e = DIR == 0 ? DXAB(ps, iz) : DYAB(ps, iz);
+    const double area_m = DIR == 0 ? DXAC(ps, iz) : DYAC(ps, iz);
+    const Fld<T> W = DIR == 0 ? VX1 : VY1;
+    const Fld<T> R1 = RHO, C1 = A.cv1, C2 = A.cv2;
+    const auto M = DM;
+    // the window enters the warm-up position c = (d0 - 1)
+    int c = IX(ix0, iy0, iz);
+    double E[2][3], F[3][3];
+    for (int e = 0; e < 3; ++e) {
+        E[0][e] = EE[c + (e - 1) * sd];
+        E[1][e] = EI[c + (e - 1) * sd];
+        F[0][e] = VX[c + (e - 1) * sd];
+        F[1][e] = VY[c + (e - 1) * sd];
+        F[2][e] = VZ[c + (e - 1) * sd];
+    }
+    double cvc_m = A.cvc1[c - sd], cvc_0 = A.cvc1[c];
+    double cv1_0 = C1[c], cv1_p = C1[c + sd];
+    double rho_0 = R1[c], rho_p = R1[c + sd];
+    double w_0 = W[c];
+    double dm_m = 0.0;
+    double DML[4], C2L[4], PL[4], P0[4];
+    for (int j = 0; j < 4; ++j) {
+        DML[j] = M[c + tj[j]];
+        C2L[j] = 0.0;
+        P0[j] = R1[c + tj[j]] * C1[c + tj[j]];
+        PL[j] = R1[c + sd + tj[j]] * C1[c + sd + tj[j]];
+    }
+    double rv_c = DIR == 0 ? sum8<DIR>(P0, PL) * 0.125 / cvc_0 : sum8<DIR>(P0, PL) * (0.125 / cvc_0);
+    double dmv_m = 0.0, flm[3] = {0.0, 0.0, 0.0}, em[2] = {0.0, 0.0};
+    for (int k = -1; k < K; ++k) {
+        const int d = d0 + k;
+        if (d > dhi) break;
+        // new values: line at c+sd / c+2sd, corner planes d+1 (DM, cv2) and d+2 (rho cv1)
+        double Ep[2] = {EE[c + 2 * sd], EI[c + 2 * sd]};
+        double Fp[3] = {VX[c + 2 * sd], VY[c + 2 * sd], VZ[c + 2 * sd]};
+        const double cvc_p = A.cvc1[c + sd];
+        const double w_p = W[c + sd];
+        const double fv = DIR == 0 ? (w_0 + W[c - so] + W[c - 1] + W[c - so - 1]) * 0.25
+                                   : (W[c - so] + w_0 + W[c - so - 1] + W[c - 1]) * 0.25;
+        double DMH[4], C2H[4], PH[4];
+        double cv1_pp = 0.0, rho_pp = 0.0;
+        for (int j = 0; j < 4; ++j) {
+            DMH[j] = M[c + sd + tj[j]];
+            C2H[j] = C2[c + sd + tj[j]];
+            const double r = R1[c + 2 * sd + tj[j]], v = C1[c + 2 * sd + tj[j]];
+            if (j == 0) {
+                rho_pp = r;
+                cv1_pp = v;
+            }
+            PH[j] = r * v;
+        }
+        // vertex density at c+sd, vertex mass flux at c
+        const double sp = sum8<DIR>(PL, PH);
+        const double rv_p = DIR == 0 ? sp * 0.125 / cvc_p : sp * (0.125 / cvc_p);
+        const double q = sum8<DIR>(DML, DMH);
+        const double dmv_c = DIR == 0 ? q * 0.125 : 0.125 * q;
+        // momentum fluxes through vertex face c
+        double flc[3];
+        {
+            double v_advect = (w_0 + w_p) * 0.5;
+            double o_v = v_advect * dt * area_m;
+            double sign_v = (o_v >= 0.0) ? 1.0 : -1.0;
+            double vad_p = (sign_v + 1.0) * 0.5;
+            double vad_m = 1.0 - vad_p;
+            double dxci = cv1_p;
+            double dxcu = cv1_0 * vad_p + cv1_pp * vad_m;
+            double dxbu = cvc_0 * vad_p + cvc_p * vad_m;
+            double rhou = rv_c * vad_p + rv_p * vad_m;
+            for (int f = 0; f < 3; ++f)
+                flc[f] = mom_flux_v<DIR>(F[f][0], F[f][1], F[f][2], Fp[f], o_v, sign_v, dxci, dxcu, dxbu, rhou,
+                                         dmv_c);
+        }
+        if (k >= 0) {
+            const double s2 = sum8<DIR>(C2L, C2H);
+            const double cv2v = DIR == 0 ? s2 * 0.125 : 0.125 * s2;
+            const double rho_v1 = (rv_c * cvc_0 + dmv_m - dmv_c) / cv2v;
+            const double den = cv2v * rho_v1;
+            A.vx2[c] = (rv_c * F[0][1] * cvc_0 + flm[0] - flc[0]) / den;
+            A.vy2[c] = (rv_c * F[1][1] * cvc_0 + flm[1] - flc[1]) / den;
+            A.vz2[c] = (rv_c * F[2][1] * cvc_0 + flm[2] - flc[2]) / den;
+        }
+        // energy fluxes through cell face c, and the cell update
+        double ec[2] = {0.0, 0.0};
+        if (tr1 && d >= 0) {
+            const double o_v = fv * dt * area_e;
+            for (int e = 0; e < 2; ++e)
+                ec[e] = energy_flux_v(E[e][0], E[e][1], E[e][2], Ep[e], fv, o_v, cvc_m, cvc_0, cvc_p, cv1_0, cv1_p,
+                                      rho_0, rho_p, DML[0]);
+            if (k >= 0 && d >= 1) {
+                const double cv2 = C2L[0];
+                const double rho = (rho_0 * cv1_0 + dm_m - DML[0]) / cv2;
+                A.rho2[c] = rho;
+                A.ee2[c] = (E[0][1] * cv1_0 * rho_0 + em[0] - ec[0]) / (cv2 * rho);
+                A.ei2[c] = (E[1][1] * cv1_0 * rho_0 + em[1] - ec[1]) / (cv2 * rho);
+            }
+        }
+        // slide the window one cell along DIR
+        for (int e = 0; e < 2; ++e) {
+            E[e][0] = E[e][1];
+            E[e][1] = E[e][2];
+            E[e][2] = Ep[e];
+            em[e] = ec[e];
+        }
+        for (int f = 0; f < 3; ++f) {
+            F[f][0] = F[f][1];
+            F[f][1] = F[f][2];
+            F[f][2] = Fp[f];
+            flm[f] = flc[f];
+        }
+        cvc_m = cvc_0;
+        cvc_0 = cvc_p;
+        cv1_0 = cv1_p;
+        cv1_p = cv1_pp;
+        rho_0 = rho_p;
+        rho_p = rho_pp;
+        w_0 = w_p;
+        dm_m = DML[0];
+        for (int j = 0; j < 4; ++j) {
+            DML[j] = DMH[j];
+            C2L[j] = C2H[j];
+            PL[j] = PH[j];
+        }
+        rv_c = rv_p;
+        dmv_m = dmv_c;
+        c += sd;
+    }
+}
+
+// Value of v held by the previous (lane - 1) or next (lane + 1) lane of the
+// wavefront (DPP wave_shr:1 / wave_shl:1 on both halves); the end lanes get
+// their own value.
+__device__ __forceinline__ double lane_prev(double v)
+{
+    const int2 h = __builtin_bit_cast(int2, v);
+    return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_update_dpp(h.x, h.x, 0x138, 0xf, 0xf, false),
+                                                __builtin_amdgcn_update_dpp(h.y, h.y, 0x138, 0xf, 0xf, false)));
+}
+__device__ __forceinline__ double lane_next(double v)
+{
+    const int2 h = __builtin_bit_cast(int2, v);
+    return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_update_dpp(h.x, h.x, 0x130, 0xf, 0xf, false),
+                                                __builtin_amdgcn_update_dpp(h.y, h.y, 0x130, 0xf, 0xf, false)));
+}
+
+// Lanes of one wavefront per z-row segment: a wave covers 64 consecutive
+// positions of a z-row flattened over (y, z), with two halo positions
+// beyond each end of the row (z0-2 .. z1+2), and successive waves overlap by
+// four lanes; lanes 2..61 write.
+const int kZLine = 2000;
+
+// R6' for the z sweep: the transport of f_update_all with every lane
+// loading only its own z (the values at z-1, z+1, z+2 and the fluxes of the
+// face below come from the neighbouring lanes), so each face flux, vertex
+// density and vertex mass flux is computed once.  The arithmetic of each
+// flux and update is the same expression on the same operands as in
+// f_update_all.  Every lane of the wave stays active through the lane
+// exchanges; lanes outside the range work on a clamped position and write
+// nothing.
+template <class T>
+KERNEL void f_update_zline(const Src<T> S)
+{
+    const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
+    const int g_ = grid_of(S.boff, S.ngrid, b_);
+    const EDev<T> &D_ = S.devs[g_];
+    const Geo G = D_.g;
+    const ArrE<T> A = state_view(D_.a, S.st);
+    const R3 R = {S.r.x0, G.nx + S.r.dx1, S.r.y0, G.ny + S.r.dy1, S.r.z0, G.nz + S.r.dz1};
+    const int nyr = R.y1 - R.y0 + 1, nzr = R.z1 - R.z0 + 1;
+    const int L = nzr + 4, P = nyr * L;
+    const int bpp = (P + 239) / 240;
+    const int lb = b_ - S.boff[g_];
+    const int xp = lb / bpp;
+    const int lane = (int)threadIdx.x & 63;
+    const int w = (lb - xp * bpp) * 4 + ((int)threadIdx.x >> 6);
+    int u = 60 * w + lane - 2;
+    const bool inside = u >= 0 && u < P;
+    u = u < 0 ? 0 : (u >= P ? P - 1 : u);
+    const int row = u / L;
+    const int ix = R.x0 + xp, iy = R.y0 + row, iz = R.z0 - 2 + (u - row * L);
+    const bool writer = inside && lane >= 2 && lane < 62 && iz >= R.z0 && iz <= R.z1;
+    const int sx = G.sx, sy = G.sy;
+    const int c = IX(ix, iy, iz);
+    const int oj[4] = {0, sx, sy, sx + sy};
+    const double dt = lare::gbl(A.s)->dt;
+    const Fld<T> R1 = RHO, C1 = A.cv1, C2 = A.cv2, W = VZ1;
+    const auto M = DM;
+    // own loads
+    double P4[4], D4[4], S4[4];
+    for (int j = 0; j < 4; ++j) {
+        P4[j] = R1[c + oj[j]] * C1[c + oj[j]];
+        D4[j] = M[c + oj[j]];
+        S4[j] = C2[c + oj[j]];
+    }
+    const double rho_0 = R1[c], cv1_0 = C1[c], cvc_0 = A.cvc1[c], w_0 = W[c];
+    const double fv = (W[c - sx] + w_0 + W[c - sx - sy] + W[c - sy]) * 0.25;
+    const double F[3] = {VX[c], VY[c], VZ[c]};
+    const double E[2] = {EE[c], EI[c]};
+    const double area_m = DYAC(ix, iz), area_e = DZAB(ix, iy);
+    // vertex density, vertex mass flux and vertex cv2 at c (corners at z+1
+    // from the next lane), in the reference's corner order
+    double sp = P4[0] + P4[1] + P4[2] + P4[3];
+    double q = D4[0] + D4[1] + D4[2] + D4[3];
+    double s2 = S4[0] + S4[1] + S4[2] + S4[3];
+    for (int j = 0; j < 4; ++j) {
+        sp = sp + lane_next(P4[j]);
+        q = q + lane_next(D4[j]);
+        s2 = s2 + lane_next(S4[j]);
+    }
+    const double rv = sp * (0.125 / cvc_0);
+    const double dmv = 0.125 * q;
+    const double cv2v = 0.125 * s2;
+    // neighbours along z
+    const double rv_p = lane_next(rv), w_p = lane_next(w_0), rho_p = lane_next(rho_0);
+    const double cv1_p = lane_next(cv1_0), cv1_pp = lane_next(cv1_p);
+    const double cvc_p = lane_next(cvc_0), cvc_m = lane_prev(cvc_0);
+    // momentum fluxes through vertex face c
+    double flc[3];
+    {
+        double v_advect = (w_0 + w_p) * 0.5;
+        double o_v = v_advect * dt * area_m;
+        double sign_v = (o_v >= 0.0) ? 1.0 : -1.0;
+        double vad_p = (sign_v + 1.0) * 0.5;
+        double vad_m = 1.0 - vad_p;
+        double dxci = cv1_p;
+        double dxcu = cv1_0 * vad_p + cv1_pp * vad_m;
+        double dxbu = cvc_0 * vad_p + cvc_p * vad_m;
+        double rhou = rv * vad_p + rv_p * vad_m;
+        for (int f = 0; f < 3; ++f) {
+            const double fp = lane_next(F[f]);
+            flc[f] = mom_flux_v<2>(lane_prev(F[f]), F[f], fp, lane_next(fp), o_v, sign_v, dxci, dxcu, dxbu, rhou,
+                                   dmv);
+        }
+    }
+    // energy fluxes through cell face c
+    double ec[2];
+    {
+        const double o_v = fv * dt * area_e;
+        for (int e = 0; e < 2; ++e) {
+            const double ep = lane_next(E[e]);
+            ec[e] = energy_flux_v(lane_prev(E[e]), E[e], ep, lane_next(ep), fv, o_v, cvc_m, cvc_0, cvc_p, cv1_0,
+                                  cv1_p, rho_0, rho_p, D4[0]);
+        }
+    }
+    // fluxes of the face below from the previous lane
+    const double flm[3] = {lane_prev(flc[0]), lane_prev(flc[1]), lane_prev(flc[2])};
+    const double em[2] = {lane_prev(ec[0]), lane_prev(ec[1])};
+    const double dmv_m = lane_prev(dmv), dm_m = lane_prev(D4[0]);
+    if (!writer) return;
+    const double rho_v1 = (rv * cvc_0 + dmv_m - dmv) / cv2v;
+    const double den = cv2v * rho_v1;
+    A.vx2[c] = (rv * F[0] * cvc_0 + flm[0] - flc[0]) / den;
+    A.vy2[c] = (rv * F[1] * cvc_0 + flm[1] - flc[1]) / den;
+    A.vz2[c] = (rv * F[2] * cvc_0 + flm[2] - flc[2]) / den;
+    if (ix >= 1 && iy >= 1 && iz >= 1) {
+        const double cv2 = S4[0];
+        const double rho = (rho_0 * cv1_0 + dm_m - D4[0]) / cv2;
+        A.rho2[c] = rho;
+        A.ee2[c] = (E[0] * cv1_0 * rho_0 + em[0] - ec[0]) / (cv2 * rho);
+        A.ei2[c] = (E[1] * cv1_0 * rho_0 + em[1] - ec[1]) / (cv2 * rho);
+    }
+}
+
 // The shifted sweep velocity the reference writes back into the registered
 // v1 component at the end of the sweep's vertex stage (xremap.cpp:193-203,
 // yremap.cpp:197-207, zremap.cpp:194-204), over (-2..n+1 along DIR, 0..n
@@ -567,24 +921,6 @@ KERNEL void f_wshift(const Src<T> S)
         w[c] = lare::gbl(A.fl[0])[c];
     else
         lare::gbl(A.fl[0])[c] = (w[c] + w[c + sdir<DIR>(G)]) * 0.5;
-    };
-    ELOOP(cell)
-}
-
-// B *= area (remap.cpp:46-50) and B /= (area + none_zero) (91-95) over each
-// component's full extent; the areas are dxab, dyab, dzab.
-template <class T, int COMP, int DIVIDE>
-KERNEL void f_b_area(const Src<T> S)
-{
-    EGRID
-    auto cell = [&](const int ix, const int iy, const int iz) {
-    const int c = IX(ix, iy, iz);
-    const Fld<T> B = COMP == 0 ? BX : COMP == 1 ? BY : BZ;
-    const double ar = COMP == 0 ? DXAB(iy, iz) : COMP == 1 ? DYAB(ix, iz) : DZAB(ix, iy);
-    if (DIVIDE)
-        B[c] /= (ar + DBL_EPSILON);
-    else
-        B[c] *= ar;
     };
     ELOOP(cell)
 }
@@ -1318,6 +1654,10 @@ void *field_ptr(const Grid &g, int f) { return g.fp32 ? field_ptr_t(g.e32.a, f) 
 
 // Block offsets of a launch over `subset` with range r (built outside graph
 // capture, by group_prepare or on first use).
+// Table order codes: 0 plane-major, 1 band-major, kMarch + 100 * DIR + K a
+// lane walking K cells along DIR (f_update_march).
+const int kMarch = 1000;
+
 Group::Tab &table(Group &gp, int subset, const RSpec &r, int order)
 {
     const std::array<int, 7> key{r.x0, r.dx1, r.y0, r.dy1, r.z0, r.dz1, order};
@@ -1333,8 +1673,18 @@ Group::Tab &table(Group &gp, int subset, const RSpec &r, int order)
                    nzr = (long)G.nz + r.dz1 - r.z0 + 1;
         boff.push_back((int)total);
         if (nxr > 0 && nyr > 0 && nzr > 0) {
-            const long bpp = (nyr * nzr + 255) / 256;
-            total += nxr * (order ? 8 * ((bpp + 7) / 8) : bpp);
+            if (order == kZLine) {
+                total += nxr * ((nyr * (nzr + 4) + 239) / 240);
+            } else if (order >= kMarch) {
+                // a lane walks K cells along DIR; the plane across DIR is
+                // (y, z) for DIR 0, (x, z) for DIR 1
+                const int mdir = (order - kMarch) / 100, K = (order - kMarch) % 100;
+                const long nd = mdir == 0 ? nxr : nyr, ns = mdir == 0 ? nyr : nxr;
+                total += ((ns * nzr + 255) / 256) * ((nd + K - 1) / K);
+            } else {
+                const long bpp = (nyr * nzr + 255) / 256;
+                total += nxr * (order ? 8 * ((bpp + 7) / 8) : bpp);
+            }
             t.cells += (double)(nxr * nyr * nzr);
         }
     }
@@ -1363,9 +1713,9 @@ std::string kname(const char *k)
 
 template <class T, typename... KA, typename... Args>
 void elaunch_(void (*k)(Src<T>, KA...), const char *name, double bpc, Group &gp, int subset, RSpec r, int st,
-              Args... args)
+              int order, Args... args)
 {
-    Group::Tab &t = table(gp, subset, r, gp.order);
+    Group::Tab &t = table(gp, subset, r, order);
     if (gp.planning || t.nblk == 0) return;
     Src<T> s{(const EDev<T> *)gp.devs[subset], t.boff, gp.n[subset], t.nblk, r, t.order, st};
     Prof *pf = gp.prof;
@@ -1386,8 +1736,8 @@ void elaunch_(void (*k)(Src<T>, KA...), const char *name, double bpc, Group &gp,
     }
 }
 // launch on the state copy 0 / on state copy st
-#define EL(k, bpc, gp, subset, r, ...) elaunch_(k, #k, bpc, gp, subset, r, 0, ##__VA_ARGS__)
-#define ELS(k, bpc, gp, subset, r, st, ...) elaunch_(k, #k, bpc, gp, subset, r, st, ##__VA_ARGS__)
+#define EL(k, bpc, gp, subset, r, ...) elaunch_(k, #k, bpc, gp, subset, r, 0, gp.order, ##__VA_ARGS__)
+#define ELS(k, bpc, gp, subset, r, st, ...) elaunch_(k, #k, bpc, gp, subset, r, st, gp.order, ##__VA_ARGS__)
 
 // Per-grid kernels (one lane per member).
 template <class T>
@@ -1598,7 +1948,10 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
     EL((f_prep<T, DIR>), 6 * w, gp, 0, rs(-1, 2, -1, 2, -1, 2), xpass, ypass, zpass);
     EL(f_cvc1<T>, 2 * w, gp, 0, rs(-1, 1, -1, 1, -1, 1));
     EL((f_ct_fluxes<T, DIR>), 7 * w + 2 * 8, gp, 0, all0);
-    EL((f_ct_update<T, DIR>), 6 * w + 2 * 8, gp, 0, all0);
+    if (DIR == 1)
+        EL((f_ct_update<T, DIR>), 6 * w + 2 * 8, gp, 0, all0);
+    else  // also the B area product (x) or quotient (z) over the B extents
+        EL((f_ct_update<T, DIR>), 6 * w + 2 * 8, gp, 0, rs(-2, 2, -2, 2, -2, 2));
     // mass flux: its reference range, widened to the ghost faces along DIR
     ELS((f_mass_flux<T, DIR>), 4 * w + 8, gp, 0,
         rs(DIR == 0 ? -1 : 0, DIR == 0 ? 2 : 1, DIR == 1 ? -1 : 0, DIR == 1 ? 2 : 1, DIR == 2 ? -1 : 0,
@@ -1607,7 +1960,19 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
     ebcs<T>(gp, VDM, DIR);
     // reads rho e_e e_i v cv1 cv2 cvc1 VX1..VZ1 (sweep component) dm;
     // writes rho e_e e_i v
-    ELS((f_update_all<T, DIR>), 16 * w + 8, gp, 0, all0, st);
+    if (DIR == 2 && gp.zline) {
+        elaunch_(f_update_zline<T>, "f_update_zline", 16 * w + 8, gp, 0, all0, st, kZLine);
+    } else if (DIR == 2 || gp.march_k <= 0) {
+        ELS((f_update_all<T, DIR>), 16 * w + 8, gp, 0, all0, st);
+    } else {
+        constexpr int MD = DIR == 2 ? 0 : DIR;
+        const int K = gp.march_k >= 16 ? 16 : gp.march_k >= 8 ? 8 : 4;
+        void (*k)(Src<T>) = K == 16  ? f_update_march<T, MD, 16>
+                            : K == 8 ? f_update_march<T, MD, 8>
+                                     : f_update_march<T, MD, 4>;
+        elaunch_(k, DIR == 0 ? "f_update_march<0>" : "f_update_march<1>", 16 * w + 8, gp, 0, all0, st,
+                 kMarch + 100 * DIR + K);
+    }
     bool wshift = false;
     for (const Grid *g : gp.mem) wshift = wshift || g->bck[9 + DIR][DIR][0] == 0 || g->bck[9 + DIR][DIR][1] == 0;
     if (wshift) {
@@ -1625,15 +1990,9 @@ template <class T>
 void end_t(Group &gp)
 {
     const double w = (double)sizeof(T);
-    EL((f_b_area<T, 0, 0>), 2 * w, gp, 0, rs(-2, 2, -1, 2, -1, 2));
-    EL((f_b_area<T, 1, 0>), 2 * w, gp, 0, rs(-1, 2, -2, 2, -1, 2));
-    EL((f_b_area<T, 2, 0>), 2 * w, gp, 0, rs(-1, 2, -1, 2, -2, 2));
     sweep_t<T, 0>(gp, 1.0, 1.0, 1.0, 1);
     sweep_t<T, 1>(gp, 0.0, 1.0, 1.0, 0);
     sweep_t<T, 2>(gp, 0.0, 0.0, 1.0, 1);
-    EL((f_b_area<T, 0, 1>), 2 * w, gp, 0, rs(-2, 2, -1, 2, -1, 2));
-    EL((f_b_area<T, 1, 1>), 2 * w, gp, 0, rs(-1, 2, -2, 2, -1, 2));
-    EL((f_b_area<T, 2, 1>), 2 * w, gp, 0, rs(-1, 2, -1, 2, -2, 2));
     ebcs<T>(gp, VB);
     per_grid<T>(g_step_done<T>, "g_step_done", gp);
 }
@@ -1647,6 +2006,8 @@ int group_init(Group &gp, const std::vector<Grid *> &members, hipStream_t stream
     gp.stream = stream;
     gp.prof = prof;
     if (const char *o = std::getenv("LARE_ORDER")) gp.order = std::atoi(o) != 0;
+    if (const char *o = std::getenv("LARE_MARCH")) gp.march_k = std::atoi(o);
+    if (const char *o = std::getenv("LARE_ZLINE")) gp.zline = std::atoi(o) != 0;
     if (members.empty()) return 0;
     gp.fp32 = members[0]->fp32;
     for (size_t m = 0; m < members.size(); ++m) {
