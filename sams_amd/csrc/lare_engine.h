@@ -48,6 +48,7 @@ struct Group {
     int order = 1;    // workgroup order within a grid (Src::order)
     int march_k = 16;   // cells per lane of the x and y transport kernels (0: one)
     bool zline = true;  // z transport with neighbours exchanged between lanes
+    int zx_k = 8;       // cells per lane of the z-line x-march kernels (0: one cell per lane)
     // one captured step of the whole group
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;

@@ -790,6 +790,49 @@ __device__ __forceinline__ double lane_next(double v)
 // four lanes; lanes 2..61 write.
 const int kZLine = 2000;
 
+// Lanes of a z-line x-march kernel: each wavefront covers 64 consecutive
+// positions of the z-rows of one (y, z) plane flattened z fastest, with H
+// halo positions beyond each end of a row; successive wavefronts overlap by
+// 2H lanes and lanes H..63-H write.  Each lane walks K cells along x, after
+// one warm-up position (x0 - 1) that writes nothing.  Values at z +- 1 come
+// from the neighbouring lanes and the values of plane x - 1 from the
+// previous position, so each value is loaded once.
+const int kZX = 3000;  // table code kZX + 100 * H + K
+
+struct ZXPos {
+    int ix0, ix1;  // first (warm-up) and last x of the lane
+    int iy, iz;
+    bool wz;       // lane writes (inside, not a halo lane, z in range)
+};
+
+template <int H, int K, class T>
+__device__ __forceinline__ ZXPos zx_map(const Src<T> &S, const Geo &G, int g_, int b_)
+{
+    const R3 R = {S.r.x0, G.nx + S.r.dx1, S.r.y0, G.ny + S.r.dy1, S.r.z0, G.nz + S.r.dz1};
+    const int nyr = R.y1 - R.y0 + 1, nzr = R.z1 - R.z0 + 1;
+    const int L = nzr + 2 * H, P = nyr * L;
+    const int per = 4 * (64 - 2 * H);
+    const int bpp = (P + per - 1) / per;
+    const int lb = b_ - S.boff[g_];
+    const int xc = lb / bpp;
+    const int lane = (int)threadIdx.x & 63;
+    const int w = (lb - xc * bpp) * 4 + ((int)threadIdx.x >> 6);
+    int u = (64 - 2 * H) * w + lane - H;
+    const bool inside = u >= 0 && u < P;
+    u = u < 0 ? 0 : (u >= P ? P - 1 : u);
+    const int row = u / L;
+    ZXPos z;
+    z.iy = R.y0 + row;
+    z.iz = R.z0 - H + (u - row * L);
+    z.wz = inside && lane >= H && lane < 64 - H && z.iz >= R.z0 && z.iz <= R.z1;
+    // a halo position beyond the box (a range reaching z = n+2) loads at
+    // n+2; only a lane above it would read its values, and none writes
+    if (z.iz > G.nz + 2) z.iz = G.nz + 2;
+    z.ix0 = R.x0 + xc * K - 1;
+    z.ix1 = z.ix0 + K < R.x1 ? z.ix0 + K : R.x1;
+    return z;
+}
+
 // R6' for the z sweep: the transport of f_update_all with every lane
 // loading only its own z (the values at z-1, z+1, z+2 and the fluxes of the
 // face below come from the neighbouring lanes), so each face flux, vertex
@@ -1146,6 +1189,139 @@ KERNEL OCC(4) void g_visc_dt(const Src<T> S)
     }
 }
 
+// L4 as a z-line x-march (H = 1): the same arithmetic as g_visc_dt.  Each
+// lane carries the velocity columns (x, y) and (x, y-1) of the previous
+// position and loads those of x + 1 ahead (the viscous force reads x + 1);
+// the squared differences of the corner ring at z - 1 are those of the lane
+// below at its z.
+template <class T, int K>
+KERNEL OCC(2) void g_visc_dt_zx(const Src<T> S)
+{
+    const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
+    const int g_ = grid_of(S.boff, S.ngrid, b_);
+    const EDev<T> &D_ = S.devs[g_];
+    const Geo G = D_.g;
+    const ArrE<T> A = state_view(D_.a, S.st);
+    const Phys &P = D_.p;
+    const double gas_gamma = P.gas_gamma, mu0 = P.mu0;
+    const ZXPos Z = zx_map<1, K>(S, G, g_, b_);
+    const int sx = G.sx, sy = G.sy;
+    const int iy = Z.iy, iz = Z.iz;
+    const Fld<T> V3[3] = {VX, VY, VZ};
+    const double dy = DYB(iy), dz = DZB(iz);
+    const bool fz = iy <= G.ny && iz <= G.nz;  // force and dt range, y and z
+    const double ax = 0.25 * DXAB(iy, iz);
+    int c = IX(Z.ix0, iy, iz);
+    // columns at own z: O = (x, y), Om = (x, y-1), M = (x-1, y), Mm = (x-1, y-1)
+    double O[3], Om[3], M[3] = {0, 0, 0}, Mm[3] = {0, 0, 0};
+    double pM[3] = {0, 0, 0}, pMm[3] = {0, 0, 0};
+    for (int k = 0; k < 3; ++k) {
+        O[k] = V3[k][c];
+        Om[k] = V3[k][c - sy];
+    }
+    double a6_m = 0.0, a9_m = 0.0, a12_m = 0.0, vxbp_m = 0.0;
+    double al2_m = 0.0, al2p_m = 0.0, al3_m = 0.0, al3y_m = 0.0;
+    double al1y_n = A.alpha1[c + sy];  // alpha1 at (x, y+1), loaded ahead
+    double m = DBL_MAX;
+    for (int ix = Z.ix0; ix <= Z.ix1; ++ix, c += sx) {
+        double N[3], Nm[3], Yp[3], pO[3], pOm[3];
+        for (int k = 0; k < 3; ++k) {
+            N[k] = V3[k][c + sx];
+            Nm[k] = V3[k][c + sx - sy];
+            Yp[k] = V3[k][c + sy];
+            pO[k] = lane_prev(O[k]);
+            pOm[k] = lane_prev(Om[k]);
+        }
+        const double al1 = A.alpha1[c], al1y = al1y_n, al2 = A.alpha2[c], al2y = A.alpha2[c + sy];
+        const double al3 = A.alpha3[c], al3y = A.alpha3[c - sy];
+        al1y_n = A.alpha1[c + sx + sy];
+        const double al1p = lane_next(al1), al1yp = lane_next(al1y), al2p = lane_next(al2);
+        const double al1ynp = lane_next(al1y_n), al2yp = lane_next(al2y), al3p = lane_next(al3);
+#define ZSQD(p, q) \
+    (((p)[0] - (q)[0]) * ((p)[0] - (q)[0]) + ((p)[1] - (q)[1]) * ((p)[1] - (q)[1]) + ((p)[2] - (q)[2]) * ((p)[2] - (q)[2]))
+        const double a5 = ZSQD(Mm, Om), a6 = ZSQD(Om, O), a7 = ZSQD(O, M), a8 = a6_m;
+        const double a9 = ZSQD(pO, O), a12 = ZSQD(pOm, Om), a10 = a9_m, a11 = a12_m;
+#undef ZSQD
+        const double a1 = lane_prev(a5), a2 = lane_prev(a6), a3 = lane_prev(a7), a4 = lane_prev(a8);
+        const double vxbp = (O[0] + Om[0] + pO[0] + pOm[0]) * ax;
+        const double fO[3] = {lane_next(O[0]), lane_next(O[1]), lane_next(O[2])};
+        if (Z.wz && ix > Z.ix0) {
+            double pv = 0.0;
+            pv = pmax(pv, -al1 * sqrt(a1));
+            pv = pmax(pv, -al2 * sqrt(a2));
+            pv = pmax(pv, -al3 * sqrt(a9));
+            const double dx = DXB(ix);
+            double vh = -0.25 * dy * dz * al1 * a1 - 0.25 * dx * dz * al2 * a2 - 0.25 * dy * dz * al1y * a3 -
+                        0.25 * dx * dz * al2_m * a4 - 0.25 * dy * dz * al1p * a5 - 0.25 * dx * dz * al2p * a6 -
+                        0.25 * dy * dz * al1yp * a7 - 0.25 * dx * dz * al2p_m * a8 - 0.25 * dx * dy * al3 * a9 -
+                        0.25 * dx * dy * al3_m * a10 - 0.25 * dx * dy * al3y_m * a11 - 0.25 * dx * dy * al3y * a12;
+            const double cv = CV(ix, iy, iz);
+            A.visc_heat[c] = vh / cv;
+            if (fz && ix <= G.nx) {
+                double b1 = al1yp * dy * dz;
+                double b2 = al1ynp * dy * dz;
+                double b3 = al2p * dx * dz;
+                double b4 = al2yp * dx * dz;
+                double b5 = al3 * dx * dy;
+                double b6 = al3p * dx * dy;
+                double cvv = 0.125 * CV8SUM(ix, iy, iz);
+                double fv[3];
+                for (int k = 0; k < 3; ++k)
+                    fv[k] = (b1 * (O[k] - M[k]) + b2 * (O[k] - N[k]) + b3 * (O[k] - Om[k]) + b4 * (O[k] - Yp[k]) +
+                             b5 * (O[k] - pO[k]) + b6 * (O[k] - fO[k])) / cvv;
+                A.fxv[c] = fv[0];
+                A.fyv[c] = fv[1];
+                A.fzv[c] = fv[2];
+                double rho0 = pmax(RHO[c], DBL_EPSILON);
+                double press = PE_OF(c) + PI_OF(c);
+                double cs2 = gas_gamma * press / rho0;
+                double w1 = (BX[c] * BX[c] + BY[c] * BY[c] + BZ[c] * BZ[c]) / mu0 / rho0;
+                double c_visc2 = pv / rho0;
+                double length = pmin(pmin(dx, dy), dz);
+                double dt1 = length / (sqrt(c_visc2) + sqrt(cs2 + w1 + c_visc2));
+                double ay = 0.25 * DYAB(ix, iz);
+                double az = 0.25 * DZAB(ix, iy);
+                double vxbm = vxbp_m;
+                double vybm = (Om[1] + Mm[1] + pOm[1] + pMm[1]) * ay;
+                double vybp = (O[1] + M[1] + pO[1] + pM[1]) * ay;
+                double vzbm = (pO[2] + pM[2] + pOm[2] + pMm[2]) * az;
+                double vzbp = (O[2] + M[2] + Om[2] + Mm[2]) * az;
+                double dvx = fabs(vxbp - vxbm), dvy = fabs(vybp - vybm), dvz = fabs(vzbp - vzbm);
+                double dt2 = cv / pmax(pmax(pmax(fabs(vxbm), fabs(vxbp)), dvx), 1.0e-10 * cv);
+                double dt3 = cv / pmax(pmax(pmax(fabs(vybm), fabs(vybp)), dvy), 1.0e-10 * cv);
+                double dt4 = cv / pmax(pmax(pmax(fabs(vzbm), fabs(vzbp)), dvz), 1.0e-10 * cv);
+                m = pmin(m, pmin(pmin(pmin(dt1, dt2), dt3), dt4));
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            M[k] = O[k];
+            Mm[k] = Om[k];
+            O[k] = N[k];
+            Om[k] = Nm[k];
+            pM[k] = pO[k];
+            pMm[k] = pOm[k];
+        }
+        a6_m = a6;
+        a9_m = a9;
+        a12_m = a12;
+        vxbp_m = vxbp;
+        al2_m = al2;
+        al2p_m = al2p;
+        al3_m = al3;
+        al3y_m = al3y;
+    }
+    for (int off = 32; off > 0; off >>= 1) m = pmin(m, __shfl_xor(m, off, 64));
+    __shared__ double wmin[4];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    if (lane == 0) wmin[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = pmin(pmin(pmin(wmin[0], wmin[1]), wmin[2]), wmin[3]);
+        if (b >= 0.0 && b < DBL_MAX) atomicMin(&A.s->dtmin_bits, (unsigned long long)__double_as_longlong(b));
+    }
+}
+
 // L5 per grid: dt = dt_multiplier * min (lagran.cpp:380-443), the runner's
 // minimum over packages (runner.h:158-169: one package here), dtr = dt and
 // the resistive substep dt = dt / (int(dt/dtr)+1) = dt / 2 (159-161).
@@ -1250,6 +1426,128 @@ KERNEL void g_predict(const Src<T> S)
     }
     };
     ELOOP(cell)
+}
+
+// L6 as a z-line x-march (H = 1): the same arithmetic as g_predict.  The
+// eight corners (x-1..x, y-1..y, z-1..z) of the velocities are this lane's
+// columns (x, y) and (x, y-1), the previous position's, and the lane below.
+template <class T, int K>
+KERNEL OCC(4) void g_predict_zx(const Src<T> S)
+{
+    const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
+    const int g_ = grid_of(S.boff, S.ngrid, b_);
+    const EDev<T> &D_ = S.devs[g_];
+    const Geo G = D_.g;
+    const ArrE<T> A = state_view(D_.a, S.st);
+    const Phys &P = D_.p;
+    const double gas_gamma = P.gas_gamma;
+    const bool res = P.resistive != 0;
+    const ZXPos Z = zx_map<1, K>(S, G, g_, b_);
+    const int sx = G.sx, sy = G.sy;
+    const int iy = Z.iy, iz = Z.iz;
+    const double dt = lare::gbl(A.s)->dt;
+    const Fld<T> V3[3] = {VX, VY, VZ};
+    enum { ooo, moo, omo, oom, mmo, mom, omm, mmm };
+    double M[3] = {0, 0, 0}, Mm[3] = {0, 0, 0}, pM[3] = {0, 0, 0}, pMm[3] = {0, 0, 0};
+    double bx_m = 0.0;
+    int c = IX(Z.ix0, iy, iz);
+    for (int ix = Z.ix0; ix <= Z.ix1; ++ix, c += sx) {
+        double O[3], Om[3], pO[3], pOm[3];
+        for (int k = 0; k < 3; ++k) {
+            O[k] = V3[k][c];
+            Om[k] = V3[k][c - sy];
+        }
+        for (int k = 0; k < 3; ++k) {
+            pO[k] = lane_prev(O[k]);
+            pOm[k] = lane_prev(Om[k]);
+        }
+        double bz_own = 0.0, bx_own = 0.0;
+        if (!res) {
+            bx_own = BX[c];
+            bz_own = BZ[c];
+        }
+        const double bz_m = lane_prev(bz_own);
+        if (Z.wz && ix > Z.ix0) {
+            double W[3][8];
+            for (int k = 0; k < 3; ++k) {
+                W[k][ooo] = O[k];
+                W[k][moo] = M[k];
+                W[k][omo] = Om[k];
+                W[k][oom] = pO[k];
+                W[k][mmo] = Mm[k];
+                W[k][mom] = pM[k];
+                W[k][omm] = pOm[k];
+                W[k][mmm] = pMm[k];
+            }
+            const double *X = W[0], *Y = W[1], *Zv = W[2];
+            double vxb = 0.25 * (X[ooo] + X[omo] + X[oom] + X[omm]);
+            double vxbm = 0.25 * (X[moo] + X[mmo] + X[mom] + X[mmm]);
+            double vyb = 0.25 * (Y[ooo] + Y[moo] + Y[oom] + Y[mom]);
+            double vybm = 0.25 * (Y[omo] + Y[mmo] + Y[omm] + Y[mmm]);
+            double vzb = 0.25 * (Zv[ooo] + Zv[moo] + Zv[omo] + Zv[mmo]);
+            double vzbm = 0.25 * (Zv[oom] + Zv[mom] + Zv[omm] + Zv[mmm]);
+            const double vol = CV(ix, iy, iz);
+            const double axx = DXAB(iy, iz), ayy = DYAB(ix, iz), azz = DZAB(ix, iy);
+            double dvxdx = (vxb * axx - vxbm * axx) / vol;
+            double dvydy = (vyb * ayy - vybm * ayy) / vol;
+            double dvzdz = (vzb * azz - vzbm * azz) / vol;
+            double dv = (dvxdx + dvydy + dvzdz) * dt / 2.0;
+            const double cv1 = vol * (1.0 + dv);
+            A.cv1[c] = cv1;
+            vxb = 0.25 * (X[ooo] + X[moo] + X[oom] + X[mom]);
+            vxbm = 0.25 * (X[omo] + X[mmo] + X[omm] + X[mmm]);
+            vyb = 0.25 * (Y[ooo] + Y[omo] + Y[oom] + Y[omm]);
+            vybm = 0.25 * (Y[moo] + Y[mmo] + Y[mom] + Y[mmm]);
+            double dvxdy = (vxb * ayy - vxbm * ayy) / vol;
+            double dvydx = (vyb * axx - vybm * axx) / vol;
+            vxb = 0.25 * (X[ooo] + X[moo] + X[omo] + X[mmo]);
+            vxbm = 0.25 * (X[oom] + X[mom] + X[omm] + X[mmm]);
+            double vzb2 = 0.25 * (Zv[ooo] + Zv[omo] + Zv[oom] + Zv[omm]);
+            double vzbm2 = 0.25 * (Zv[moo] + Zv[mmo] + Zv[mom] + Zv[mmm]);
+            double dvxdz = (vxb * azz - vxbm * azz) / vol;
+            double dvzdx = (vzb2 * axx - vzbm2 * axx) / vol;
+            double vyb3 = 0.25 * (Y[ooo] + Y[moo] + Y[omo] + Y[mmo]);
+            double vybm3 = 0.25 * (Y[oom] + Y[mom] + Y[omm] + Y[mmm]);
+            double vzb3 = 0.25 * (Zv[ooo] + Zv[moo] + Zv[oom] + Zv[mom]);
+            double vzbm3 = 0.25 * (Zv[omo] + Zv[mmo] + Zv[omm] + Zv[mmm]);
+            double dvydz = (vyb3 * azz - vybm3 * azz) / vol;
+            double dvzdy = (vzb3 * ayy - vzbm3 * ayy) / vol;
+            double bx1, by1, bz1;
+            if (res) {
+                bx1 = A.bx1[c];
+                by1 = A.by1[c];
+                bz1 = A.bz1[c];
+            } else {
+                bx1 = 0.5 * (bx_own + bx_m);
+                by1 = 0.5 * (BY[c] + BY[c - sy]);
+                bz1 = 0.5 * (bz_own + bz_m);
+            }
+            double w3 = bx1 * dvxdx + by1 * dvxdy + bz1 * dvxdz;
+            double w4 = bx1 * dvydx + by1 * dvydy + bz1 * dvydz;
+            double w5 = bx1 * dvzdx + by1 * dvzdy + bz1 * dvzdz;
+            A.bx1[c] = (bx1 + w3 * dt / 2.0) / (1.0 + dv) * cv1;
+            A.by1[c] = (by1 + w4 * dt / 2.0) / (1.0 + dv) * cv1;
+            A.bz1[c] = (bz1 + w5 * dt / 2.0) / (1.0 + dv) * cv1;
+            if (ix >= 0 && iy >= 0 && iz >= 0 && ix <= G.nx + 1 && iy <= G.ny + 1 && iz <= G.nz + 1) {
+                double rho = RHO[c];
+                double dvp = cv1 / vol - 1.0;
+                double pe0 = res ? (double)A.p_e[c] : PE_OF(c);
+                double pi0 = PI_OF(c);
+                double e1_e = EE[c] - pe0 * dvp / rho;
+                double e1_i = EI[c] - pi0 * dvp / rho;
+                e1_i += A.visc_heat[c] * dt / 2.0 / rho;
+                A.p_e[c] = e1_e * (gas_gamma - 1.0) * rho * vol / cv1;
+                A.p_i[c] = e1_i * (gas_gamma - 1.0) * rho * vol / cv1;
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            M[k] = O[k];
+            Mm[k] = Om[k];
+            pM[k] = pO[k];
+            pMm[k] = pOm[k];
+        }
+        bx_m = bx_own;
+    }
 }
 
 // L7 over (0..n)^3: forces and half-step velocities (lagran.cpp:619-731);
@@ -1383,6 +1681,102 @@ KERNEL OCC(2) void g_correct(const Src<T> S)
     A.rho2[c] = rho / (1.0 + dv);
     };
     ELOOP(cell)
+}
+
+// C (corrector) as a z-line x-march (H = 1): the same arithmetic as g_correct.
+// Corner values of the shock heating at (x, y, z) come from this lane (z),
+// the lanes at z -+ 1 and the previous position (x - 1); each DOT product
+// is formed once and reused by the cells that share it (a DOT of two
+// points is the same value in either order: both differences change sign).
+template <class T, int K>
+KERNEL OCC(4) void g_correct_zx(const Src<T> S)
+{
+    const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
+    const int g_ = grid_of(S.boff, S.ngrid, b_);
+    const EDev<T> &D_ = S.devs[g_];
+    const Geo G = D_.g;
+    const ArrE<T> A = state_view(D_.a, S.st);
+    const ZXPos Z = zx_map<1, K>(S, G, g_, b_);
+    const int sx = G.sx, sy = G.sy;
+    const int iy = Z.iy, iz = Z.iz;
+    const double dt = lare::gbl(A.s)->dt;
+    const Fld<T> V6[6] = {VX, VY, VZ, VX1, VY1, VZ1};
+    const double dy = DYB(iy), dz = DZB(iz);
+    const bool cyz = iy >= 1 && iz >= 1;
+    // plane x - 1: column (x-1, y) = M, (x-1, y-1) = Mm at z; their z-1
+    // values of VY1, VZ1; the carried products and coefficients
+    double M[6] = {0, 0, 0, 0, 0, 0}, Mm[6] = {0, 0, 0, 0, 0, 0};
+    double pM_vy1 = 0.0, pM_vz1 = 0.0, pMm_vy1 = 0.0, pMm_vz1 = 0.0;
+    double a2_m = 0.0, a9_m = 0.0, a12_m = 0.0, vxb_m = 0.0;
+    double al2_m = 0.0, al2p_m = 0.0, al3_m = 0.0, al3y_m = 0.0;
+    int c = IX(Z.ix0, iy, iz);
+    for (int ix = Z.ix0; ix <= Z.ix1; ++ix, c += sx) {
+        double O[6], Om[6], pO[6], pOm[6];
+        for (int k = 0; k < 6; ++k) {
+            O[k] = V6[k][c];
+            Om[k] = V6[k][c - sy];
+        }
+        for (int k = 0; k < 6; ++k) {
+            pO[k] = lane_prev(O[k]);
+            pOm[k] = lane_prev(Om[k]);
+        }
+        const double al1 = A.alpha1[c], al1y = A.alpha1[c + sy], al2 = A.alpha2[c], al3 = A.alpha3[c],
+                     al3y = A.alpha3[c - sy];
+        const double al1p = lane_next(al1), al1yp = lane_next(al1y), al2p = lane_next(al2);
+#define ZDOT(p, q) \
+    (1 * (((p)[0] - (q)[0]) * ((p)[3] - (q)[3]) + ((p)[1] - (q)[1]) * ((p)[4] - (q)[4]) + ((p)[2] - (q)[2]) * ((p)[5] - (q)[5])))
+        const double a1 = ZDOT(Mm, Om), a2 = ZDOT(Om, O), a3 = ZDOT(O, M), a4 = a2_m;
+        const double a9 = ZDOT(pO, O), a12 = ZDOT(pOm, Om), a10 = a9_m, a11 = a12_m;
+#undef ZDOT
+        const double a5 = lane_next(a1), a6 = lane_next(a2), a7 = lane_next(a3), a8 = lane_next(a4);
+        const double vxb = 0.25 * (O[3] + Om[3] + pO[3] + pOm[3]);
+        if (Z.wz && ix > Z.ix0) {
+            const double rv = A.rho_v[c];
+            A.vx2[c] = O[0] + dt * A.fxv[c] / rv;
+            A.vy2[c] = O[1] + dt * A.fyv[c] / rv;
+            A.vz2[c] = O[2] + dt * A.fzv[c] / rv;
+            if (ix >= 1 && cyz) {
+                const double dx = DXB(ix);
+                const double vh =
+                    (-0.25 * dy * dz * al1 * a1 - 0.25 * dx * dz * al2 * a2 - 0.25 * dy * dz * al1y * a3 -
+                     0.25 * dx * dz * al2_m * a4 - 0.25 * dy * dz * al1p * a5 - 0.25 * dx * dz * al2p * a6 -
+                     0.25 * dy * dz * al1yp * a7 - 0.25 * dx * dz * al2p_m * a8 - 0.25 * dy * dy * al3 * a9 -
+                     0.25 * dx * dy * al3_m * a10 - 0.25 * dy * dy * al3y_m * a11 - 0.25 * dx * dy * al3y * a12);
+                const double vol = CV(ix, iy, iz);
+                const double visc_heat = pmax(vh / vol, 0.0);
+                const double vxbm = vxb_m;
+                const double vyb = 0.25 * (O[4] + M[4] + pO[4] + pM_vy1);
+                const double vybm = 0.25 * (Om[4] + Mm[4] + pOm[4] + pMm_vy1);
+                const double vzb = 0.25 * (O[5] + M[5] + Om[5] + Mm[5]);
+                const double vzbm = 0.25 * (pO[5] + pM_vz1 + pOm[5] + pMm_vz1);
+                const double axx = DXAB(iy, iz), ayy = DYAB(ix, iz), azz = DZAB(ix, iy);
+                double dvxdx = (vxb * axx - vxbm * axx) / vol;
+                double dvydy = (vyb * ayy - vybm * ayy) / vol;
+                double dvzdz = (vzb * azz - vzbm * azz) / vol;
+                double dv = (dvxdx + dvydy + dvzdz) * dt;
+                double rho = RHO[c];
+                A.ee2[c] = EE[c] - dv * A.p_e[c] / rho;
+                A.ei2[c] = EI[c] + (dt * visc_heat - dv * A.p_i[c]) / rho;
+                A.rho2[c] = rho / (1.0 + dv);
+            }
+        }
+        for (int k = 0; k < 6; ++k) {
+            M[k] = O[k];
+            Mm[k] = Om[k];
+        }
+        pM_vy1 = pO[4];
+        pM_vz1 = pO[5];
+        pMm_vy1 = pOm[4];
+        pMm_vz1 = pOm[5];
+        a2_m = a2;
+        a9_m = a9;
+        a12_m = a12;
+        vxb_m = vxb;
+        al2_m = al2;
+        al2p_m = al2p;
+        al3_m = al3;
+        al3y_m = al3y;
+    }
 }
 
 // ------------------------------------------------- resistive substeps
@@ -1673,7 +2067,11 @@ Group::Tab &table(Group &gp, int subset, const RSpec &r, int order)
                    nzr = (long)G.nz + r.dz1 - r.z0 + 1;
         boff.push_back((int)total);
         if (nxr > 0 && nyr > 0 && nzr > 0) {
-            if (order == kZLine) {
+            if (order >= kZX) {
+                const int H = (order - kZX) / 100, K = (order - kZX) % 100;
+                const long per = 4 * (64 - 2 * H);
+                total += ((nxr + K - 1) / K) * ((nyr * (nzr + 2 * H) + per - 1) / per);
+            } else if (order == kZLine) {
                 total += nxr * ((nyr * (nzr + 4) + 239) / 240);
             } else if (order >= kMarch) {
                 // a lane walks K cells along DIR; the plane across DIR is
@@ -1903,7 +2301,10 @@ void start_a_t(Group &gp)
     EL(g_cs<T>, 7 * w, gp, 0, C);
     EL(g_vertex<T>, 4 * w, gp, 0, V);
     EL(g_alpha<T>, 8 * w, gp, 0, rs(-1, 1, -1, 2, 0, 2));
-    EL(g_visc_dt<T>, 16 * w, gp, 0, rs(0, 1, 0, 1, 0, 1));
+    if (gp.zx_k > 0)
+        elaunch_(g_visc_dt_zx<T, 8>, "g_visc_dt", 16 * w, gp, 0, rs(0, 1, 0, 1, 0, 1), 0, kZX + 100 + 8);
+    else
+        EL(g_visc_dt<T>, 16 * w, gp, 0, rs(0, 1, 0, 1, 0, 1));
     per_grid<T>(g_dt_final<T>, "g_dt_final", gp);
 }
 
@@ -1920,7 +2321,10 @@ void start_b_t(Group &gp, int substeps)
             ebcs<T>(gp, VBE, -1, 1);
         }
     }
-    EL(g_predict<T>, 16 * w, gp, 0, C);
+    if (gp.zx_k > 0)
+        elaunch_(g_predict_zx<T, 8>, "g_predict", 16 * w, gp, 0, C, 0, kZX + 100 + 8);
+    else
+        EL(g_predict<T>, 16 * w, gp, 0, C);
     EL(g_forces<T>, 19 * w, gp, 0, rs(0, 0, 0, 0, 0, 0));
     ebcs<T>(gp, VV1);
 }
@@ -1933,7 +2337,10 @@ void half_t(Group &gp)
     const double w = (double)sizeof(T);
     // reads v v1 alpha1..3 rho_v fx_visc..fz_visc rho e_e e_i p_e p_i;
     // writes v rho e_e e_i
-    EL(g_correct<T>, 22 * w, gp, 0, rs(0, 0, 0, 0, 0, 0));
+    if (gp.zx_k > 0)
+        elaunch_(g_correct_zx<T, 8>, "g_correct", 22 * w, gp, 0, rs(0, 0, 0, 0, 0, 0), 0, kZX + 100 + 8);
+    else
+        EL(g_correct<T>, 22 * w, gp, 0, rs(0, 0, 0, 0, 0, 0));
     // velocity_bcs, energy_bcs, density_bcs (independent variables)
     ebcs<T>(gp, VCORR_2);
     for (Grid *g : gp.mem) g->st = 1;
@@ -2008,6 +2415,7 @@ int group_init(Group &gp, const std::vector<Grid *> &members, hipStream_t stream
     if (const char *o = std::getenv("LARE_ORDER")) gp.order = std::atoi(o) != 0;
     if (const char *o = std::getenv("LARE_MARCH")) gp.march_k = std::atoi(o);
     if (const char *o = std::getenv("LARE_ZLINE")) gp.zline = std::atoi(o) != 0;
+    if (const char *o = std::getenv("LARE_ZX")) gp.zx_k = std::atoi(o);
     if (members.empty()) return 0;
     gp.fp32 = members[0]->fp32;
     for (size_t m = 0; m < members.size(); ++m) {
