@@ -134,8 +134,12 @@ int lare_steps(lare_ctx *ctx, int32_t nsteps, double *dt_seq);
 int lare_sync(lare_ctx *ctx);
 
 /* Kernel set (no counterpart in the reference): 1 (default) runs the fused
- * kernels, 0 one kernel per reference kernel over the reference's ranges.
- * Both give bit-identical results; 0 exists to localise a parity failure. */
+ * kernels, choosing the lane-walking forms (a lane advancing several cells
+ * and reusing the values its neighbours share) for launches of 300000
+ * cells or more; 2 uses the lane-walking forms on every launch, 3
+ * never; 0 runs one kernel per reference kernel over the reference's
+ * ranges.  All give bit-identical results; 0, 2 and 3 exist to localise a
+ * parity failure and to test every form on small grids. */
 int lare_set_kernel_set(lare_ctx *ctx, int32_t kernel_set);
 
 /* Whether lare_steps replays one captured HIP graph of the step (default 1;

@@ -83,11 +83,14 @@ void drop_graphs(lare_ctx *c)
 
 int ensure_solo(lare_ctx *c)
 {
-    if (c->solo_ready) return 0;
-    std::string err;
-    int rc = lare::group_init(c->solo, {&c->g}, c->g.stream, &c->g.prof, err);
-    if (rc) return fail(rc, "grid group: " + err);
-    c->solo_ready = true;
+    if (!c->solo_ready) {
+        std::string err;
+        int rc = lare::group_init(c->solo, {&c->g}, c->g.stream, &c->g.prof, err);
+        if (rc) return fail(rc, "grid group: " + err);
+        c->solo_ready = true;
+    }
+    // kernel set 1: lane-walking kernels by launch size; 2: always; 3: never
+    c->solo.walk_mode = c->g.kernel_set == 2 ? 1 : c->g.kernel_set == 3 ? 2 : 0;
     return 0;
 }
 
@@ -149,7 +152,7 @@ hipError_t capture_step(lare_ctx *c, hipGraphExec_t *exec)
     drop_graphs(c);
     hipGraph_t *graph = &c->g.graph;
     hipGraphExec_t *gexec = &c->g.graph_exec;
-    if (c->g.kernel_set == 1) {
+    if (c->g.kernel_set != 0) {
         if (ensure_solo(c)) return hipErrorInvalidValue;
         lare::group_prepare(c->solo);
         graph = &c->solo.graph;
@@ -165,7 +168,7 @@ hipError_t capture_step(lare_ctx *c, hipGraphExec_t *exec)
     return e;
 }
 
-hipGraphExec_t captured(lare_ctx *c) { return c->g.kernel_set == 1 ? c->solo.graph_exec : c->g.graph_exec; }
+hipGraphExec_t captured(lare_ctx *c) { return c->g.kernel_set != 0 ? c->solo.graph_exec : c->g.graph_exec; }
 
 }  // namespace
 
@@ -685,7 +688,7 @@ extern "C" int lare_profile_get(lare_ctx *c, int32_t i, char *name, int32_t name
 extern "C" int lare_set_kernel_set(lare_ctx *c, int32_t kernel_set)
 {
     if (!c) return fail(-1, "lare_set_kernel_set: null context");
-    if (kernel_set < 0 || kernel_set > 1) return fail(-1, "lare_set_kernel_set: unknown kernel set");
+    if (kernel_set < 0 || kernel_set > 3) return fail(-1, "lare_set_kernel_set: unknown kernel set");
     HIPCHK(hipStreamSynchronize(c->g.stream));
     if (c->g.st != 0) return fail(-1, "lare_set_kernel_set: only between steps");
     if (kernel_set == 0) {

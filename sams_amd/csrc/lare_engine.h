@@ -49,6 +49,8 @@ struct Group {
     int march_k = 16;   // cells per lane of the x and y transport kernels (0: one)
     bool zline = true;  // z transport with neighbours exchanged between lanes
     int zx_k = 8;       // cells per lane of the z-line x-march kernels (0: one cell per lane)
+    double walk_min_cells = 3.0e5;  // launches over fewer cells use one cell per lane
+    int walk_mode = 0;              // lane-walking kernels: 0 by launch size, 1 always, 2 never
     // one captured step of the whole group
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;

@@ -1868,6 +1868,119 @@ KERNEL void r_b_energy(const Src<T> S)
     ELOOP(cell)
 }
 
+// rkstep's current, resistivity, flux and Ohmic term (lagran.cpp:445-535) at
+// one vertex from the B values around it: bzAB / byAB / bxAB is the
+// component at (x + A, y + B), the z suffix its value at z + 1.  The curlb
+// returned is rounded to the storage type, as stored by r_rkstep.
+template <class T>
+__device__ __forceinline__ void rk_vertex(double bz00, double bz01, double bz10, double bz11, double by00,
+                                          double by00z, double by10, double by10z, double bx00, double bx00z,
+                                          double bx01, double bx01z, double dxc, double dyc, double dzc,
+                                          const Phys &P, double f[4])
+{
+    double jx1 = (bz01 - bz00) / dyc - (by00z - by00) / dzc;
+    double jx2 = (bz11 - bz10) / dyc - (by10z - by10) / dzc;
+    double jy1 = (bx00z - bx00) / dzc - (bz10 - bz00) / dxc;
+    double jy2 = (bx01z - bx01) / dzc - (bz11 - bz01) / dxc;
+    double jz1 = (by10 - by00) / dxc - (bx01 - bx00) / dyc;
+    double jz2 = (by10z - by00z) / dxc - (bx01z - bx00z) / dyc;
+    double jx = 0.5 * (jx1 + jx2);
+    double jy = 0.5 * (jy1 + jy2);
+    double jz = 0.5 * (jz1 + jz2);
+    double j_local = sqrt(jx * jx + jy * jy + jz * jz);
+    double eta = j_local > P.j_max ? P.eta_background + P.eta0 : P.eta_background;
+    f[0] = -jx * eta * dxc * 0.5;
+    f[1] = -jy * eta * dyc * 0.5;
+    f[2] = -jz * eta * dzc * 0.5;
+    f[3] = (double)(T)(eta * (jx * jx + jy * jy + jz * jz));
+}
+
+// One resistive substep (r_rkstep then r_b_energy, lagran.cpp:490-590) as a
+// z-line x-march (H = 1) over (0..n)^3, B read from one copy and written to
+// the other (to_bx1: B -> bx1..bz1, else bx1..bz1 -> B), so no lane reads a
+// B another lane has updated.  Each lane computes the rkstep fluxes and
+// Ohmic terms of its vertex column (x, y) and of (x, y-1); those at x - 1
+// are carried and those at z - 1 come from the lane below.  The same
+// arithmetic as r_rkstep and r_b_energy; B outside each component's update
+// range is left to the boundary fill that follows.
+template <class T, int K>
+KERNEL OCC(4) void r_substep_zx(const Src<T> S, int to_bx1)
+{
+    const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
+    const int g_ = grid_of(S.boff, S.ngrid, b_);
+    const EDev<T> &D_ = S.devs[g_];
+    const Geo G = D_.g;
+    const ArrE<T> A = state_view(D_.a, S.st);
+    const Phys &P = D_.p;
+    const ZXPos Z = zx_map<1, K>(S, G, g_, b_);
+    const int sx = G.sx, sy = G.sy;
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const int iy = Z.iy, iz = Z.iz;
+    const double dt = lare::gbl(A.s)->dt_sub;
+    const Fld<T> IX_ = to_bx1 ? BX : A.bx1, IY_ = to_bx1 ? BY : A.by1, IZ_ = to_bx1 ? BZ : A.bz1;
+    const Fld<T> OX_ = to_bx1 ? A.bx1 : BX, OY_ = to_bx1 ? A.by1 : BY, OZ_ = to_bx1 ? A.bz1 : BZ;
+    const double dyc0 = DYC(iy), dycm = DYC(iy - 1), dzc = DZC(iz);
+    int c = IX(Z.ix0, iy, iz);
+    // plane x: BY at (x, y-1), (x, y) and their z+1; BZ at (x, y-1..y+1)
+    double by_m = IY_[c - sy], by_0 = IY_[c];
+    double by_mz = lane_next(by_m), by_0z = lane_next(by_0);
+    double bz_m = IZ_[c - sy], bz_0 = IZ_[c], bz_p = IZ_[c + sy];
+    // rkstep terms (fx, fy, fz, curlb) of plane x - 1 at (y) and (y-1), and
+    // of the lane below
+    double F0x[4] = {0, 0, 0, 0}, Fmx[4] = {0, 0, 0, 0}, pF0x[4] = {0, 0, 0, 0}, pFmx[4] = {0, 0, 0, 0};
+    for (int ix = Z.ix0; ix <= Z.ix1; ++ix, c += sx) {
+        const double bx_m = IX_[c - sy], bx_0 = IX_[c], bx_p = IX_[c + sy];
+        const double bx_mz = lane_next(bx_m), bx_0z = lane_next(bx_0), bx_pz = lane_next(bx_p);
+        const double byN_m = IY_[c + sx - sy], byN_0 = IY_[c + sx];
+        const double byN_mz = lane_next(byN_m), byN_0z = lane_next(byN_0);
+        const double bzN_m = IZ_[c + sx - sy], bzN_0 = IZ_[c + sx], bzN_p = IZ_[c + sx + sy];
+        const double dxc = DXC(ix);
+        double F0[4], Fm[4];
+        rk_vertex<T>(bz_0, bz_p, bzN_0, bzN_p, by_0, by_0z, byN_0, byN_0z, bx_0, bx_0z, bx_p, bx_pz, dxc, dyc0, dzc, P,
+                     F0);
+        rk_vertex<T>(bz_m, bz_0, bzN_m, bzN_0, by_m, by_mz, byN_m, byN_mz, bx_m, bx_mz, bx_0, bx_0z, dxc, dycm, dzc, P,
+                     Fm);
+        double pF0[4], pFm[4];
+        for (int q = 0; q < 4; ++q) {
+            pF0[q] = lane_prev(F0[q]);
+            pFm[q] = lane_prev(Fm[q]);
+        }
+        if (Z.wz && ix > Z.ix0) {
+            // fx = [0], fy = [1], fz = [2], curlb = [3]; c, c-sy, c-1, c-sy-1 at
+            // this x (F0, Fm, pF0, pFm) and at x - 1 (F0x, Fmx, pF0x, pFmx)
+            if (iy >= 1 && iz >= 1) {
+                double area = DYB(iy) * DZB(iz);
+                OX_[c] = bx_0 + (F0[2] - Fm[2] + pF0[2] - pFm[2] - F0[1] + pF0[1] - Fm[1] + pFm[1]) * dt / area;
+            }
+            if (ix >= 1 && iz >= 1) {
+                double area = DXB(ix) * DZB(iz);
+                OY_[c] = by_0 + (F0[0] - pF0[0] + F0x[0] - pF0x[0] - F0[2] + F0x[2] - pF0[2] + pF0x[2]) * dt / area;
+            }
+            if (ix >= 1 && iy >= 1) {
+                double area = DXB(ix) * DYB(iy);
+                OZ_[c] = bz_0 + (F0[1] - F0x[1] + Fm[1] - Fmx[1] - F0[0] + Fm[0] - F0x[0] + Fmx[0]) * dt / area;
+            }
+            if (ix >= 1 && iy >= 1 && iz >= 1 && ix <= nx && iy <= ny && iz <= nz) {
+                double sum_curlb = F0[3] + F0x[3] + Fm[3] + Fmx[3] + pF0[3] + pF0x[3] + pFm[3] + pFmx[3];
+                EE[c] += sum_curlb * dt / (8.0 * RHO[c]);
+            }
+        }
+        for (int q = 0; q < 4; ++q) {
+            F0x[q] = F0[q];
+            Fmx[q] = Fm[q];
+            pF0x[q] = pF0[q];
+            pFmx[q] = pFm[q];
+        }
+        by_m = byN_m;
+        by_0 = byN_0;
+        by_mz = byN_mz;
+        by_0z = byN_0z;
+        bz_m = bzN_m;
+        bz_0 = bzN_0;
+        bz_p = bzN_p;
+    }
+}
+
 // ------------------------------------------------- boundary conditions
 
 // One edge of one built-in boundary condition along a line of a variable
@@ -2033,15 +2146,16 @@ const int kStagE[13][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 1, 1}, {1, 1, 1}
 inline RSpec rs(int x0, int dx1, int y0, int dy1, int z0, int dz1) { return RSpec{x0, dx1, y0, dy1, z0, dz1}; }
 
 // Variable ids of the boundary fills: 0..12 the registered variables (state
-// in the first copy), 13..18 the second copy of rho, e_e, e_i, vx, vy, vz.
+// in the first copy), 13..18 the second copy of rho, e_e, e_i, vx, vy, vz,
+// 19..21 bx1..bz1 as the second copy of B (resistive substeps).
 const int kSecond = 13;
 inline int base_var(int v) { return v >= kSecond ? v - kSecond : v; }
 
 template <class T>
 void *field_ptr_t(const ArrE<T> &a, int f)
 {
-    const Fld<T> fs[19] = {a.rho, a.ee, a.ei, a.vx, a.vy, a.vz, a.bx, a.by, a.bz, a.vx1, a.vy1, a.vz1, {nullptr},
-                           a.rho2, a.ee2, a.ei2, a.vx2, a.vy2, a.vz2};
+    const Fld<T> fs[22] = {a.rho, a.ee, a.ei, a.vx,  a.vy,  a.vz,  a.bx,  a.by,  a.bz,  a.vx1,  a.vy1,
+                           a.vz1, {nullptr}, a.rho2, a.ee2, a.ei2, a.vx2, a.vy2, a.vz2, a.bx1, a.by1, a.bz1};
     return f == 12 ? (void *)a.dm : (void *)fs[f].p;
 }
 void *field_ptr(const Grid &g, int f) { return g.fp32 ? field_ptr_t(g.e32.a, f) : field_ptr_t(g.e64.a, f); }
@@ -2160,7 +2274,7 @@ void per_grid(void (*k)(const EDev<T> *, int), const char *name, Group &gp)
 // Variable groups of the boundary fills (boundary.cpp:27-96): the variables
 // filled and, for a variable just written into the other state copy, the
 // copy it was read from (-1: updated in place).
-enum { VB, VE, VRHO, VV, VV1, VDM, VALL, VERHO, VBE, VCORR_2, VALL_2, VALL_1, NVG };
+enum { VB, VE, VRHO, VV, VV1, VDM, VALL, VERHO, VBE, VCORR_2, VALL_2, VALL_1, VBE_1, VBE_0, NVG };
 struct VarGroup {
     int n;
     int dst[9];
@@ -2183,6 +2297,10 @@ const VarGroup kGroups[NVG] = {
     {9, {6, 7, 8, 14, 15, 13, 16, 17, 18}, {-1, -1, -1, 1, 2, 0, 3, 4, 5}},
     // boundary_conditions after a sweep that wrote the first copy
     {9, {6, 7, 8, 1, 2, 0, 3, 4, 5}, {-1, -1, -1, 14, 15, 13, 16, 17, 18}},
+    // the resistive substep's bfield_bcs and energy_bcs after it wrote B
+    // into bx1..bz1, and after it wrote B back
+    {5, {19, 20, 21, 1, 2}, {6, 7, 8, -1, -1}},
+    {5, {6, 7, 8, 1, 2}, {19, 20, 21, -1, -1}},
 };
 
 void drop_btabs(Group &gp)
@@ -2290,6 +2408,26 @@ void check_bcs(Group &gp)
 
 // ------------------------------------------------------------ the events
 
+// Cells per launch of subset (0 all members, 1 resistive ones).  The lane-
+// walking kernels (x/y march, z-line, z-line x-march) give each lane several
+// cells, so below a few hundred thousand cells a launch would not fill the chip and
+// the one-cell-per-lane kernels are used instead.
+double subset_cells(const Group &gp, int subset)
+{
+    double n = 0.0;
+    for (const Grid *g : gp.mem)
+        if (subset == 0 || g->prm.resistive) n += (double)g->dev.g.nx * g->dev.g.ny * g->dev.g.nz;
+    return n;
+}
+bool walk_ok(const Group &gp, int subset)
+{
+    if (gp.walk_mode) return gp.walk_mode == 1;
+    return subset_cells(gp, subset) >= gp.walk_min_cells;
+}
+bool use_zx(const Group &gp, int subset) { return gp.zx_k > 0 && walk_ok(gp, subset); }
+bool use_walk(const Group &gp) { return walk_ok(gp, 0); }
+
+
 template <class T>
 void start_a_t(Group &gp)
 {
@@ -2301,7 +2439,7 @@ void start_a_t(Group &gp)
     EL(g_cs<T>, 7 * w, gp, 0, C);
     EL(g_vertex<T>, 4 * w, gp, 0, V);
     EL(g_alpha<T>, 8 * w, gp, 0, rs(-1, 1, -1, 2, 0, 2));
-    if (gp.zx_k > 0)
+    if (use_zx(gp, 0))
         elaunch_(g_visc_dt_zx<T, 8>, "g_visc_dt", 16 * w, gp, 0, rs(0, 1, 0, 1, 0, 1), 0, kZX + 100 + 8);
     else
         EL(g_visc_dt<T>, 16 * w, gp, 0, rs(0, 1, 0, 1, 0, 1));
@@ -2313,7 +2451,18 @@ void start_b_t(Group &gp, int substeps)
 {
     const double w = (double)sizeof(T);
     const RSpec C = rs(-1, 2, -1, 2, -1, 2);
-    if (gp.n[1] > 0) {
+    if (gp.n[1] > 0 && use_zx(gp, 1)) {
+        // B alternates between its arrays and bx1..bz1 and ends in its own;
+        // the last substep reads bx1..bz1, which then hold B at its start
+        // (the copy of lagran.cpp:482).  An odd count starts from that copy.
+        if (substeps % 2) EL(r_copy<T>, 6 * w, gp, 1, C);
+        for (int i = 0; i < substeps; ++i) {
+            const int to_bx1 = (substeps - 1 - i) % 2;
+            elaunch_(r_substep_zx<T, 8>, "r_substep", 12 * w, gp, 1, rs(0, 0, 0, 0, 0, 0), 0, kZX + 100 + 8,
+                     to_bx1);
+            ebcs<T>(gp, to_bx1 ? VBE_1 : VBE_0, -1, 1);
+        }
+    } else if (gp.n[1] > 0) {
         for (int i = 0; i < substeps; ++i) {
             if (i == substeps - 1) EL(r_copy<T>, 6 * w, gp, 1, C);
             EL(r_rkstep<T>, 4 * w + 3 * 8, gp, 1, rs(0, 0, 0, 0, 0, 0));
@@ -2321,7 +2470,7 @@ void start_b_t(Group &gp, int substeps)
             ebcs<T>(gp, VBE, -1, 1);
         }
     }
-    if (gp.zx_k > 0)
+    if (use_zx(gp, 0))
         elaunch_(g_predict_zx<T, 8>, "g_predict", 16 * w, gp, 0, C, 0, kZX + 100 + 8);
     else
         EL(g_predict<T>, 16 * w, gp, 0, C);
@@ -2337,7 +2486,7 @@ void half_t(Group &gp)
     const double w = (double)sizeof(T);
     // reads v v1 alpha1..3 rho_v fx_visc..fz_visc rho e_e e_i p_e p_i;
     // writes v rho e_e e_i
-    if (gp.zx_k > 0)
+    if (use_zx(gp, 0))
         elaunch_(g_correct_zx<T, 8>, "g_correct", 22 * w, gp, 0, rs(0, 0, 0, 0, 0, 0), 0, kZX + 100 + 8);
     else
         EL(g_correct<T>, 22 * w, gp, 0, rs(0, 0, 0, 0, 0, 0));
@@ -2367,9 +2516,9 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
     ebcs<T>(gp, VDM, DIR);
     // reads rho e_e e_i v cv1 cv2 cvc1 VX1..VZ1 (sweep component) dm;
     // writes rho e_e e_i v
-    if (DIR == 2 && gp.zline) {
+    if (DIR == 2 && gp.zline && use_walk(gp)) {
         elaunch_(f_update_zline<T>, "f_update_zline", 16 * w + 8, gp, 0, all0, st, kZLine);
-    } else if (DIR == 2 || gp.march_k <= 0) {
+    } else if (DIR == 2 || gp.march_k <= 0 || !use_walk(gp)) {
         ELS((f_update_all<T, DIR>), 16 * w + 8, gp, 0, all0, st);
     } else {
         constexpr int MD = DIR == 2 ? 0 : DIR;
@@ -2415,6 +2564,7 @@ int group_init(Group &gp, const std::vector<Grid *> &members, hipStream_t stream
     if (const char *o = std::getenv("LARE_ORDER")) gp.order = std::atoi(o) != 0;
     if (const char *o = std::getenv("LARE_MARCH")) gp.march_k = std::atoi(o);
     if (const char *o = std::getenv("LARE_ZLINE")) gp.zline = std::atoi(o) != 0;
+    if (const char *o = std::getenv("LARE_WALK_MIN_CELLS")) gp.walk_min_cells = std::atof(o);
     if (const char *o = std::getenv("LARE_ZX")) gp.zx_k = std::atoi(o);
     if (members.empty()) return 0;
     gp.fp32 = members[0]->fp32;

@@ -87,6 +87,21 @@ def test_fp32_storage_batch_equals_alone():
             assert bit_diff(mem[i].download(f), g.download(f)) is None, (i, f)
 
 
+def test_fp32_storage_kernel_forms_agree():
+    """The lane-walking kernels (set 2) and the one-cell-per-lane kernels
+    (set 3) round to FP32 at the same stores: bit-identical FP32 runs."""
+    ps = [L.problem("ot", n=12, ny=10, nz=9, seed=3, amp=0.05, precision=L.FP32_STORAGE),
+          L.problem("bw", n=24, ny=5, nz=4, seed=2, noise=0.01, eta=1e-3, precision=L.FP32_STORAGE)]
+    for i, p in enumerate(ps):
+        runs = [L.Lare.from_problem(p) for _ in range(2)]
+        for k, g in zip((2, 3), runs):
+            g.set_kernel_set(k)
+        da, db = runs[0].steps(6), runs[1].steps(6)
+        assert [d.hex() for d in da] == [d.hex() for d in db], i
+        for f in L.STATE_FIELDS:
+            assert bit_diff(runs[0].download(f), runs[1].download(f)) is None, (i, f)
+
+
 def test_fp32_and_fp64_do_not_share_a_batch():
     a = L.Lare.from_problem(L.problem("ot", n=8))
     b = L.Lare.from_problem(L.problem("ot", n=8, precision=L.FP32_STORAGE))

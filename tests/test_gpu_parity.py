@@ -26,13 +26,14 @@ def _state_diffs(dev, orc, fields=L.STATE_FIELDS):
     return out
 
 
-@pytest.mark.parametrize("kernel_set", [0, 1])
+@pytest.mark.parametrize("kernel_set", [0, 1, 2])
 @pytest.mark.parametrize("name", ["ot_pert_10x8x12", "bw_zerograd_24x4x6", "bw_walls_16x5x3",
                                   "bw_mirror_16x4x4"])
 def test_events_of_first_two_steps_match_oracle(name, kernel_set):
     """Event by event (startOfTimestep, halfTimestep, endOfTimestep) for two
     steps: dt, the half-step velocities and every state field; for the
-    one-kernel-per-reference-kernel set (0) and the fused set (1)."""
+    one-kernel-per-reference-kernel set (0), the fused set (1: one cell per
+    lane at these sizes) and the fused set with lane-walking kernels (2)."""
     p = case_problem(CASES[name])
     dev = L.Lare.from_problem(p)
     dev.set_kernel_set(kernel_set)
@@ -53,7 +54,7 @@ def test_events_of_first_two_steps_match_oracle(name, kernel_set):
         assert not diffs, f"step {step} after endOfTimestep: {diffs}"
 
 
-@pytest.mark.parametrize("kernel_set", [0, 1])
+@pytest.mark.parametrize("kernel_set", [0, 1, 2])
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_reference_goldens(name, kernel_set):
     """Whole runs against the reference: the dt sequence and state digests."""
@@ -83,7 +84,7 @@ def test_oracle_parity_random_perturbed_orszag_tang():
     assert not _state_diffs(dev, orc)
 
 
-@pytest.mark.parametrize("kernel_set", [0, 1])
+@pytest.mark.parametrize("kernel_set", [0, 1, 2])
 def test_resistive_brio_wu_matches_oracle(kernel_set):
     """Resistive MHD (eta = 1e-3): two half-dt substeps per step."""
     p = L.problem("bw", n=24, ny=5, nz=4, seed=4, noise=0.01, eta=1e-3)
@@ -116,13 +117,16 @@ def test_uniform_state_first_dt_closed_form():
     assert dt == pytest.approx(expect, rel=1e-15)
 
 
-def test_graph_replays_match_eager_steps():
+@pytest.mark.parametrize("kernel_set", [1, 2])
+def test_graph_replays_match_eager_steps(kernel_set):
     """The captured-step HIP graph, replayed with host synchronisation and
     downloads between replays, gives after every replay the same bits as the
     eagerly launched step and as the oracle."""
     p = L.problem("bw", n=20, ny=6, nz=5, seed=8, noise=0.02, eta=1e-3)
     g = L.Lare.from_problem(p)
     e = L.Lare.from_problem(p)
+    for d in (g, e):
+        d.set_kernel_set(kernel_set)
     e.set_graph(False)
     orc = make_oracle(p)
     for s in range(5):
@@ -134,7 +138,7 @@ def test_graph_replays_match_eager_steps():
         assert not _state_diffs(g, orc), f"replay {s + 1} vs oracle"
 
 
-@pytest.mark.parametrize("kernel_set", [0, 1])
+@pytest.mark.parametrize("kernel_set", [0, 1, 2])
 def test_split_start_of_timestep_matches_whole_event(kernel_set):
     """lare_viscosity_dt followed by lare_lagrange with the dt it returned
     (the runner's timestep protocol, runner.h:158-169) gives the same bits
@@ -158,7 +162,33 @@ def test_split_start_of_timestep_matches_whole_event(kernel_set):
         assert not _state_diffs(split, whole), f"step {s + 1}"
 
 
-@pytest.mark.parametrize("kernel_set", [0, 1])
+@pytest.mark.parametrize("kernel_set", [2, 3])
+@pytest.mark.parametrize("ratio", [0.5, 2.0])
+def test_odd_substep_counts_match_kernel_set0(ratio, kernel_set):
+    """A runner dt other than the package's own (the minimum over packages,
+    runner.h:158-169) changes the resistive substep count int(dt/dtr)+1:
+    1 substep for dt = dtr/2, 3 for dt = 2 dtr.  The fused sets (2: B
+    alternating with bx1..bz1 in one kernel per substep; 3: one cell per
+    lane) must equal the one-kernel-per-reference-kernel set, whose substeps
+    follow lagran.cpp:479-500 literally (its events are pinned to the oracle
+    above)."""
+    p = L.problem("bw", n=20, ny=4, nz=6, seed=9, noise=0.02, eta=1e-3)
+    devs = [L.Lare.from_problem(p) for _ in range(2)]
+    for k, d in zip((0, kernel_set), devs):
+        d.set_kernel_set(k)
+    for s in range(2):
+        dts = [d.viscosity_dt() for d in devs]
+        assert dts[0][0].hex() == dts[1][0].hex(), f"dt of step {s + 1}"
+        for d in devs:
+            d.lagrange(dts[0][0] * ratio)
+        assert not _state_diffs(devs[0], devs[1], L.STATE_FIELDS + ["vx1", "vy1", "vz1"]), f"step {s + 1}"
+        for d in devs:
+            d.half_timestep()
+            d.end_of_timestep()
+        assert not _state_diffs(devs[0], devs[1]), f"step {s + 1}"
+
+
+@pytest.mark.parametrize("kernel_set", [0, 1, 2])
 def test_edges_without_conditions_match_oracle(kernel_set):
     """Edges with no boundary condition keep their ghost values (the
     reference fills nothing there): all z edges, and the upper x edge of rho
