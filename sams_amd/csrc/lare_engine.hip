@@ -2379,7 +2379,7 @@ void ebcs(Group &gp, int vg, int only_dim = -1, int subset = 0)
         Group::BTab &t = btable(gp, vg, d, subset);
         if (gp.planning || !t.any || t.nblk == 0) continue;
         Prof *pf = gp.prof;
-        Prof::Rec rec{"e_bc", 0.0, nullptr, nullptr};
+        Prof::Rec rec{std::string("e_bc<") + char('0' + d) + ">", 0.0, nullptr, nullptr};
         if (pf && pf->on) {
             (void)hipEventCreate(&rec.a);
             (void)hipEventCreate(&rec.b);
@@ -2458,7 +2458,7 @@ void start_b_t(Group &gp, int substeps)
         if (substeps % 2) EL(r_copy<T>, 6 * w, gp, 1, C);
         for (int i = 0; i < substeps; ++i) {
             const int to_bx1 = (substeps - 1 - i) % 2;
-            elaunch_(r_substep_zx<T, 8>, "r_substep", 12 * w, gp, 1, rs(0, 0, 0, 0, 0, 0), 0, kZX + 100 + 8,
+            elaunch_(r_substep_zx<T, 8>, "r_substep", 9 * w, gp, 1, rs(0, 0, 0, 0, 0, 0), 0, kZX + 100 + 8,
                      to_bx1);
             ebcs<T>(gp, to_bx1 ? VBE_1 : VBE_0, -1, 1);
         }
