@@ -237,6 +237,29 @@ __device__ __forceinline__ double b_area(const ArrE<T> &A, int ix, int iy, int i
     return SC == 0 ? DXAB(iy, iz) : SC == 1 ? DYAB(ix, iz) : DZAB(ix, iy);
 }
 
+// The constrained-transport flux from the values along DIR: f = B / db at
+// c-1..c+2, the averaged widths db at c and c+1, the advection velocity, and
+// cvc1 at c-1..c+1, cv1 at c, c+1.
+__device__ __forceinline__ double ct_flux_v(double fm, double fi, double fp, double fp2, double dbx, double dbxp,
+                                            double v_advect, double dt, double cvc_m, double cvc_0, double cvc_p,
+                                            double cv1_0, double cv1_p)
+{
+    double o_v = v_advect * dt;
+    double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
+    double sign_v = (v_advect >= 0.0) ? 1.0 : -1.0;
+    double vad_p = (sign_v + 1.0) * 0.5;
+    double vad_m = 1.0 - vad_p;
+    double fu = fi * vad_p + fp * vad_m;
+    double dfu = dfm * vad_p + dfp * vad_m;
+    double dxci = cvc_0;
+    double dxcu = cvc_m * vad_p + cvc_p * vad_m;
+    double dxbu = cv1_0 * vad_p + cv1_p * vad_m;
+    double dxu = dbx * vad_p + dbxp * vad_m;
+    double phi = fabs(o_v) / dxu;
+    double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
+    return (fu + Di * (1.0 - phi)) * o_v;
+}
+
 // Constrained-transport flux of face magnetic flux B along DIR: v_advect is
 // the mean of w at c and c+ow, db1 is averaged with c+tw (vx_by_flux,
 // xremap.cpp:269-319, and its five twins).
@@ -255,7 +278,6 @@ __device__ __forceinline__ double ct_flux(const ArrE<T> &A, const Geo &G, int c,
         return b;
     };
     double v_advect = (w[c] + w[c + ow]) * 0.5;
-    double o_v = v_advect * dt;
     double dbx = (db1[c] + db1[c + tw]) * 0.5;
     double dbxp = (db1[c + sd] + db1[c + sd + tw]) * 0.5;
     double dbxp2 = (db1[c + 2 * sd] + db1[c + 2 * sd + tw]) * 0.5;
@@ -264,19 +286,8 @@ __device__ __forceinline__ double ct_flux(const ArrE<T> &A, const Geo &G, int c,
     double fi = B(0) / dbx;
     double fp = B(1) / dbxp;
     double fp2 = B(2) / dbxp2;
-    double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
-    double sign_v = (v_advect >= 0.0) ? 1.0 : -1.0;
-    double vad_p = (sign_v + 1.0) * 0.5;
-    double vad_m = 1.0 - vad_p;
-    double fu = fi * vad_p + fp * vad_m;
-    double dfu = dfm * vad_p + dfp * vad_m;
-    double dxci = A.cvc1[c];
-    double dxcu = A.cvc1[c - sd] * vad_p + A.cvc1[c + sd] * vad_m;
-    double dxbu = A.cv1[c] * vad_p + A.cv1[c + sd] * vad_m;
-    double dxu = dbx * vad_p + dbxp * vad_m;
-    double phi = fabs(o_v) / dxu;
-    double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
-    return (fu + Di * (1.0 - phi)) * o_v;
+    return ct_flux_v(fm, fi, fp, fp2, dbx, dbxp, v_advect, dt, A.cvc1[c - sd], A.cvc1[c], A.cvc1[c + sd], A.cv1[c],
+                     A.cv1[c + sd]);
 }
 
 // R3: both constrained-transport fluxes of the sweep, over (0..n)^3
@@ -943,6 +954,132 @@ KERNEL void f_update_zline(const Src<T> S)
         A.ee2[c] = (E[0] * cv1_0 * rho_0 + em[0] - ec[0]) / (cv2 * rho);
         A.ei2[c] = (E[1] * cv1_0 * rho_0 + em[1] - ec[1]) / (cv2 * rho);
     }
+}
+
+// R3 for the x and y sweeps with each lane walking K vertices along DIR (the
+// mapping of f_update_march): B / db along DIR, cvc1 and cv1 are carried as
+// a window, so each is loaded once; the same arithmetic as f_ct_fluxes.
+// The x sweep reads B unscaled and multiplies by the component's area at
+// each position (remap.cpp:46-50), as ct_flux<0, SC> does.
+template <class T, int DIR, int K>
+KERNEL void f_ct_march(const Src<T> S)
+{
+    const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
+    const int g_ = grid_of(S.boff, S.ngrid, b_);
+    const EDev<T> &D_ = S.devs[g_];
+    const Geo G = D_.g;
+    const ArrE<T> A = state_view(D_.a, S.st);
+    const R3 R = {S.r.x0, G.nx + S.r.dx1, S.r.y0, G.ny + S.r.dy1, S.r.z0, G.nz + S.r.dz1};
+    const int ps0 = DIR == 0 ? R.y0 : R.x0, ps1 = DIR == 0 ? R.y1 : R.x1;
+    const int nzr = R.z1 - R.z0 + 1;
+    const int plane = (ps1 - ps0 + 1) * nzr;
+    const int bpp = (plane + 255) >> 8;
+    const int lb = b_ - S.boff[g_];
+    const int mb = lb / bpp;
+    const int t = (lb - mb * bpp) * 256 + (int)threadIdx.x;
+    if (t >= plane) return;
+    const int ps = ps0 + t / nzr, iz = R.z0 + t % nzr;
+    const int dlo = DIR == 0 ? R.x0 : R.y0, dhi = DIR == 0 ? R.x1 : R.y1;
+    const int d0 = dlo + mb * K;
+    const int sx = G.sx, sy = G.sy;
+    const int sd = sdir<DIR>(G);
+    const double dt = lare::gbl(A.s)->dt;
+    // flux a: by (x sweep) or bx (y sweep); flux b: bz
+    const Fld<T> Ba = DIR == 0 ? BY : BX, Bb = BZ;
+    const Fld<T> W = DIR == 0 ? VX1 : VY1;
+    const int owa = -1, owb = DIR == 0 ? -sy : -sx;
+    const int twa = DIR == 0 ? sy : sx, twb = 1;
+    const Fld<T> db1 = A.db1;
+    // B area of the x sweep at position d along x: dyab(d, iz), dzab(d, iy)
+    auto area_a = [&](int d) -> double { return DIR == 0 ? DXB(d) * DZB(iz) : 1.0; };
+    auto area_b = [&](int d) -> double { return DIR == 0 ? DXB(d) * DYB(ps) : 1.0; };
+    auto fval = [&](const Fld<T> Bf, int p, int tw, int d, bool a, double &db) -> double {
+        db = (db1[p] + db1[p + tw]) * 0.5;
+        double b = Bf[p];
+        if (DIR == 0) b = b * (a ? area_a(d) : area_b(d));
+        return b / db;
+    };
+    int c = DIR == 0 ? IX(d0, ps, iz) : IX(ps, d0, iz);
+    double fa[3], fb[3], da[2], db[2], dtmp;
+    fa[0] = fval(Ba, c - sd, twa, d0 - 1, true, dtmp);
+    fa[1] = fval(Ba, c, twa, d0, true, da[0]);
+    fa[2] = fval(Ba, c + sd, twa, d0 + 1, true, da[1]);
+    fb[0] = fval(Bb, c - sd, twb, d0 - 1, false, dtmp);
+    fb[1] = fval(Bb, c, twb, d0, false, db[0]);
+    fb[2] = fval(Bb, c + sd, twb, d0 + 1, false, db[1]);
+    double cvc_m = A.cvc1[c - sd], cvc_0 = A.cvc1[c], cv1_0 = A.cv1[c];
+    const auto FA = lare::gbl(A.fl[0]), FB = lare::gbl(A.fl[1]);
+    for (int k = 0; k < K; ++k) {
+        const int d = d0 + k;
+        if (d > dhi) break;
+        double da2, db2;
+        const double fa3 = fval(Ba, c + 2 * sd, twa, d + 2, true, da2);
+        const double fb3 = fval(Bb, c + 2 * sd, twb, d + 2, false, db2);
+        const double cvc_p = A.cvc1[c + sd], cv1_p = A.cv1[c + sd];
+        const double w0 = W[c];
+        const double va = (w0 + W[c + owa]) * 0.5, vb = (w0 + W[c + owb]) * 0.5;
+        FA[c] = ct_flux_v(fa[0], fa[1], fa[2], fa3, da[0], da[1], va, dt, cvc_m, cvc_0, cvc_p, cv1_0, cv1_p);
+        FB[c] = ct_flux_v(fb[0], fb[1], fb[2], fb3, db[0], db[1], vb, dt, cvc_m, cvc_0, cvc_p, cv1_0, cv1_p);
+        fa[0] = fa[1];
+        fa[1] = fa[2];
+        fa[2] = fa3;
+        fb[0] = fb[1];
+        fb[1] = fb[2];
+        fb[2] = fb3;
+        da[0] = da[1];
+        da[1] = da2;
+        db[0] = db[1];
+        db[1] = db2;
+        cvc_m = cvc_0;
+        cvc_0 = cvc_p;
+        cv1_0 = cv1_p;
+        c += sd;
+    }
+}
+
+// R3 for the z sweep with the z-line lane mapping of f_update_zline: each
+// lane loads its own z and takes B / db, db, cvc1 and cv1 at z-1, z+1, z+2
+// from its neighbour lanes; the same arithmetic as f_ct_fluxes<2>.
+template <class T>
+KERNEL void f_ct_zline(const Src<T> S)
+{
+    const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
+    const int g_ = grid_of(S.boff, S.ngrid, b_);
+    const EDev<T> &D_ = S.devs[g_];
+    const Geo G = D_.g;
+    const ArrE<T> A = state_view(D_.a, S.st);
+    const R3 R = {S.r.x0, G.nx + S.r.dx1, S.r.y0, G.ny + S.r.dy1, S.r.z0, G.nz + S.r.dz1};
+    const int nyr = R.y1 - R.y0 + 1, nzr = R.z1 - R.z0 + 1;
+    const int L = nzr + 4, P = nyr * L;
+    const int bpp = (P + 239) / 240;
+    const int lb = b_ - S.boff[g_];
+    const int xp = lb / bpp;
+    const int lane = (int)threadIdx.x & 63;
+    const int w = (lb - xp * bpp) * 4 + ((int)threadIdx.x >> 6);
+    int u = 60 * w + lane - 2;
+    const bool inside = u >= 0 && u < P;
+    u = u < 0 ? 0 : (u >= P ? P - 1 : u);
+    const int row = u / L;
+    const int ix = R.x0 + xp, iy = R.y0 + row, iz = R.z0 - 2 + (u - row * L);
+    const bool writer = inside && lane >= 2 && lane < 62 && iz >= R.z0 && iz <= R.z1;
+    const int sx = G.sx, sy = G.sy;
+    const int c = IX(ix, iy, iz);
+    const double dt = lare::gbl(A.s)->dt;
+    const Fld<T> db1 = A.db1, W = VZ1;
+    const double d0 = db1[c];
+    // flux a: bx (tw = sx, ow = -sy); flux b: by (tw = sy, ow = -sx)
+    const double dba = (d0 + db1[c + sx]) * 0.5, dbb = (d0 + db1[c + sy]) * 0.5;
+    const double fa = BX[c] / dba, fb = BY[c] / dbb;
+    const double w0 = W[c];
+    const double va = (w0 + W[c - sy]) * 0.5, vb = (w0 + W[c - sx]) * 0.5;
+    const double cvc_0 = A.cvc1[c], cv1_0 = A.cv1[c];
+    const double cvc_m = lane_prev(cvc_0), cvc_p = lane_next(cvc_0), cv1_p = lane_next(cv1_0);
+    const double dbap = lane_next(dba), dbbp = lane_next(dbb);
+    const double fam = lane_prev(fa), fap = lane_next(fa), fap2 = lane_next(fap);
+    const double fbm = lane_prev(fb), fbp = lane_next(fb), fbp2 = lane_next(fbp);
+    if (!writer) return;
+    lare::gbl(A.fl[0])[c] = ct_flux_v(fam, fa, fap, fap2, dba, dbap, va, dt, cvc_m, cvc_0, cvc_p, cv1_0, cv1_p);
+    lare::gbl(A.fl[1])[c] = ct_flux_v(fbm, fb, fbp, fbp2, dbb, dbbp, vb, dt, cvc_m, cvc_0, cvc_p, cv1_0, cv1_p);
 }
 
 // The shifted sweep velocity the reference writes back into the registered
@@ -2503,7 +2640,15 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
     if (gp.prof) gp.prof->dir = DIR;
     EL((f_prep<T, DIR>), 6 * w, gp, 0, rs(-1, 2, -1, 2, -1, 2), xpass, ypass, zpass);
     EL(f_cvc1<T>, 2 * w, gp, 0, rs(-1, 1, -1, 1, -1, 1));
-    EL((f_ct_fluxes<T, DIR>), 7 * w + 2 * 8, gp, 0, all0);
+    if (DIR == 2 && gp.zline && use_walk(gp)) {
+        elaunch_(f_ct_zline<T>, "f_ct_fluxes<2>", 7 * w + 2 * 8, gp, 0, all0, 0, kZLine);
+    } else if (DIR != 2 && gp.march_k > 0 && use_walk(gp)) {
+        constexpr int MD = DIR == 2 ? 0 : DIR;
+        elaunch_(f_ct_march<T, MD, 16>, DIR == 0 ? "f_ct_fluxes<0>" : "f_ct_fluxes<1>", 7 * w + 2 * 8, gp, 0, all0,
+                 0, kMarch + 100 * DIR + 16);
+    } else {
+        EL((f_ct_fluxes<T, DIR>), 7 * w + 2 * 8, gp, 0, all0);
+    }
     if (DIR == 1)
         EL((f_ct_update<T, DIR>), 6 * w + 2 * 8, gp, 0, all0);
     else  // also the B area product (x) or quotient (z) over the B extents
