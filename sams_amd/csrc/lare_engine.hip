@@ -1165,34 +1165,47 @@ KERNEL void g_vertex(const Src<T> S)
 }
 
 // edge_viscosity (lagran.cpp:56-93) between c1 and c1 + s.
-template <class T>
-__device__ __forceinline__ double g_edge_visc(const ArrE<T> &A, double dt, double visc1, double visc2_norm,
-                                              double dvdots, double dx, double dxm, double dxp, double cs_edge,
-                                              int c1, int s)
+// The same from values: the velocity at the four points c0..c3 of the edge
+// line (v[k][0..2] = vx, vy, vz) and rho_v at c1, c2.
+__device__ __forceinline__ double edge_visc_v(double dt, double visc1, double visc2_norm, double dvdots, double dx,
+                                              double dxm, double dxp, double cs_edge, double rv1, double rv2,
+                                              const double *v0, const double *v1, const double *v2,
+                                              const double *v3)
 {
-    const int c0 = c1 - s, c2 = c1 + s, c3 = c2 + s;
     dvdots = pmin(0.0, dvdots);
-    double rho_edge = 2.0 * A.rho_v[c1] * A.rho_v[c2] / (A.rho_v[c1] + A.rho_v[c2]);
-    double dvx = VX[c1] - VX[c2];
-    double dvy = VY[c1] - VY[c2];
-    double dvz = VZ[c1] - VZ[c2];
+    double rho_edge = 2.0 * rv1 * rv2 / (rv1 + rv2);
+    double dvx = v1[0] - v2[0];
+    double dvy = v1[1] - v2[1];
+    double dvz = v1[2] - v2[2];
     double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
     double dv = sqrt(dv2);
     double psi = 0.0;
     const bool small = dv * dt / dx < 1.e-14;
     dvdots = small ? 0.0 : dvdots / dv;
-    double dvxm = VX[c0] - VX[c1];
-    double dvxp = VX[c2] - VX[c3];
-    double dvym = VY[c0] - VY[c1];
-    double dvyp = VY[c2] - VY[c3];
-    double dvzm = VZ[c0] - VZ[c1];
-    double dvzp = VZ[c2] - VZ[c3];
+    double dvxm = v0[0] - v1[0];
+    double dvxp = v2[0] - v3[0];
+    double dvym = v0[1] - v1[1];
+    double dvyp = v2[1] - v3[1];
+    double dvzm = v0[2] - v1[2];
+    double dvzp = v2[2] - v3[2];
     double rl = small ? 1.0 : (dvxp * dvx + dvyp * dvy + dvzp * dvz) * dx / (dxp * dv2);
     double rr = small ? 1.0 : (dvxm * dvx + dvym * dvy + dvzm * dvz) * dx / (dxm * dv2);
     psi = pmin_list4(0.5 * (rr + rl), 2.0 * rl, 2.0 * rr, 1.0);
     psi = pmax(0.0, psi);
     double q_k_bar = rho_edge * (visc2_norm * dv + sqrt(visc2_norm * visc2_norm * dv2 + (visc1 * cs_edge) * (visc1 * cs_edge)));
     return q_k_bar * (1.0 - psi) * dvdots;
+}
+
+template <class T>
+__device__ __forceinline__ double g_edge_visc(const ArrE<T> &A, double dt, double visc1, double visc2_norm,
+                                              double dvdots, double dx, double dxm, double dxp, double cs_edge,
+                                              int c1, int s)
+{
+    const int c0 = c1 - s, c2 = c1 + s, c3 = c2 + s;
+    const double v0[3] = {VX[c0], VY[c0], VZ[c0]}, v1[3] = {VX[c1], VY[c1], VZ[c1]};
+    const double v2[3] = {VX[c2], VY[c2], VZ[c2]}, v3[3] = {VX[c3], VY[c3], VZ[c3]};
+    return edge_visc_v(dt, visc1, visc2_norm, dvdots, dx, dxm, dxp, cs_edge, A.rho_v[c1], A.rho_v[c2], v0, v1, v2,
+                       v3);
 }
 
 // L3: alpha1, alpha2, alpha3 (lagran.cpp:211-274), each over its own range
