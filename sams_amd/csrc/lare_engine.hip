@@ -160,6 +160,55 @@ __device__ __forceinline__ int sdir(const Geo &G)
 
 // =================================================================== remap
 
+// R1 at one cell from the eight corners (x-1..x, y-1..y, z-1..z) of v1:
+// V[k][p] is component k at corner p (P_OOO ... P_MMM); out = cv1, cv2, db1.
+enum { P_OOO, P_MOO, P_OMO, P_OOM, P_MMO, P_MOM, P_OMM, P_MMM };
+template <int DIR, class T>
+__device__ __forceinline__ void prep_v(const ArrE<T> &A, int ix, int iy, int iz, const double V[3][8], double dt,
+                                       double xpass, double ypass, double zpass, double out[3])
+{
+    const double *X = V[0], *Y = V[1], *Zv = V[2];
+    double vxb = (X[P_OOO] + X[P_OMO] + X[P_OOM] + X[P_OMM]) * 0.25;
+    double vxbm = (X[P_MOO] + X[P_MMO] + X[P_MOM] + X[P_MMM]) * 0.25;
+    double vyb = (Y[P_OOO] + Y[P_MOO] + Y[P_OOM] + Y[P_MOM]) * 0.25;
+    double vybm = (Y[P_OMO] + Y[P_MMO] + Y[P_OMM] + Y[P_MMM]) * 0.25;
+    double vzb = (Zv[P_OOO] + Zv[P_MOO] + Zv[P_OMO] + Zv[P_MMO]) * 0.25;
+    double vzbm = (Zv[P_OOM] + Zv[P_MOM] + Zv[P_OMM] + Zv[P_MMM]) * 0.25;
+    const double vol = CV(ix, iy, iz);
+    const double axp = DXAB(iy, iz), axm = DXAB(iy, iz);
+    const double ayp = DYAB(ix, iz), aym = DYAB(ix, iz);
+    const double azp = DZAB(ix, iy), azm = DZAB(ix, iy);
+    double dv;
+    if (DIR == 0) {
+        double dvxdx = (vxb * axp - vxbm * axm) / vol;
+        double dvydy = ypass * (vyb * ayp - vybm * aym) / vol;
+        double dvzdz = zpass * (vzb * azp - vzbm * azm) / vol;
+        dv = (dvydy + dvzdz) * dt;
+        out[1] = vol * (1.0 + dv);
+        dv = dv + dvxdx * dt;
+        out[0] = vol * (1.0 + dv);
+        out[2] = DXB(ix) + (vxb - vxbm) * dt;
+    } else if (DIR == 1) {
+        double dvxdx = xpass * (vxb * axp - vxbm * axm) / vol;
+        double dvydy = (vyb * ayp - vybm * aym) / vol;
+        double dvzdz = zpass * (vzb * azp - vzbm * azm) / vol;
+        dv = (dvxdx + dvzdz) * dt;
+        out[1] = vol * (1.0 + dv);
+        dv = dv + dvydy * dt;
+        out[0] = vol * (1.0 + dv);
+        out[2] = DYB(iy) + (vyb - vybm) * dt;
+    } else {
+        double dvxdx = xpass * (vxb * axp - vxbm * axm) / vol;
+        double dvydy = ypass * (vyb * ayp - vybm * aym) / vol;
+        double dvzdz = (vzb * azp - vzbm * azm) / vol;
+        dv = (dvxdx + dvydy) * dt;
+        out[1] = vol * (1.0 + dv);
+        dv = dv + dvzdz * dt;
+        out[0] = vol * (1.0 + dv);
+        out[2] = DZB(iz) + (vzb - vzbm) * dt;
+    }
+}
+
 // R1: cell volumes before (cv1) and after (cv2) the remap of DIR and the
 // Lagrangian cell width db1 (xremap.cpp:26-70 and twins), centred extent.
 template <class T, int DIR>
@@ -172,45 +221,18 @@ KERNEL void f_prep(const Src<T> S, double xpass, double ypass, double zpass)
     const double dt = lare::gbl(A.s)->dt;
     const int ooo = c, moo = c - sx, omo = c - sy, oom = c - 1, mmo = c - sx - sy, mom = c - sx - 1,
               omm = c - sy - 1, mmm = c - sx - sy - 1;
-    double vxb = (VX1[ooo] + VX1[omo] + VX1[oom] + VX1[omm]) * 0.25;
-    double vxbm = (VX1[moo] + VX1[mmo] + VX1[mom] + VX1[mmm]) * 0.25;
-    double vyb = (VY1[ooo] + VY1[moo] + VY1[oom] + VY1[mom]) * 0.25;
-    double vybm = (VY1[omo] + VY1[mmo] + VY1[omm] + VY1[mmm]) * 0.25;
-    double vzb = (VZ1[ooo] + VZ1[moo] + VZ1[omo] + VZ1[mmo]) * 0.25;
-    double vzbm = (VZ1[oom] + VZ1[mom] + VZ1[omm] + VZ1[mmm]) * 0.25;
-    const double vol = CV(ix, iy, iz);
-    const double axp = DXAB(iy, iz), axm = DXAB(iy, iz);
-    const double ayp = DYAB(ix, iz), aym = DYAB(ix, iz);
-    const double azp = DZAB(ix, iy), azm = DZAB(ix, iy);
-    double dv;
-    if (DIR == 0) {
-        double dvxdx = (vxb * axp - vxbm * axm) / vol;
-        double dvydy = ypass * (vyb * ayp - vybm * aym) / vol;
-        double dvzdz = zpass * (vzb * azp - vzbm * azm) / vol;
-        dv = (dvydy + dvzdz) * dt;
-        A.cv2[c] = vol * (1.0 + dv);
-        dv = dv + dvxdx * dt;
-        A.cv1[c] = vol * (1.0 + dv);
-        A.db1[c] = DXB(ix) + (vxb - vxbm) * dt;
-    } else if (DIR == 1) {
-        double dvxdx = xpass * (vxb * axp - vxbm * axm) / vol;
-        double dvydy = (vyb * ayp - vybm * aym) / vol;
-        double dvzdz = zpass * (vzb * azp - vzbm * azm) / vol;
-        dv = (dvxdx + dvzdz) * dt;
-        A.cv2[c] = vol * (1.0 + dv);
-        dv = dv + dvydy * dt;
-        A.cv1[c] = vol * (1.0 + dv);
-        A.db1[c] = DYB(iy) + (vyb - vybm) * dt;
-    } else {
-        double dvxdx = xpass * (vxb * axp - vxbm * axm) / vol;
-        double dvydy = ypass * (vyb * ayp - vybm * aym) / vol;
-        double dvzdz = (vzb * azp - vzbm * azm) / vol;
-        dv = (dvxdx + dvydy) * dt;
-        A.cv2[c] = vol * (1.0 + dv);
-        dv = dv + dvzdz * dt;
-        A.cv1[c] = vol * (1.0 + dv);
-        A.db1[c] = DZB(iz) + (vzb - vzbm) * dt;
+    const int pos[8] = {ooo, moo, omo, oom, mmo, mom, omm, mmm};
+    double V[3][8];
+    for (int p = 0; p < 8; ++p) {
+        V[0][p] = VX1[pos[p]];
+        V[1][p] = VY1[pos[p]];
+        V[2][p] = VZ1[pos[p]];
     }
+    double out[3];
+    prep_v<DIR>(A, ix, iy, iz, V, dt, xpass, ypass, zpass, out);
+    A.cv1[c] = out[0];
+    A.cv2[c] = out[1];
+    A.db1[c] = out[2];
     };
     ELOOP(cell)
 }
