@@ -88,6 +88,9 @@ def bench_batched(steps, seq_steps, ngrids=256, seed=2024):
     setup = time.perf_counter() - t0
     cells = float(sum(n ** 3 for n in sizes))
     b = L.Batch(grids)
+    t0 = time.perf_counter()
+    tune = b.autotune(steps=2)  # launch configuration for this batch (state restored)
+    tune_s = time.perf_counter() - t0
     b.steps(2)  # capture and warm
     b.sync()
     t0 = time.perf_counter()
@@ -106,13 +109,13 @@ def bench_batched(steps, seq_steps, ngrids=256, seed=2024):
         g.close()
     rb, rs = cells * steps / tb, cells * seq_steps / ts
     log(f"bench: batched {ngrids} grids ({cells / 1e6:.1f} M cells): {rb / 1e9:.3f} G cell-updates/s batched, "
-        f"{rs / 1e9:.3f} sequential, x{rb / rs:.2f} (set-up {setup:.1f} s)")
+        f"{rs / 1e9:.3f} sequential, x{rb / rs:.2f} (set-up {setup:.1f} s, autotune {tune_s:.1f} s: {tune})")
     return {"workload": f"{ngrids} Orszag-Tang grids, n log-uniform in [32, 128], beta U[0.1, 10], "
                         f"amp U[0, 0.05], FP64, seed {seed}",
             "cells": cells, "value": rb, "unit": "cell-updates/s", "steps": steps,
             "ms_per_step": 1e3 * tb / steps, "sequential_value": rs, "sequential_steps_per_grid": seq_steps,
             "speedup_vs_sequential": rb / rs, "launches_per_step": launches,
-            "kernel_ms_per_step_eager": prof_ms}
+            "kernel_ms_per_step_eager": prof_ms, "tune": tune, "autotune_s": tune_s}
 
 
 def bench_serving(qps, duration, seed, budget, k):
@@ -225,6 +228,8 @@ def main():
     ap.add_argument("--fp32-n", type=int, default=256)
     ap.add_argument("--fp32-steps", type=int, default=100)
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch configuration instead of tuning it before the timed steps")
     ap.add_argument("--eager", action="store_true",
                     help="launch every kernel from the host instead of replaying the captured step graph")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -250,6 +255,10 @@ def main():
     p = L.problem("bw", n=n, seed=1, noise=0.01, eta=1e-3)
     log(f"bench: Brio-Wu {n}^3 resistive FP64, warmup {a.warmup}, steps {a.steps}")
     dev = L.Lare.from_problem(p)
+    tune = None
+    if not a.no_autotune:
+        tune = dev.autotune(steps=3)  # state saved and restored around the candidates
+        log(f"bench: autotuned launch configuration {tune}")
     if a.eager:
         dev.set_graph(False)
     if a.warmup:
@@ -290,7 +299,7 @@ def main():
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"brio-wu {n}^3 resistive (eta=1e-3, 1% density noise), FP64",
                       "nx": n, "ny": n, "nz": n, "parallelism": "single-gpu"},
-           "roofline": roofline, "cpu_baseline": cpu}
+           "roofline": roofline, "cpu_baseline": cpu, "tune": tune}
     out.update(extra)
     print(json.dumps(out), flush=True)
 

@@ -192,6 +192,34 @@ int lare_batch_profile_begin(lare_batch *batch);
 int lare_batch_profile_end(lare_batch *batch, double *total_ms, int32_t *launches);
 
 /* ------------------------------------------------------------------------
+ * Launch configuration and autotuner (no counterpart in the reference).
+ * The fused kernel set has several forms of its stencil kernels (lanes
+ * walking several cells and reusing the values neighbours share, or one
+ * cell per lane) and workgroup orders; every configuration gives
+ * bit-identical results, so the fastest can be chosen by measurement.
+ */
+typedef struct lare_tune {
+    int32_t walk;       /* lane-walking kernels: 0 by launch size, 1 always, 2 never */
+    int32_t march_k;    /* cells per lane of the x/y transport and CT-flux kernels: 8 or 16 */
+    int32_t zx_k;       /* cells per lane of the z-line x-march kernels: 8 or 16 */
+    int32_t zline;      /* z-sweep kernels exchanging z-neighbours between lanes: 1, or 0 */
+    int32_t order;      /* workgroup order within a grid: 1 band-major, 0 plane-major */
+    double ms_per_step; /* the chosen configuration's measured step time (autotune) */
+} lare_tune;
+/* Times one captured step of every candidate configuration (coordinate
+ * descent over the fields above, `steps` replays each after one warm-up),
+ * keeps the fastest and returns it.  The members' device state is saved
+ * before and restored after, so tuning changes no result; it needs device
+ * memory for one copy of each member's arrays.  Only between steps, on the
+ * fused kernel set.  A later lare_set_kernel_set resets the configuration. */
+int lare_autotune(lare_ctx *ctx, int32_t steps, lare_tune *out);
+int lare_batch_autotune(lare_batch *batch, int32_t steps, lare_tune *out);
+int lare_get_tune(lare_ctx *ctx, lare_tune *out);
+int lare_set_tune(lare_ctx *ctx, const lare_tune *t);
+int lare_batch_get_tune(lare_batch *batch, lare_tune *out);
+int lare_batch_set_tune(lare_batch *batch, const lare_tune *t);
+
+/* ------------------------------------------------------------------------
  * Problem set-up (the initial-condition packages' role: controlVariables,
  * setDomain, initialConditions, setBoundaryConditions of
  * src/InitialConditions/OrszagTang3D/OrszagTang3D.cpp:28-158 and

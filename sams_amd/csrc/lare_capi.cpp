@@ -89,8 +89,126 @@ int ensure_solo(lare_ctx *c)
         if (rc) return fail(rc, "grid group: " + err);
         c->solo_ready = true;
     }
-    // kernel set 1: lane-walking kernels by launch size; 2: always; 3: never
-    c->solo.walk_mode = c->g.kernel_set == 2 ? 1 : c->g.kernel_set == 3 ? 2 : 0;
+    // kernel set 2: lane-walking kernels always; 3: never; 1: by launch
+    // size or as tuned (lare_set_tune, lare_autotune)
+    if (c->g.kernel_set == 2) c->solo.walk_mode = 1;
+    if (c->g.kernel_set == 3) c->solo.walk_mode = 2;
+    return 0;
+}
+
+// Launch configuration of a group (the fields of lare_tune).
+lare_tune default_tune()
+{
+    lare::Group d;
+    return lare_tune{d.walk_mode, d.march_k, d.zx_k, d.zline ? 1 : 0, d.order, 0.0};
+}
+lare_tune get_tune(const lare::Group &gp)
+{
+    return lare_tune{gp.walk_mode, gp.march_k, gp.zx_k, gp.zline ? 1 : 0, gp.order, 0.0};
+}
+bool valid_tune(const lare_tune &t)
+{
+    return t.walk >= 0 && t.walk <= 2 && (t.march_k == 8 || t.march_k == 16) && (t.zx_k == 8 || t.zx_k == 16) &&
+           (t.zline == 0 || t.zline == 1) && (t.order == 0 || t.order == 1);
+}
+void apply_tune(lare::Group &gp, const lare_tune &t)
+{
+    gp.walk_mode = t.walk;
+    gp.march_k = t.march_k;
+    gp.zx_k = t.zx_k;
+    gp.zline = t.zline != 0;
+    gp.order = t.order;
+    lare::group_drop_graph(gp);
+}
+
+// Autotuner: coordinate descent over the launch configuration of group gp,
+// each candidate timed as `steps` replays of one captured step (after one
+// warm-up replay) with HIP events on `stream`.  Every member's device pool
+// (its whole state, scalars included) is copied aside first and restored at
+// the end, so the members continue exactly where they were.
+int tune_group(lare::Group &gp, hipStream_t stream, int steps, lare_tune *best_out, const char *who)
+{
+    const std::string w = who;
+    if (steps < 1) return fail(-1, w + ": steps must be at least 1");
+    if (gp.prof && gp.prof->on) return fail(-1, w + ": not while profiling");
+    for (lare::Grid *g : gp.mem)
+        if (g->st != 0) return fail(-1, w + ": only between steps");
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<void *> snap(gp.mem.size(), nullptr);
+    auto release = [&]() {
+        for (void *p : snap)
+            if (p) (void)hipFree(p);
+    };
+    for (size_t m = 0; m < gp.mem.size(); ++m) {
+        hipError_t e = hipMalloc(&snap[m], gp.mem[m]->pool_bytes);
+        if (e != hipSuccess) {
+            snap[m] = nullptr;
+            release();
+            return fail(-4, w + ": no device memory for the copy of the state: " + hipGetErrorString(e));
+        }
+        e = hipMemcpyAsync(snap[m], gp.mem[m]->pool, gp.mem[m]->pool_bytes, hipMemcpyDeviceToDevice, stream);
+        if (e != hipSuccess) {
+            release();
+            return fail(-3, w + ": saving the state: " + hipGetErrorString(e));
+        }
+    }
+    hipEvent_t ea = nullptr, eb = nullptr;
+    (void)hipEventCreate(&ea);
+    (void)hipEventCreate(&eb);
+    auto measure = [&](const lare_tune &t, double &ms) -> hipError_t {
+        apply_tune(gp, t);
+        lare::group_prepare(gp);
+        hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        lare::group_step(gp);
+        e = hipStreamEndCapture(stream, &gp.graph);
+        if (e == hipSuccess) e = hipGraphInstantiate(&gp.graph_exec, gp.graph, nullptr, nullptr, 0);
+        if (e == hipSuccess) e = hipGraphLaunch(gp.graph_exec, stream);
+        if (e == hipSuccess) e = hipEventRecord(ea, stream);
+        for (int i = 0; i < steps && e == hipSuccess; ++i) e = hipGraphLaunch(gp.graph_exec, stream);
+        if (e == hipSuccess) e = hipEventRecord(eb, stream);
+        if (e == hipSuccess) e = hipEventSynchronize(eb);
+        float f = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&f, ea, eb);
+        ms = (double)f / steps;
+        lare::group_drop_graph(gp);
+        return e;
+    };
+    lare_tune best = get_tune(gp);
+    double best_ms = 0.0;
+    hipError_t e = measure(best, best_ms);
+    // one field at a time, keeping whatever is fastest so far
+    const int walks[2] = {1, 2}, ks[2] = {8, 16}, flags[2] = {0, 1};
+    for (int field = 0; field < 5 && e == hipSuccess; ++field) {
+        for (int v = 0; v < 2 && e == hipSuccess; ++v) {
+            lare_tune t = best;
+            if (field == 0) t.walk = walks[v];
+            if (field == 1) t.march_k = ks[v];
+            if (field == 2) t.zx_k = ks[v];
+            if (field == 3) t.zline = flags[v];
+            if (field == 4) t.order = flags[v];
+            if (t.walk == best.walk && t.march_k == best.march_k && t.zx_k == best.zx_k && t.zline == best.zline &&
+                t.order == best.order)
+                continue;
+            double ms = 0.0;
+            e = measure(t, ms);
+            if (e == hipSuccess && ms < best_ms) {
+                best = t;
+                best_ms = ms;
+            }
+        }
+    }
+    (void)hipEventDestroy(ea);
+    (void)hipEventDestroy(eb);
+    apply_tune(gp, best);
+    for (size_t m = 0; m < gp.mem.size(); ++m)
+        (void)hipMemcpyAsync(gp.mem[m]->pool, snap[m], gp.mem[m]->pool_bytes, hipMemcpyDeviceToDevice, stream);
+    hipError_t e2 = hipStreamSynchronize(stream);
+    release();
+    if (e != hipSuccess) return fail(-3, w + ": " + hipGetErrorString(e));
+    if (e2 != hipSuccess) return fail(-3, w + ": restoring the state: " + hipGetErrorString(e2));
+    best.ms_per_step = best_ms;
+    if (best_out) *best_out = best;
     return 0;
 }
 
@@ -696,6 +814,7 @@ extern "C" int lare_set_kernel_set(lare_ctx *c, int32_t kernel_set)
         if (rc) return rc;
     }
     c->g.kernel_set = kernel_set;
+    if (c->solo_ready) apply_tune(c->solo, default_tune());
     drop_graphs(c);
     return 0;
 }
@@ -866,6 +985,61 @@ extern "C" int lare_energy(lare_ctx *c, double *out)
     rc = lare::group_energy(c->solo, e, err);
     if (rc) return fail(rc, err);
     std::memcpy(out, e.data(), 4 * sizeof(double));
+    return 0;
+}
+
+extern "C" int lare_autotune(lare_ctx *c, int32_t steps, lare_tune *out)
+{
+    if (!c) return fail(-1, "lare_autotune: null context");
+    if (c->g.kernel_set != 1) return fail(-1, "lare_autotune: only on kernel set 1");
+    int rc = ensure_solo(c);
+    if (rc) return rc;
+    drop_graphs(c);
+    return tune_group(c->solo, c->g.stream, steps, out, "lare_autotune");
+}
+
+extern "C" int lare_get_tune(lare_ctx *c, lare_tune *out)
+{
+    if (!c || !out) return fail(-1, "lare_get_tune: null argument");
+    int rc = ensure_solo(c);
+    if (rc) return rc;
+    *out = get_tune(c->solo);
+    return 0;
+}
+
+extern "C" int lare_set_tune(lare_ctx *c, const lare_tune *t)
+{
+    if (!c || !t) return fail(-1, "lare_set_tune: null argument");
+    if (!valid_tune(*t)) return fail(-1, "lare_set_tune: invalid configuration");
+    if (c->g.kernel_set != 1) return fail(-1, "lare_set_tune: only on kernel set 1");
+    int rc = ensure_solo(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->g.stream));
+    drop_graphs(c);
+    apply_tune(c->solo, *t);
+    return 0;
+}
+
+extern "C" int lare_batch_autotune(lare_batch *b, int32_t steps, lare_tune *out)
+{
+    if (!b) return fail(-1, "lare_batch_autotune: null batch");
+    for (lare_ctx *c : b->members) HIPCHK(hipStreamSynchronize(c->g.stream));
+    return tune_group(b->gp, b->stream, steps, out, "lare_batch_autotune");
+}
+
+extern "C" int lare_batch_get_tune(lare_batch *b, lare_tune *out)
+{
+    if (!b || !out) return fail(-1, "lare_batch_get_tune: null argument");
+    *out = get_tune(b->gp);
+    return 0;
+}
+
+extern "C" int lare_batch_set_tune(lare_batch *b, const lare_tune *t)
+{
+    if (!b || !t) return fail(-1, "lare_batch_set_tune: null argument");
+    if (!valid_tune(*t)) return fail(-1, "lare_batch_set_tune: invalid configuration");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    apply_tune(b->gp, *t);
     return 0;
 }
 

@@ -149,7 +149,7 @@ __device__ __forceinline__ int grid_of(const int *boff, int n, int b)
     const int iz_ = R.z0 + t_ % nzr_;                                                              \
     (void)P;
 // Run cell(ix, iy, iz) for the cell of this lane.
-#define ELOOP(cell) \
+#define ECELL(cell) \
     if (ok_) cell(ix_, iy_, iz_);
 
 template <int DIR>
@@ -234,7 +234,7 @@ KERNEL void f_prep(const Src<T> S, double xpass, double ypass, double zpass)
     A.cv2[c] = out[1];
     A.db1[c] = out[2];
     };
-    ELOOP(cell)
+    ECELL(cell)
 }
 
 // R2: vertex volume before remap, cvc1 = 0.125 * (8 cells of cv1).
@@ -249,7 +249,7 @@ KERNEL void f_cvc1(const Src<T> S)
     A.cvc1[c] = 0.125 * (C1[c] + C1[c + sx] + C1[c + sy] + C1[c + sx + sy] + C1[c + 1] + C1[c + sx + 1] +
                          C1[c + sy + 1] + C1[c + sx + sy + 1]);
     };
-    ELOOP(cell)
+    ECELL(cell)
 }
 
 // Face area of B component SC (remap.cpp:46-50): dxab, dyab, dzab.
@@ -335,7 +335,7 @@ KERNEL void f_ct_fluxes(const Src<T> S)
         lare::gbl(A.fl[1])[c] = ct_flux<2>(A, G, c, BY, VZ1, -sx, sy, dt);
     }
     };
-    ELOOP(cell)
+    ECELL(cell)
 }
 
 // R4: the four CT updates of the sweep in their reference order, each over
@@ -392,7 +392,7 @@ KERNEL void f_ct_update(const Src<T> S)
         if (ebz) BZ[c] = bz / (b_area<2>(A, ix, iy, iz) + DBL_EPSILON);
     }
     };
-    ELOOP(cell)
+    ECELL(cell)
 }
 
 // Face-averaged sweep velocity of the mass and energy fluxes.
@@ -448,7 +448,7 @@ KERNEL void f_mass_flux(const Src<T> S)
     double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
     DM[c] = (fu + Di * (1.0 - phi)) * o_v;
     };
-    ELOOP(cell)
+    ECELL(cell)
 }
 
 // Energy flux of specific energy E on face c (x_energy_flux,
@@ -623,7 +623,7 @@ KERNEL void f_update_all(const Src<T> S)
         A.ei2[c] = (EI[c] * cv1 * rho1 + em[1] - ec[1]) / (cv2 * rho);
     }
     };
-    ELOOP(cell)
+    ECELL(cell)
 }
 
 // Sum of the eight corner values of vertex c in the reference's order (c,
@@ -1124,7 +1124,7 @@ KERNEL void f_wshift(const Src<T> S)
     else
         lare::gbl(A.fl[0])[c] = (w[c] + w[c + sdir<DIR>(G)]) * 0.5;
     };
-    ELOOP(cell)
+    ECELL(cell)
 }
 
 // ============================================================= Lagrangian
@@ -1154,7 +1154,7 @@ KERNEL void g_cs(const Src<T> S)
     double b2 = bx1 * bx1 + by1 * by1 + bz1 * bz1;
     A.cs[c] = sqrt((gas_gamma * p + b2) / rmin);
     };
-    ELOOP(cell)
+    ECELL(cell)
 }
 
 #define CV8SUM(ix, iy, iz)                                                                               \
@@ -1183,7 +1183,7 @@ KERNEL void g_vertex(const Src<T> S)
                  s[c + sx + 1] * c101 + s[c + sy + 1] * c011 + s[c + sx + sy + 1] * c111;
     A.cs_v[c] = 0.125 * sum / cv_v;
     };
-    ELOOP(cell)
+    ECELL(cell)
 }
 
 // edge_viscosity (lagran.cpp:56-93) between c1 and c1 + s.
@@ -1261,7 +1261,7 @@ KERNEL void g_alpha(const Src<T> S)
         A.alpha3[c] = g_edge_visc(A, dt, visc1, visc2_norm, dvdots, DZB(iz), DZB(iz - 1), DZB(iz + 1), cs_edge, c1, 1);
     }
     };
-    ELOOP(cell)
+    ECELL(cell)
 }
 
 #define SQD(p, q) \
@@ -1597,7 +1597,7 @@ KERNEL void g_predict(const Src<T> S)
         A.p_i[c] = e1_i * (gas_gamma - 1.0) * rho * vol / cv1;
     }
     };
-    ELOOP(cell)
+    ECELL(cell)
 }
 
 // L6 as a z-line x-march (H = 1): the same arithmetic as g_predict.  The
@@ -1788,7 +1788,7 @@ KERNEL void g_forces(const Src<T> S)
     VY1[c] = VY[c] + dt / 2.0 * syv / rv;
     VZ1[c] = VZ[c] + dt / 2.0 * szv / rv;
     };
-    ELOOP(cell)
+    ECELL(cell)
 }
 
 #define DOT(p, q)                                                                                   \
@@ -1852,7 +1852,7 @@ KERNEL OCC(2) void g_correct(const Src<T> S)
     A.ei2[c] = EI[c] + (dt * visc_heat - dv * A.p_i[c]) / rho;
     A.rho2[c] = rho / (1.0 + dv);
     };
-    ELOOP(cell)
+    ECELL(cell)
 }
 
 // C (corrector) as a z-line x-march (H = 1): the same arithmetic as g_correct.
@@ -1969,7 +1969,7 @@ KERNEL void r_copy(const Src<T> S)
     A.by1[c] = BY[c];
     A.bz1[c] = BZ[c];
     };
-    ELOOP(cell)
+    ECELL(cell)
 }
 
 // eta_calc (lagran.cpp:445-476) and rkstep (lagran.cpp:507-535) over
@@ -2001,7 +2001,7 @@ KERNEL void r_rkstep(const Src<T> S)
     lare::gbl(A.flux_z)[c] = -jz * eta * dzc * 0.5;
     A.curlb[c] = eta * (jx * jx + jy * jy + jz * jz);
     };
-    ELOOP(cell)
+    ECELL(cell)
 }
 
 // bstep's three components (lagran.cpp:537-590) with the Ohmic heating of
@@ -2037,7 +2037,7 @@ KERNEL void r_b_energy(const Src<T> S)
         EE[c] += sum_curlb * dt / (8.0 * RHO[c]);
     }
     };
-    ELOOP(cell)
+    ECELL(cell)
 }
 
 // rkstep's current, resistivity, flux and Ohmic term (lagran.cpp:445-535) at
@@ -2277,7 +2277,7 @@ KERNEL void e_energy_part(const Src<T> S, double *part)
         const double bx = 0.5 * (BX[c] + BX[c - sx]), by = 0.5 * (BY[c] + BY[c - sy]), bz = 0.5 * (BZ[c] + BZ[c - 1]);
         e[2] += (bx * bx + by * by + bz * bz) / (2.0 * P.mu0) * cv;
     };
-    ELOOP(cell)
+    ECELL(cell)
     __shared__ double sh[3][256];
     for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] = e[k];
     __syncthreads();
@@ -2598,6 +2598,7 @@ bool walk_ok(const Group &gp, int subset)
 }
 bool use_zx(const Group &gp, int subset) { return gp.zx_k > 0 && walk_ok(gp, subset); }
 bool use_walk(const Group &gp) { return walk_ok(gp, 0); }
+int zxk(const Group &gp) { return gp.zx_k >= 16 ? 16 : 8; }
 
 
 template <class T>
@@ -2612,7 +2613,8 @@ void start_a_t(Group &gp)
     EL(g_vertex<T>, 4 * w, gp, 0, V);
     EL(g_alpha<T>, 8 * w, gp, 0, rs(-1, 1, -1, 2, 0, 2));
     if (use_zx(gp, 0))
-        elaunch_(g_visc_dt_zx<T, 8>, "g_visc_dt", 16 * w, gp, 0, rs(0, 1, 0, 1, 0, 1), 0, kZX + 100 + 8);
+        elaunch_(gp.zx_k >= 16 ? g_visc_dt_zx<T, 16> : g_visc_dt_zx<T, 8>, "g_visc_dt", 16 * w, gp, 0,
+                 rs(0, 1, 0, 1, 0, 1), 0, kZX + 100 + zxk(gp));
     else
         EL(g_visc_dt<T>, 16 * w, gp, 0, rs(0, 1, 0, 1, 0, 1));
     per_grid<T>(g_dt_final<T>, "g_dt_final", gp);
@@ -2630,7 +2632,8 @@ void start_b_t(Group &gp, int substeps)
         if (substeps % 2) EL(r_copy<T>, 6 * w, gp, 1, C);
         for (int i = 0; i < substeps; ++i) {
             const int to_bx1 = (substeps - 1 - i) % 2;
-            elaunch_(r_substep_zx<T, 8>, "r_substep", 9 * w, gp, 1, rs(0, 0, 0, 0, 0, 0), 0, kZX + 100 + 8,
+            elaunch_(gp.zx_k >= 16 ? r_substep_zx<T, 16> : r_substep_zx<T, 8>, "r_substep", 9 * w, gp, 1,
+                     rs(0, 0, 0, 0, 0, 0), 0, kZX + 100 + zxk(gp),
                      to_bx1);
             ebcs<T>(gp, to_bx1 ? VBE_1 : VBE_0, -1, 1);
         }
@@ -2643,7 +2646,8 @@ void start_b_t(Group &gp, int substeps)
         }
     }
     if (use_zx(gp, 0))
-        elaunch_(g_predict_zx<T, 8>, "g_predict", 16 * w, gp, 0, C, 0, kZX + 100 + 8);
+        elaunch_(gp.zx_k >= 16 ? g_predict_zx<T, 16> : g_predict_zx<T, 8>, "g_predict", 16 * w, gp, 0, C, 0,
+                 kZX + 100 + zxk(gp));
     else
         EL(g_predict<T>, 16 * w, gp, 0, C);
     EL(g_forces<T>, 19 * w, gp, 0, rs(0, 0, 0, 0, 0, 0));
@@ -2659,7 +2663,8 @@ void half_t(Group &gp)
     // reads v v1 alpha1..3 rho_v fx_visc..fz_visc rho e_e e_i p_e p_i;
     // writes v rho e_e e_i
     if (use_zx(gp, 0))
-        elaunch_(g_correct_zx<T, 8>, "g_correct", 22 * w, gp, 0, rs(0, 0, 0, 0, 0, 0), 0, kZX + 100 + 8);
+        elaunch_(gp.zx_k >= 16 ? g_correct_zx<T, 16> : g_correct_zx<T, 8>, "g_correct", 22 * w, gp, 0,
+                 rs(0, 0, 0, 0, 0, 0), 0, kZX + 100 + zxk(gp));
     else
         EL(g_correct<T>, 22 * w, gp, 0, rs(0, 0, 0, 0, 0, 0));
     // velocity_bcs, energy_bcs, density_bcs (independent variables)
@@ -2679,8 +2684,9 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
         elaunch_(f_ct_zline<T>, "f_ct_fluxes<2>", 7 * w + 2 * 8, gp, 0, all0, 0, kZLine);
     } else if (DIR != 2 && gp.march_k > 0 && use_walk(gp)) {
         constexpr int MD = DIR == 2 ? 0 : DIR;
-        elaunch_(f_ct_march<T, MD, 16>, DIR == 0 ? "f_ct_fluxes<0>" : "f_ct_fluxes<1>", 7 * w + 2 * 8, gp, 0, all0,
-                 0, kMarch + 100 * DIR + 16);
+        const int K = gp.march_k >= 16 ? 16 : 8;
+        elaunch_(K == 16 ? f_ct_march<T, MD, 16> : f_ct_march<T, MD, 8>, DIR == 0 ? "f_ct_fluxes<0>" : "f_ct_fluxes<1>",
+                 7 * w + 2 * 8, gp, 0, all0, 0, kMarch + 100 * DIR + K);
     } else {
         EL((f_ct_fluxes<T, DIR>), 7 * w + 2 * 8, gp, 0, all0);
     }
@@ -2725,7 +2731,6 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
 template <class T>
 void end_t(Group &gp)
 {
-    const double w = (double)sizeof(T);
     sweep_t<T, 0>(gp, 1.0, 1.0, 1.0, 1);
     sweep_t<T, 1>(gp, 0.0, 1.0, 1.0, 0);
     sweep_t<T, 2>(gp, 0.0, 0.0, 1.0, 1);

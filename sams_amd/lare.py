@@ -41,6 +41,15 @@ class LareDesc(ctypes.Structure):
                 ("eta0", ctypes.c_double), ("j_max", ctypes.c_double)]
 
 
+class LareTune(ctypes.Structure):
+    """lare_tune: launch configuration of the fused kernel set."""
+    _fields_ = [("walk", ctypes.c_int32), ("march_k", ctypes.c_int32), ("zx_k", ctypes.c_int32),
+                ("zline", ctypes.c_int32), ("order", ctypes.c_int32), ("ms_per_step", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class LareProblem(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("bcx", ctypes.c_int32),
                 ("nx", ctypes.c_int64), ("ny", ctypes.c_int64), ("nz", ctypes.c_int64),
@@ -108,6 +117,12 @@ def lib():
             "lare_batch_energy": (ctypes.c_int, [vp, P(ctypes.c_double)]),
             "lare_batch_profile_begin": (ctypes.c_int, [vp]),
             "lare_batch_profile_end": (ctypes.c_int, [vp, P(ctypes.c_double), P(ctypes.c_int32)]),
+            "lare_autotune": (ctypes.c_int, [vp, ctypes.c_int32, P(LareTune)]),
+            "lare_get_tune": (ctypes.c_int, [vp, P(LareTune)]),
+            "lare_set_tune": (ctypes.c_int, [vp, P(LareTune)]),
+            "lare_batch_autotune": (ctypes.c_int, [vp, ctypes.c_int32, P(LareTune)]),
+            "lare_batch_get_tune": (ctypes.c_int, [vp, P(LareTune)]),
+            "lare_batch_set_tune": (ctypes.c_int, [vp, P(LareTune)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -198,8 +213,29 @@ class Lare:
         _check(lib().lare_problem_bcs(self.h, ctypes.byref(p)), "lare_problem_bcs")
 
     def set_kernel_set(self, k):
-        """1: fused kernels (default); 0: one kernel per reference kernel."""
+        """1: fused kernels (default; lane-walking forms by launch size or as
+        tuned); 2 / 3: fused with the lane-walking forms always / never;
+        0: one kernel per reference kernel."""
         _check(lib().lare_set_kernel_set(self.h, int(k)), "lare_set_kernel_set")
+
+    def autotune(self, steps=3):
+        """Time every candidate launch configuration (state saved and
+        restored), keep the fastest; returns it as a dict."""
+        t = LareTune()
+        _check(lib().lare_autotune(self.h, int(steps), ctypes.byref(t)), "lare_autotune")
+        return t.as_dict()
+
+    def get_tune(self):
+        t = LareTune()
+        _check(lib().lare_get_tune(self.h, ctypes.byref(t)), "lare_get_tune")
+        return t.as_dict()
+
+    def set_tune(self, **kw):
+        t = LareTune()
+        _check(lib().lare_get_tune(self.h, ctypes.byref(t)), "lare_get_tune")
+        for k, v in kw.items():
+            setattr(t, k, v)
+        _check(lib().lare_set_tune(self.h, ctypes.byref(t)), "lare_set_tune")
 
     def set_graph(self, on):
         """Replay one captured HIP graph per step in steps() (default on)."""
@@ -313,6 +349,25 @@ class Batch:
 
     def set_graph(self, on):
         _check(lib().lare_batch_set_graph(self.h, int(bool(on))), "lare_batch_set_graph")
+
+    def autotune(self, steps=3):
+        """lare_batch_autotune: the fastest launch configuration for this
+        batch (members' state saved and restored); returns it as a dict."""
+        t = LareTune()
+        _check(lib().lare_batch_autotune(self.h, int(steps), ctypes.byref(t)), "lare_batch_autotune")
+        return t.as_dict()
+
+    def get_tune(self):
+        t = LareTune()
+        _check(lib().lare_batch_get_tune(self.h, ctypes.byref(t)), "lare_batch_get_tune")
+        return t.as_dict()
+
+    def set_tune(self, **kw):
+        t = LareTune()
+        _check(lib().lare_batch_get_tune(self.h, ctypes.byref(t)), "lare_batch_get_tune")
+        for k, v in kw.items():
+            setattr(t, k, v)
+        _check(lib().lare_batch_set_tune(self.h, ctypes.byref(t)), "lare_batch_set_tune")
 
     def energy(self):
         out = (ctypes.c_double * (4 * len(self.members)))()
