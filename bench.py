@@ -225,7 +225,7 @@ def main():
     ap.add_argument("--serve-duration", type=float, default=30.0)
     ap.add_argument("--serve-steps-per-tick", type=int, default=8)
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--fp32-n", type=int, default=256)
+    ap.add_argument("--fp32-n", type=int, default=512)
     ap.add_argument("--fp32-steps", type=int, default=100)
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
