@@ -100,16 +100,16 @@ int ensure_solo(lare_ctx *c)
 lare_tune default_tune()
 {
     lare::Group d;
-    return lare_tune{d.walk_mode, d.march_k, d.zx_k, d.zline ? 1 : 0, d.order, 0.0};
+    return lare_tune{d.walk_mode, d.march_k, d.zx_k, d.zline ? 1 : 0, d.order, d.ct_fuse ? 1 : 0, 0.0};
 }
 lare_tune get_tune(const lare::Group &gp)
 {
-    return lare_tune{gp.walk_mode, gp.march_k, gp.zx_k, gp.zline ? 1 : 0, gp.order, 0.0};
+    return lare_tune{gp.walk_mode, gp.march_k, gp.zx_k, gp.zline ? 1 : 0, gp.order, gp.ct_fuse ? 1 : 0, 0.0};
 }
 bool valid_tune(const lare_tune &t)
 {
     return t.walk >= 0 && t.walk <= 2 && (t.march_k == 8 || t.march_k == 16) && (t.zx_k == 8 || t.zx_k == 16) &&
-           (t.zline == 0 || t.zline == 1) && (t.order == 0 || t.order == 1);
+           (t.zline == 0 || t.zline == 1) && (t.order == 0 || t.order == 1) && (t.ct_fuse == 0 || t.ct_fuse == 1);
 }
 void apply_tune(lare::Group &gp, const lare_tune &t)
 {
@@ -118,6 +118,7 @@ void apply_tune(lare::Group &gp, const lare_tune &t)
     gp.zx_k = t.zx_k;
     gp.zline = t.zline != 0;
     gp.order = t.order;
+    gp.ct_fuse = t.ct_fuse;
     lare::group_drop_graph(gp);
 }
 
@@ -179,7 +180,7 @@ int tune_group(lare::Group &gp, hipStream_t stream, int steps, lare_tune *best_o
     hipError_t e = measure(best, best_ms);
     // one field at a time, keeping whatever is fastest so far
     const int walks[2] = {1, 2}, ks[2] = {8, 16}, flags[2] = {0, 1};
-    for (int field = 0; field < 5 && e == hipSuccess; ++field) {
+    for (int field = 0; field < 6 && e == hipSuccess; ++field) {
         for (int v = 0; v < 2 && e == hipSuccess; ++v) {
             lare_tune t = best;
             if (field == 0) t.walk = walks[v];
@@ -187,8 +188,9 @@ int tune_group(lare::Group &gp, hipStream_t stream, int steps, lare_tune *best_o
             if (field == 2) t.zx_k = ks[v];
             if (field == 3) t.zline = flags[v];
             if (field == 4) t.order = flags[v];
+            if (field == 5) t.ct_fuse = flags[v];
             if (t.walk == best.walk && t.march_k == best.march_k && t.zx_k == best.zx_k && t.zline == best.zline &&
-                t.order == best.order)
+                t.order == best.order && t.ct_fuse == best.ct_fuse)
                 continue;
             double ms = 0.0;
             e = measure(t, ms);

@@ -74,13 +74,22 @@ __device__ __forceinline__ int xcd_swizzle(int b, int nblk)
 }
 
 // The arrays with the state copies named for this launch: rho .. vz the
-// copy holding the current state (the second copy when st is 1), rho2 ..
-// vz2 the other one.
+// copy holding the current state (the second copy when bit 0 of st is set),
+// rho2 .. vz2 the other one; with bit 1 set, bx .. bz name bx1 .. bz1 (B's
+// second copy during the remap) and bx1 .. bz1 name B's own arrays.
 template <class T>
 __device__ __forceinline__ ArrE<T> state_view(const ArrE<T> &a, int st)
 {
     ArrE<T> v = a;
-    if (st) {
+    if (st & 2) {
+        v.bx = a.bx1;
+        v.by = a.by1;
+        v.bz = a.bz1;
+        v.bx1 = a.bx;
+        v.by1 = a.by;
+        v.bz1 = a.bz;
+    }
+    if (st & 1) {
         v.rho = a.rho2;
         v.ee = a.ee2;
         v.ei = a.ei2;
@@ -858,9 +867,11 @@ __device__ __forceinline__ ZXPos zx_map(const Src<T> &S, const Geo &G, int g_, i
     z.iy = R.y0 + row;
     z.iz = R.z0 - H + (u - row * L);
     z.wz = inside && lane >= H && lane < 64 - H && z.iz >= R.z0 && z.iz <= R.z1;
-    // a halo position beyond the box (a range reaching z = n+2) loads at
-    // n+2; only a lane above it would read its values, and none writes
+    // a halo position beyond the box (a range reaching z = -2 or n+2) loads
+    // at the box edge; only a lane beyond it would read its values, and
+    // none writes
     if (z.iz > G.nz + 2) z.iz = G.nz + 2;
+    if (z.iz < -2) z.iz = -2;
     z.ix0 = R.x0 + xc * K - 1;
     z.ix1 = z.ix0 + K < R.x1 ? z.ix0 + K : R.x1;
     return z;
@@ -1102,6 +1113,191 @@ KERNEL void f_ct_zline(const Src<T> S)
     if (!writer) return;
     lare::gbl(A.fl[0])[c] = ct_flux_v(fam, fa, fap, fap2, dba, dbap, va, dt, cvc_m, cvc_0, cvc_p, cv1_0, cv1_p);
     lare::gbl(A.fl[1])[c] = ct_flux_v(fbm, fb, fbp, fbp2, dbb, dbbp, vb, dt, cvc_m, cvc_0, cvc_p, cv1_0, cv1_p);
+}
+
+// R3 + R4 of the x sweep in one z-line x-march (H = 1) over the union of the
+// B extents (-2..n+2): B is read from its own arrays and written, times its
+// face area and CT-updated, into bx1..bz1 (B's second copy during the
+// remap), so no lane reads a B another lane has written.  The by flux is
+// evaluated on this lane's column (y) and on (y-1) (for bx), the bz flux on
+// (y) only; their values at x-1 are carried and the bz flux at z-1 comes
+// from the lane below.  Each flux and update is the expression of
+// ct_flux<0, SC> and f_ct_update<0>.  Loads beyond the box are clamped;
+// they feed only fluxes outside (0..n)^3, which no update reads.
+template <class T, int K>
+KERNEL OCC(3) void f_ctx_fused(const Src<T> S)
+{
+    const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
+    const int g_ = grid_of(S.boff, S.ngrid, b_);
+    const EDev<T> &D_ = S.devs[g_];
+    const Geo G = D_.g;
+    const ArrE<T> A = state_view(D_.a, S.st);
+    const ZXPos Z = zx_map<1, K>(S, G, g_, b_);
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const int iy = Z.iy, iz = Z.iz;
+    const double dt = lare::gbl(A.s)->dt;
+    auto clx = [&](int x) { return x < -2 ? -2 : (x > nx + 2 ? nx + 2 : x); };
+    const int ym = iy - 1 < -2 ? -2 : iy - 1, yp = iy + 1 > ny + 2 ? ny + 2 : iy + 1;
+    const Fld<T> db1 = A.db1;
+    const double dzb = DZB(iz), dyb = DYB(iy);
+    // f = B * area / db and db along x at x+k, for the by flux on rows y
+    // (a) and y-1 (m) and the bz flux on row y (b); area: dyab = dxb * dzb,
+    // dzab = dxb * dyb (remap.cpp:46-50)
+    auto col = [&](int xq, double &fa, double &da, double &sa, double &fm, double &dm, double &fb, double &db,
+                   double &sb) {
+        const int x = clx(xq);
+        const int c = IX(x, iy, iz);
+        const double d0 = db1[c], dbz = lane_next(d0);
+        da = (d0 + db1[IX(x, yp, iz)]) * 0.5;
+        dm = (db1[IX(x, ym, iz)] + d0) * 0.5;
+        db = (d0 + dbz) * 0.5;
+        const double dxb = DXB(x);
+        sa = BY[c] * (dxb * dzb);
+        fa = sa / da;
+        fm = BY[IX(x, ym, iz)] * (dxb * dzb) / dm;
+        sb = BZ[c] * (dxb * dyb);
+        fb = sb / db;
+    };
+    double fa[3], da[2], sa[2], fm[3], dmm[2], fb[3], dbb[2], sb[2], t[5];
+    col(Z.ix0 - 1, fa[0], t[0], t[1], fm[0], t[2], fb[0], t[3], t[4]);  // distinct outputs: col writes them in turn
+    col(Z.ix0, fa[1], da[0], sa[0], fm[1], dmm[0], fb[1], dbb[0], sb[0]);
+    col(Z.ix0 + 1, fa[2], da[1], sa[1], fm[2], dmm[1], fb[2], dbb[1], sb[1]);
+    auto ca = [&](int x, int y) { return A.cvc1[IX(clx(x), y, iz)]; };
+    auto cv = [&](int x, int y) { return A.cv1[IX(clx(x), y, iz)]; };
+    double cva_m = ca(Z.ix0 - 1, iy), cva_0 = ca(Z.ix0, iy), cvm_m = ca(Z.ix0 - 1, ym), cvm_0 = ca(Z.ix0, ym);
+    double c1a_0 = cv(Z.ix0, iy), c1m_0 = cv(Z.ix0, ym);
+    double Fa_x = 0.0, Fb_x = 0.0;  // fluxes at x-1
+    for (int ix = Z.ix0; ix <= Z.ix1; ++ix) {
+        const int c = IX(ix, iy, iz);
+        double fa3, da2, sa2, fm3, dm2, fb3, db2, sb2;
+        col(ix + 2, fa3, da2, sa2, fm3, dm2, fb3, db2, sb2);
+        const double cva_p = ca(ix + 1, iy), cvm_p = ca(ix + 1, ym), c1a_p = cv(ix + 1, iy), c1m_p = cv(ix + 1, ym);
+        const int xc = clx(ix);
+        const double w0 = VX1[IX(xc, iy, iz)], wm = VX1[IX(xc, ym, iz)];
+        const double w0z = lane_prev(w0), wmz = lane_prev(wm);
+        // by flux (x_by: v = (vx1[c] + vx1[c-1]) / 2) at (y) and (y-1);
+        // bz flux (v = (vx1[c] + vx1[c-sy]) / 2) at (y)
+        const double Fa = ct_flux_v(fa[0], fa[1], fa[2], fa3, da[0], da[1], (w0 + w0z) * 0.5, dt, cva_m, cva_0, cva_p,
+                                    c1a_0, c1a_p);
+        const double Fm = ct_flux_v(fm[0], fm[1], fm[2], fm3, dmm[0], dmm[1], (wm + wmz) * 0.5, dt, cvm_m, cvm_0,
+                                    cvm_p, c1m_0, c1m_p);
+        const double Fb = ct_flux_v(fb[0], fb[1], fb[2], fb3, dbb[0], dbb[1], (w0 + wm) * 0.5, dt, cva_m, cva_0,
+                                    cva_p, c1a_0, c1a_p);
+        const double Fbz = lane_prev(Fb);
+        if (Z.wz && ix > Z.ix0) {
+            const bool x1 = ix >= 1 && ix <= nx, x0 = ix >= 0 && ix <= nx;
+            const bool y1 = iy >= 1 && iy <= ny, y0 = iy >= 0 && iy <= ny;
+            const bool z1 = iz >= 1 && iz <= nz, z0 = iz >= 0 && iz <= nz;
+            const bool ebx = iy >= -1 && iz >= -1, eby = ix >= -1 && iz >= -1, ebz = ix >= -1 && iy >= -1;
+            double bx = ebx ? BX[c] * b_area<0>(A, ix, iy, iz) : 0.0;
+            double by = eby ? sa[0] : 0.0;
+            double bz = ebz ? sb[0] : 0.0;
+            if (x1 && y0 && z1) by = by - Fa + Fa_x;
+            const bool bxr = x0 && y1 && z1;
+            if (bxr) bx = bx + Fa - Fm;
+            if (x1 && y1 && z0) bz = bz - Fb + Fb_x;
+            if (bxr) bx = bx + Fb - Fbz;
+            if (ebx) A.bx1[c] = bx;
+            if (eby) A.by1[c] = by;
+            if (ebz) A.bz1[c] = bz;
+        }
+        Fa_x = Fa;
+        Fb_x = Fb;
+        fa[0] = fa[1];
+        fa[1] = fa[2];
+        fa[2] = fa3;
+        fm[0] = fm[1];
+        fm[1] = fm[2];
+        fm[2] = fm3;
+        fb[0] = fb[1];
+        fb[1] = fb[2];
+        fb[2] = fb3;
+        da[0] = da[1];
+        da[1] = da2;
+        dmm[0] = dmm[1];
+        dmm[1] = dm2;
+        dbb[0] = dbb[1];
+        dbb[1] = db2;
+        sa[0] = sa[1];
+        sa[1] = sa2;
+        sb[0] = sb[1];
+        sb[1] = sb2;
+        cva_m = cva_0;
+        cva_0 = cva_p;
+        cvm_m = cvm_0;
+        cvm_0 = cvm_p;
+        c1a_0 = c1a_p;
+        c1m_0 = c1m_p;
+    }
+}
+
+// R3 + R4 of the z sweep in one z-line x-march (H = 2) over the union of the
+// B extents: B is read from bx1..bz1 (where the x and y sweeps left it),
+// CT-updated and divided by its face area (remap.cpp:91-95) into its own
+// arrays.  The bx flux is evaluated on this lane's column, the by flux on
+// (y) and (y-1); z neighbours come from the lanes around, the bx flux at
+// x-1 is carried.  Each flux and update is the expression of ct_flux<2>
+// and f_ct_update<2>.
+template <class T, int K>
+KERNEL OCC(4) void f_ctz_fused(const Src<T> S)
+{
+    const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
+    const int g_ = grid_of(S.boff, S.ngrid, b_);
+    const EDev<T> &D_ = S.devs[g_];
+    const Geo G = D_.g;
+    const ArrE<T> A = state_view(D_.a, S.st);
+    const ZXPos Z = zx_map<2, K>(S, G, g_, b_);
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const int iy = Z.iy, iz = Z.iz;
+    const double dt = lare::gbl(A.s)->dt;
+    auto clx = [&](int x) { return x < -2 ? -2 : (x > nx + 2 ? nx + 2 : x); };
+    const int ym = iy - 1 < -2 ? -2 : iy - 1, yp = iy + 1 > ny + 2 ? ny + 2 : iy + 1;
+    const Fld<T> db1 = A.db1, W = VZ1, B1x = A.bx1, B1y = A.by1, B1z = A.bz1;
+    // the z flux of f = B / db at this lane (ct_flux<2>: fm..fp2 and db,
+    // dbp from the lanes at z-1..z+2, v from w and its offset column)
+    auto zflux = [&](double b, double d, double w, double wo, double cvc, double cv1) -> double {
+        const double f = b / d;
+        const double fp = lane_next(f);
+        return ct_flux_v(lane_prev(f), f, fp, lane_next(fp), d, lane_next(d), (w + wo) * 0.5, dt, lane_prev(cvc), cvc,
+                         lane_next(cvc), cv1, lane_next(cv1));
+    };
+    double w_xm = W[IX(clx(Z.ix0 - 1), iy, iz)], wm_xm = W[IX(clx(Z.ix0 - 1), ym, iz)];
+    double Fa_x = 0.0;
+    for (int ix = Z.ix0; ix <= Z.ix1; ++ix) {
+        const int x = clx(ix);
+        const int c = IX(x, iy, iz);
+        const double d0 = db1[c];
+        const double w0 = W[c], wm = W[IX(x, ym, iz)];
+        const double cvc0 = A.cvc1[c], cv10 = A.cv1[c];
+        const double bx0 = B1x[c], by0 = B1y[c];
+        // bx flux (tw = sx, ow = -sy); by flux (tw = sy, ow = -sx) at (y)
+        const double Fa = zflux(bx0, (d0 + db1[IX(clx(ix + 1), iy, iz)]) * 0.5, w0, wm, cvc0, cv10);
+        const double Fb = zflux(by0, (d0 + db1[IX(x, yp, iz)]) * 0.5, w0, w_xm, cvc0, cv10);
+        // by flux at (y-1)
+        const int cm = IX(x, ym, iz);
+        const double Fbm = zflux(B1y[cm], (db1[cm] + d0) * 0.5, wm, wm_xm, A.cvc1[cm], A.cv1[cm]);
+        const double Faz = lane_prev(Fa), Fbz = lane_prev(Fb);
+        if (Z.wz && ix > Z.ix0) {
+            const bool x1 = ix >= 1 && ix <= nx, x0 = ix >= 0 && ix <= nx;
+            const bool y1 = iy >= 1 && iy <= ny, y0 = iy >= 0 && iy <= ny;
+            const bool z1 = iz >= 1 && iz <= nz, z0 = iz >= 0 && iz <= nz;
+            const bool ebx = iy >= -1 && iz >= -1, eby = ix >= -1 && iz >= -1, ebz = ix >= -1 && iy >= -1;
+            double bx = ebx ? bx0 : 0.0;
+            double by = eby ? by0 : 0.0;
+            double bz = ebz ? (double)B1z[c] : 0.0;
+            if (x0 && y1 && z1) bx = bx - Fa + Faz;
+            const bool bzr = x1 && y1 && z0;
+            if (bzr) bz = bz + Fa - Fa_x;
+            if (x1 && y0 && z1) by = by - Fb + Fbz;
+            if (bzr) bz = bz + Fb - Fbm;
+            if (ebx) BX[c] = bx / (b_area<0>(A, ix, iy, iz) + DBL_EPSILON);
+            if (eby) BY[c] = by / (b_area<1>(A, ix, iy, iz) + DBL_EPSILON);
+            if (ebz) BZ[c] = bz / (b_area<2>(A, ix, iy, iz) + DBL_EPSILON);
+        }
+        Fa_x = Fa;
+        w_xm = w0;
+        wm_xm = wm;
+    }
 }
 
 // The shifted sweep velocity the reference writes back into the registered
@@ -2446,7 +2642,7 @@ void per_grid(void (*k)(const EDev<T> *, int), const char *name, Group &gp)
 // Variable groups of the boundary fills (boundary.cpp:27-96): the variables
 // filled and, for a variable just written into the other state copy, the
 // copy it was read from (-1: updated in place).
-enum { VB, VE, VRHO, VV, VV1, VDM, VALL, VERHO, VBE, VCORR_2, VALL_2, VALL_1, VBE_1, VBE_0, NVG };
+enum { VB, VE, VRHO, VV, VV1, VDM, VALL, VERHO, VBE, VCORR_2, VALL_2, VALL_1, VBE_1, VBE_0, VALL_2B, VALL_1B, NVG };
 struct VarGroup {
     int n;
     int dst[9];
@@ -2473,6 +2669,10 @@ const VarGroup kGroups[NVG] = {
     // into bx1..bz1, and after it wrote B back
     {5, {19, 20, 21, 1, 2}, {6, 7, 8, -1, -1}},
     {5, {6, 7, 8, 1, 2}, {19, 20, 21, -1, -1}},
+    // boundary_conditions after the x / y sweep with B in bx1..bz1 (fused
+    // CT kernels); the state as in VALL_2 / VALL_1
+    {9, {19, 20, 21, 14, 15, 13, 16, 17, 18}, {-1, -1, -1, 1, 2, 0, 3, 4, 5}},
+    {9, {19, 20, 21, 1, 2, 0, 3, 4, 5}, {-1, -1, -1, 14, 15, 13, 16, 17, 18}},
 };
 
 void drop_btabs(Group &gp)
@@ -2599,6 +2799,7 @@ bool walk_ok(const Group &gp, int subset)
 bool use_zx(const Group &gp, int subset) { return gp.zx_k > 0 && walk_ok(gp, subset); }
 bool use_walk(const Group &gp) { return walk_ok(gp, 0); }
 int zxk(const Group &gp) { return gp.zx_k >= 16 ? 16 : 8; }
+bool ct_fused(const Group &gp) { return gp.ct_fuse != 0 && use_zx(gp, 0); }
 
 
 template <class T>
@@ -2680,20 +2881,39 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
     if (gp.prof) gp.prof->dir = DIR;
     EL((f_prep<T, DIR>), 6 * w, gp, 0, rs(-1, 2, -1, 2, -1, 2), xpass, ypass, zpass);
     EL(f_cvc1<T>, 2 * w, gp, 0, rs(-1, 1, -1, 1, -1, 1));
-    if (DIR == 2 && gp.zline && use_walk(gp)) {
-        elaunch_(f_ct_zline<T>, "f_ct_fluxes<2>", 7 * w + 2 * 8, gp, 0, all0, 0, kZLine);
-    } else if (DIR != 2 && gp.march_k > 0 && use_walk(gp)) {
-        constexpr int MD = DIR == 2 ? 0 : DIR;
-        const int K = gp.march_k >= 16 ? 16 : 8;
-        elaunch_(K == 16 ? f_ct_march<T, MD, 16> : f_ct_march<T, MD, 8>, DIR == 0 ? "f_ct_fluxes<0>" : "f_ct_fluxes<1>",
-                 7 * w + 2 * 8, gp, 0, all0, 0, kMarch + 100 * DIR + K);
+    // fused CT (flux and update in one kernel): B moves to bx1..bz1 in the x
+    // sweep, is updated there in place by the y sweep (B view bit 2) and
+    // comes back in the z sweep
+    const bool fuse = ct_fused(gp);
+    const int bst = fuse && DIR == 1 ? 2 : 0;
+    const RSpec ext = rs(-2, 2, -2, 2, -2, 2);
+    if (fuse && DIR != 1) {
+        const int K = zxk(gp);
+        if (DIR == 0)
+            elaunch_(K == 16 ? f_ctx_fused<T, 16> : f_ctx_fused<T, 8>, "f_ct<0>", 10 * w, gp, 0, ext, 0,
+                     kZX + 100 + K);
+        else
+            elaunch_(K == 16 ? f_ctz_fused<T, 16> : f_ctz_fused<T, 8>, "f_ct<2>", 10 * w, gp, 0, ext, 0,
+                     kZX + 200 + K);
     } else {
-        EL((f_ct_fluxes<T, DIR>), 7 * w + 2 * 8, gp, 0, all0);
+        if (DIR == 2 && gp.zline && use_walk(gp)) {
+            elaunch_(f_ct_zline<T>, "f_ct_fluxes<2>", 7 * w + 2 * 8, gp, 0, all0, bst, kZLine);
+        } else if (DIR != 2 && gp.march_k > 0 && use_walk(gp)) {
+            constexpr int MD = DIR == 2 ? 0 : DIR;
+            const int K = gp.march_k >= 16 ? 16 : 8;
+            elaunch_(K == 16 ? f_ct_march<T, MD, 16> : f_ct_march<T, MD, 8>,
+                     DIR == 0 ? "f_ct_fluxes<0>" : "f_ct_fluxes<1>", 7 * w + 2 * 8, gp, 0, all0, bst,
+                     kMarch + 100 * DIR + K);
+        } else {
+            elaunch_(f_ct_fluxes<T, DIR>, DIR == 0 ? "f_ct_fluxes<0>" : DIR == 1 ? "f_ct_fluxes<1>" : "f_ct_fluxes<2>",
+                     7 * w + 2 * 8, gp, 0, all0, bst, gp.order);
+        }
+        const char *un = DIR == 0 ? "f_ct_update<0>" : DIR == 1 ? "f_ct_update<1>" : "f_ct_update<2>";
+        if (DIR == 1)
+            elaunch_(f_ct_update<T, DIR>, un, 6 * w + 2 * 8, gp, 0, all0, bst, gp.order);
+        else  // also the B area product (x) or quotient (z) over the B extents
+            elaunch_(f_ct_update<T, DIR>, un, 6 * w + 2 * 8, gp, 0, ext, bst, gp.order);
     }
-    if (DIR == 1)
-        EL((f_ct_update<T, DIR>), 6 * w + 2 * 8, gp, 0, all0);
-    else  // also the B area product (x) or quotient (z) over the B extents
-        EL((f_ct_update<T, DIR>), 6 * w + 2 * 8, gp, 0, rs(-2, 2, -2, 2, -2, 2));
     // mass flux: its reference range, widened to the ghost faces along DIR
     ELS((f_mass_flux<T, DIR>), 4 * w + 8, gp, 0,
         rs(DIR == 0 ? -1 : 0, DIR == 0 ? 2 : 1, DIR == 1 ? -1 : 0, DIR == 1 ? 2 : 1, DIR == 2 ? -1 : 0,
@@ -2724,7 +2944,10 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
         EL((f_wshift<T, DIR, 1>), w + 8, gp, 0, sh);
     }
     if (gp.prof) gp.prof->dir = -1;
-    ebcs<T>(gp, st ? VALL_1 : VALL_2);
+    if (fuse && DIR != 2)
+        ebcs<T>(gp, st ? VALL_1B : VALL_2B);
+    else
+        ebcs<T>(gp, st ? VALL_1 : VALL_2);
     for (Grid *g : gp.mem) g->st = 1 - st;
 }
 
@@ -2751,6 +2974,7 @@ int group_init(Group &gp, const std::vector<Grid *> &members, hipStream_t stream
     if (const char *o = std::getenv("LARE_ZLINE")) gp.zline = std::atoi(o) != 0;
     if (const char *o = std::getenv("LARE_WALK_MIN_CELLS")) gp.walk_min_cells = std::atof(o);
     if (const char *o = std::getenv("LARE_ZX")) gp.zx_k = std::atoi(o);
+    if (const char *o = std::getenv("LARE_CTFUSE")) gp.ct_fuse = std::atoi(o);
     if (members.empty()) return 0;
     gp.fp32 = members[0]->fp32;
     for (size_t m = 0; m < members.size(); ++m) {

@@ -44,7 +44,8 @@ class LareDesc(ctypes.Structure):
 class LareTune(ctypes.Structure):
     """lare_tune: launch configuration of the fused kernel set."""
     _fields_ = [("walk", ctypes.c_int32), ("march_k", ctypes.c_int32), ("zx_k", ctypes.c_int32),
-                ("zline", ctypes.c_int32), ("order", ctypes.c_int32), ("ms_per_step", ctypes.c_double)]
+                ("zline", ctypes.c_int32), ("order", ctypes.c_int32), ("ct_fuse", ctypes.c_int32),
+                ("ms_per_step", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -11,10 +11,10 @@ from lare_testlib import bit_diff
 pytestmark = pytest.mark.gpu
 
 CONFIGS = [
-    dict(walk=1, march_k=8, zx_k=8, zline=1, order=1),
-    dict(walk=1, march_k=16, zx_k=16, zline=0, order=0),
-    dict(walk=1, march_k=8, zx_k=16, zline=1, order=0),
-    dict(walk=2, march_k=16, zx_k=8, zline=1, order=1),
+    dict(walk=1, march_k=8, zx_k=8, zline=1, order=1, ct_fuse=1),
+    dict(walk=1, march_k=16, zx_k=16, zline=0, order=0, ct_fuse=1),
+    dict(walk=1, march_k=8, zx_k=16, zline=1, order=0, ct_fuse=0),
+    dict(walk=2, march_k=16, zx_k=8, zline=1, order=1, ct_fuse=1),
 ]
 
 
