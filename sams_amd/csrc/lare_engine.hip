@@ -2382,6 +2382,62 @@ __device__ __forceinline__ void bc_edge_e(E *p, int c0, int sd, int n, int s, in
     }
 }
 
+// Both edges of a line at once, with every value the fills read loaded
+// before any is written, so the loads of a lane overlap instead of waiting
+// on the stores before them.  The values are those of bc_edge_e applied to
+// the lower edge and then the upper edge: every source of either fill is an
+// unfilled cell (n >= 3) except one, the upper periodic fill of a staggered
+// line at g = n, which copies index 0 as the lower fill has just set it.
+template <class E>
+__device__ __forceinline__ void bc_line_e(E *p, int c0, int sd, int n, int s, int k0, double v0, int k1, double v1)
+{
+    const int gl = s ? -2 : -1, nl = 1 - gl;  // lower ghosts gl..0
+    const int gh = n + 1 - s, nh = n + 3 - gh;  // upper ghosts gh..n+2
+    const int LB = s ? 0 : 1, UB = n;
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    if (k0 == 3 || k0 == 2) {
+        const double bv = (double)p[c0 + LB * sd];
+        const double boundary = (bv + bv) / 2.0;
+        for (int j = 0; j < nl; ++j) {
+            const int g = gl + j;
+            if (k0 == 2) {
+                lo[j] = bv;
+            } else {
+                const double delta = (double)p[c0 + (LB - g) * sd] - boundary;
+                lo[j] = boundary - delta;
+            }
+        }
+    } else if (k0 == 1) {
+        for (int j = 0; j < nl; ++j) lo[j] = (double)p[c0 + (gl + j + n) * sd];
+    } else if (k0 != 0) {  // clamp
+        for (int j = 0; j < nl; ++j) lo[j] = v0;
+    }
+    if (k1 == 3 || k1 == 2) {
+        const double bv = (double)p[c0 + UB * sd];
+        const double boundary = (bv + bv) / 2.0;
+        for (int j = 0; j < nh; ++j) {
+            const int g = gh + j;
+            if (k1 == 2) {
+                hi[j] = bv;
+            } else {
+                const double delta = (double)p[c0 + (UB - (g - gh)) * sd] - boundary;
+                hi[j] = boundary - delta;
+            }
+        }
+    } else if (k1 == 1) {
+        for (int j = 0; j < nh; ++j) {
+            const int src = gh + j - n;
+            hi[j] = (k0 != 0 && src <= 0) ? (double)(E)lo[src - gl] : (double)p[c0 + src * sd];
+        }
+    } else if (k1 != 0) {  // clamp
+        for (int j = 0; j < nh; ++j) hi[j] = v1;
+    }
+    if (k0)
+        for (int j = 0; j < nl; ++j) p[c0 + (gl + j) * sd] = (E)lo[j];
+    if (k1)
+        for (int j = 0; j < nh; ++j) p[c0 + (gh + j) * sd] = (E)hi[j];
+}
+
 // Workgroups per grid: nvar variables x ceil(lines / 256); each lane owns
 // one line along dim (spanning the other dimensions' full extents) and
 // applies the lower edge, then the upper edge.
@@ -2436,12 +2492,20 @@ KERNEL void e_bc(const EDev<T> *devs, const lare::BCSetE *sets, const int *boff,
             for (int g = n + 1; g <= n + 2; ++g) q[c0 + g * sd] = r[c0 + g * sd];
         }
     }
-    if (V.f64) {
-        bc_edge_e((double *)V.p, c0, sd, n, V.sd, 0, V.k0, V.v0);
-        bc_edge_e((double *)V.p, c0, sd, n, V.sd, 1, V.k1, V.v1);
+    // z lines (one scattered row end per lane) gain from overlapping their
+    // loads; x and y lines are coalesced across lanes already
+    if (V.src || dim != 2) {  // in the fills' own order (after the carry-over)
+        if (V.f64) {
+            bc_edge_e((double *)V.p, c0, sd, n, V.sd, 0, V.k0, V.v0);
+            bc_edge_e((double *)V.p, c0, sd, n, V.sd, 1, V.k1, V.v1);
+        } else {
+            bc_edge_e((T *)V.p, c0, sd, n, V.sd, 0, V.k0, V.v0);
+            bc_edge_e((T *)V.p, c0, sd, n, V.sd, 1, V.k1, V.v1);
+        }
+    } else if (V.f64) {
+        bc_line_e((double *)V.p, c0, sd, n, V.sd, V.k0, V.v0, V.k1, V.v1);
     } else {
-        bc_edge_e((T *)V.p, c0, sd, n, V.sd, 0, V.k0, V.v0);
-        bc_edge_e((T *)V.p, c0, sd, n, V.sd, 1, V.k1, V.v1);
+        bc_line_e((T *)V.p, c0, sd, n, V.sd, V.k0, V.v0, V.k1, V.v1);
     }
 }
 
