@@ -1563,7 +1563,7 @@ KERNEL OCC(4) void g_visc_dt(const Src<T> S)
 // the squared differences of the corner ring at z - 1 are those of the lane
 // below at its z.
 template <class T, int K>
-KERNEL OCC(2) void g_visc_dt_zx(const Src<T> S)
+KERNEL OCC(3) void g_visc_dt_zx(const Src<T> S)
 {
     const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
     const int g_ = grid_of(S.boff, S.ngrid, b_);
@@ -1581,21 +1581,18 @@ KERNEL OCC(2) void g_visc_dt_zx(const Src<T> S)
     const double ax = 0.25 * DXAB(iy, iz);
     int c = IX(Z.ix0, iy, iz);
     // columns at own z: O = (x, y), Om = (x, y-1), M = (x-1, y), Mm = (x-1, y-1)
-    double O[3], Om[3], M[3] = {0, 0, 0}, Mm[3] = {0, 0, 0};
+    double M[3] = {0, 0, 0}, Mm[3] = {0, 0, 0};
     double pM[3] = {0, 0, 0}, pMm[3] = {0, 0, 0};
-    for (int k = 0; k < 3; ++k) {
-        O[k] = V3[k][c];
-        Om[k] = V3[k][c - sy];
-    }
     double a6_m = 0.0, a9_m = 0.0, a12_m = 0.0, vxbp_m = 0.0;
     double al2_m = 0.0, al2p_m = 0.0, al3_m = 0.0, al3y_m = 0.0;
     double al1y_n = A.alpha1[c + sy];  // alpha1 at (x, y+1), loaded ahead
     double m = DBL_MAX;
     for (int ix = Z.ix0; ix <= Z.ix1; ++ix, c += sx) {
-        double N[3], Nm[3], Yp[3], pO[3], pOm[3];
+        double O[3], Om[3], N[3], Yp[3], pO[3], pOm[3];
         for (int k = 0; k < 3; ++k) {
+            O[k] = V3[k][c];
+            Om[k] = V3[k][c - sy];
             N[k] = V3[k][c + sx];
-            Nm[k] = V3[k][c + sx - sy];
             Yp[k] = V3[k][c + sy];
             pO[k] = lane_prev(O[k]);
             pOm[k] = lane_prev(Om[k]);
@@ -1664,8 +1661,6 @@ KERNEL OCC(2) void g_visc_dt_zx(const Src<T> S)
         for (int k = 0; k < 3; ++k) {
             M[k] = O[k];
             Mm[k] = Om[k];
-            O[k] = N[k];
-            Om[k] = Nm[k];
             pM[k] = pO[k];
             pMm[k] = pOm[k];
         }
