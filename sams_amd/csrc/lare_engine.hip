@@ -422,6 +422,26 @@ __device__ __forceinline__ double face_area(const ArrE<T> &A, int ix, int iy, in
     return DZAB(ix, iy);
 }
 
+// Mass flux through a face from the values along DIR: rho at c-1..c+2, the
+// face velocity and o_v, cvc1 at c-1..c+1, cv1 at c, c+1.
+__device__ __forceinline__ double mass_flux_v(double fm, double fi, double fp, double fp2, double v_advect,
+                                              double o_v, double cvc_m, double cvc_0, double cvc_p, double cv1_0,
+                                              double cv1_p)
+{
+    double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
+    double sign_v = (v_advect >= 0.0) ? 1.0 : -1.0;
+    double vad_p = (sign_v + 1.0) * 0.5;
+    double vad_m = 1.0 - vad_p;
+    double fu = fi * vad_p + fp * vad_m;
+    double dfu = dfm * vad_p + dfp * vad_m;
+    double dxci = cvc_0;
+    double dxcu = cvc_m * vad_p + cvc_p * vad_m;
+    double dxbu = cv1_0 * vad_p + cv1_p * vad_m;
+    double phi = fabs(o_v) / dxbu;
+    double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
+    return (fu + Di * (1.0 - phi)) * o_v;
+}
+
 // R5: mass flux dm on the faces of DIR (x_mass_flux, xremap.cpp:373-422 and
 // twins) over its reference range; along DIR the range is widened to the
 // ghost faces, where dm is set to the zero of the reference's pw::assign(dm,
@@ -443,19 +463,8 @@ KERNEL void f_mass_flux(const Src<T> S)
     double area = face_area<DIR>(A, ix, iy, iz);
     double v_advect = face_v<DIR>(A, G, c);
     double o_v = v_advect * dt * area;
-    double fm = RHO[c - sd], fi = RHO[c], fp = RHO[c + sd], fp2 = RHO[c + 2 * sd];
-    double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
-    double sign_v = (v_advect >= 0.0) ? 1.0 : -1.0;
-    double vad_p = (sign_v + 1.0) * 0.5;
-    double vad_m = 1.0 - vad_p;
-    double fu = fi * vad_p + fp * vad_m;
-    double dfu = dfm * vad_p + dfp * vad_m;
-    double dxci = A.cvc1[c];
-    double dxcu = A.cvc1[c - sd] * vad_p + A.cvc1[c + sd] * vad_m;
-    double dxbu = A.cv1[c] * vad_p + A.cv1[c + sd] * vad_m;
-    double phi = fabs(o_v) / dxbu;
-    double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
-    DM[c] = (fu + Di * (1.0 - phi)) * o_v;
+    DM[c] = mass_flux_v(RHO[c - sd], RHO[c], RHO[c + sd], RHO[c + 2 * sd], v_advect, o_v, A.cvc1[c - sd], A.cvc1[c],
+                        A.cvc1[c + sd], A.cv1[c], A.cv1[c + sd]);
     };
     ECELL(cell)
 }
@@ -1167,6 +1176,8 @@ KERNEL OCC(3) void f_ctx_fused(const Src<T> S)
     double cva_m = ca(Z.ix0 - 1, iy), cva_0 = ca(Z.ix0, iy), cvm_m = ca(Z.ix0 - 1, ym), cvm_0 = ca(Z.ix0, ym);
     double c1a_0 = cv(Z.ix0, iy), c1m_0 = cv(Z.ix0, ym);
     double Fa_x = 0.0, Fb_x = 0.0;  // fluxes at x-1
+    // rho (current state copy) along x at x-1, x, x+1 for the mass flux
+    double ro[3] = {RHO[IX(clx(Z.ix0 - 1), iy, iz)], RHO[IX(clx(Z.ix0), iy, iz)], RHO[IX(clx(Z.ix0 + 1), iy, iz)]};
     for (int ix = Z.ix0; ix <= Z.ix1; ++ix) {
         const int c = IX(ix, iy, iz);
         double fa3, da2, sa2, fm3, dm2, fb3, db2, sb2;
@@ -1184,7 +1195,18 @@ KERNEL OCC(3) void f_ctx_fused(const Src<T> S)
         const double Fb = ct_flux_v(fb[0], fb[1], fb[2], fb3, dbb[0], dbb[1], (w0 + wm) * 0.5, dt, cva_m, cva_0,
                                     cva_p, c1a_0, c1a_p);
         const double Fbz = lane_prev(Fb);
+        const double ro3 = RHO[IX(clx(ix + 2), iy, iz)];
         if (Z.wz && ix > Z.ix0) {
+            // R5: the x mass flux over its range (dm = 0 on the ghost faces)
+            if (ix >= -1 && ix <= nx + 2 && iy >= 0 && iy <= ny + 1 && iz >= 0 && iz <= nz + 1) {
+                double dmf = 0.0;
+                if (ix >= 0 && ix <= nx) {
+                    const double v_advect = (w0 + wm + w0z + wmz) * 0.25;  // face_v<0>
+                    const double o_v = v_advect * dt * DXAB(iy, iz);
+                    dmf = mass_flux_v(ro[0], ro[1], ro[2], ro3, v_advect, o_v, cva_m, cva_0, cva_p, c1a_0, c1a_p);
+                }
+                DM[c] = dmf;
+            }
             const bool x1 = ix >= 1 && ix <= nx, x0 = ix >= 0 && ix <= nx;
             const bool y1 = iy >= 1 && iy <= ny, y0 = iy >= 0 && iy <= ny;
             const bool z1 = iz >= 1 && iz <= nz, z0 = iz >= 0 && iz <= nz;
@@ -1203,6 +1225,9 @@ KERNEL OCC(3) void f_ctx_fused(const Src<T> S)
         }
         Fa_x = Fa;
         Fb_x = Fb;
+        ro[0] = ro[1];
+        ro[1] = ro[2];
+        ro[2] = ro3;
         fa[0] = fa[1];
         fa[1] = fa[2];
         fa[2] = fa3;
@@ -1277,7 +1302,20 @@ KERNEL OCC(4) void f_ctz_fused(const Src<T> S)
         const int cm = IX(x, ym, iz);
         const double Fbm = zflux(B1y[cm], (db1[cm] + d0) * 0.5, wm, wm_xm, A.cvc1[cm], A.cv1[cm]);
         const double Faz = lane_prev(Fa), Fbz = lane_prev(Fb);
+        // R5: the z mass flux from rho (current state copy) at z-1..z+2
+        const double r0 = RHO[c], rp = lane_next(r0);
+        const double rm = lane_prev(r0), rp2 = lane_next(rp);
+        const double cvc_m = lane_prev(cvc0), cvc_p = lane_next(cvc0), cv1_p = lane_next(cv10);
         if (Z.wz && ix > Z.ix0) {
+            if (ix >= 0 && ix <= nx + 1 && iy >= 0 && iy <= ny + 1 && iz >= -1 && iz <= nz + 2) {
+                double dmf = 0.0;
+                if (iz >= 0 && iz <= nz) {
+                    const double v_advect = (w_xm + w0 + wm_xm + wm) * 0.25;  // face_v<2>
+                    const double o_v = v_advect * dt * DZAB(ix, iy);
+                    dmf = mass_flux_v(rm, r0, rp, rp2, v_advect, o_v, cvc_m, cvc0, cvc_p, cv10, cv1_p);
+                }
+                DM[c] = dmf;
+            }
             const bool x1 = ix >= 1 && ix <= nx, x0 = ix >= 0 && ix <= nx;
             const bool y1 = iy >= 1 && iy <= ny, y0 = iy >= 0 && iy <= ny;
             const bool z1 = iz >= 1 && iz <= nz, z0 = iz >= 0 && iz <= nz;
@@ -2948,11 +2986,12 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
     const RSpec ext = rs(-2, 2, -2, 2, -2, 2);
     if (fuse && DIR != 1) {
         const int K = zxk(gp);
+        // also the sweep's mass flux (rho from the current state copy: st)
         if (DIR == 0)
-            elaunch_(K == 16 ? f_ctx_fused<T, 16> : f_ctx_fused<T, 8>, "f_ct<0>", 10 * w, gp, 0, ext, 0,
+            elaunch_(K == 16 ? f_ctx_fused<T, 16> : f_ctx_fused<T, 8>, "f_ct<0>", 11 * w + 8, gp, 0, ext, st,
                      kZX + 100 + K);
         else
-            elaunch_(K == 16 ? f_ctz_fused<T, 16> : f_ctz_fused<T, 8>, "f_ct<2>", 10 * w, gp, 0, ext, 0,
+            elaunch_(K == 16 ? f_ctz_fused<T, 16> : f_ctz_fused<T, 8>, "f_ct<2>", 11 * w + 8, gp, 0, ext, st,
                      kZX + 200 + K);
     } else {
         if (DIR == 2 && gp.zline && use_walk(gp)) {
@@ -2974,10 +3013,12 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
             elaunch_(f_ct_update<T, DIR>, un, 6 * w + 2 * 8, gp, 0, ext, bst, gp.order);
     }
     // mass flux: its reference range, widened to the ghost faces along DIR
-    ELS((f_mass_flux<T, DIR>), 4 * w + 8, gp, 0,
-        rs(DIR == 0 ? -1 : 0, DIR == 0 ? 2 : 1, DIR == 1 ? -1 : 0, DIR == 1 ? 2 : 1, DIR == 2 ? -1 : 0,
-           DIR == 2 ? 2 : 1),
-        st);
+    // (the fused CT kernels of the x and z sweeps compute it themselves)
+    if (!(fuse && DIR != 1))
+        ELS((f_mass_flux<T, DIR>), 4 * w + 8, gp, 0,
+            rs(DIR == 0 ? -1 : 0, DIR == 0 ? 2 : 1, DIR == 1 ? -1 : 0, DIR == 1 ? 2 : 1, DIR == 2 ? -1 : 0,
+               DIR == 2 ? 2 : 1),
+            st);
     ebcs<T>(gp, VDM, DIR);
     // reads rho e_e e_i v cv1 cv2 cvc1 VX1..VZ1 (sweep component) dm;
     // writes rho e_e e_i v
