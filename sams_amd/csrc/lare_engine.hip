@@ -1079,6 +1079,98 @@ KERNEL void f_ct_march(const Src<T> S)
     }
 }
 
+// R3 and R5 of the y sweep in one march (the mapping of f_update_march, each
+// lane walking K positions along y) over the union of their ranges (x
+// 0..n+1, y -1..n+2, z 0..n+1): the CT fluxes of f_ct_march<1> on (0..n)^3
+// and the y mass flux of f_mass_flux<1> (zero on the ghost faces), sharing
+// the cvc1 and cv1 windows; rho (the current state copy) is a window along
+// y too.  Positions beyond the box along y are clamped; they feed only
+// fluxes that are not stored.
+template <class T, int K>
+KERNEL void f_cty_mass(const Src<T> S)
+{
+    const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
+    const int g_ = grid_of(S.boff, S.ngrid, b_);
+    const EDev<T> &D_ = S.devs[g_];
+    const Geo G = D_.g;
+    const ArrE<T> A = state_view(D_.a, S.st);
+    const R3 R = {S.r.x0, G.nx + S.r.dx1, S.r.y0, G.ny + S.r.dy1, S.r.z0, G.nz + S.r.dz1};
+    const int nzr = R.z1 - R.z0 + 1;
+    const int plane = (R.x1 - R.x0 + 1) * nzr;
+    const int bpp = (plane + 255) >> 8;
+    const int lb = b_ - S.boff[g_];
+    const int mb = lb / bpp;
+    const int t = (lb - mb * bpp) * 256 + (int)threadIdx.x;
+    if (t >= plane) return;
+    const int ix = R.x0 + t / nzr, iz = R.z0 + t % nzr;
+    const int d0 = R.y0 + mb * K;
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const int sx = G.sx;
+    const double dt = lare::gbl(A.s)->dt;
+    const Fld<T> Ba = BX, Bb = BZ, W = VY1, db1 = A.db1;
+    auto cy = [&](int y) { return y > ny + 2 ? ny + 2 : y; };
+    auto at = [&](int y) { return IX(ix, cy(y), iz); };
+    // f = B / db at y (bx flux: tw = sx; bz flux: tw = 1)
+    auto fval = [&](const Fld<T> Bf, int p, int tw, double &db) -> double {
+        db = (db1[p] + db1[p + tw]) * 0.5;
+        return (double)Bf[p] / db;
+    };
+    double fa[3], fb[3], da[2], db[2], dtmp;
+    fa[0] = fval(Ba, at(d0 - 1), sx, dtmp);
+    fa[1] = fval(Ba, at(d0), sx, da[0]);
+    fa[2] = fval(Ba, at(d0 + 1), sx, da[1]);
+    fb[0] = fval(Bb, at(d0 - 1), 1, dtmp);
+    fb[1] = fval(Bb, at(d0), 1, db[0]);
+    fb[2] = fval(Bb, at(d0 + 1), 1, db[1]);
+    double cvc_m = A.cvc1[at(d0 - 1)], cvc_0 = A.cvc1[at(d0)], cv1_0 = A.cv1[at(d0)];
+    double ro[3] = {RHO[at(d0 - 1)], RHO[at(d0)], RHO[at(d0 + 1)]};
+    const double area_m = DYAB(ix, iz);  // face area of the y mass flux
+    const bool ct_xz = ix <= nx && iz <= nz;
+    const bool mf_xz = ix <= nx + 1 && iz <= nz + 1;
+    const auto FA = lare::gbl(A.fl[0]), FB = lare::gbl(A.fl[1]);
+    for (int k = 0; k < K; ++k) {
+        const int d = d0 + k;
+        if (d > R.y1) break;
+        const int c = at(d);
+        double da2, db2;
+        const double fa3 = fval(Ba, at(d + 2), sx, da2);
+        const double fb3 = fval(Bb, at(d + 2), 1, db2);
+        const double cvc_p = A.cvc1[at(d + 1)], cv1_p = A.cv1[at(d + 1)];
+        const double ro3 = RHO[at(d + 2)];
+        const double w0 = W[c];
+        if (ct_xz && d >= 0 && d <= ny) {
+            const double va = (w0 + W[c - 1]) * 0.5, vb = (w0 + W[c - sx]) * 0.5;
+            FA[c] = ct_flux_v(fa[0], fa[1], fa[2], fa3, da[0], da[1], va, dt, cvc_m, cvc_0, cvc_p, cv1_0, cv1_p);
+            FB[c] = ct_flux_v(fb[0], fb[1], fb[2], fb3, db[0], db[1], vb, dt, cvc_m, cvc_0, cvc_p, cv1_0, cv1_p);
+        }
+        if (mf_xz) {
+            double dmf = 0.0;
+            if (d >= 0 && d <= ny) {
+                const double v_advect = (W[c - sx] + w0 + W[c - sx - 1] + W[c - 1]) * 0.25;  // face_v<1>
+                const double o_v = v_advect * dt * area_m;
+                dmf = mass_flux_v(ro[0], ro[1], ro[2], ro3, v_advect, o_v, cvc_m, cvc_0, cvc_p, cv1_0, cv1_p);
+            }
+            DM[c] = dmf;
+        }
+        fa[0] = fa[1];
+        fa[1] = fa[2];
+        fa[2] = fa3;
+        fb[0] = fb[1];
+        fb[1] = fb[2];
+        fb[2] = fb3;
+        da[0] = da[1];
+        da[1] = da2;
+        db[0] = db[1];
+        db[1] = db2;
+        ro[0] = ro[1];
+        ro[1] = ro[2];
+        ro[2] = ro3;
+        cvc_m = cvc_0;
+        cvc_0 = cvc_p;
+        cv1_0 = cv1_p;
+    }
+}
+
 // R3 for the z sweep with the z-line lane mapping of f_update_zline: each
 // lane loads its own z and takes B / db, db, cvc1 and cv1 at z-1, z+1, z+2
 // from its neighbour lanes; the same arithmetic as f_ct_fluxes<2>.
@@ -2996,12 +3088,15 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
     } else {
         if (DIR == 2 && gp.zline && use_walk(gp)) {
             elaunch_(f_ct_zline<T>, "f_ct_fluxes<2>", 7 * w + 2 * 8, gp, 0, all0, bst, kZLine);
-        } else if (DIR != 2 && gp.march_k > 0 && use_walk(gp)) {
-            constexpr int MD = DIR == 2 ? 0 : DIR;
+        } else if (DIR == 1 && gp.march_k > 0 && use_walk(gp)) {
+            // with the y mass flux, over the union of the two ranges
             const int K = gp.march_k >= 16 ? 16 : 8;
-            elaunch_(K == 16 ? f_ct_march<T, MD, 16> : f_ct_march<T, MD, 8>,
-                     DIR == 0 ? "f_ct_fluxes<0>" : "f_ct_fluxes<1>", 7 * w + 2 * 8, gp, 0, all0, bst,
-                     kMarch + 100 * DIR + K);
+            elaunch_(K == 16 ? f_cty_mass<T, 16> : f_cty_mass<T, 8>, "f_ct_mass<1>", 9 * w + 3 * 8, gp, 0,
+                     rs(0, 1, -1, 2, 0, 1), bst | st, kMarch + 100 * DIR + K);
+        } else if (DIR == 0 && gp.march_k > 0 && use_walk(gp)) {
+            const int K = gp.march_k >= 16 ? 16 : 8;
+            elaunch_(K == 16 ? f_ct_march<T, 0, 16> : f_ct_march<T, 0, 8>, "f_ct_fluxes<0>", 7 * w + 2 * 8, gp, 0,
+                     all0, bst, kMarch + K);
         } else {
             elaunch_(f_ct_fluxes<T, DIR>, DIR == 0 ? "f_ct_fluxes<0>" : DIR == 1 ? "f_ct_fluxes<1>" : "f_ct_fluxes<2>",
                      7 * w + 2 * 8, gp, 0, all0, bst, gp.order);
@@ -3014,7 +3109,8 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
     }
     // mass flux: its reference range, widened to the ghost faces along DIR
     // (the fused CT kernels of the x and z sweeps compute it themselves)
-    if (!(fuse && DIR != 1))
+    const bool mass_done = (fuse && DIR != 1) || (DIR == 1 && gp.march_k > 0 && use_walk(gp));
+    if (!mass_done)
         ELS((f_mass_flux<T, DIR>), 4 * w + 8, gp, 0,
             rs(DIR == 0 ? -1 : 0, DIR == 0 ? 2 : 1, DIR == 1 ? -1 : 0, DIR == 1 ? 2 : 1, DIR == 2 ? -1 : 0,
                DIR == 2 ? 2 : 1),
