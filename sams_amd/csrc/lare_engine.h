@@ -52,6 +52,7 @@ struct Group {
     double walk_min_cells = 3.0e5;  // launches over fewer cells use one cell per lane
     int walk_mode = 0;              // lane-walking kernels: 0 by launch size, 1 always, 2 never
     int ct_fuse = 1;                // x and z sweeps' CT flux and update in one kernel (lane-walking only)
+    bool prep_all = true;           // the three sweeps' cell volumes and widths in one kernel
     // one captured step of the whole group
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;

@@ -76,11 +76,19 @@ __device__ __forceinline__ int xcd_swizzle(int b, int nblk)
 // The arrays with the state copies named for this launch: rho .. vz the
 // copy holding the current state (the second copy when bit 0 of st is set),
 // rho2 .. vz2 the other one; with bit 1 set, bx .. bz name bx1 .. bz1 (B's
-// second copy during the remap) and bx1 .. bz1 name B's own arrays.
+// second copy during the remap) and bx1 .. bz1 name B's own arrays; bits 2
+// and 3 name the y or z sweep's cv1, cv2, db1 (kept in alpha1..3 or
+// fxv..fzv by f_prep_all).
 template <class T>
 __device__ __forceinline__ ArrE<T> state_view(const ArrE<T> &a, int st)
 {
     ArrE<T> v = a;
+    if (st & 12) {  // prep set of the y (4) or z (8) sweep
+        const bool y = (st & 12) == 4;
+        v.cv1 = y ? a.alpha1 : a.fxv;
+        v.cv2 = y ? a.alpha2 : a.fyv;
+        v.db1 = y ? a.alpha3 : a.fzv;
+    }
     if (st & 2) {
         v.bx = a.bx1;
         v.by = a.by1;
@@ -172,17 +180,38 @@ __device__ __forceinline__ int sdir(const Geo &G)
 // R1 at one cell from the eight corners (x-1..x, y-1..y, z-1..z) of v1:
 // V[k][p] is component k at corner p (P_OOO ... P_MMM); out = cv1, cv2, db1.
 enum { P_OOO, P_MOO, P_OMO, P_OOM, P_MMO, P_MOM, P_OMM, P_MMM };
+__device__ __forceinline__ void prep_avg(const double V[3][8], double avg[6]);
+template <int DIR, class T>
+__device__ __forceinline__ void prep_out(const ArrE<T> &A, int ix, int iy, int iz, const double avg[6], double dt,
+                                         double xpass, double ypass, double zpass, double out[3]);
+
 template <int DIR, class T>
 __device__ __forceinline__ void prep_v(const ArrE<T> &A, int ix, int iy, int iz, const double V[3][8], double dt,
                                        double xpass, double ypass, double zpass, double out[3])
 {
+    double avg[6];
+    prep_avg(V, avg);
+    prep_out<DIR>(A, ix, iy, iz, avg, dt, xpass, ypass, zpass, out);
+}
+
+// The six face-averaged v1 of R1 (vxb, vxbm, vyb, vybm, vzb, vzbm).
+__device__ __forceinline__ void prep_avg(const double V[3][8], double avg[6])
+{
     const double *X = V[0], *Y = V[1], *Zv = V[2];
-    double vxb = (X[P_OOO] + X[P_OMO] + X[P_OOM] + X[P_OMM]) * 0.25;
-    double vxbm = (X[P_MOO] + X[P_MMO] + X[P_MOM] + X[P_MMM]) * 0.25;
-    double vyb = (Y[P_OOO] + Y[P_MOO] + Y[P_OOM] + Y[P_MOM]) * 0.25;
-    double vybm = (Y[P_OMO] + Y[P_MMO] + Y[P_OMM] + Y[P_MMM]) * 0.25;
-    double vzb = (Zv[P_OOO] + Zv[P_MOO] + Zv[P_OMO] + Zv[P_MMO]) * 0.25;
-    double vzbm = (Zv[P_OOM] + Zv[P_MOM] + Zv[P_OMM] + Zv[P_MMM]) * 0.25;
+    avg[0] = (X[P_OOO] + X[P_OMO] + X[P_OOM] + X[P_OMM]) * 0.25;
+    avg[1] = (X[P_MOO] + X[P_MMO] + X[P_MOM] + X[P_MMM]) * 0.25;
+    avg[2] = (Y[P_OOO] + Y[P_MOO] + Y[P_OOM] + Y[P_MOM]) * 0.25;
+    avg[3] = (Y[P_OMO] + Y[P_MMO] + Y[P_OMM] + Y[P_MMM]) * 0.25;
+    avg[4] = (Zv[P_OOO] + Zv[P_MOO] + Zv[P_OMO] + Zv[P_MMO]) * 0.25;
+    avg[5] = (Zv[P_OOM] + Zv[P_MOM] + Zv[P_OMM] + Zv[P_MMM]) * 0.25;
+}
+
+// R1's outputs of sweep DIR from the face averages.
+template <int DIR, class T>
+__device__ __forceinline__ void prep_out(const ArrE<T> &A, int ix, int iy, int iz, const double avg[6], double dt,
+                                         double xpass, double ypass, double zpass, double out[3])
+{
+    const double vxb = avg[0], vxbm = avg[1], vyb = avg[2], vybm = avg[3], vzb = avg[4], vzbm = avg[5];
     const double vol = CV(ix, iy, iz);
     const double axp = DXAB(iy, iz), axm = DXAB(iy, iz);
     const double ayp = DYAB(ix, iz), aym = DYAB(ix, iz);
@@ -242,6 +271,46 @@ KERNEL void f_prep(const Src<T> S, double xpass, double ypass, double zpass)
     A.cv1[c] = out[0];
     A.cv2[c] = out[1];
     A.db1[c] = out[2];
+    };
+    ECELL(cell)
+}
+
+// R1 of all three sweeps at once, in the x sweep: each sweep's prep reads
+// the same v1 face averages (the only v1 a sweep writes back is its own
+// component, which later sweeps' preps weight by a pass of 0: xpass in y and
+// z, ypass in z), with the passes of the x, y and z sweeps (1,1,1), (0,1,1),
+// (0,0,1).  The y and z sweeps' cv1, cv2, db1 go into alpha1..3 and
+// fxv..fzv, which nothing reads between the corrector and the next step's
+// viscosity; their kernels see them through the prep-set bits of the view.
+template <class T>
+KERNEL void f_prep_all(const Src<T> S)
+{
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
+    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    const double dt = lare::gbl(A.s)->dt;
+    const int pos[8] = {c, c - sx, c - sy, c - 1, c - sx - sy, c - sx - 1, c - sy - 1, c - sx - sy - 1};
+    double V[3][8];
+    for (int p = 0; p < 8; ++p) {
+        V[0][p] = VX1[pos[p]];
+        V[1][p] = VY1[pos[p]];
+        V[2][p] = VZ1[pos[p]];
+    }
+    double avg[6], o[3];
+    prep_avg(V, avg);
+    prep_out<0>(A, ix, iy, iz, avg, dt, 1.0, 1.0, 1.0, o);
+    A.cv1[c] = o[0];
+    A.cv2[c] = o[1];
+    A.db1[c] = o[2];
+    prep_out<1>(A, ix, iy, iz, avg, dt, 0.0, 1.0, 1.0, o);
+    A.alpha1[c] = o[0];
+    A.alpha2[c] = o[1];
+    A.alpha3[c] = o[2];
+    prep_out<2>(A, ix, iy, iz, avg, dt, 0.0, 0.0, 1.0, o);
+    A.fxv[c] = o[0];
+    A.fyv[c] = o[1];
+    A.fzv[c] = o[2];
     };
     ECELL(cell)
 }
@@ -3068,8 +3137,14 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
     const double w = (double)sizeof(T);
     const RSpec all0 = rs(0, 0, 0, 0, 0, 0);
     if (gp.prof) gp.prof->dir = DIR;
-    EL((f_prep<T, DIR>), 6 * w, gp, 0, rs(-1, 2, -1, 2, -1, 2), xpass, ypass, zpass);
-    EL(f_cvc1<T>, 2 * w, gp, 0, rs(-1, 1, -1, 1, -1, 1));
+    // R1 of the three sweeps at once in the x sweep (f_prep_all); the y and
+    // z sweeps' kernels see their sets through view bits 4 and 8
+    const int pv = gp.prep_all ? DIR << 2 : 0;
+    if (!gp.prep_all)
+        EL((f_prep<T, DIR>), 6 * w, gp, 0, rs(-1, 2, -1, 2, -1, 2), xpass, ypass, zpass);
+    else if (DIR == 0)
+        elaunch_(f_prep_all<T>, "f_prep_all", 12 * w, gp, 0, rs(-1, 2, -1, 2, -1, 2), 0, gp.order);
+    elaunch_(f_cvc1<T>, "f_cvc1", 2 * w, gp, 0, rs(-1, 1, -1, 1, -1, 1), pv, gp.order);
     // fused CT (flux and update in one kernel): B moves to bx1..bz1 in the x
     // sweep, is updated there in place by the y sweep (B view bit 2) and
     // comes back in the z sweep
@@ -3080,32 +3155,32 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
         const int K = zxk(gp);
         // also the sweep's mass flux (rho from the current state copy: st)
         if (DIR == 0)
-            elaunch_(K == 16 ? f_ctx_fused<T, 16> : f_ctx_fused<T, 8>, "f_ct<0>", 11 * w + 8, gp, 0, ext, st,
+            elaunch_(K == 16 ? f_ctx_fused<T, 16> : f_ctx_fused<T, 8>, "f_ct<0>", 11 * w + 8, gp, 0, ext, st | pv,
                      kZX + 100 + K);
         else
-            elaunch_(K == 16 ? f_ctz_fused<T, 16> : f_ctz_fused<T, 8>, "f_ct<2>", 11 * w + 8, gp, 0, ext, st,
+            elaunch_(K == 16 ? f_ctz_fused<T, 16> : f_ctz_fused<T, 8>, "f_ct<2>", 11 * w + 8, gp, 0, ext, st | pv,
                      kZX + 200 + K);
     } else {
         if (DIR == 2 && gp.zline && use_walk(gp)) {
-            elaunch_(f_ct_zline<T>, "f_ct_fluxes<2>", 7 * w + 2 * 8, gp, 0, all0, bst, kZLine);
+            elaunch_(f_ct_zline<T>, "f_ct_fluxes<2>", 7 * w + 2 * 8, gp, 0, all0, bst | pv, kZLine);
         } else if (DIR == 1 && gp.march_k > 0 && use_walk(gp)) {
             // with the y mass flux, over the union of the two ranges
             const int K = gp.march_k >= 16 ? 16 : 8;
             elaunch_(K == 16 ? f_cty_mass<T, 16> : f_cty_mass<T, 8>, "f_ct_mass<1>", 9 * w + 3 * 8, gp, 0,
-                     rs(0, 1, -1, 2, 0, 1), bst | st, kMarch + 100 * DIR + K);
+                     rs(0, 1, -1, 2, 0, 1), bst | st | pv, kMarch + 100 * DIR + K);
         } else if (DIR == 0 && gp.march_k > 0 && use_walk(gp)) {
             const int K = gp.march_k >= 16 ? 16 : 8;
             elaunch_(K == 16 ? f_ct_march<T, 0, 16> : f_ct_march<T, 0, 8>, "f_ct_fluxes<0>", 7 * w + 2 * 8, gp, 0,
-                     all0, bst, kMarch + K);
+                     all0, bst | pv, kMarch + K);
         } else {
             elaunch_(f_ct_fluxes<T, DIR>, DIR == 0 ? "f_ct_fluxes<0>" : DIR == 1 ? "f_ct_fluxes<1>" : "f_ct_fluxes<2>",
-                     7 * w + 2 * 8, gp, 0, all0, bst, gp.order);
+                     7 * w + 2 * 8, gp, 0, all0, bst | pv, gp.order);
         }
         const char *un = DIR == 0 ? "f_ct_update<0>" : DIR == 1 ? "f_ct_update<1>" : "f_ct_update<2>";
         if (DIR == 1)
-            elaunch_(f_ct_update<T, DIR>, un, 6 * w + 2 * 8, gp, 0, all0, bst, gp.order);
+            elaunch_(f_ct_update<T, DIR>, un, 6 * w + 2 * 8, gp, 0, all0, bst | pv, gp.order);
         else  // also the B area product (x) or quotient (z) over the B extents
-            elaunch_(f_ct_update<T, DIR>, un, 6 * w + 2 * 8, gp, 0, ext, bst, gp.order);
+            elaunch_(f_ct_update<T, DIR>, un, 6 * w + 2 * 8, gp, 0, ext, bst | pv, gp.order);
     }
     // mass flux: its reference range, widened to the ghost faces along DIR
     // (the fused CT kernels of the x and z sweeps compute it themselves)
@@ -3114,21 +3189,21 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
         ELS((f_mass_flux<T, DIR>), 4 * w + 8, gp, 0,
             rs(DIR == 0 ? -1 : 0, DIR == 0 ? 2 : 1, DIR == 1 ? -1 : 0, DIR == 1 ? 2 : 1, DIR == 2 ? -1 : 0,
                DIR == 2 ? 2 : 1),
-            st);
+            st | pv);
     ebcs<T>(gp, VDM, DIR);
     // reads rho e_e e_i v cv1 cv2 cvc1 VX1..VZ1 (sweep component) dm;
     // writes rho e_e e_i v
     if (DIR == 2 && gp.zline && use_walk(gp)) {
-        elaunch_(f_update_zline<T>, "f_update_zline", 16 * w + 8, gp, 0, all0, st, kZLine);
+        elaunch_(f_update_zline<T>, "f_update_zline", 16 * w + 8, gp, 0, all0, st | pv, kZLine);
     } else if (DIR == 2 || gp.march_k <= 0 || !use_walk(gp)) {
-        ELS((f_update_all<T, DIR>), 16 * w + 8, gp, 0, all0, st);
+        ELS((f_update_all<T, DIR>), 16 * w + 8, gp, 0, all0, st | pv);
     } else {
         constexpr int MD = DIR == 2 ? 0 : DIR;
         const int K = gp.march_k >= 16 ? 16 : gp.march_k >= 8 ? 8 : 4;
         void (*k)(Src<T>) = K == 16  ? f_update_march<T, MD, 16>
                             : K == 8 ? f_update_march<T, MD, 8>
                                      : f_update_march<T, MD, 4>;
-        elaunch_(k, DIR == 0 ? "f_update_march<0>" : "f_update_march<1>", 16 * w + 8, gp, 0, all0, st,
+        elaunch_(k, DIR == 0 ? "f_update_march<0>" : "f_update_march<1>", 16 * w + 8, gp, 0, all0, st | pv,
                  kMarch + 100 * DIR + K);
     }
     bool wshift = false;
@@ -3136,8 +3211,8 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
     if (wshift) {
         const RSpec sh = rs(DIR == 0 ? -2 : 0, DIR == 0 ? 1 : 0, DIR == 1 ? -2 : 0, DIR == 1 ? 1 : 0,
                             DIR == 2 ? -2 : 0, DIR == 2 ? 1 : 0);
-        EL((f_wshift<T, DIR, 0>), w + 8, gp, 0, sh);
-        EL((f_wshift<T, DIR, 1>), w + 8, gp, 0, sh);
+        ELS((f_wshift<T, DIR, 0>), w + 8, gp, 0, sh, pv);
+        ELS((f_wshift<T, DIR, 1>), w + 8, gp, 0, sh, pv);
     }
     if (gp.prof) gp.prof->dir = -1;
     if (fuse && DIR != 2)
@@ -3171,6 +3246,7 @@ int group_init(Group &gp, const std::vector<Grid *> &members, hipStream_t stream
     if (const char *o = std::getenv("LARE_WALK_MIN_CELLS")) gp.walk_min_cells = std::atof(o);
     if (const char *o = std::getenv("LARE_ZX")) gp.zx_k = std::atoi(o);
     if (const char *o = std::getenv("LARE_CTFUSE")) gp.ct_fuse = std::atoi(o);
+    if (const char *o = std::getenv("LARE_PREPALL")) gp.prep_all = std::atoi(o) != 0;
     if (members.empty()) return 0;
     gp.fp32 = members[0]->fp32;
     for (size_t m = 0; m < members.size(); ++m) {
