@@ -174,6 +174,22 @@ struct BCVarE {
 struct BCSetE {
     BCVarE v[9];
 };
+// One variable's fills along all three dimensions in one pass (every ghost
+// cell computed from the cells it finally derives from): staggering, kinds
+// and clamp values per dimension, and the variable's workgroups (b0: first
+// within its grid; nsh: ghost cells).
+struct BCVar3 {
+    void *p;
+    int f64;
+    int s[3];
+    int k0[3], k1[3];
+    double v0[3], v1[3];
+    int b0, nsh;
+};
+struct BCSet3 {
+    BCVar3 v[9];
+    int nv;
+};
 
 }  // namespace lare
 

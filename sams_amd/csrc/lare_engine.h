@@ -36,11 +36,12 @@ struct Group {
     std::map<std::pair<int, std::array<int, 7>>, Tab> tabs;
     struct BTab {
         BCSetE *sets = nullptr;
+        BCSet3 *sets3 = nullptr;  // all-dimension fill (dim 3 tables)
         int *boff = nullptr;
         int nblk = 0;
         bool any = false;
     };
-    std::map<std::pair<int, int>, BTab> btabs;  // (variable group, dim)
+    std::map<std::pair<int, int>, BTab> btabs;  // (variable group, dim); dim 3: all at once
     std::vector<long> bc_seen;                  // members' bc_version when btabs were built
     hipStream_t stream = nullptr;
     Prof *prof = nullptr;
@@ -53,6 +54,7 @@ struct Group {
     int walk_mode = 0;              // lane-walking kernels: 0 by launch size, 1 always, 2 never
     int ct_fuse = 1;                // x and z sweeps' CT flux and update in one kernel (lane-walking only)
     bool prep_all = true;           // the three sweeps' cell volumes and widths in one kernel
+    bool bc_all = true;             // a variable group's fills of all three dimensions in one kernel
     // one captured step of the whole group
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;

@@ -1990,6 +1990,70 @@ KERNEL void g_predict(const Src<T> S)
     ECELL(cell)
 }
 
+// b_field_and_cv1_update and the energy predictor of g_predict at one cell,
+// from the eight v corners W[k][p] (p: ooo, moo, omo, oom, mmo, mom, omm,
+// mmm), the cell's face-averaged B (bx1..bz1: the resistive copy or the
+// average of its faces) and, when `energy`, its rho, e_e, e_i, the
+// predictor's p_e input pe0 and visc_heat.  out = cv1, bx1..bz1 (times
+// cv1), p_e, p_i.
+template <class T>
+__device__ __forceinline__ void predict_cell(const ArrE<T> &A, double gas_gamma, int ix, int iy, int iz,
+                                             const double W[3][8], double bx1, double by1, double bz1, bool energy,
+                                             double rho, double ee, double ei, double pe0, double vh, double dt,
+                                             double out[6])
+{
+    enum { ooo, moo, omo, oom, mmo, mom, omm, mmm };
+    const double *X = W[0], *Y = W[1], *Zv = W[2];
+    double vxb = 0.25 * (X[ooo] + X[omo] + X[oom] + X[omm]);
+    double vxbm = 0.25 * (X[moo] + X[mmo] + X[mom] + X[mmm]);
+    double vyb = 0.25 * (Y[ooo] + Y[moo] + Y[oom] + Y[mom]);
+    double vybm = 0.25 * (Y[omo] + Y[mmo] + Y[omm] + Y[mmm]);
+    double vzb = 0.25 * (Zv[ooo] + Zv[moo] + Zv[omo] + Zv[mmo]);
+    double vzbm = 0.25 * (Zv[oom] + Zv[mom] + Zv[omm] + Zv[mmm]);
+    const double vol = CV(ix, iy, iz);
+    const double axx = DXAB(iy, iz), ayy = DYAB(ix, iz), azz = DZAB(ix, iy);
+    double dvxdx = (vxb * axx - vxbm * axx) / vol;
+    double dvydy = (vyb * ayy - vybm * ayy) / vol;
+    double dvzdz = (vzb * azz - vzbm * azz) / vol;
+    double dv = (dvxdx + dvydy + dvzdz) * dt / 2.0;
+    const double cv1 = vol * (1.0 + dv);
+    vxb = 0.25 * (X[ooo] + X[moo] + X[oom] + X[mom]);
+    vxbm = 0.25 * (X[omo] + X[mmo] + X[omm] + X[mmm]);
+    vyb = 0.25 * (Y[ooo] + Y[omo] + Y[oom] + Y[omm]);
+    vybm = 0.25 * (Y[moo] + Y[mmo] + Y[mom] + Y[mmm]);
+    double dvxdy = (vxb * ayy - vxbm * ayy) / vol;
+    double dvydx = (vyb * axx - vybm * axx) / vol;
+    vxb = 0.25 * (X[ooo] + X[moo] + X[omo] + X[mmo]);
+    vxbm = 0.25 * (X[oom] + X[mom] + X[omm] + X[mmm]);
+    double vzb2 = 0.25 * (Zv[ooo] + Zv[omo] + Zv[oom] + Zv[omm]);
+    double vzbm2 = 0.25 * (Zv[moo] + Zv[mmo] + Zv[mom] + Zv[mmm]);
+    double dvxdz = (vxb * azz - vxbm * azz) / vol;
+    double dvzdx = (vzb2 * axx - vzbm2 * axx) / vol;
+    double vyb3 = 0.25 * (Y[ooo] + Y[moo] + Y[omo] + Y[mmo]);
+    double vybm3 = 0.25 * (Y[oom] + Y[mom] + Y[omm] + Y[mmm]);
+    double vzb3 = 0.25 * (Zv[ooo] + Zv[moo] + Zv[oom] + Zv[mom]);
+    double vzbm3 = 0.25 * (Zv[omo] + Zv[mmo] + Zv[omm] + Zv[mmm]);
+    double dvydz = (vyb3 * azz - vybm3 * azz) / vol;
+    double dvzdy = (vzb3 * ayy - vzbm3 * ayy) / vol;
+    double w3 = bx1 * dvxdx + by1 * dvxdy + bz1 * dvxdz;
+    double w4 = bx1 * dvydx + by1 * dvydy + bz1 * dvydz;
+    double w5 = bx1 * dvzdx + by1 * dvzdy + bz1 * dvzdz;
+    out[0] = cv1;
+    out[1] = (bx1 + w3 * dt / 2.0) / (1.0 + dv) * cv1;
+    out[2] = (by1 + w4 * dt / 2.0) / (1.0 + dv) * cv1;
+    out[3] = (bz1 + w5 * dt / 2.0) / (1.0 + dv) * cv1;
+    out[4] = out[5] = 0.0;
+    if (energy) {
+        double dvp = cv1 / vol - 1.0;
+        double pi0 = (gas_gamma - 1.0) * rho * ei;
+        double e1_e = ee - pe0 * dvp / rho;
+        double e1_i = ei - pi0 * dvp / rho;
+        e1_i += vh * dt / 2.0 / rho;
+        out[4] = e1_e * (gas_gamma - 1.0) * rho * vol / cv1;
+        out[5] = e1_i * (gas_gamma - 1.0) * rho * vol / cv1;
+    }
+}
+
 // L6 as a z-line x-march (H = 1): the same arithmetic as g_predict.  The
 // eight corners (x-1..x, y-1..y, z-1..z) of the velocities are this lane's
 // columns (x, y) and (x, y-1), the previous position's, and the lane below.
@@ -2041,39 +2105,6 @@ KERNEL OCC(4) void g_predict_zx(const Src<T> S)
                 W[k][omm] = pOm[k];
                 W[k][mmm] = pMm[k];
             }
-            const double *X = W[0], *Y = W[1], *Zv = W[2];
-            double vxb = 0.25 * (X[ooo] + X[omo] + X[oom] + X[omm]);
-            double vxbm = 0.25 * (X[moo] + X[mmo] + X[mom] + X[mmm]);
-            double vyb = 0.25 * (Y[ooo] + Y[moo] + Y[oom] + Y[mom]);
-            double vybm = 0.25 * (Y[omo] + Y[mmo] + Y[omm] + Y[mmm]);
-            double vzb = 0.25 * (Zv[ooo] + Zv[moo] + Zv[omo] + Zv[mmo]);
-            double vzbm = 0.25 * (Zv[oom] + Zv[mom] + Zv[omm] + Zv[mmm]);
-            const double vol = CV(ix, iy, iz);
-            const double axx = DXAB(iy, iz), ayy = DYAB(ix, iz), azz = DZAB(ix, iy);
-            double dvxdx = (vxb * axx - vxbm * axx) / vol;
-            double dvydy = (vyb * ayy - vybm * ayy) / vol;
-            double dvzdz = (vzb * azz - vzbm * azz) / vol;
-            double dv = (dvxdx + dvydy + dvzdz) * dt / 2.0;
-            const double cv1 = vol * (1.0 + dv);
-            A.cv1[c] = cv1;
-            vxb = 0.25 * (X[ooo] + X[moo] + X[oom] + X[mom]);
-            vxbm = 0.25 * (X[omo] + X[mmo] + X[omm] + X[mmm]);
-            vyb = 0.25 * (Y[ooo] + Y[omo] + Y[oom] + Y[omm]);
-            vybm = 0.25 * (Y[moo] + Y[mmo] + Y[mom] + Y[mmm]);
-            double dvxdy = (vxb * ayy - vxbm * ayy) / vol;
-            double dvydx = (vyb * axx - vybm * axx) / vol;
-            vxb = 0.25 * (X[ooo] + X[moo] + X[omo] + X[mmo]);
-            vxbm = 0.25 * (X[oom] + X[mom] + X[omm] + X[mmm]);
-            double vzb2 = 0.25 * (Zv[ooo] + Zv[omo] + Zv[oom] + Zv[omm]);
-            double vzbm2 = 0.25 * (Zv[moo] + Zv[mmo] + Zv[mom] + Zv[mmm]);
-            double dvxdz = (vxb * azz - vxbm * azz) / vol;
-            double dvzdx = (vzb2 * axx - vzbm2 * axx) / vol;
-            double vyb3 = 0.25 * (Y[ooo] + Y[moo] + Y[omo] + Y[mmo]);
-            double vybm3 = 0.25 * (Y[oom] + Y[mom] + Y[omm] + Y[mmm]);
-            double vzb3 = 0.25 * (Zv[ooo] + Zv[moo] + Zv[oom] + Zv[mom]);
-            double vzbm3 = 0.25 * (Zv[omo] + Zv[mmo] + Zv[omm] + Zv[mmm]);
-            double dvydz = (vyb3 * azz - vybm3 * azz) / vol;
-            double dvzdy = (vzb3 * ayy - vzbm3 * ayy) / vol;
             double bx1, by1, bz1;
             if (res) {
                 bx1 = A.bx1[c];
@@ -2084,22 +2115,23 @@ KERNEL OCC(4) void g_predict_zx(const Src<T> S)
                 by1 = 0.5 * (BY[c] + BY[c - sy]);
                 bz1 = 0.5 * (bz_own + bz_m);
             }
-            double w3 = bx1 * dvxdx + by1 * dvxdy + bz1 * dvxdz;
-            double w4 = bx1 * dvydx + by1 * dvydy + bz1 * dvydz;
-            double w5 = bx1 * dvzdx + by1 * dvzdy + bz1 * dvzdz;
-            A.bx1[c] = (bx1 + w3 * dt / 2.0) / (1.0 + dv) * cv1;
-            A.by1[c] = (by1 + w4 * dt / 2.0) / (1.0 + dv) * cv1;
-            A.bz1[c] = (bz1 + w5 * dt / 2.0) / (1.0 + dv) * cv1;
-            if (ix >= 0 && iy >= 0 && iz >= 0 && ix <= G.nx + 1 && iy <= G.ny + 1 && iz <= G.nz + 1) {
-                double rho = RHO[c];
-                double dvp = cv1 / vol - 1.0;
-                double pe0 = res ? (double)A.p_e[c] : PE_OF(c);
-                double pi0 = PI_OF(c);
-                double e1_e = EE[c] - pe0 * dvp / rho;
-                double e1_i = EI[c] - pi0 * dvp / rho;
-                e1_i += A.visc_heat[c] * dt / 2.0 / rho;
-                A.p_e[c] = e1_e * (gas_gamma - 1.0) * rho * vol / cv1;
-                A.p_i[c] = e1_i * (gas_gamma - 1.0) * rho * vol / cv1;
+            const bool energy = ix >= 0 && iy >= 0 && iz >= 0 && ix <= G.nx + 1 && iy <= G.ny + 1 && iz <= G.nz + 1;
+            double o[6];
+            if (energy) {
+                const double rho = RHO[c], ee = EE[c], ei = EI[c];
+                const double pe0 = res ? (double)A.p_e[c] : (gas_gamma - 1.0) * rho * ee;
+                predict_cell(A, gas_gamma, ix, iy, iz, W, bx1, by1, bz1, true, rho, ee, ei, pe0,
+                             (double)A.visc_heat[c], dt, o);
+            } else {
+                predict_cell(A, gas_gamma, ix, iy, iz, W, bx1, by1, bz1, false, 0.0, 0.0, 0.0, 0.0, 0.0, dt, o);
+            }
+            A.cv1[c] = o[0];
+            A.bx1[c] = o[1];
+            A.by1[c] = o[2];
+            A.bz1[c] = o[3];
+            if (energy) {
+                A.p_e[c] = o[4];
+                A.p_i[c] = o[5];
             }
         }
         for (int k = 0; k < 3; ++k) {
@@ -2703,6 +2735,145 @@ KERNEL void e_bc(const EDev<T> *devs, const lare::BCSetE *sets, const int *boff,
     }
 }
 
+// The fills of dims 0, 1, 2 in turn (variableDef.h:601-605) composed: the
+// value at a cell after the fills of dims 0..D-1, computed from the values
+// before any fill.  The fill of dim d rewrites the cells whose index along d
+// lies in a ghost range of an edge with a condition (lower gl..0, upper
+// n+1-s..n+2), each from cells of the same line, over the full extents of
+// the other dims; so a cell's final value is its edge rule applied to
+// values after the fills of the dims before d, at the source indices the
+// rule names, with every stage rounded to the storage type as a store
+// would.  Rules as bc_edge_e: periodic (1), zero-gradient (2), mirror (3),
+// clamp (4).  The one source a fill reads that the same fill has written is
+// index 0 for the upper periodic edge of a staggered line (the lower edge's
+// value, recomputed here).
+template <class E, int D>
+__device__ __forceinline__ double bc3_val(const lare::BCVar3 &V, const E *p, const Geo &G, int i0, int i1, int i2)
+{
+    if constexpr (D == 0) {
+        return (double)p[IX(i0, i1, i2)];
+    } else {
+        constexpr int d = D - 1;
+        const int x = d == 0 ? i0 : d == 1 ? i1 : i2;
+        const int n = d == 0 ? G.nx : d == 1 ? G.ny : G.nz;
+        const int s = V.s[d], k0 = V.k0[d], k1 = V.k1[d];
+        // the rule as a form (0: copy of q1; 1: mirror about q1 of q2; 2:
+        // the constant c) and its source indices
+        int form = 0, q1 = x, q2 = x;
+        double c = 0.0;
+        auto lower = [&](int g) {
+            const int LB = s ? 0 : 1;
+            if (k0 == 1) {
+                q1 = g + n;
+            } else if (k0 == 2) {
+                q1 = LB;
+            } else if (k0 == 3) {
+                form = 1;
+                q1 = LB;
+                q2 = LB - g;
+            } else {
+                form = 2;
+                c = V.v0[d];
+            }
+        };
+        const int gh = n + 1 - s;
+        if (k0 && x <= 0) {
+            lower(x);
+        } else if (k1 && x >= gh) {
+            if (k1 == 1) {
+                const int src = x - n;
+                if (src <= 0 && k0)
+                    lower(src);
+                else
+                    q1 = src;
+            } else if (k1 == 2) {
+                q1 = n;
+            } else if (k1 == 3) {
+                form = 1;
+                q1 = n;
+                q2 = n - (x - gh);
+            } else {
+                form = 2;
+                c = V.v1[d];
+            }
+        }
+        if (form == 2) return (double)(E)c;
+        const double a = bc3_val<E, D - 1>(V, p, G, d == 0 ? q1 : i0, d == 1 ? q1 : i1, d == 2 ? q1 : i2);
+        if (form == 0) return a;
+        const double boundary = (a + a) / 2.0;
+        const double delta = bc3_val<E, D - 1>(V, p, G, d == 0 ? q2 : i0, d == 1 ? q2 : i1, d == 2 ? q2 : i2) - boundary;
+        return (double)(E)(boundary - delta);
+    }
+}
+
+// Workgroups per grid: each variable's ghost cells (the box lo..n+2 of
+// every dim less the cells 1..n-s of all three), 256 per workgroup, one
+// cell per lane: first the x slabs (all y, z), then the y slabs (x
+// inside), then the z slabs (x, y inside), z fastest.  A cell whose every
+// index lies outside the ghost ranges of edges with a condition is left
+// as it is.  Every value read is of a cell no fill rewrites, or one whose
+// rewrite stores the value it holds (the boundary point of a staggered
+// line under zero-gradient or mirror, or under periodic at both edges);
+// the host takes the per-dim fills for a staggered line with a periodic
+// lower and a clamped upper edge, whose lower fill reads the point n that
+// the upper fill then changes.
+template <class T>
+KERNEL void e_bc3(const EDev<T> *devs, const lare::BCSet3 *sets, const int *boff, int ngrid, int nblk)
+{
+    const int b = xcd_swizzle((int)blockIdx.x, nblk);
+    const int g = grid_of(boff, ngrid, b);
+    const Geo G = devs[g].g;
+    const int lb = b - boff[g];
+    const lare::BCSet3 &S = sets[g];
+    int vi = 0;
+    while (vi + 1 < S.nv && S.v[vi + 1].b0 <= lb) ++vi;
+    const lare::BCVar3 &V = S.v[vi];
+    const int t = (lb - V.b0) * 256 + (int)threadIdx.x;
+    if (t >= V.nsh) return;
+    const int n3[3] = {G.nx, G.ny, G.nz};
+    int lo[3], ext[3], wl[3], wd[3], in[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = V.s[d] ? -2 : -1;
+        ext[d] = n3[d] + 3 - lo[d];
+        wl[d] = 1 - lo[d];
+        wd[d] = wl[d] + 2 + V.s[d];
+        in[d] = n3[d] - V.s[d];
+    }
+    auto slab = [&](int d, int a) { return a < wl[d] ? lo[d] + a : n3[d] + 1 - V.s[d] + (a - wl[d]); };
+    int x, y, z;
+    const int cx = wd[0] * ext[1] * ext[2], cy = in[0] * wd[1] * ext[2];
+    if (t < cx) {
+        const int a = t / (ext[1] * ext[2]), r = t - a * (ext[1] * ext[2]);
+        x = slab(0, a);
+        y = lo[1] + r / ext[2];
+        z = lo[2] + r % ext[2];
+    } else if (t < cx + cy) {
+        const int u = t - cx;
+        const int a = u / (wd[1] * ext[2]), r = u - a * (wd[1] * ext[2]);
+        x = 1 + a;
+        y = slab(1, r / ext[2]);
+        z = lo[2] + r % ext[2];
+    } else {
+        const int u = t - cx - cy;
+        const int a = u / (in[1] * wd[2]), r = u - a * (in[1] * wd[2]);
+        x = 1 + a;
+        y = 1 + r / wd[2];
+        z = slab(2, r % wd[2]);
+    }
+    const int pos[3] = {x, y, z};
+    bool filled = false;
+    for (int d = 0; d < 3; ++d)
+        filled = filled || (V.k0[d] && pos[d] <= 0) || (V.k1[d] && pos[d] >= n3[d] + 1 - V.s[d]);
+    if (!filled) return;
+    if (V.f64) {
+        double *p = (double *)V.p;
+        p[IX(x, y, z)] = bc3_val<double, 3>(V, p, G, x, y, z);
+    } else {
+        T *p = (T *)V.p;
+        p[IX(x, y, z)] = (T)bc3_val<T, 3>(V, p, G, x, y, z);
+    }
+}
+
 // ------------------------------------------------- energy diagnostic
 
 // Per-workgroup partial sums over the domain cells (1..n)^3 of one grid:
@@ -2900,7 +3071,7 @@ void per_grid(void (*k)(const EDev<T> *, int), const char *name, Group &gp)
 // Variable groups of the boundary fills (boundary.cpp:27-96): the variables
 // filled and, for a variable just written into the other state copy, the
 // copy it was read from (-1: updated in place).
-enum { VB, VE, VRHO, VV, VV1, VDM, VALL, VERHO, VBE, VCORR_2, VALL_2, VALL_1, VBE_1, VBE_0, VALL_2B, VALL_1B, NVG };
+enum { VB, VE, VRHO, VV, VV1, VDM, VALL, VERHO, VBE, VCORR_2, VALL_2, VALL_1, VBE_1, VBE_0, VALL_2B, VALL_1B, VV_2, NVG };
 struct VarGroup {
     int n;
     int dst[9];
@@ -2931,12 +3102,15 @@ const VarGroup kGroups[NVG] = {
     // CT kernels); the state as in VALL_2 / VALL_1
     {9, {19, 20, 21, 14, 15, 13, 16, 17, 18}, {-1, -1, -1, 1, 2, 0, 3, 4, 5}},
     {9, {19, 20, 21, 1, 2, 0, 3, 4, 5}, {-1, -1, -1, 14, 15, 13, 16, 17, 18}},
+    // the corrector's first velocity_bcs, into the second copy
+    {3, {16, 17, 18}, {3, 4, 5}},
 };
 
 void drop_btabs(Group &gp)
 {
     for (auto &kv : gp.btabs) {
         if (kv.second.sets) (void)hipFree(kv.second.sets);
+        if (kv.second.sets3) (void)hipFree(kv.second.sets3);
         if (kv.second.boff) (void)hipFree(kv.second.boff);
     }
     gp.btabs.clear();
@@ -2999,11 +3173,97 @@ Group::BTab &btable(Group &gp, int vg, int dim, int subset)
     return gp.btabs.emplace(std::make_pair(key, 0), t).first->second;
 }
 
+// All-dimension fill table of variable group vg over a member subset, or
+// one with nblk = -1 when some member needs the per-dim fills: a variable
+// whose fill carries values over from the other state copy (an edge
+// without a condition), or a staggered line with a periodic lower and a
+// clamped upper edge.
+Group::BTab &btable3(Group &gp, int vg, int subset)
+{
+    const int key = vg * 2 + subset;
+    auto it = gp.btabs.find({key, 3});
+    if (it != gp.btabs.end()) return it->second;
+    Group::BTab t;
+    std::vector<const Grid *> mem;
+    for (size_t m = 0; m < gp.mem.size(); ++m)
+        if (subset == 0 || gp.mem[m]->prm.resistive) mem.push_back(gp.mem[m]);
+    std::vector<BCSet3> sets(mem.size());
+    std::vector<int> boff(mem.size());
+    long total = 0;
+    bool ok = true;
+    for (size_t m = 0; m < mem.size(); ++m) {
+        const Grid &g = *mem[m];
+        BCSet3 &S = sets[m];
+        std::memset(&S, 0, sizeof(S));
+        S.nv = kGroups[vg].n;
+        const int n3[3] = {g.dev.g.nx, g.dev.g.ny, g.dev.g.nz};
+        int nb = 0;
+        for (int i = 0; i < kGroups[vg].n; ++i) {
+            const int v = kGroups[vg].dst[i], u = base_var(v), src = kGroups[vg].src[i];
+            BCVar3 &b = S.v[i];
+            b.p = field_ptr(g, v);
+            b.f64 = (u == 12 || !g.fp32) ? 1 : 0;
+            bool none_edge = false;
+            for (int d = 0; d < 3; ++d) {
+                b.s[d] = kStagE[u][d];
+                b.k0[d] = g.bck[u][d][0];
+                b.k1[d] = g.bck[u][d][1];
+                b.v0[d] = g.bcv[u][d][0];
+                b.v1[d] = g.bcv[u][d][1];
+                none_edge = none_edge || b.k0[d] == 0 || b.k1[d] == 0;
+                if (b.s[d] && b.k0[d] == 1 && b.k1[d] == 4) ok = false;
+                if (b.k0[d] || b.k1[d]) t.any = true;
+            }
+            if (src >= 0 && none_edge) ok = false;
+            long nsh = 1, inner = 1;
+            for (int d = 0; d < 3; ++d) {
+                nsh *= n3[d] + 3 - (b.s[d] ? -2 : -1);
+                inner *= n3[d] - b.s[d];
+            }
+            b.nsh = (int)(nsh - inner);
+            b.b0 = nb;
+            nb += (b.nsh + 255) / 256;
+        }
+        boff[m] = (int)total;
+        total += nb;
+    }
+    t.nblk = ok ? (int)total : -1;
+    if (!mem.empty() && ok) {
+        (void)hipMalloc(&t.sets3, sets.size() * sizeof(BCSet3));
+        (void)hipMemcpyAsync(t.sets3, sets.data(), sets.size() * sizeof(BCSet3), hipMemcpyHostToDevice, gp.stream);
+        (void)hipMalloc(&t.boff, boff.size() * sizeof(int));
+        (void)hipMemcpyAsync(t.boff, boff.data(), boff.size() * sizeof(int), hipMemcpyHostToDevice, gp.stream);
+        (void)hipStreamSynchronize(gp.stream);
+    }
+    return gp.btabs.emplace(std::make_pair(key, 3), t).first->second;
+}
+
 // variableRegistry::applyBoundaryConditions for a variable group: dims 0, 1,
 // 2 in turn (variableDef.h:601-605), or one dim.
 template <class T>
 void ebcs(Group &gp, int vg, int only_dim = -1, int subset = 0)
 {
+    if (only_dim < 0 && gp.bc_all) {
+        Group::BTab &t = btable3(gp, vg, subset);
+        if (t.nblk >= 0) {
+            if (gp.planning || !t.any || t.nblk == 0) return;
+            Prof *pf = gp.prof;
+            Prof::Rec rec{"e_bc3", 0.0, nullptr, nullptr};
+            if (pf && pf->on) {
+                (void)hipEventCreate(&rec.a);
+                (void)hipEventCreate(&rec.b);
+                (void)hipEventRecord(rec.a, gp.stream);
+            }
+            hipLaunchKernelGGL(e_bc3<T>, dim3((unsigned)t.nblk), dim3(256), 0, gp.stream,
+                               (const EDev<T> *)gp.devs[subset], (const BCSet3 *)t.sets3, (const int *)t.boff,
+                               gp.n[subset], t.nblk);
+            if (pf && pf->on) {
+                (void)hipEventRecord(rec.b, gp.stream);
+                pf->recs.push_back(rec);
+            }
+            return;
+        }
+    }
     for (int d = 0; d < 3; ++d) {
         if (only_dim >= 0 && d != only_dim) continue;
         Group::BTab &t = btable(gp, vg, d, subset);
@@ -3126,7 +3386,17 @@ void half_t(Group &gp)
                  rs(0, 0, 0, 0, 0, 0), 0, kZX + 100 + zxk(gp));
     else
         EL(g_correct<T>, 22 * w, gp, 0, rs(0, 0, 0, 0, 0, 0));
-    // velocity_bcs, energy_bcs, density_bcs (independent variables)
+    // velocity_bcs, energy_bcs, density_bcs (independent variables).  The
+    // reference fills the velocities twice (lagran.cpp:759 and 803); the
+    // second fill repeats the first except on a staggered line with a
+    // periodic lower and a clamped upper edge, where the lower fill then
+    // copies the clamped boundary point, so only then is the first issued.
+    bool twice = false;
+    for (const Grid *g : gp.mem)
+        for (int v = 3; v <= 5; ++v)
+            for (int d = 0; d < 3; ++d)
+                twice = twice || (kStagE[v][d] && g->bck[v][d][0] == 1 && g->bck[v][d][1] == 4);
+    if (twice) ebcs<T>(gp, VV_2);
     ebcs<T>(gp, VCORR_2);
     for (Grid *g : gp.mem) g->st = 1;
 }
@@ -3247,6 +3517,7 @@ int group_init(Group &gp, const std::vector<Grid *> &members, hipStream_t stream
     if (const char *o = std::getenv("LARE_ZX")) gp.zx_k = std::atoi(o);
     if (const char *o = std::getenv("LARE_CTFUSE")) gp.ct_fuse = std::atoi(o);
     if (const char *o = std::getenv("LARE_PREPALL")) gp.prep_all = std::atoi(o) != 0;
+    if (const char *o = std::getenv("LARE_BCALL")) gp.bc_all = std::atoi(o) != 0;
     if (members.empty()) return 0;
     gp.fp32 = members[0]->fp32;
     for (size_t m = 0; m < members.size(); ++m) {

@@ -1269,9 +1269,10 @@ void step_start_b(Grid &g, int substeps)
 
 // LARE3D::corrector_step (lagran.cpp:739-804).  The division of bx1..bz1 by
 // cv1 at lagran.cpp:746-749 is not performed: nothing reads bx1..bz1 before
-// the next step's prologue overwrites them.  The final velocity_bcs
-// (lagran.cpp:802) repeats the fill of lagran.cpp:759 on unchanged velocities
-// and is idempotent for the built-in conditions, so it is issued once.
+// the next step's prologue overwrites them.  The velocity fills run twice,
+// as in the reference (lagran.cpp:759 and 803): a second fill changes the
+// result where a staggered line has a periodic lower and a clamped upper
+// edge (the lower fill then copies the clamped boundary point).
 void step_half(Grid &g)
 {
     const int nx = g.dev.g.nx, ny = g.dev.g.ny, nz = g.dev.g.nz;
@@ -1281,6 +1282,7 @@ void step_half(Grid &g)
     launch3(k_rho_e_update, 136, g, rng(1, nx, 1, ny, 1, nz));
     energy_bcs(g);
     density_bcs(g);
+    velocity_bcs(g);
 }
 
 // LARE3D::endOfTimestep (shared_data.h:413-419): eulerian_remap
