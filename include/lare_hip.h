@@ -70,6 +70,8 @@ typedef struct lare_desc {
     int32_t resistive;      /* resistiveMHD */
     int32_t precision;      /* enum lare_precision */
     double eta_background, eta0, j_max;
+    int32_t rke;            /* remap kinetic-energy correction (shared_data.h:118) */
+    int32_t reserved;
 } lare_desc;
 
 typedef struct lare_ctx lare_ctx;
@@ -121,7 +123,11 @@ int lare_lagrange(lare_ctx *ctx, double dt);
 
 /* LARE3D::halfTimestep (shared_data.h:405-407 -> lagran.cpp:739-804). */
 int lare_half_timestep(lare_ctx *ctx);
-/* LARE3D::endOfTimestep (shared_data.h:413-419 -> remap.cpp:23-111). */
+/* LARE3D::endOfTimestep (shared_data.h:413-419 -> remap.cpp:23-111), with
+ * energy_correction (diagnostics.cpp:125-135) when desc.rke is set.  The
+ * correction's delta_ke is summed per cell in the order one thread of the
+ * reference adds it (its atomic adds order a multi-threaded run
+ * arbitrarily); kernel set 0 does not run it. */
 int lare_end_of_timestep(lare_ctx *ctx);
 
 /* One runner iteration (runner.h:564-586): the three events in order.
@@ -253,7 +259,7 @@ typedef struct lare_problem {
     double noise;       /* Brio-Wu: relative uniform density noise */
     double eta;         /* Brio-Wu: background resistivity; > 0 enables resistive MHD */
     int32_t precision;  /* enum lare_precision */
-    int32_t reserved;
+    int32_t rke;        /* remap kinetic-energy correction on */
 } lare_problem;
 
 /* Grid and physics scalars of the problem. */

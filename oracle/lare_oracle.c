@@ -82,7 +82,7 @@ struct lo_ctx {
     long nx, ny, nz;
     double gas_gamma, visc1, visc2, visc2_norm, mu0, dt_multiplier;
     double none_zero, largest_number;
-    int resistive;
+    int resistive, rke;
     double eta_background, eta0, j_max;
     double dt, dtr;
     double xpass, ypass, zpass;
@@ -101,6 +101,8 @@ struct lo_ctx {
     a3 cs, cs_v;
     /* remap scratch (remap.cpp:30-35, xremap.cpp:22, yremap.cpp:22, zremap.cpp:23) */
     a3 rho1, cv2, cvc1, db1, rrho_v, rho_v1, rflux_x, rflux_y, rflux_z;
+    /* remap kinetic-energy correction (setup.cpp:249-252), when rke */
+    a3 delta_ke;
 };
 
 #define RHO(c) ((c)->var[LO_RHO])
@@ -227,6 +229,7 @@ lo_ctx *lo_create(const lo_desc *d)
     c->none_zero = DBL_EPSILON;   /* shared_data.h:92 */
     c->largest_number = DBL_MAX;  /* shared_data.h:93 */
     c->resistive = d->resistive;
+    c->rke = d->rke;
     c->eta_background = d->eta_background;
     c->eta0 = d->eta0;
     c->j_max = d->j_max;
@@ -292,6 +295,8 @@ lo_ctx *lo_create(const lo_desc *d)
     c->rflux_x = a3_new(-2, nx + 2, -1, ny + 2, -1, nz + 2, NaN);
     c->rflux_y = a3_new(-1, nx + 2, -2, ny + 2, -1, nz + 2, NaN);
     c->rflux_z = a3_new(-1, nx + 2, -1, ny + 2, -2, nz + 2, NaN);
+    /* zeroed by LARE3D::defaultVariables (control.cpp:48-51) */
+    if (c->rke) c->delta_ke = a3_new(CCC, 0.0);
 #undef CCC
 #undef D0N
     make_metric(c);
@@ -308,6 +313,7 @@ void lo_destroy(lo_ctx *c)
                  &c->flux_y, &c->flux_z, &c->curlb, &c->cs, &c->cs_v, &c->rho1, &c->cv2, &c->cvc1,
                  &c->db1, &c->rrho_v, &c->rho_v1, &c->rflux_x, &c->rflux_y, &c->rflux_z};
     for (size_t i = 0; i < sizeof(all) / sizeof(all[0]); ++i) free(all[i]->p);
+    if (c->rke) free(c->delta_ke.p);
     for (int v = 0; v < LO_NVAR; ++v) free(c->var[v].p);
     a1 *ax[] = {&c->xb, &c->yb, &c->zb, &c->xc, &c->yc, &c->zc, &c->dxb, &c->dyb, &c->dzb,
                 &c->dxc, &c->dyc, &c->dzc};
@@ -338,9 +344,10 @@ double *lo_array(lo_ctx *c, const char *name, long lb[3], long ub[3])
         {"visc_heat", &c->visc_heat}, {"pressure", &c->pressure}, {"p_e", &c->p_e}, {"p_i", &c->p_i},
         {"rho_v", &c->rho_v}, {"cv_v", &c->cv_v}, {"fx", &c->fx}, {"fy", &c->fy}, {"fz", &c->fz},
         {"fx_visc", &c->fx_visc}, {"fy_visc", &c->fy_visc}, {"fz_visc", &c->fz_visc},
-        {"cs", &c->cs}, {"cs_v", &c->cs_v}, {"cv2", &c->cv2}, {"cvc1", &c->cvc1}, {"db1", &c->db1}};
+        {"cs", &c->cs}, {"cs_v", &c->cs_v}, {"cv2", &c->cv2}, {"cvc1", &c->cvc1}, {"db1", &c->db1},
+        {"delta_ke", c->rke ? &c->delta_ke : NULL}};
     for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); ++i)
-        if (strcmp(t[i].n, name) == 0) return a3_out(t[i].a, lb, ub);
+        if (strcmp(t[i].n, name) == 0) return t[i].a ? a3_out(t[i].a, lb, ub) : NULL;
     return NULL;
 }
 
@@ -1291,6 +1298,64 @@ static void energy_flux(lo_ctx *c, int d, a3 e, a3 flux)
     }
 }
 
+/* The remap kinetic-energy correction's accumulation for one velocity
+ * component f, between a momentum-flux kernel and its multiplication by dm
+ * (xremap.cpp:534-558, yremap.cpp:558-584, zremap.cpp:514-537): dk of the
+ * vertex face (ix, iy, iz) is added to four delta_ke cells.  The loops are
+ * sequential, in the order one thread runs the reference's loop (ix, iy, iz
+ * ascending), so each cell's sum has that order; the reference's atomic
+ * adds make the order of a multi-threaded run arbitrary (y and z: two
+ * ix iterations add to one cell), and that of one thread is this one. */
+static void rke_accumulate(lo_ctx *c, int d, a3 f, a3 flux)
+{
+    const long nx = c->nx, ny = c->ny, nz = c->nz;
+    const int di = d == 0, dj = d == 1, dk = d == 2;
+    const a3 rho_v1 = c->rho_v1, cv2 = c->cv2, dm = DM(c), ke = c->delta_ke;
+    const long x1 = d == 0 ? nx - 1 : nx, y1 = d == 1 ? ny - 1 : ny, z1 = d == 2 ? nz - 1 : nz;
+    for (long ix = 0; ix <= x1; ++ix)
+        for (long iy = 0; iy <= y1; ++iy)
+            for (long iz = 0; iz <= z1; ++iz) {
+                const long ixp = ix + 1, iyp = iy + 1, izp = iz + 1;
+                double m = A(rho_v1, ix, iy, iz) * A(cv2, ix, iy, iz);
+                double mp = AP(rho_v1, 1) * AP(cv2, 1);
+                double f0 = A(f, ix, iy, iz), fp = AP(f, 1);
+                double flm = AP(flux, -1), fl0 = A(flux, ix, iy, iz), flp = AP(flux, 1);
+                double dmm = AP(dm, -1), dm0 = A(dm, ix, iy, iz), dmp = AP(dm, 1);
+                double ai = (f0 - flm) * dmm / m - (f0 - fl0) * dm0 / m;
+                double aip = (fp - fl0) * dm0 / mp - (fp - flp) * dmp / mp;
+                double e = (fp - f0) * (fl0 - 0.5 * (fp + f0)) - 0.5 * ai * (f0 - fl0) + 0.5 * aip * (fp - fl0);
+                e = e * dm0 * 0.5;
+                if (d == 0) {
+                    A(ke, ixp, iy, iz) += e;
+                    A(ke, ixp, iyp, iz) += e;
+                    A(ke, ixp, iyp, izp) += e;
+                    A(ke, ixp, iy, izp) += e;
+                } else if (d == 1) {
+                    A(ke, ixp, iyp, iz) += e;
+                    A(ke, ix, iyp, iz) += e;
+                    A(ke, ixp, iyp, izp) += e;
+                    A(ke, ix, iyp, izp) += e;
+                } else {
+                    A(ke, ix, iy, izp) += e;
+                    A(ke, ixp, iy, izp) += e;
+                    A(ke, ix, iyp, izp) += e;
+                    A(ke, ixp, iyp, izp) += e;
+                }
+            }
+}
+
+/* LARE3D::energy_correction (diagnostics.cpp:125-135). */
+static void energy_correction(lo_ctx *c)
+{
+    const long nx = c->nx, ny = c->ny, nz = c->nz;
+    FOR3(ix, 1, nx, iy, 1, ny, iz, 1, nz)
+    {
+        double dke = pmax(-A(c->delta_ke, ix, iy, iz), 0.0) / (A(RHO(c), ix, iy, iz) * A(c->cv, ix, iy, iz));
+        A(EE(c), ix, iy, iz) += 0.5 * dke;
+        A(EI(c), ix, iy, iz) += 0.5 * dke;
+    }
+}
+
 /* x_mom_flux (xremap.cpp:485-566), y_mom_flux (yremap.cpp:508-588),
  * z_mom_flux (zremap.cpp:461-541) for one velocity component f; the flux is
  * multiplied by dm afterwards as in the reference's second kernel.
@@ -1326,6 +1391,7 @@ static void mom_flux(lo_ctx *c, int d, a3 f, a3 flux)
         double dmu = d == 2 ? fabs(A(dm, ix, iy, iz)) / (dxbu * rhou) : fabs(A(dm, ix, iy, iz)) / dxbu / rhou;
         A(flux, ix, iy, iz) = fu + Di * (1.0 - dmu);
     }
+    if (c->rke) rke_accumulate(c, d, f, flux);
     FOR3(ix, x0, nx, iy, y0, ny, iz, z0, nz)
     {
         A(flux, ix, iy, iz) *= A(dm, ix, iy, iz);
@@ -1475,6 +1541,7 @@ static void remap_sweep(lo_ctx *c, int d)
 void lo_end_of_timestep(lo_ctx *c)
 {
     const long nx = c->nx, ny = c->ny, nz = c->nz;
+    if (c->rke) a3_fill(c->delta_ke, 0.0); /* remap.cpp:38-41 */
     c->xpass = 1.0;
     c->ypass = 1.0;
     c->zpass = 1.0;
@@ -1488,6 +1555,7 @@ void lo_end_of_timestep(lo_ctx *c)
     FOR3(ix, -1, nx + 2, iy, -2, ny + 2, iz, -1, nz + 2) { A(BY(c), ix, iy, iz) /= (A(c->dyab, ix, iy, iz) + c->none_zero); }
     FOR3(ix, -1, nx + 2, iy, -1, ny + 2, iz, -2, nz + 2) { A(BZ(c), ix, iy, iz) /= (A(c->dzab, ix, iy, iz) + c->none_zero); }
     bfield_bcs(c);
+    if (c->rke) energy_correction(c); /* shared_data.h:415-417 */
     /* eta_calc (shared_data.h:418) only feeds the resistive path, which
      * recomputes it before use; it runs here for fidelity when resistive. */
     if (c->resistive) eta_calc(c);

@@ -38,6 +38,7 @@ typedef struct lo_desc {
     double gas_gamma, visc1, visc2, mu0, dt_multiplier;
     int resistive;              /* resistiveMHD */
     double eta_background, eta0, j_max;
+    int rke;                    /* remap kinetic-energy correction */
 } lo_desc;
 
 typedef struct lo_ctx lo_ctx;

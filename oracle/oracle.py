@@ -33,7 +33,7 @@ class LoDesc(ctypes.Structure):
                 ("visc2", ctypes.c_double), ("mu0", ctypes.c_double),
                 ("dt_multiplier", ctypes.c_double), ("resistive", ctypes.c_int),
                 ("eta_background", ctypes.c_double), ("eta0", ctypes.c_double),
-                ("j_max", ctypes.c_double)]
+                ("j_max", ctypes.c_double), ("rke", ctypes.c_int)]
 
 
 _lib = None
@@ -87,7 +87,7 @@ class Oracle:
         d = LoDesc(desc.nx, desc.ny, desc.nz, desc.x_min, desc.x_max, desc.y_min, desc.y_max,
                    desc.z_min, desc.z_max, desc.gas_gamma, desc.visc1, desc.visc2, desc.mu0,
                    desc.dt_multiplier, int(desc.resistive), desc.eta_background, desc.eta0,
-                   desc.j_max)
+                   desc.j_max, int(getattr(desc, "rke", 0)))
         self.h = lib().lo_create(ctypes.byref(d))
         self.nx, self.ny, self.nz = desc.nx, desc.ny, desc.nz
 
@@ -177,7 +177,7 @@ def problem_bcs(o, kind, bcx=0):
 
 
 def run_reference(out_dir, problem="ot", nx=16, ny=None, nz=None, steps=10, dumps=(), seed=1,
-                  amp=0.0, noise=0.0, bcx=None, threads=4, fast=False, beta=-1.0, exe=None):
+                  amp=0.0, noise=0.0, bcx=None, threads=4, fast=False, beta=-1.0, exe=None, rke=False):
     """Run the reference driver (oracle/_ref, built from the reference's own
     sources; exe selects another build of it, such as DROPIN_DRIVER) and
     return its meta dict.  Raises if it has not been built."""
@@ -193,6 +193,8 @@ def run_reference(out_dir, problem="ot", nx=16, ny=None, nz=None, steps=10, dump
         args += ["--dump", ",".join(str(s) for s in dumps)]
     if bcx:
         args += ["--bcx", bcx]
+    if rke:
+        args += ["--rke"]
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     subprocess.run(args, cwd=out_dir, env=env, check=True, capture_output=True)
     with open(os.path.join(out_dir, "meta.json")) as f:

@@ -19,7 +19,7 @@
 //   ref_driver --problem ot|bw --nx N [--ny N --nz N] --steps K
 //              [--dump s1,s2,...] [--out DIR] [--seed S] [--amp A] [--beta B]
 //              [--noise F] [--eta E] [--bcx periodic|zerograd|mirror|clamp]
-//              [--time]
+//              [--time] [--rke]
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -69,6 +69,7 @@ namespace refdrv
         bool bcxSet = false;
         bool time = false;
         bool phases = false;
+        bool rke = false;   // remap kinetic-energy correction
         std::vector<double> dts;
         double elapsed = 0.0;
         std::chrono::steady_clock::time_point t0, tLast;
@@ -196,7 +197,7 @@ namespace refdrv
             data.visc2 = 1.0;
             data.mf = 1.2;
             data.mu0 = 1.0;
-            data.rke = false;
+            data.rke = c.rke;
             data.resistiveMHD = false;
             data.eta_background = 0.0;
             data.j_max = 1.0e30;
@@ -421,6 +422,8 @@ int main(int argc, char *argv[])
             c.eta = std::stod(next());
         else if (a == "--time")
             c.time = true;
+        else if (a == "--rke")
+            c.rke = true;
         else if (a == "--phases")
         {
 #if defined(LARE_DROPIN)

@@ -442,6 +442,7 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
     g.prm.eta_background = d->eta_background;
     g.prm.eta0 = d->eta0;
     g.prm.j_max = d->j_max;
+    g.prm.rke = d->rke != 0;
     g.kernel_set = 1;
     std::memset(g.bck, 0, sizeof(g.bck));
     std::memset(g.bcv, 0, sizeof(g.bcv));
@@ -449,7 +450,8 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
     // one pool: the engine's 3D arrays, the six 1D delta arrays, the scalars
     const size_t tsz = g.fp32 ? sizeof(float) : sizeof(double);
     const size_t lmax = (size_t)std::max(d->nx, std::max(d->ny, d->nz)) + 5;
-    g.pool_bytes = box * (kEngineT * tsz + kEngineD * sizeof(double)) + 6 * lmax * sizeof(double) + 256;
+    g.pool_bytes = box * (kEngineT * tsz + (kEngineD + (g.prm.rke ? 1 : 0)) * sizeof(double)) +
+                   6 * lmax * sizeof(double) + 256;
     hipError_t e = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -470,6 +472,11 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
         carve_engine(g.e64.a, p, g.box);
         alias_set0(a, g.e64.a);
     }
+    double *dke = nullptr;  // delta_ke of the kinetic-energy correction
+    if (g.prm.rke) {
+        dke = (double *)p;
+        p += box * sizeof(double);
+    }
     for (int k = 0; k < 6; ++k) {
         d1[k] = (double *)p;
         p += lmax * sizeof(double);
@@ -481,8 +488,11 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
     a.dyc = d1[4];
     a.dzc = d1[5];
     a.s = (lare::Scal *)p;
+    const double eta_sum = g.prm.eta_background + g.prm.eta0;
+    const int eta_fixed = std::memcmp(&eta_sum, &g.prm.eta_background, sizeof(double)) == 0;
     lare::Phys ph{g.prm.gas_gamma, g.prm.visc1, g.prm.visc2_norm, g.prm.mu0, g.prm.dt_multiplier,
-                  g.prm.eta_background, g.prm.eta0, g.prm.j_max, g.prm.resistive};
+                  g.prm.eta_background, g.prm.eta0, g.prm.j_max, g.prm.resistive, eta_fixed,
+                  g.prm.resistive && g.prm.rke};
     auto finish_view = [&](auto &ev) {
         ev.g = g.dev.g;
         ev.p = ph;
@@ -492,6 +502,7 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
         ev.a.dxc = a.dxc;
         ev.a.dyc = a.dyc;
         ev.a.dzc = a.dzc;
+        ev.a.dke = dke;
         ev.a.s = a.s;
     };
     if (g.fp32)
@@ -812,6 +823,8 @@ extern "C" int lare_set_kernel_set(lare_ctx *c, int32_t kernel_set)
     HIPCHK(hipStreamSynchronize(c->g.stream));
     if (c->g.st != 0) return fail(-1, "lare_set_kernel_set: only between steps");
     if (kernel_set == 0) {
+        if (c->g.prm.rke)
+            return fail(-1, "lare_set_kernel_set: the kinetic-energy correction (rke) runs on kernel sets 1-3 only");
         int rc = ensure_set0(c);
         if (rc) return rc;
     }

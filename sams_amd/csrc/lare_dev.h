@@ -80,6 +80,14 @@ struct Phys {
     double gas_gamma, visc1, visc2_norm, mu0, dt_multiplier;
     double eta_background, eta0, j_max;
     int resistive;
+    // eta_background + eta0 equals eta_background bit for bit (eta0 = 0):
+    // eta_calc's choice between them cannot change eta, so |j| is not needed
+    int eta_fixed;
+    // resistive with the kinetic-energy correction: the predictor reads the
+    // prologue's p_i (lagran.cpp:121-125, 606), which differs from one
+    // recomputed from e_i on ghost cells: the correction leaves them stale
+    // and the resistive energy fill (lagran.cpp:502) refreshes them between
+    int keep_pi;
 };
 
 // Device arrays are global memory: accessing them through address-space-1
@@ -134,6 +142,8 @@ struct ArrE {
     // constrained-transport fluxes
     Fld<T> cv2, cvc1, db1;
     double *fl[2];
+    // the remap kinetic-energy correction's delta_ke (FP64; only with rke)
+    double *dke;
     // 1D axis deltas, index i stored at [i + 2]
     double *dxb, *dyb, *dzb, *dxc, *dyc, *dzc;
     Scal *s;

@@ -23,9 +23,11 @@ struct Group {
     bool fp32 = false;
     std::vector<Grid *> mem;  // members, in order
     std::vector<int> res;     // indices of resistive members
-    // device arrays of EDev<T>: [0] all members, [1] resistive members
-    void *devs[2] = {nullptr, nullptr};
-    int n[2] = {0, 0};
+    std::vector<int> rke;     // indices of members with the kinetic-energy correction
+    // device arrays of EDev<T>: [0] all members, [1] resistive members,
+    // [2] members with the kinetic-energy correction
+    void *devs[3] = {nullptr, nullptr, nullptr};
+    int n[3] = {0, 0, 0};
     struct Tab {
         int *boff = nullptr;
         int nblk = 0;

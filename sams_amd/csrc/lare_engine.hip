@@ -587,9 +587,11 @@ __device__ __forceinline__ double energy_flux_face(const ArrE<T> &A, const Geo &
 // Momentum flux of velocity component f on vertex face c (x_mom_flux,
 // xremap.cpp:485-566 and twins), multiplied by dm.  z takes dyac as its area
 // (zremap.cpp:475) and divides by (dzbu*rhou) (zremap.cpp:505).
+// mom_fl_v is the flux before its multiplication by dm (the value the
+// kinetic-energy correction reads, xremap.cpp:534-558).
 template <int DIR>
-__device__ __forceinline__ double mom_flux_v(double fm, double fi, double fp, double fp2, double o_v, double sign_v,
-                                             double dxci, double dxcu, double dxbu, double rhou, double dm)
+__device__ __forceinline__ double mom_fl_v(double fm, double fi, double fp, double fp2, double o_v, double sign_v,
+                                           double dxci, double dxcu, double dxbu, double rhou, double dm)
 {
     double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
     double vad_p = (sign_v + 1.0) * 0.5;
@@ -599,8 +601,13 @@ __device__ __forceinline__ double mom_flux_v(double fm, double fi, double fp, do
     double phi = fabs(o_v) / dxbu;
     double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
     double dmu = DIR == 2 ? fabs(dm) / (dxbu * rhou) : fabs(dm) / dxbu / rhou;
-    double fl = fu + Di * (1.0 - dmu);
-    return fl * dm;
+    return fu + Di * (1.0 - dmu);
+}
+template <int DIR>
+__device__ __forceinline__ double mom_flux_v(double fm, double fi, double fp, double fp2, double o_v, double sign_v,
+                                             double dxci, double dxcu, double dxbu, double rhou, double dm)
+{
+    return mom_fl_v<DIR>(fm, fi, fp, fp2, o_v, sign_v, dxci, dxcu, dxbu, rhou, dm) * dm;
 }
 
 template <int DIR, class T>
@@ -640,7 +647,7 @@ __device__ __forceinline__ double dmv_at(const ArrE<T> &A, const Geo &G, int c)
 // The three momentum fluxes through vertex face f = (fx, fy, fz) along DIR
 // (x_mom_flux, xremap.cpp:485-566 and twins), given the vertex densities
 // at f and f + sd.  z takes dyac as its area (zremap.cpp:475).
-template <int DIR, class T>
+template <int DIR, class T, bool TIMES_DM = true>
 __device__ __forceinline__ void mom_fluxes_at(const ArrE<T> &A, const Geo &G, int f, int fx, int fy, int fz,
                                               double rv0, double rv1, double dm, double dt, double out[3])
 {
@@ -656,9 +663,13 @@ __device__ __forceinline__ void mom_fluxes_at(const ArrE<T> &A, const Geo &G, in
     double dxcu = A.cv1[f] * vad_p + A.cv1[f + 2 * sd] * vad_m;
     double dxbu = A.cvc1[f] * vad_p + A.cvc1[f + sd] * vad_m;
     double rhou = rv0 * vad_p + rv1 * vad_m;
-    out[0] = mom_flux<DIR>(G, f, VX, o_v, sign_v, dxci, dxcu, dxbu, rhou, dm);
-    out[1] = mom_flux<DIR>(G, f, VY, o_v, sign_v, dxci, dxcu, dxbu, rhou, dm);
-    out[2] = mom_flux<DIR>(G, f, VZ, o_v, sign_v, dxci, dxcu, dxbu, rhou, dm);
+    const Fld<T> V3[3] = {VX, VY, VZ};
+    for (int k = 0; k < 3; ++k) {
+        const Fld<T> v = V3[k];
+        const double fl = mom_fl_v<DIR>(v[f - sd], v[f], v[f + sd], v[f + 2 * sd], o_v, sign_v, dxci, dxcu, dxbu,
+                                        rhou, dm);
+        out[k] = TIMES_DM ? fl * dm : fl;
+    }
 }
 
 // R6 over (0..n)^3: the whole transport of the sweep from the current state
@@ -709,6 +720,123 @@ KERNEL void f_update_all(const Src<T> S)
         A.ee2[c] = (EE[c] * cv1 * rho1 + em[0] - ec[0]) / (cv2 * rho);
         A.ei2[c] = (EI[c] * cv1 * rho1 + em[1] - ec[1]) / (cv2 * rho);
     }
+    };
+    ECELL(cell)
+}
+
+// ------------------------------------------ remap kinetic-energy correction
+//
+// With rke the reference adds, after each momentum-flux kernel and before
+// its multiplication by dm, a kinetic-energy term dk of every vertex face to
+// four delta_ke cells with atomic adds (xremap.cpp:534-558,
+// yremap.cpp:558-584, zremap.cpp:514-537), and energy_correction
+// (diagnostics.cpp:125-135) heats the cells after the remap.  Here the sum
+// is a gather in a fixed order: r_ke_dk stores dk of the three velocity
+// components per vertex face, r_ke_add adds each cell's contributions in
+// the order one thread of the reference adds them (component by component,
+// the faces in ascending (ix, iy, iz)).  Atomic adds of a multi-threaded
+// reference run sum in an arbitrary order; a one-thread run sums in this one.
+
+// dk of the vertex face c = (ix, iy, iz) along DIR for the velocity
+// components (lagged in the state copy the transport read), over
+// (0..n-1 along DIR, 0..n across), into flux_x, flux_y, flux_z (free during
+// the remap).  m, mp are the vertex masses after the remap at c, c + sd;
+// the fluxes at c - sd, c, c + sd are the momentum fluxes before their
+// multiplication by dm, each from the same expressions as the transport.
+template <class T, int DIR>
+KERNEL void r_ke_dk(const Src<T> S)
+{
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
+    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    const int sd = sdir<DIR>(G);
+    const double dt = lare::gbl(A.s)->dt;
+    const int ox = DIR == 0, oy = DIR == 1, oz = DIR == 2;
+    double rv[4], dmv[3], cv2v[2];
+    for (int k = 0; k < 4; ++k) rv[k] = rrho_v_at<DIR>(A, G, c + (k - 1) * sd);
+    for (int k = 0; k < 3; ++k) dmv[k] = dmv_at<DIR>(A, G, c + (k - 1) * sd);
+    const Fld<T> C2 = A.cv2;
+    for (int k = 0; k < 2; ++k) {
+        const int q = c + k * sd;
+        double s2 = C2[q] + C2[q + sx] + C2[q + sy] + C2[q + sx + sy] + C2[q + 1] + C2[q + sx + 1] + C2[q + sy + 1] +
+                    C2[q + sx + sy + 1];
+        cv2v[k] = DIR == 0 ? s2 * 0.125 : 0.125 * s2;
+    }
+    const double rho_v1_0 = (rv[1] * A.cvc1[c] + dmv[0] - dmv[1]) / cv2v[0];
+    const double rho_v1_p = (rv[2] * A.cvc1[c + sd] + dmv[1] - dmv[2]) / cv2v[1];
+    const double m = rho_v1_0 * cv2v[0];
+    const double mp = rho_v1_p * cv2v[1];
+    double flm[3], fl0[3], flp[3];
+    mom_fluxes_at<DIR, T, false>(A, G, c - sd, ix - ox, iy - oy, iz - oz, rv[0], rv[1], dmv[0], dt, flm);
+    mom_fluxes_at<DIR, T, false>(A, G, c, ix, iy, iz, rv[1], rv[2], dmv[1], dt, fl0);
+    mom_fluxes_at<DIR, T, false>(A, G, c + sd, ix + ox, iy + oy, iz + oz, rv[2], rv[3], dmv[2], dt, flp);
+    const Fld<T> V3[3] = {VX, VY, VZ};
+    double *const out[3] = {A.flux_x, A.flux_y, A.flux_z};
+    for (int k = 0; k < 3; ++k) {
+        const double f0 = V3[k][c], fp = V3[k][c + sd];
+        double ai = (f0 - flm[k]) * dmv[0] / m - (f0 - fl0[k]) * dmv[1] / m;
+        double aip = (fp - fl0[k]) * dmv[1] / mp - (fp - flp[k]) * dmv[2] / mp;
+        double dk = (fp - f0) * (fl0[k] - 0.5 * (fp + f0)) - 0.5 * ai * (f0 - fl0[k]) + 0.5 * aip * (fp - fl0[k]);
+        dk = dk * dmv[1] * 0.5;
+        lare::gbl(out[k])[c] = dk;
+    }
+    };
+    ECELL(cell)
+}
+
+// delta_ke of cell (ix, iy, iz) += the dk of the faces that reach it: along
+// DIR the face at index - 1, across it the four faces at offsets -1 and 0,
+// in ascending order, each if inside r_ke_dk's range; component by
+// component.
+template <class T, int DIR>
+KERNEL void r_ke_add(const Src<T> S)
+{
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
+    const int c = IX(ix, iy, iz);
+    const int sx = G.sx, sy = G.sy;
+    // the two axes across DIR, in loop order (x before y before z)
+    const int s1 = DIR == 0 ? sy : sx, s2 = DIR == 2 ? sy : 1;
+    const int i1 = DIR == 0 ? iy : ix, i2 = DIR == 2 ? iy : iz;
+    const int n1 = DIR == 0 ? G.ny : G.nx, n2 = DIR == 2 ? G.ny : G.nz;
+    const int f = c - sdir<DIR>(G);
+    auto dke = lare::gbl(A.dke);
+    double acc = dke[c];
+    const double *const dk3[3] = {A.flux_x, A.flux_y, A.flux_z};
+    for (int k = 0; k < 3; ++k) {
+        const auto dk = lare::gbl(dk3[k]);
+        for (int a = -1; a <= 0; ++a)
+            for (int b = -1; b <= 0; ++b) {
+                const int j1 = i1 + a, j2 = i2 + b;
+                if (j1 >= 0 && j1 <= n1 && j2 >= 0 && j2 <= n2) acc = acc + dk[f + a * s1 + b * s2];
+            }
+    }
+    dke[c] = acc;
+    };
+    ECELL(cell)
+}
+
+// delta_ke = 0 before the first sweep (remap.cpp:38-41).
+template <class T>
+KERNEL void r_ke_zero(const Src<T> S)
+{
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) { lare::gbl(A.dke)[IX(ix, iy, iz)] = 0.0; };
+    ECELL(cell)
+}
+
+// energy_correction (diagnostics.cpp:125-135) over (1..n)^3 after the remap:
+// half of max(-delta_ke, 0) / (rho cv) to each energy.
+template <class T>
+KERNEL void r_ke_energy(const Src<T> S)
+{
+    EGRID
+    auto cell = [&](const int ix, const int iy, const int iz) {
+    const int c = IX(ix, iy, iz);
+    const double dke = pmax(-lare::gbl(A.dke)[c], 0.0) / (RHO[c] * CV(ix, iy, iz));
+    EE[c] += 0.5 * dke;
+    EI[c] += 0.5 * dke;
     };
     ECELL(cell)
 }
@@ -1530,7 +1658,8 @@ KERNEL void f_wshift(const Src<T> S)
 // L1 (centred extent): sound speed (lagran.cpp:190-197) from the prologue
 // values of lagran.cpp:113-125; resets the CFL minimum.  With resistive MHD
 // it also stores p_e: the predictor reads the prologue value after the
-// resistive heating changed e_e.
+// resistive heating changed e_e; and p_i when the kinetic-energy correction
+// is on too (the resistive energy fill then changes e_i on ghost cells).
 template <class T>
 KERNEL void g_cs(const Src<T> S)
 {
@@ -1545,6 +1674,7 @@ KERNEL void g_cs(const Src<T> S)
     double pe = PE_OF(c), pi = PI_OF(c);
     double p = pe + pi;
     if (P.resistive) A.p_e[c] = pe;
+    if (P.keep_pi) A.p_i[c] = pi;
     double rmin = pmax(RHO[c], DBL_EPSILON);
     double b2 = bx1 * bx1 + by1 * by1 + bz1 * bz1;
     A.cs[c] = sqrt((gas_gamma * p + b2) / rmin);
@@ -1979,7 +2109,7 @@ KERNEL void g_predict(const Src<T> S)
         double rho = RHO[c];
         double dvp = cv1 / vol - 1.0;
         double pe0 = res ? (double)A.p_e[c] : PE_OF(c);
-        double pi0 = PI_OF(c);
+        double pi0 = P.keep_pi ? (double)A.p_i[c] : PI_OF(c);
         double e1_e = EE[c] - pe0 * dvp / rho;
         double e1_i = EI[c] - pi0 * dvp / rho;
         e1_i += A.visc_heat[c] * dt / 2.0 / rho;
@@ -1999,8 +2129,8 @@ KERNEL void g_predict(const Src<T> S)
 template <class T>
 __device__ __forceinline__ void predict_cell(const ArrE<T> &A, double gas_gamma, int ix, int iy, int iz,
                                              const double W[3][8], double bx1, double by1, double bz1, bool energy,
-                                             double rho, double ee, double ei, double pe0, double vh, double dt,
-                                             double out[6])
+                                             double rho, double ee, double ei, double pe0, double pi0, double vh,
+                                             double dt, double out[6])
 {
     enum { ooo, moo, omo, oom, mmo, mom, omm, mmm };
     const double *X = W[0], *Y = W[1], *Zv = W[2];
@@ -2045,7 +2175,6 @@ __device__ __forceinline__ void predict_cell(const ArrE<T> &A, double gas_gamma,
     out[4] = out[5] = 0.0;
     if (energy) {
         double dvp = cv1 / vol - 1.0;
-        double pi0 = (gas_gamma - 1.0) * rho * ei;
         double e1_e = ee - pe0 * dvp / rho;
         double e1_i = ei - pi0 * dvp / rho;
         e1_i += vh * dt / 2.0 / rho;
@@ -2120,10 +2249,12 @@ KERNEL OCC(4) void g_predict_zx(const Src<T> S)
             if (energy) {
                 const double rho = RHO[c], ee = EE[c], ei = EI[c];
                 const double pe0 = res ? (double)A.p_e[c] : (gas_gamma - 1.0) * rho * ee;
-                predict_cell(A, gas_gamma, ix, iy, iz, W, bx1, by1, bz1, true, rho, ee, ei, pe0,
+                const double pi0 = P.keep_pi ? (double)A.p_i[c] : (gas_gamma - 1.0) * rho * ei;
+                predict_cell(A, gas_gamma, ix, iy, iz, W, bx1, by1, bz1, true, rho, ee, ei, pe0, pi0,
                              (double)A.visc_heat[c], dt, o);
             } else {
-                predict_cell(A, gas_gamma, ix, iy, iz, W, bx1, by1, bz1, false, 0.0, 0.0, 0.0, 0.0, 0.0, dt, o);
+                predict_cell(A, gas_gamma, ix, iy, iz, W, bx1, by1, bz1, false, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, dt,
+                             o);
             }
             A.cv1[c] = o[0];
             A.bx1[c] = o[1];
@@ -2406,21 +2537,24 @@ KERNEL void r_rkstep(const Src<T> S)
     auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
-    double dxc = DXC(ix), dyc = DYC(iy), dzc = DZC(iz);
-    double jx1 = (BZ[c + sy] - BZ[c]) / dyc - (BY[c + 1] - BY[c]) / dzc;
-    double jx2 = (BZ[c + sx + sy] - BZ[c + sx]) / dyc - (BY[c + sx + 1] - BY[c + sx]) / dzc;
-    double jy1 = (BX[c + 1] - BX[c]) / dzc - (BZ[c + sx] - BZ[c]) / dxc;
-    double jy2 = (BX[c + sy + 1] - BX[c + sy]) / dzc - (BZ[c + sx + sy] - BZ[c + sy]) / dxc;
-    double jz1 = (BY[c + sx] - BY[c]) / dxc - (BX[c + sy] - BX[c]) / dyc;
-    double jz2 = (BY[c + sx + 1] - BY[c + 1]) / dxc - (BX[c + sy + 1] - BX[c + 1]) / dyc;
+    const QDiv qx = qdiv_of(DXC(ix)), qy = qdiv_of(DYC(iy)), qz = qdiv_of(DZC(iz));
+    double jx1 = qdiv(BZ[c + sy] - BZ[c], qy) - qdiv(BY[c + 1] - BY[c], qz);
+    double jx2 = qdiv(BZ[c + sx + sy] - BZ[c + sx], qy) - qdiv(BY[c + sx + 1] - BY[c + sx], qz);
+    double jy1 = qdiv(BX[c + 1] - BX[c], qz) - qdiv(BZ[c + sx] - BZ[c], qx);
+    double jy2 = qdiv(BX[c + sy + 1] - BX[c + sy], qz) - qdiv(BZ[c + sx + sy] - BZ[c + sy], qx);
+    double jz1 = qdiv(BY[c + sx] - BY[c], qx) - qdiv(BX[c + sy] - BX[c], qy);
+    double jz2 = qdiv(BY[c + sx + 1] - BY[c + 1], qx) - qdiv(BX[c + sy + 1] - BX[c + 1], qy);
     double jx = 0.5 * (jx1 + jx2);
     double jy = 0.5 * (jy1 + jy2);
     double jz = 0.5 * (jz1 + jz2);
-    double j_local = sqrt(jx * jx + jy * jy + jz * jz);
-    double eta = j_local > P.j_max ? P.eta_background + P.eta0 : P.eta_background;
-    lare::gbl(A.flux_x)[c] = -jx * eta * dxc * 0.5;
-    lare::gbl(A.flux_y)[c] = -jy * eta * dyc * 0.5;
-    lare::gbl(A.flux_z)[c] = -jz * eta * dzc * 0.5;
+    double eta = P.eta_background;
+    if (!P.eta_fixed) {
+        double j_local = sqrt(jx * jx + jy * jy + jz * jz);
+        eta = j_local > P.j_max ? P.eta_background + P.eta0 : P.eta_background;
+    }
+    lare::gbl(A.flux_x)[c] = -jx * eta * qx.d * 0.5;
+    lare::gbl(A.flux_y)[c] = -jy * eta * qy.d * 0.5;
+    lare::gbl(A.flux_z)[c] = -jz * eta * qz.d * 0.5;
     A.curlb[c] = eta * (jx * jx + jy * jy + jz * jz);
     };
     ECELL(cell)
@@ -2466,27 +2600,49 @@ KERNEL void r_b_energy(const Src<T> S)
 // one vertex from the B values around it: bzAB / byAB / bxAB is the
 // component at (x + A, y + B), the z suffix its value at z + 1.  The curlb
 // returned is rounded to the storage type, as stored by r_rkstep.
-template <class T>
-__device__ __forceinline__ void rk_vertex(double bz00, double bz01, double bz10, double bz11, double by00,
-                                          double by00z, double by10, double by10z, double bx00, double bx00z,
-                                          double bx01, double bx01z, double dxc, double dyc, double dzc,
-                                          const Phys &P, double f[4])
+template <class T, bool FAST>
+__device__ __forceinline__ void rk_vertex_q(double bz00, double bz01, double bz10, double bz11, double by00,
+                                            double by00z, double by10, double by10z, double bx00, double bx00z,
+                                            double bx01, double bx01z, const QDiv &qx, const QDiv &qy,
+                                            const QDiv &qz, const Phys &P, double f[4], bool &slow)
 {
-    double jx1 = (bz01 - bz00) / dyc - (by00z - by00) / dzc;
-    double jx2 = (bz11 - bz10) / dyc - (by10z - by10) / dzc;
-    double jy1 = (bx00z - bx00) / dzc - (bz10 - bz00) / dxc;
-    double jy2 = (bx01z - bx01) / dzc - (bz11 - bz01) / dxc;
-    double jz1 = (by10 - by00) / dxc - (bx01 - bx00) / dyc;
-    double jz2 = (by10z - by00z) / dxc - (bx01z - bx00z) / dyc;
+    auto D = [&](double a, const QDiv &R) -> double {
+        if constexpr (FAST)
+            return qdiv_f(a, R, slow);
+        else
+            return a / R.d;
+    };
+    double jx1 = D(bz01 - bz00, qy) - D(by00z - by00, qz);
+    double jx2 = D(bz11 - bz10, qy) - D(by10z - by10, qz);
+    double jy1 = D(bx00z - bx00, qz) - D(bz10 - bz00, qx);
+    double jy2 = D(bx01z - bx01, qz) - D(bz11 - bz01, qx);
+    double jz1 = D(by10 - by00, qx) - D(bx01 - bx00, qy);
+    double jz2 = D(by10z - by00z, qx) - D(bx01z - bx00z, qy);
     double jx = 0.5 * (jx1 + jx2);
     double jy = 0.5 * (jy1 + jy2);
     double jz = 0.5 * (jz1 + jz2);
-    double j_local = sqrt(jx * jx + jy * jy + jz * jz);
-    double eta = j_local > P.j_max ? P.eta_background + P.eta0 : P.eta_background;
-    f[0] = -jx * eta * dxc * 0.5;
-    f[1] = -jy * eta * dyc * 0.5;
-    f[2] = -jz * eta * dzc * 0.5;
+    double eta = P.eta_background;
+    if (!P.eta_fixed) {
+        double j_local = sqrt(jx * jx + jy * jy + jz * jz);
+        eta = j_local > P.j_max ? P.eta_background + P.eta0 : P.eta_background;
+    }
+    f[0] = -jx * eta * qx.d * 0.5;
+    f[1] = -jy * eta * qy.d * 0.5;
+    f[2] = -jz * eta * qz.d * 0.5;
     f[3] = (double)(T)(eta * (jx * jx + jy * jy + jz * jz));
+}
+template <class T>
+__device__ __forceinline__ void rk_vertex(double bz00, double bz01, double bz10, double bz11, double by00,
+                                          double by00z, double by10, double by10z, double bx00, double bx00z,
+                                          double bx01, double bx01z, const QDiv &qx, const QDiv &qy,
+                                          const QDiv &qz, const Phys &P, double f[4])
+{
+    bool slow = false;
+    rk_vertex_q<T, true>(bz00, bz01, bz10, bz11, by00, by00z, by10, by10z, bx00, bx00z, bx01, bx01z, qx, qy, qz, P,
+                         f, slow);
+    if (__builtin_expect(slow, 0))
+        rk_vertex_q<T, false>(bz00, bz01, bz10, bz11, by00, by00z, by10, by10z, bx00, bx00z, bx01, bx01z, qx, qy,
+                              qz, P, f, slow);
 }
 
 // One resistive substep (r_rkstep then r_b_energy, lagran.cpp:490-590) as a
@@ -2498,7 +2654,7 @@ __device__ __forceinline__ void rk_vertex(double bz00, double bz01, double bz10,
 // arithmetic as r_rkstep and r_b_energy; B outside each component's update
 // range is left to the boundary fill that follows.
 template <class T, int K>
-KERNEL OCC(4) void r_substep_zx(const Src<T> S, int to_bx1)
+KERNEL OCC(3) void r_substep_zx(const Src<T> S, int to_bx1)
 {
     const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
     const int g_ = grid_of(S.boff, S.ngrid, b_);
@@ -2513,7 +2669,7 @@ KERNEL OCC(4) void r_substep_zx(const Src<T> S, int to_bx1)
     const double dt = lare::gbl(A.s)->dt_sub;
     const Fld<T> IX_ = to_bx1 ? BX : A.bx1, IY_ = to_bx1 ? BY : A.by1, IZ_ = to_bx1 ? BZ : A.bz1;
     const Fld<T> OX_ = to_bx1 ? A.bx1 : BX, OY_ = to_bx1 ? A.by1 : BY, OZ_ = to_bx1 ? A.bz1 : BZ;
-    const double dyc0 = DYC(iy), dycm = DYC(iy - 1), dzc = DZC(iz);
+    const QDiv qy0 = qdiv_of(DYC(iy)), qym = qdiv_of(DYC(iy - 1)), qz = qdiv_of(DZC(iz));
     int c = IX(Z.ix0, iy, iz);
     // plane x: BY at (x, y-1), (x, y) and their z+1; BZ at (x, y-1..y+1)
     double by_m = IY_[c - sy], by_0 = IY_[c];
@@ -2528,11 +2684,11 @@ KERNEL OCC(4) void r_substep_zx(const Src<T> S, int to_bx1)
         const double byN_m = IY_[c + sx - sy], byN_0 = IY_[c + sx];
         const double byN_mz = lane_next(byN_m), byN_0z = lane_next(byN_0);
         const double bzN_m = IZ_[c + sx - sy], bzN_0 = IZ_[c + sx], bzN_p = IZ_[c + sx + sy];
-        const double dxc = DXC(ix);
+        const QDiv qx = qdiv_of(DXC(ix));
         double F0[4], Fm[4];
-        rk_vertex<T>(bz_0, bz_p, bzN_0, bzN_p, by_0, by_0z, byN_0, byN_0z, bx_0, bx_0z, bx_p, bx_pz, dxc, dyc0, dzc, P,
+        rk_vertex<T>(bz_0, bz_p, bzN_0, bzN_p, by_0, by_0z, byN_0, byN_0z, bx_0, bx_0z, bx_p, bx_pz, qx, qy0, qz, P,
                      F0);
-        rk_vertex<T>(bz_m, bz_0, bzN_m, bzN_0, by_m, by_mz, byN_m, byN_mz, bx_m, bx_mz, bx_0, bx_0z, dxc, dycm, dzc, P,
+        rk_vertex<T>(bz_m, bz_0, bzN_m, bzN_0, by_m, by_mz, byN_m, byN_mz, bx_m, bx_mz, bx_0, bx_0z, qx, qym, qz, P,
                      Fm);
         double pF0[4], pFm[4];
         for (int q = 0; q < 4; ++q) {
@@ -2973,6 +3129,7 @@ Group::Tab &table(Group &gp, int subset, const RSpec &r, int order)
     long total = 0;
     for (size_t m = 0; m < gp.mem.size(); ++m) {
         if (subset == 1 && std::find(gp.res.begin(), gp.res.end(), (int)m) == gp.res.end()) continue;
+        if (subset == 2 && std::find(gp.rke.begin(), gp.rke.end(), (int)m) == gp.rke.end()) continue;
         const Geo &G = gp.mem[m]->dev.g;
         const long nxr = (long)G.nx + r.dx1 - r.x0 + 1, nyr = (long)G.ny + r.dy1 - r.y0 + 1,
                    nzr = (long)G.nz + r.dz1 - r.z0 + 1;
@@ -3476,6 +3633,14 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
         elaunch_(k, DIR == 0 ? "f_update_march<0>" : "f_update_march<1>", 16 * w + 8, gp, 0, all0, st | pv,
                  kMarch + 100 * DIR + K);
     }
+    if (gp.n[2]) {
+        // the kinetic-energy correction's sums, before v1 is shifted
+        const RSpec faces = rs(0, DIR == 0 ? -1 : 0, 0, DIR == 1 ? -1 : 0, 0, DIR == 2 ? -1 : 0);
+        const RSpec cells = rs(DIR == 0 ? 1 : 0, DIR == 0 ? 0 : 1, DIR == 1 ? 1 : 0, DIR == 1 ? 0 : 1,
+                               DIR == 2 ? 1 : 0, DIR == 2 ? 0 : 1);
+        ELS((r_ke_dk<T, DIR>), 14 * w + 3 * 8, gp, 2, faces, st | pv);
+        ELS((r_ke_add<T, DIR>), 4 * 8, gp, 2, cells, st | pv);
+    }
     bool wshift = false;
     for (const Grid *g : gp.mem) wshift = wshift || g->bck[9 + DIR][DIR][0] == 0 || g->bck[9 + DIR][DIR][1] == 0;
     if (wshift) {
@@ -3495,10 +3660,15 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
 template <class T>
 void end_t(Group &gp)
 {
+    const double w = (double)sizeof(T);
+    if (gp.n[2]) EL(r_ke_zero<T>, 8, gp, 2, rs(-1, 2, -1, 2, -1, 2));
     sweep_t<T, 0>(gp, 1.0, 1.0, 1.0, 1);
     sweep_t<T, 1>(gp, 0.0, 1.0, 1.0, 0);
     sweep_t<T, 2>(gp, 0.0, 0.0, 1.0, 1);
     ebcs<T>(gp, VB);
+    // energy_correction after the remap's last fill (shared_data.h:415-417);
+    // no fill follows it in the reference either
+    if (gp.n[2]) EL(r_ke_energy<T>, 4 * w + 8, gp, 2, rs(1, 0, 1, 0, 1, 0));
     per_grid<T>(g_step_done<T>, "g_step_done", gp);
 }
 
@@ -3526,13 +3696,15 @@ int group_init(Group &gp, const std::vector<Grid *> &members, hipStream_t stream
             return -1;
         }
         if (members[m]->prm.resistive) gp.res.push_back((int)m);
+        if (members[m]->prm.rke) gp.rke.push_back((int)m);
     }
     const size_t es = gp.fp32 ? sizeof(EDev<float>) : sizeof(EDev<double>);
-    for (int sub = 0; sub < 2; ++sub) {
+    for (int sub = 0; sub < 3; ++sub) {
         std::vector<char> host;
         int n = 0;
         for (size_t m = 0; m < members.size(); ++m) {
             if (sub == 1 && !members[m]->prm.resistive) continue;
+            if (sub == 2 && !members[m]->prm.rke) continue;
             const char *src = gp.fp32 ? (const char *)&members[m]->e32 : (const char *)&members[m]->e64;
             host.insert(host.end(), src, src + es);
             ++n;
@@ -3567,13 +3739,14 @@ void group_free(Group &gp)
     gp.tabs.clear();
     drop_btabs(gp);
     gp.bc_seen.clear();
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < 3; ++s) {
         if (gp.devs[s]) (void)hipFree(gp.devs[s]);
         gp.devs[s] = nullptr;
         gp.n[s] = 0;
     }
     gp.mem.clear();
     gp.res.clear();
+    gp.rke.clear();
 }
 
 void group_start_a(Group &gp)

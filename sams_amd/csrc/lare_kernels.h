@@ -27,6 +27,7 @@ struct Params {
     double gas_gamma, visc1, visc2, visc2_norm, mu0, dt_multiplier;
     int resistive;
     double eta_background, eta0, j_max;
+    int rke;  // remap kinetic-energy correction
 };
 
 // Optional per-kernel timing with HIP events on the grid's stream.

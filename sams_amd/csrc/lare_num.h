@@ -30,6 +30,50 @@ __device__ __forceinline__ double pmin_list4(double a, double b, double c, doubl
 
 static constexpr double kSixth = 1.0 / 6.0;  // constants.h:46
 
+// Quotients by a shared divisor, equal bit for bit to IEEE a / d.
+// hipcc lowers a / d (FP64, gfx950) to: v_div_scale of d and of a,
+// y = v_rcp_f64(d') refined by two Newton steps (e = fma(-d', y, 1),
+// y = fma(y, e, y)), m = a' * y, r = fma(-d', m, a'), q = v_div_fmas(r, y,
+// m), then v_div_fixup.  The two scale steps return their operand unchanged
+// and v_div_fmas is a plain fma unless an operand or the quotient is near the
+// exponent limits, and v_div_fixup then only sets the sign.  QDiv keeps d and
+// its refined reciprocal y (computed once, as above), so a quotient costs
+// m = a * y and q = fma(-fma(d, m, -a), y, m): the same operations on the
+// same values (fma(d, m, -a) is exactly -r), for |a| and d in
+// [2^-300, 2^301).  For d > 0 a zero a gives the signed zero a / d gives
+// (-fma(d, m, -a) is -0 there, so the sum keeps m's sign).  Any other a, and
+// every a when d is negative or out of that range, marks the quotient slow:
+// the caller recomputes it with the full division.
+struct QDiv {
+    double d, y;
+    bool ok;
+};
+__device__ __forceinline__ unsigned qdiv_expo(double v) { return ((unsigned)__double2hiint(v) >> 20) & 0x7ffu; }
+__device__ __forceinline__ QDiv qdiv_of(double d)
+{
+    double y = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-d, y, 1.0);
+    y = fma(y, e, y);
+    return QDiv{d, y, d > 0.0 && qdiv_expo(d) - 723u <= 600u};
+}
+// The quotient, with `slow` set where it may differ from a / d.
+__device__ __forceinline__ double qdiv_f(double a, const QDiv &R, bool &slow)
+{
+    const double m = a * R.y;
+    const double q = fma(-fma(R.d, m, -a), R.y, m);
+    slow = slow || !R.ok || !(qdiv_expo(a) - 723u <= 600u || a == 0.0);
+    return q;
+}
+__device__ __forceinline__ double qdiv(double a, const QDiv &R)
+{
+    bool slow = false;
+    double q = qdiv_f(a, R, slow);
+    if (__builtin_expect(slow, 0)) q = a / R.d;
+    return q;
+}
+
 // The limited-slope core shared by the fifteen remap flux kernels
 // (xremap.cpp:304-316 and twins).
 __device__ __forceinline__ double vl_di(double sign_v, double dfi, double dfu, double phi, double dxci, double dxcu, double dxbu)

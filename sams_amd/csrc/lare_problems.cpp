@@ -64,6 +64,7 @@ extern "C" int lare_problem_desc(const lare_problem *p, lare_desc *d)
     d->mu0 = 1.0;
     d->dt_multiplier = 0.8;
     d->precision = p->precision;
+    d->rke = p->rke != 0;
     d->j_max = 1.0e30;
     if (p->kind == LARE_PROBLEM_ORSZAG_TANG) {
         d->x_min = 0.0;

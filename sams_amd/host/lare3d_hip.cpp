@@ -14,8 +14,10 @@
 //       controlFunctions::calculateTimestep (runner.h:158-169, which calls
 //       back set_dt and getTimestep), then lare_lagrange with the runner's dt
 //   halfTimestep    -> corrector_step: lare_half_timestep
-//   endOfTimestep   -> eulerian_remap: lare_end_of_timestep; eta_calc is
-//       recomputed on the device inside the resistive substeps
+//   endOfTimestep   -> eulerian_remap: lare_end_of_timestep, with the
+//       kinetic-energy correction when rke is set (energy_correction is then
+//       a no-op); eta_calc is recomputed on the device inside the resistive
+//       substeps
 //
 // Boundary conditions: the built-in BC objects attached to each variable are
 // private to variableDef (variableDef.h:56).  They are identified once, at
@@ -197,8 +199,6 @@ namespace LARE
                 return g;
             if (d.geometry != geometryType::Cartesian)
                 throw std::runtime_error("LARE3D (MI355X): only Cartesian geometry is implemented");
-            if (d.rke)
-                throw std::runtime_error("LARE3D (MI355X): the remap kinetic-energy correction (rke) is not implemented");
             lare_desc desc{};
             desc.nx = d.nx;
             desc.ny = d.ny;
@@ -219,6 +219,7 @@ namespace LARE
             desc.eta_background = d.eta_background;
             desc.eta0 = d.eta0;
             desc.j_max = d.j_max;
+            desc.rke = d.rke ? 1 : 0;
             // boundary conditions first (host only), so that a condition the
             // device does not implement is reported before any device work
             std::vector<edgeBC> bcs;
@@ -330,9 +331,11 @@ namespace LARE
         throw std::runtime_error("LARE3D (MI355X): remap_z runs inside eulerian_remap");
     }
 
+    // endOfTimestep calls this after eulerian_remap when rke is set
+    // (shared_data.h:415-417); lare_end_of_timestep has already applied the
+    // correction on the device, after the remap's last boundary fill
     void LARE3D::energy_correction(simulationData &)
     {
-        throw std::runtime_error("LARE3D (MI355X): the remap kinetic-energy correction (rke) is not implemented");
     }
 
     // ---- output events (diagnostics.cpp:50-107): the same datasets, read

@@ -38,7 +38,8 @@ class LareDesc(ctypes.Structure):
                 ("visc2", ctypes.c_double), ("mu0", ctypes.c_double),
                 ("dt_multiplier", ctypes.c_double), ("resistive", ctypes.c_int32),
                 ("precision", ctypes.c_int32), ("eta_background", ctypes.c_double),
-                ("eta0", ctypes.c_double), ("j_max", ctypes.c_double)]
+                ("eta0", ctypes.c_double), ("j_max", ctypes.c_double),
+                ("rke", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class LareTune(ctypes.Structure):
@@ -56,15 +57,16 @@ class LareProblem(ctypes.Structure):
                 ("nx", ctypes.c_int64), ("ny", ctypes.c_int64), ("nz", ctypes.c_int64),
                 ("seed", ctypes.c_uint64), ("amp", ctypes.c_double), ("beta", ctypes.c_double),
                 ("noise", ctypes.c_double), ("eta", ctypes.c_double),
-                ("precision", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("precision", ctypes.c_int32), ("rke", ctypes.c_int32)]
 
 
 def problem(kind="ot", n=16, ny=None, nz=None, seed=1, amp=0.0, beta=-1.0, noise=0.0, eta=0.0,
-            bcx=0, precision=FP64):
-    """A LareProblem: kind 'ot' (Orszag-Tang) or 'bw' (Brio-Wu)."""
+            bcx=0, precision=FP64, rke=False):
+    """A LareProblem: kind 'ot' (Orszag-Tang) or 'bw' (Brio-Wu); rke turns the
+    remap kinetic-energy correction on."""
     k = {"ot": ORSZAG_TANG, "bw": BRIO_WU}[kind] if isinstance(kind, str) else int(kind)
     return LareProblem(k, bcx, n, ny if ny is not None else n, nz if nz is not None else n,
-                       seed, amp, beta, noise, eta, precision, 0)
+                       seed, amp, beta, noise, eta, precision, int(bool(rke)))
 
 
 _lib = None

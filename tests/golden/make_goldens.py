@@ -8,6 +8,11 @@ field's raw little-endian float64 bytes over its full reference extent
 (ghosts included), with L-inf/L2 norms for reading.  Requires the reference
 tree; the JSON it writes is the fixture the tests use.
 
+Cases with the remap kinetic-energy correction (rke) run the reference on one
+OpenMP thread: its atomic adds into delta_ke (xremap.cpp:552-555 and twins)
+sum in an arbitrary order when threads share a cell, and one thread's order
+is the one the oracle and the kernels reproduce.
+
     make -C oracle ref && python tests/golden/make_goldens.py
 """
 import hashlib
@@ -34,6 +39,11 @@ CASES = [
     ("bw_walls_16x5x3", "bw", 16, 5, 3, 8, (0, 8), {"bcx": "clamp", "noise": 0.02, "seed": 11}),
     ("bw_plain_32x3x3", "bw", 32, 3, 3, 20, (0, 20), {}),
     ("ot_64_config0", "ot", 64, 64, 64, 100, (1, 100), {}),
+    ("ot_rke_12", "ot", 12, 12, 12, 10, (0, 1, 10), {"amp": 0.05, "seed": 7, "rke": True}),
+    ("ot_rke_10x8x12", "ot", 10, 8, 12, 12, (0, 12), {"amp": 0.02, "seed": 2, "rke": True}),
+    ("bw_rke_zerograd_24x4x6", "bw", 24, 4, 6, 15, (0, 15), {"noise": 0.01, "seed": 3, "rke": True}),
+    ("bw_rke_mirror_16x5x4", "bw", 16, 5, 4, 10, (0, 10), {"bcx": "mirror", "noise": 0.02, "seed": 4,
+                                                           "rke": True}),
 ]
 
 
@@ -50,13 +60,15 @@ def main():
         with tempfile.TemporaryDirectory() as td:
             meta = O.run_reference(td, problem=prob, nx=nx, ny=ny, nz=nz, steps=steps, dumps=dumps,
                                    seed=extra.get("seed", 1), amp=extra.get("amp", 0.0),
-                                   noise=extra.get("noise", 0.0), bcx=extra.get("bcx"), threads=8)
+                                   noise=extra.get("noise", 0.0), bcx=extra.get("bcx"),
+                                   threads=1 if extra.get("rke") else 8, rke=extra.get("rke", False))
             snaps = {}
             for s in dumps:
                 snaps[str(s)] = {f: digest(os.path.join(td, f"s{s:05d}_{f}.f64")) for f in FIELDS}
         out["cases"][name] = {"problem": prob, "nx": nx, "ny": ny, "nz": nz, "steps": steps,
                               "seed": extra.get("seed", 1), "amp": extra.get("amp", 0.0),
                               "noise": extra.get("noise", 0.0), "bcx": extra.get("bcx", ""),
+                              "rke": bool(extra.get("rke", False)),
                               "dt": meta["dt"], "snapshots": snaps}
         print(name, "done", flush=True)
     with open(os.path.join(ROOT, "tests", "golden", "goldens.json"), "w") as f:

@@ -22,7 +22,7 @@ def goldens():
 def case_problem(c):
     """LareProblem for a golden case."""
     return L.problem(c["problem"], n=c["nx"], ny=c["ny"], nz=c["nz"], seed=c["seed"],
-                     amp=c["amp"], noise=c["noise"], bcx=BCX[c["bcx"]])
+                     amp=c["amp"], noise=c["noise"], bcx=BCX[c["bcx"]], rke=c.get("rke", False))
 
 
 def make_oracle(p):

@@ -37,7 +37,7 @@ def test_dropin_reproduces_reference(name):
         meta = O.run_reference(td, problem=c["problem"], nx=c["nx"], ny=c["ny"], nz=c["nz"],
                                steps=c["steps"], dumps=dumps, seed=c["seed"], amp=c["amp"],
                                noise=c["noise"], bcx=c["bcx"] or None, threads=2,
-                               exe=O.DROPIN_DRIVER)
+                               exe=O.DROPIN_DRIVER, rke=c.get("rke", False))
         assert meta["dt"] == c["dt"], "dt sequence"
         for s in dumps:
             for f in FIELDS:

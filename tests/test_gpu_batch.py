@@ -1,6 +1,7 @@
 """Batches: grids of uneven sizes, problems, boundary conditions and physics
-(ideal and resistive) stepped together by one launch per kernel give, per
-grid, the same bits as the grid stepped alone and as the oracle."""
+(ideal and resistive, with and without the kinetic-energy correction)
+stepped together by one launch per kernel give, per grid, the same bits as
+the grid stepped alone and as the oracle."""
 import numpy as np
 import pytest
 
@@ -16,6 +17,8 @@ MIX = [
     L.problem("bw", n=16, ny=5, nz=3, seed=11, noise=0.02, bcx=L.BC_CLAMP),
     L.problem("bw", n=16, ny=4, nz=4, bcx=L.BC_MIRROR),
     L.problem("ot", n=8, ny=10, nz=6, seed=1, amp=0.05),
+    L.problem("ot", n=11, ny=13, nz=9, seed=6, amp=0.08, rke=True),
+    L.problem("bw", n=18, ny=4, nz=6, seed=8, noise=0.02, eta=1e-3, rke=True),
 ]
 
 
@@ -46,7 +49,7 @@ def test_batch_matches_each_grid_alone(graph):
 
 def test_batch_matches_oracle_and_continues():
     """Two batch calls in a row, then the members continue on their own."""
-    sel = [MIX[0], MIX[1], MIX[4]]
+    sel = [MIX[0], MIX[1], MIX[4], MIX[6], MIX[7]]
     mem = [L.Lare.from_problem(p) for p in sel]
     orc = [make_oracle(p) for p in sel]
     b = L.Batch(mem)

@@ -26,14 +26,21 @@ def _state_diffs(dev, orc, fields=L.STATE_FIELDS):
     return out
 
 
+def _skip_set0_rke(c, kernel_set):
+    if kernel_set == 0 and c.get("rke"):
+        pytest.skip("the kinetic-energy correction runs on kernel sets 1-3 only")
+
+
 @pytest.mark.parametrize("kernel_set", [0, 1, 2])
 @pytest.mark.parametrize("name", ["ot_pert_10x8x12", "bw_zerograd_24x4x6", "bw_walls_16x5x3",
-                                  "bw_mirror_16x4x4"])
+                                  "bw_mirror_16x4x4", "ot_rke_12", "bw_rke_mirror_16x5x4"])
 def test_events_of_first_two_steps_match_oracle(name, kernel_set):
     """Event by event (startOfTimestep, halfTimestep, endOfTimestep) for two
     steps: dt, the half-step velocities and every state field; for the
     one-kernel-per-reference-kernel set (0), the fused set (1: one cell per
-    lane at these sizes) and the fused set with lane-walking kernels (2)."""
+    lane at these sizes) and the fused set with lane-walking kernels (2);
+    the rke cases add the kinetic-energy correction to endOfTimestep."""
+    _skip_set0_rke(CASES[name], kernel_set)
     p = case_problem(CASES[name])
     dev = L.Lare.from_problem(p)
     dev.set_kernel_set(kernel_set)
@@ -59,6 +66,7 @@ def test_events_of_first_two_steps_match_oracle(name, kernel_set):
 def test_reference_goldens(name, kernel_set):
     """Whole runs against the reference: the dt sequence and state digests."""
     c = CASES[name]
+    _skip_set0_rke(c, kernel_set)
     dev = L.Lare.from_problem(case_problem(c))
     dev.set_kernel_set(kernel_set)
     snaps = c["snapshots"]
@@ -94,6 +102,32 @@ def test_resistive_brio_wu_matches_oracle(kernel_set):
     for s in range(6):
         assert dev.step().hex() == orc.step().hex(), f"dt of step {s + 1}"
     assert not _state_diffs(dev, orc)
+
+
+def test_kinetic_energy_correction_changes_only_the_energies():
+    """rke on and off from the same state for one step: rho, v and B are
+    bit-identical (the correction only heats), e_e and e_i differ and never
+    decrease, and the device matches the oracle in both runs."""
+    runs = {}
+    for rke in (False, True):
+        p = L.problem("ot", n=14, ny=12, nz=10, seed=5, amp=0.1, rke=rke)
+        dev = L.Lare.from_problem(p)
+        orc = make_oracle(p)
+        for s in range(3):
+            assert dev.step().hex() == orc.step().hex(), f"rke={rke} dt of step {s + 1}"
+        assert not _state_diffs(dev, orc), f"rke={rke}"
+        runs[rke] = dev
+    # same first steps: the correction enters the state only through the energies
+    p0 = L.problem("ot", n=14, ny=12, nz=10, seed=5, amp=0.1)
+    a, b = L.Lare.from_problem(p0), L.Lare.from_problem(L.problem("ot", n=14, ny=12, nz=10, seed=5, amp=0.1,
+                                                                    rke=True))
+    a.step()
+    b.step()
+    for f in ("rho", "vx", "vy", "vz", "bx", "by", "bz"):
+        assert bit_diff(a.download(f), b.download(f)) is None, f
+    for f in ("energy_electron", "energy_ion"):
+        ea, eb = a.download(f)[2:-2, 2:-2, 2:-2], b.download(f)[2:-2, 2:-2, 2:-2]
+        assert np.all(eb >= ea) and np.any(eb > ea), f
 
 
 def test_uniform_state_first_dt_closed_form():
