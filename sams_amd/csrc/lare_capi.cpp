@@ -9,6 +9,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -316,19 +317,34 @@ void make_axis(long n, double mn, double mx, std::vector<double> &db, std::vecto
 
 namespace {
 
+// Bytes between consecutive 3D arrays of the pool: the array rounded up to
+// `align` bytes plus `skew` (array layout knobs, LARE_ARRAY_ALIGN and
+// LARE_ARRAY_SKEW; defaults 256 and 0).
+struct PoolLayout {
+    size_t align = 256, skew = 0;
+    size_t stride(size_t bytes) const { return (bytes + align - 1) / align * align + skew; }
+};
+PoolLayout pool_layout()
+{
+    PoolLayout L;
+    if (const char *v = std::getenv("LARE_ARRAY_ALIGN")) L.align = std::max<size_t>(256, std::strtoull(v, nullptr, 10));
+    if (const char *v = std::getenv("LARE_ARRAY_SKEW")) L.skew = std::strtoull(v, nullptr, 10) / 256 * 256;
+    return L;
+}
+
 // Engine arrays (lare::ArrE) of one grid in storage type T: 39 T arrays and
 // 6 FP64 flux arrays carved from the pool at p.
 template <class T>
-void carve_engine(lare::ArrE<T> &a, char *&p, size_t box)
+void carve_engine(lare::ArrE<T> &a, char *&p, size_t box, const PoolLayout &L)
 {
     auto tk = [&]() {
         lare::Fld<T> r{(T *)p};
-        p += box * sizeof(T);
+        p += L.stride(box * sizeof(T));
         return r;
     };
     auto dk = [&]() {
         double *r = (double *)p;
-        p += box * sizeof(double);
+        p += L.stride(box * sizeof(double));
         return r;
     };
     lare::Fld<T> *ts[] = {&a.rho, &a.ee, &a.ei, &a.vx, &a.vy, &a.vz, &a.bx, &a.by, &a.bz, &a.vx1, &a.vy1,
@@ -411,7 +427,9 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
         return fail(-1, "lare_ctx_create: each of nx, ny, nz must be at least 3");
     if (d->precision != LARE_FP64 && d->precision != LARE_FP32_STORAGE)
         return fail(-1, "lare_ctx_create: unknown storage precision");
-    const long pz = ((d->nz + 5) + 7) / 8 * 8;
+    long zp = 8;  // row pitch rounded to this many elements (LARE_ZPITCH)
+    if (const char *v = std::getenv("LARE_ZPITCH")) zp = std::max(1L, std::atol(v));
+    const long pz = ((d->nz + 5) + zp - 1) / zp * zp;
     const long sx = (d->ny + 5) * pz;
     const long box = (d->nx + 5) * sx;
     if (box >= (1L << 31)) return fail(-1, "lare_ctx_create: grid too large for 32-bit indexing");
@@ -450,7 +468,8 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
     // one pool: the engine's 3D arrays, the six 1D delta arrays, the scalars
     const size_t tsz = g.fp32 ? sizeof(float) : sizeof(double);
     const size_t lmax = (size_t)std::max(d->nx, std::max(d->ny, d->nz)) + 5;
-    g.pool_bytes = box * (kEngineT * tsz + (kEngineD + (g.prm.rke ? 1 : 0)) * sizeof(double)) +
+    const PoolLayout lay = pool_layout();
+    g.pool_bytes = kEngineT * lay.stride(box * tsz) + (kEngineD + (g.prm.rke ? 1 : 0)) * lay.stride(box * sizeof(double)) +
                    6 * lmax * sizeof(double) + 256;
     hipError_t e = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -467,15 +486,15 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
     lare::Arr &a = g.dev.a;
     double *d1[6];
     if (g.fp32) {
-        carve_engine(g.e32.a, p, g.box);
+        carve_engine(g.e32.a, p, g.box, lay);
     } else {
-        carve_engine(g.e64.a, p, g.box);
+        carve_engine(g.e64.a, p, g.box, lay);
         alias_set0(a, g.e64.a);
     }
     double *dke = nullptr;  // delta_ke of the kinetic-energy correction
     if (g.prm.rke) {
         dke = (double *)p;
-        p += box * sizeof(double);
+        p += lay.stride(box * sizeof(double));
     }
     for (int k = 0; k < 6; ++k) {
         d1[k] = (double *)p;

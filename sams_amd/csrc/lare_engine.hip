@@ -3509,8 +3509,7 @@ void start_b_t(Group &gp, int substeps)
         for (int i = 0; i < substeps; ++i) {
             const int to_bx1 = (substeps - 1 - i) % 2;
             elaunch_(gp.zx_k >= 16 ? r_substep_zx<T, 16> : r_substep_zx<T, 8>, "r_substep", 9 * w, gp, 1,
-                     rs(0, 0, 0, 0, 0, 0), 0, kZX + 100 + zxk(gp),
-                     to_bx1);
+                     rs(0, 0, 0, 0, 0, 0), 0, kZX + 100 + zxk(gp), to_bx1);
             ebcs<T>(gp, to_bx1 ? VBE_1 : VBE_0, -1, 1);
         }
     } else if (gp.n[1] > 0) {

@@ -52,12 +52,17 @@ class Request:
 def device_bytes(p):
     """HBM held by one grid of problem p: 39 state/work arrays in the storage
     type and 6 FP64 flux arrays over the padded box (7 with the kinetic-energy
-    correction), plus axes and scalars (the pool of lare_ctx_create)."""
+    correction), each rounded up to 256 bytes, plus axes and scalars (the
+    pool of lare_ctx_create with its default layout)."""
     pz = ((int(p.nz) + 5) + 7) // 8 * 8
     box = (int(p.nx) + 5) * (int(p.ny) + 5) * pz
     t = 4 if p.precision == L.FP32_STORAGE else 8
     lmax = max(int(p.nx), int(p.ny), int(p.nz)) + 5
-    return box * (39 * t + (7 if p.rke else 6) * 8) + 6 * lmax * 8 + 256
+
+    def arr(nbytes):
+        return (nbytes + 255) // 256 * 256
+
+    return 39 * arr(box * t) + (7 if p.rke else 6) * arr(box * 8) + 6 * lmax * 8 + 256
 
 
 def poisson_requests(qps, duration, seed, nmin=32, nmax=192, smin=50, smax=500):
