@@ -47,8 +47,13 @@ enum lare_bc_kind {
     LARE_BC_PERIODIC = 1,  /* simplePeriodicBC */
     LARE_BC_ZERO_GRADIENT = 2, /* simpleZeroGradientBC */
     LARE_BC_MIRROR = 3,    /* simpleMirrorBC */
-    LARE_BC_CLAMP = 4      /* simpleClamp(value) */
+    LARE_BC_CLAMP = 4,     /* simpleClamp(value) */
+    LARE_BC_MAP = 5        /* layer by layer copies and constants: lare_set_bc_map */
 };
+
+/* Entries of a lare_set_bc_map layer table besides a source index. */
+#define LARE_BC_MAP_KEEP (-1000)  /* the layer is left as it is */
+#define LARE_BC_MAP_VALUE (-1001) /* the layer is set to the map's value */
 
 /* Storage precision of the state fields. */
 enum lare_precision {
@@ -100,6 +105,18 @@ int lare_download_field(lare_ctx *ctx, int field, double *host);
  * dimension of a field.  Replaces variableDef::addBoundaryCondition
  * (variableDef.h:402-425) for the built-in classes. */
 int lare_set_bc(lare_ctx *ctx, int field, int dim, int edge, int kind, double value);
+/* A condition given layer by layer on one edge: the edge's ghost layers, the
+ * lower edge's -2..0 (staggered along dim) or -1..0 (centred), the upper
+ * edge's n..n+2 or n+1..n+2, in that order, each take src[k]: the value of
+ * layer src[k] of the same line (reference indexing along dim, read before
+ * the edge writes), LARE_BC_MAP_VALUE, or LARE_BC_MAP_KEEP (src[2] of a
+ * centred variable is ignored).  Replaces LAREBoundaryConditions
+ * (src/InitialConditions/LARE/lareBoundaryClass.h:10-34) carrying the
+ * LARE3DInitialConditions BC_OTHER rules
+ * (src/InitialConditions/LARE/boundary.cpp:31-1120): copies such as
+ * bx(-2) = bx(2), constants such as vx(-2..0) = 0.  Kernel set 0 does not
+ * run it. */
+int lare_set_bc_map(lare_ctx *ctx, int field, int dim, int edge, const int32_t src[3], double value);
 
 /* dt of the previous step (the value the next step's shock viscosity reads,
  * lagran.cpp:72,83-84); 0 for a fresh context. */

@@ -88,6 +88,7 @@ struct lo_ctx {
     double xpass, ypass, zpass;
     int bck[LO_NVAR][3][2];
     double bcv[LO_NVAR][3][2];
+    int bcm[LO_NVAR][3][2][3]; /* layer tables of LO_BC_MAP edges */
 
     a1 xb, yb, zb, xc, yc, zc, dxb, dyb, dzb, dxc, dyc, dzc;
     /* registered variables (setup.cpp:38-70) */
@@ -375,6 +376,15 @@ int lo_set_bc(lo_ctx *c, int var, int dim, int edge, int kind, double value)
     return 0;
 }
 
+int lo_set_bc_map(lo_ctx *c, int var, int dim, int edge, const int src[3], double value)
+{
+    if (var < 0 || var >= LO_NVAR || dim < 0 || dim > 2 || edge < 0 || edge > 1) return -1;
+    for (int k = 0; k < 3; ++k) c->bcm[var][dim][edge][k] = src[k];
+    c->bck[var][dim][edge] = LO_BC_MAP;
+    c->bcv[var][dim][edge] = value;
+    return 0;
+}
+
 double lo_get_dt(const lo_ctx *c) { return c->dt; }
 void lo_set_dt(lo_ctx *c, double dt) { c->dt = dt; }
 
@@ -420,6 +430,32 @@ static void bc_apply_edge(lo_ctx *c, int v, int dim, int edge)
     lo[dim] = g0;
     hi[dim] = g1;
     const double val = c->bcv[v][dim][edge];
+    if (kind == LO_BC_MAP) {
+        /* a LAREBoundaryConditions function (lareBoundaryClass.h:10-34)
+         * written as its effect on one edge: per ghost layer g0 + t a source
+         * layer, the constant or nothing; every source is read before the
+         * edge writes (the rules of src/InitialConditions/LARE/boundary.cpp
+         * never read a layer they write) */
+        const int *m = c->bcm[v][dim][edge];
+        lo[dim] = hi[dim] = 0;
+        for (long i = lo[0]; i <= hi[0]; ++i)
+            for (long j = lo[1]; j <= hi[1]; ++j)
+                for (long k = lo[2]; k <= hi[2]; ++k) {
+                    double out[3] = {0.0, 0.0, 0.0};
+                    for (long g = g0; g <= g1; ++g) {
+                        long s3[3] = {i, j, k};
+                        s3[dim] = m[g - g0];
+                        if (m[g - g0] == LO_BC_MAP_VALUE) out[g - g0] = val;
+                        else if (m[g - g0] != LO_BC_MAP_KEEP) out[g - g0] = A(a, s3[0], s3[1], s3[2]);
+                    }
+                    for (long g = g0; g <= g1; ++g) {
+                        long t3[3] = {i, j, k};
+                        t3[dim] = g;
+                        if (m[g - g0] != LO_BC_MAP_KEEP) A(a, t3[0], t3[1], t3[2]) = out[g - g0];
+                    }
+                }
+        return;
+    }
     for (long i = lo[0]; i <= hi[0]; ++i)
         for (long j = lo[1]; j <= hi[1]; ++j)
             for (long k = lo[2]; k <= hi[2]; ++k) {

@@ -30,7 +30,10 @@ enum lo_field_id {
 };
 
 /* Built-in boundary conditions (builtInBoundaryConditions.h). */
-enum lo_bc_kind { LO_BC_NONE = 0, LO_BC_PERIODIC, LO_BC_ZEROGRAD, LO_BC_MIRROR, LO_BC_CLAMP };
+enum lo_bc_kind { LO_BC_NONE = 0, LO_BC_PERIODIC, LO_BC_ZEROGRAD, LO_BC_MIRROR, LO_BC_CLAMP, LO_BC_MAP };
+/* Entries of a map edge's layer table besides a source layer. */
+#define LO_BC_MAP_KEEP (-1000)
+#define LO_BC_MAP_VALUE (-1001)
 
 typedef struct lo_desc {
     long nx, ny, nz;
@@ -54,6 +57,10 @@ double *lo_array(lo_ctx *c, const char *name, long lb[3], long ub[3]);
 double *lo_axis(lo_ctx *c, const char *name, long *lb, long *ub);
 
 int lo_set_bc(lo_ctx *c, int var, int dim, int edge, int kind, double value);
+/* A map edge: ghost layers g0..g1 of the edge (lower -2..0 or -1..0, upper
+ * n..n+2 or n+1..n+2) take src[t]: a layer of the same line (read before the
+ * edge writes), LO_BC_MAP_VALUE (value) or LO_BC_MAP_KEEP. */
+int lo_set_bc_map(lo_ctx *c, int var, int dim, int edge, const int src[3], double value);
 
 /* dt used by the shock viscosity of the next step (the previous step's dt). */
 double lo_get_dt(const lo_ctx *c);

@@ -22,6 +22,7 @@ DROPIN_DRIVER = os.path.join(HERE, "_ref", "lare3d_dropin")
 FIELDS = ["rho", "energy_electron", "energy_ion", "vx", "vy", "vz", "bx", "by", "bz",
           "vx1", "vy1", "vz1", "dm"]
 STATE_FIELDS = FIELDS[:9]
+MAP_KEEP, MAP_VALUE = -1000, -1001  # map-edge table entries besides a source layer
 
 
 class LoDesc(ctypes.Structure):
@@ -62,6 +63,7 @@ def lib():
         L.lo_axis.restype = P(ctypes.c_double)
         L.lo_axis.argtypes = [vp, ctypes.c_char_p, P(ctypes.c_long), P(ctypes.c_long)]
         L.lo_set_bc.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.lo_set_bc_map.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_int), ctypes.c_double]
         L.lo_get_dt.restype = ctypes.c_double
         L.lo_get_dt.argtypes = [vp]
         L.lo_set_dt.argtypes = [vp, ctypes.c_double]
@@ -138,6 +140,11 @@ class Oracle:
     def set_bc(self, name, dim, edge, kind, value=0.0):
         lib().lo_set_bc(self.h, FIELDS.index(name), dim, edge, kind, value)
 
+    def set_bc_map(self, name, dim, edge, src, value=0.0):
+        """Map edge: src = 3 layer entries (index, MAP_KEEP or MAP_VALUE)."""
+        arr = (ctypes.c_int * 3)(*[int(x) for x in (list(src) + [MAP_KEEP] * 3)[:3]])
+        lib().lo_set_bc_map(self.h, FIELDS.index(name), dim, edge, arr, value)
+
     def get_dt(self):
         return lib().lo_get_dt(self.h)
 
@@ -177,7 +184,8 @@ def problem_bcs(o, kind, bcx=0):
 
 
 def run_reference(out_dir, problem="ot", nx=16, ny=None, nz=None, steps=10, dumps=(), seed=1,
-                  amp=0.0, noise=0.0, bcx=None, threads=4, fast=False, beta=-1.0, exe=None, rke=False):
+                  amp=0.0, noise=0.0, bcx=None, threads=4, fast=False, beta=-1.0, exe=None, rke=False,
+                  lare_bc=False):
     """Run the reference driver (oracle/_ref, built from the reference's own
     sources; exe selects another build of it, such as DROPIN_DRIVER) and
     return its meta dict.  Raises if it has not been built."""
@@ -195,6 +203,8 @@ def run_reference(out_dir, problem="ot", nx=16, ny=None, nz=None, steps=10, dump
         args += ["--bcx", bcx]
     if rke:
         args += ["--rke"]
+    if lare_bc:
+        args += ["--lare-bc"]
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     subprocess.run(args, cwd=out_dir, env=env, check=True, capture_output=True)
     with open(os.path.join(out_dir, "meta.json")) as f:

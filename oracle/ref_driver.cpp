@@ -19,7 +19,7 @@
 //   ref_driver --problem ot|bw --nx N [--ny N --nz N] --steps K
 //              [--dump s1,s2,...] [--out DIR] [--seed S] [--amp A] [--beta B]
 //              [--noise F] [--eta E] [--bcx periodic|zerograd|mirror|clamp]
-//              [--time] [--rke]
+//              [--time] [--rke] [--lare-bc]
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -36,6 +36,7 @@
 #include "runner.h"
 #include "shared_data.h"
 #include "builtInBoundaryConditions.h"
+#include "lareic.h"
 
 #if defined(LARE_DROPIN)
 // Built with the MI355X package (sams_amd/host/lare3d_hip.cpp) in place of
@@ -70,6 +71,7 @@ namespace refdrv
         bool time = false;
         bool phases = false;
         bool rke = false;   // remap kinetic-energy correction
+        bool lareBC = false; // LARE3DInitialConditions' BC_OTHER conditions on every edge
         std::vector<double> dts;
         double elapsed = 0.0;
         std::chrono::steady_clock::time_point t0, tLast;
@@ -198,6 +200,12 @@ namespace refdrv
             data.mf = 1.2;
             data.mu0 = 1.0;
             data.rke = c.rke;
+            if (c.lareBC)
+            {
+                data.xbc_min = data.xbc_max = LARE::BCType::BC_OTHER;
+                data.ybc_min = data.ybc_max = LARE::BCType::BC_OTHER;
+                data.zbc_min = data.zbc_max = LARE::BCType::BC_OTHER;
+            }
             data.resistiveMHD = false;
             data.eta_background = 0.0;
             data.j_max = 1.0e30;
@@ -237,8 +245,17 @@ namespace refdrv
             ax.setDomain("Z", data.nz, data.z_min, data.z_max);
         }
 
-        void setBoundaryConditions(SAMS::harness &harness)
+        void setBoundaryConditions(SAMS::harness &harness, LARE::simulationData &data, SAMS::timeState &time)
         {
+            if (cfg().lareBC)
+            {
+                // the LARE-style conditions of the LARE initial-condition
+                // package (src/InitialConditions/LARE/lareic.h:124-146,
+                // boundary.cpp), attached exactly as that package does
+                LARE::LARE3DInitialConditions lare;
+                lare.setBoundaryConditions(harness, data, time);
+                return;
+            }
             const char *vars[] = {"bx", "by", "bz", "energy_ion", "energy_electron", "rho",
                                   "vx", "vy", "vz", "LARE/vx1", "LARE/vy1", "LARE/vz1", "LARE/dm"};
             for (const char *v : vars)
@@ -424,6 +441,8 @@ int main(int argc, char *argv[])
             c.time = true;
         else if (a == "--rke")
             c.rke = true;
+        else if (a == "--lare-bc")
+            c.lareBC = true;
         else if (a == "--phases")
         {
 #if defined(LARE_DROPIN)

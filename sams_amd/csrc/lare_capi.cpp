@@ -661,6 +661,28 @@ extern "C" int lare_set_bc(lare_ctx *c, int field, int dim, int edge, int kind, 
     return 0;
 }
 
+extern "C" int lare_set_bc_map(lare_ctx *c, int field, int dim, int edge, const int32_t src[3], double value)
+{
+    if (!c || !src) return fail(-1, "lare_set_bc_map: null argument");
+    if (field < 0 || field >= LARE_NFIELDS || dim < 0 || dim > 2 || edge < 0 || edge > 1)
+        return fail(-1, "lare_set_bc_map: field, dim or edge out of range");
+    if (c->g.kernel_set == 0) return fail(-1, "lare_set_bc_map: map conditions run on kernel sets 1-3 only");
+    const int64_t n3[3] = {c->desc.nx, c->desc.ny, c->desc.nz};
+    const int s = kStag[field][dim];
+    const int nl = s ? 3 : 2;
+    for (int k = 0; k < 3; ++k) {
+        int m = k < nl ? src[k] : LARE_BC_MAP_KEEP;
+        if (m != LARE_BC_MAP_KEEP && m != LARE_BC_MAP_VALUE && (m < (s ? -2 : -1) || m > n3[dim] + 2))
+            return fail(-1, "lare_set_bc_map: source layer outside the field's extent");
+        c->g.bcm[field][dim][edge][k] = m;
+    }
+    c->g.bck[field][dim][edge] = LARE_BC_MAP;
+    c->g.bcv[field][dim][edge] = value;
+    c->g.bc_version += 1;
+    drop_graphs(c);
+    return 0;
+}
+
 extern "C" int lare_get_dt(lare_ctx *c, double *dt)
 {
     if (!c || !dt) return fail(-1, "lare_get_dt: null argument");
@@ -844,6 +866,11 @@ extern "C" int lare_set_kernel_set(lare_ctx *c, int32_t kernel_set)
     if (kernel_set == 0) {
         if (c->g.prm.rke)
             return fail(-1, "lare_set_kernel_set: the kinetic-energy correction (rke) runs on kernel sets 1-3 only");
+        for (int f = 0; f < LARE_NFIELDS; ++f)
+            for (int d = 0; d < 3; ++d)
+                for (int e = 0; e < 2; ++e)
+                    if (c->g.bck[f][d][e] == LARE_BC_MAP)
+                        return fail(-1, "lare_set_kernel_set: map boundary conditions run on kernel sets 1-3 only");
         int rc = ensure_set0(c);
         if (rc) return rc;
     }

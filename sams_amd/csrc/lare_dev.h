@@ -180,6 +180,7 @@ struct BCVarE {
     int sd, sa, sb;
     int k0, k1;
     double v0, v1;
+    int m0[3], m1[3];  // layer tables of map edges (kind 5)
 };
 struct BCSetE {
     BCVarE v[9];
@@ -194,6 +195,7 @@ struct BCVar3 {
     int s[3];
     int k0[3], k1[3];
     double v0[3], v1[3];
+    int m0[3][3], m1[3][3];  // layer tables of map edges (kind 5), per dimension
     int b0, nsh;
 };
 struct BCSet3 {

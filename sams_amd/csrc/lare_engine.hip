@@ -2737,13 +2737,29 @@ KERNEL OCC(3) void r_substep_zx(const Src<T> S, int to_bx1)
 // (variableDef.h:539-605, builtInBoundaryConditions.h:54-300): ghost layers
 // -1..0 (centred) or -2..0 (staggered, the boundary point included) at the
 // lower edge, n+1..n+2 or n..n+2 at the upper.
+// Map edges (kind 5; LARE_BC_MAP) give each ghost layer g0 + k a source
+// layer m[k] (read before the edge writes), a constant (kMapValue) or
+// nothing (kMapKeep).
+constexpr int kMapKeep = -1000, kMapValue = -1001;
+
 template <class E>
-__device__ __forceinline__ void bc_edge_e(E *p, int c0, int sd, int n, int s, int edge, int kind, double val)
+__device__ __forceinline__ void bc_edge_e(E *p, int c0, int sd, int n, int s, int edge, int kind, double val,
+                                          const int *m)
 {
     if (kind == 0) return;
     const int LB = s ? 0 : 1, UB = n;
     const int g0 = edge ? n + 1 - s : (s ? -2 : -1);
     const int g1 = edge ? n + 2 : 0;
+    if (kind == 5) {
+        double out[3];
+        for (int g = g0; g <= g1; ++g) {
+            const int q = m[g - g0];
+            out[g - g0] = q == kMapValue ? val : q == kMapKeep ? 0.0 : (double)p[c0 + q * sd];
+        }
+        for (int g = g0; g <= g1; ++g)
+            if (m[g - g0] != kMapKeep) p[c0 + g * sd] = (E)out[g - g0];
+        return;
+    }
     const int first = edge ? g0 : g1;
     const int last = edge ? UB : LB;
     for (int g = g0; g <= g1; ++g) {
@@ -2771,8 +2787,14 @@ __device__ __forceinline__ void bc_edge_e(E *p, int c0, int sd, int n, int s, in
 // unfilled cell (n >= 3) except one, the upper periodic fill of a staggered
 // line at g = n, which copies index 0 as the lower fill has just set it.
 template <class E>
-__device__ __forceinline__ void bc_line_e(E *p, int c0, int sd, int n, int s, int k0, double v0, int k1, double v1)
+__device__ __forceinline__ void bc_line_e(E *p, int c0, int sd, int n, int s, int k0, double v0, int k1, double v1,
+                                          const int *m0, const int *m1)
 {
+    if (k0 == 5 || k1 == 5) {  // map edges: the two edges in turn
+        bc_edge_e(p, c0, sd, n, s, 0, k0, v0, m0);
+        bc_edge_e(p, c0, sd, n, s, 1, k1, v1, m1);
+        return;
+    }
     const int gl = s ? -2 : -1, nl = 1 - gl;  // lower ghosts gl..0
     const int gh = n + 1 - s, nh = n + 3 - gh;  // upper ghosts gh..n+2
     const int LB = s ? 0 : 1, UB = n;
@@ -2878,16 +2900,16 @@ KERNEL void e_bc(const EDev<T> *devs, const lare::BCSetE *sets, const int *boff,
     // loads; x and y lines are coalesced across lanes already
     if (V.src || dim != 2) {  // in the fills' own order (after the carry-over)
         if (V.f64) {
-            bc_edge_e((double *)V.p, c0, sd, n, V.sd, 0, V.k0, V.v0);
-            bc_edge_e((double *)V.p, c0, sd, n, V.sd, 1, V.k1, V.v1);
+            bc_edge_e((double *)V.p, c0, sd, n, V.sd, 0, V.k0, V.v0, V.m0);
+            bc_edge_e((double *)V.p, c0, sd, n, V.sd, 1, V.k1, V.v1, V.m1);
         } else {
-            bc_edge_e((T *)V.p, c0, sd, n, V.sd, 0, V.k0, V.v0);
-            bc_edge_e((T *)V.p, c0, sd, n, V.sd, 1, V.k1, V.v1);
+            bc_edge_e((T *)V.p, c0, sd, n, V.sd, 0, V.k0, V.v0, V.m0);
+            bc_edge_e((T *)V.p, c0, sd, n, V.sd, 1, V.k1, V.v1, V.m1);
         }
     } else if (V.f64) {
-        bc_line_e((double *)V.p, c0, sd, n, V.sd, V.k0, V.v0, V.k1, V.v1);
+        bc_line_e((double *)V.p, c0, sd, n, V.sd, V.k0, V.v0, V.k1, V.v1, V.m0, V.m1);
     } else {
-        bc_line_e((T *)V.p, c0, sd, n, V.sd, V.k0, V.v0, V.k1, V.v1);
+        bc_line_e((T *)V.p, c0, sd, n, V.sd, V.k0, V.v0, V.k1, V.v1, V.m0, V.m1);
     }
 }
 
@@ -2917,9 +2939,19 @@ __device__ __forceinline__ double bc3_val(const lare::BCVar3 &V, const E *p, con
         // the constant c) and its source indices
         int form = 0, q1 = x, q2 = x;
         double c = 0.0;
+        auto mapped = [&](int q, double value) {  // map layer: source q, the constant, or kept
+            if (q == kMapValue) {
+                form = 2;
+                c = value;
+            } else if (q != kMapKeep) {
+                q1 = q;
+            }
+        };
         auto lower = [&](int g) {
             const int LB = s ? 0 : 1;
-            if (k0 == 1) {
+            if (k0 == 5) {
+                mapped(V.m0[d][g - (s ? -2 : -1)], V.v0[d]);
+            } else if (k0 == 1) {
                 q1 = g + n;
             } else if (k0 == 2) {
                 q1 = LB;
@@ -2933,7 +2965,9 @@ __device__ __forceinline__ double bc3_val(const lare::BCVar3 &V, const E *p, con
             }
         };
         const int gh = n + 1 - s;
-        if (k0 && x <= 0) {
+        if (k1 == 5 && x >= gh) {
+            mapped(V.m1[d][x - gh], V.v1[d]);
+        } else if (k0 && x <= 0) {
             lower(x);
         } else if (k1 && x >= gh) {
             if (k1 == 1) {
@@ -3097,6 +3131,42 @@ const int kStagE[13][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 1, 1}, {1, 1, 1}
                            {0, 1, 0}, {0, 0, 1}, {1, 1, 1}, {1, 1, 1}, {1, 1, 1}, {0, 0, 0}};
 
 inline RSpec rs(int x0, int dx1, int y0, int dy1, int z0, int dz1) { return RSpec{x0, dx1, y0, dy1, z0, dz1}; }
+
+// Whether the fill of edge e of variable u along d writes every ghost layer
+// of the edge (a map edge may keep some).
+bool edge_writes_all(const Grid &g, int u, int d, int e)
+{
+    const int k = g.bck[u][d][e];
+    if (k == 0) return false;
+    if (k != 5) return true;
+    const int nl = kStagE[u][d] ? 3 : 2;
+    for (int j = 0; j < nl; ++j)
+        if (g.bcm[u][d][e][j] == kMapKeep) return false;
+    return true;
+}
+
+// Whether a map edge of u along d reads a layer that a fill of d rewrites
+// (the all-dimension fill reads values before any fill of the launch).
+bool map_reads_written(const Grid &g, int u, int d, int n)
+{
+    const int s = kStagE[u][d], gl = s ? -2 : -1, gh = n + 1 - s;
+    auto written = [&](int q) {
+        for (int e = 0; e < 2; ++e) {
+            const int k = g.bck[u][d][e], g0 = e ? gh : gl, g1 = e ? n + 2 : 0;
+            if (k == 0 || q < g0 || q > g1) continue;
+            if (k != 5 || g.bcm[u][d][e][q - g0] != kMapKeep) return true;
+        }
+        return false;
+    };
+    for (int e = 0; e < 2; ++e) {
+        if (g.bck[u][d][e] != 5) continue;
+        for (int j = 0; j < (s ? 3 : 2); ++j) {
+            const int q = g.bcm[u][d][e][j];
+            if (q != kMapKeep && q != kMapValue && written(q)) return true;
+        }
+    }
+    return false;
+}
 
 // Variable ids of the boundary fills: 0..12 the registered variables (state
 // in the first copy), 13..18 the second copy of rho, e_e, e_i, vx, vy, vz,
@@ -3303,7 +3373,7 @@ Group::BTab &btable(Group &gp, int vg, int dim, int subset)
             // lies, from values that derive from the updated cells alone.
             bool none_edge = false;
             for (int d = 0; d < 3; ++d)
-                for (int e = 0; e < 2; ++e) none_edge = none_edge || g.bck[u][d][e] == 0;
+                for (int e = 0; e < 2; ++e) none_edge = none_edge || !edge_writes_all(g, u, d, e);
             b.src = (src >= 0 && dim == 0 && none_edge) ? field_ptr(g, src) : nullptr;
             b.f64 = (u == 12 || !g.fp32) ? 1 : 0;
             b.sd = kStagE[u][dim];
@@ -3313,6 +3383,10 @@ Group::BTab &btable(Group &gp, int vg, int dim, int subset)
             b.k1 = g.bck[u][dim][1];
             b.v0 = g.bcv[u][dim][0];
             b.v1 = g.bcv[u][dim][1];
+            for (int j = 0; j < 3; ++j) {
+                b.m0[j] = g.bcm[u][dim][0][j];
+                b.m1[j] = g.bcm[u][dim][1][j];
+            }
             if (b.k0 || b.k1 || b.src) t.any = true;
         }
         const int n3[3] = {g.dev.g.nx, g.dev.g.ny, g.dev.g.nz};
@@ -3367,8 +3441,13 @@ Group::BTab &btable3(Group &gp, int vg, int subset)
                 b.k1[d] = g.bck[u][d][1];
                 b.v0[d] = g.bcv[u][d][0];
                 b.v1[d] = g.bcv[u][d][1];
-                none_edge = none_edge || b.k0[d] == 0 || b.k1[d] == 0;
+                for (int j = 0; j < 3; ++j) {
+                    b.m0[d][j] = g.bcm[u][d][0][j];
+                    b.m1[d][j] = g.bcm[u][d][1][j];
+                }
+                none_edge = none_edge || !edge_writes_all(g, u, d, 0) || !edge_writes_all(g, u, d, 1);
                 if (b.s[d] && b.k0[d] == 1 && b.k1[d] == 4) ok = false;
+                if (map_reads_written(g, u, d, n3[d])) ok = false;
                 if (b.k0[d] || b.k1[d]) t.any = true;
             }
             if (src >= 0 && none_edge) ok = false;
@@ -3463,7 +3542,8 @@ double subset_cells(const Group &gp, int subset)
 {
     double n = 0.0;
     for (const Grid *g : gp.mem)
-        if (subset == 0 || g->prm.resistive) n += (double)g->dev.g.nx * g->dev.g.ny * g->dev.g.nz;
+        if (subset == 0 || (subset == 1 && g->prm.resistive) || (subset == 2 && g->prm.rke))
+            n += (double)g->dev.g.nx * g->dev.g.ny * g->dev.g.nz;
     return n;
 }
 bool walk_ok(const Group &gp, int subset)
@@ -3641,7 +3721,8 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
         ELS((r_ke_add<T, DIR>), 4 * 8, gp, 2, cells, st | pv);
     }
     bool wshift = false;
-    for (const Grid *g : gp.mem) wshift = wshift || g->bck[9 + DIR][DIR][0] == 0 || g->bck[9 + DIR][DIR][1] == 0;
+    for (const Grid *g : gp.mem)
+        wshift = wshift || !edge_writes_all(*g, 9 + DIR, DIR, 0) || !edge_writes_all(*g, 9 + DIR, DIR, 1);
     if (wshift) {
         const RSpec sh = rs(DIR == 0 ? -2 : 0, DIR == 0 ? 1 : 0, DIR == 1 ? -2 : 0, DIR == 1 ? 1 : 0,
                             DIR == 2 ? -2 : 0, DIR == 2 ? 1 : 0);

@@ -48,6 +48,7 @@ struct Grid {
     Params prm;
     int bck[13][3][2];
     double bcv[13][3][2];
+    int bcm[13][3][2][3];  // layer tables of LARE_BC_MAP edges
     hipStream_t stream;
     size_t box;          // doubles per 3D array
     void *pool;          // one allocation holding every device array

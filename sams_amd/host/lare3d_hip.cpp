@@ -26,7 +26,9 @@
 // the array and reading which kind of fill each edge performed; the array is
 // restored afterwards.  Conditions that match none of the built-in fills
 // (simplePeriodicBC, simpleZeroGradientBC, simpleMirrorBC, simpleClamp) are
-// reported as an error.
+// identified layer by layer as copies of other layers of the line, constants
+// or untouched layers (the LARE-style conditions of the LARE initial-condition
+// package); any other effect is reported as an error.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -127,6 +129,7 @@ namespace LARE
         {
             int field, dim, edge, kind;
             double value;
+            int32_t src[3]; // layer table of a map edge
         };
 
         edgeBC probeEdge(SAMS::harness &h, simulationData &d, int f, int dim, int edge)
@@ -169,6 +172,13 @@ namespace LARE
                         return false;
                 return true;
             };
+            // a value equal to some layer's mark is a copy of that layer, not a constant
+            auto isMark = [&](double v) {
+                for (long t = s ? -2 : -1; t <= n + 2; ++t)
+                    if (v == mark(t))
+                        return true;
+                return false;
+            };
             int kind;
             double value = 0.0;
             if (all([&](long g) { return mark(g); }))
@@ -179,17 +189,44 @@ namespace LARE
                 kind = LARE_BC_ZERO_GRADIENT;
             else if (all([&](long g) { return mark(last) - (mark(last - (g - first)) - mark(last)); }))
                 kind = LARE_BC_MIRROR;
-            else if (all([&](long) { return at(g0); }))
+            else if (all([&](long) { return at(g0); }) && !isMark(at(g0)))
             {
                 kind = LARE_BC_CLAMP;
                 value = at(g0);
             }
             else
-                throw std::runtime_error(std::string("LARE3D (MI355X): the boundary condition of ") + kVarNames[f] +
-                                         " on the " + (edge ? "upper" : "lower") + " edge of dimension " +
-                                         std::to_string(dim) +
-                                         " is not a built-in periodic, zero-gradient, mirror or clamp condition");
-            return edgeBC{f, dim, edge, kind, value};
+            {
+                // a LARE-style condition (LAREBoundaryConditions,
+                // src/InitialConditions/LARE/lareBoundaryClass.h:10-34): each
+                // ghost layer a copy of one layer of the line, a constant, or
+                // left as it was; the marks name the source layer
+                edgeBC b{f, dim, edge, LARE_BC_MAP, 0.0, {LARE_BC_MAP_KEEP, LARE_BC_MAP_KEEP, LARE_BC_MAP_KEEP}};
+                bool haveValue = false;
+                for (long g = g0; g <= g1; ++g)
+                {
+                    const double v = at(g);
+                    int32_t src = LARE_BC_MAP_VALUE;
+                    if (v == mark(g))
+                        src = LARE_BC_MAP_KEEP;
+                    else
+                        for (long t = s ? -2 : -1; t <= n + 2; ++t)
+                            if (v == mark(t))
+                                src = static_cast<int32_t>(t);
+                    if (src == LARE_BC_MAP_VALUE)
+                    {
+                        if (haveValue && v != b.value)
+                            throw std::runtime_error(std::string("LARE3D (MI355X): the boundary condition of ") +
+                                                     kVarNames[f] + " on the " + (edge ? "upper" : "lower") +
+                                                     " edge of dimension " + std::to_string(dim) +
+                                                     " sets two ghost layers to different constants");
+                        b.value = v;
+                        haveValue = true;
+                    }
+                    b.src[g - g0] = src;
+                }
+                return b;
+            }
+            return edgeBC{f, dim, edge, kind, value, {0, 0, 0}};
         }
 
         deviceGrid &bind(SAMS::harness &h, simulationData &d)
@@ -233,14 +270,18 @@ namespace LARE
                     {
                         bcs.push_back(probeEdge(h, d, f, dim, edge));
                         if (report)
-                            std::fprintf(stderr, "LARE3D (MI355X) bc %s %d %d %d %.17g\n", kVarNames[f], dim, edge,
-                                         bcs.back().kind, bcs.back().value);
+                            std::fprintf(stderr, "LARE3D (MI355X) bc %s %d %d %d %.17g %d %d %d\n", kVarNames[f], dim,
+                                         edge, bcs.back().kind, bcs.back().value, bcs.back().src[0],
+                                         bcs.back().src[1], bcs.back().src[2]);
                     }
                 fromBuffer(*stateArray(d, f), buf); // the probes overwrote the host copy
             }
             check(lare_ctx_create(&desc, &g.ctx), "lare_ctx_create");
             for (const edgeBC &b : bcs)
-                check(lare_set_bc(g.ctx, b.field, b.dim, b.edge, b.kind, b.value), "lare_set_bc");
+                if (b.kind == LARE_BC_MAP)
+                    check(lare_set_bc_map(g.ctx, b.field, b.dim, b.edge, b.src, b.value), "lare_set_bc_map");
+                else
+                    check(lare_set_bc(g.ctx, b.field, b.dim, b.edge, b.kind, b.value), "lare_set_bc");
             for (int f = 0; f < LARE_NFIELDS; ++f)
             {
                 toBuffer(*stateArray(d, f), buf);

@@ -24,7 +24,8 @@ STAGGER = {"rho": (0, 0, 0), "energy_electron": (0, 0, 0), "energy_ion": (0, 0, 
            "vx": (1, 1, 1), "vy": (1, 1, 1), "vz": (1, 1, 1), "bx": (1, 0, 0), "by": (0, 1, 0),
            "bz": (0, 0, 1), "vx1": (1, 1, 1), "vy1": (1, 1, 1), "vz1": (1, 1, 1), "dm": (0, 0, 0)}
 
-BC_NONE, BC_PERIODIC, BC_ZERO_GRADIENT, BC_MIRROR, BC_CLAMP = 0, 1, 2, 3, 4
+BC_NONE, BC_PERIODIC, BC_ZERO_GRADIENT, BC_MIRROR, BC_CLAMP, BC_MAP = 0, 1, 2, 3, 4, 5
+MAP_KEEP, MAP_VALUE = -1000, -1001  # map-edge layer entries besides a source layer
 FP64, FP32_STORAGE = 0, 1
 ORSZAG_TANG, BRIO_WU = 0, 1
 
@@ -88,6 +89,8 @@ def lib():
             "lare_field_bounds": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_int64), P(ctypes.c_int64)]),
             "lare_upload_field": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_double)]),
             "lare_download_field": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_double)]),
+            "lare_set_bc_map": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_int32),
+                                               ctypes.c_double]),
             "lare_set_bc": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_double]),
             "lare_get_dt": (ctypes.c_int, [vp, P(ctypes.c_double)]),
@@ -211,6 +214,12 @@ class Lare:
 
     def set_bc(self, name, dim, edge, kind, value=0.0):
         _check(lib().lare_set_bc(self.h, FIELD_ID[name], dim, edge, kind, value), "lare_set_bc")
+
+    def set_bc_map(self, name, dim, edge, src, value=0.0):
+        """A map edge (LARE-style condition): src lists the edge's ghost
+        layers' sources (layer index, MAP_VALUE or MAP_KEEP)."""
+        arr = (ctypes.c_int32 * 3)(*[int(x) for x in (list(src) + [MAP_KEEP] * 3)[:3]])
+        _check(lib().lare_set_bc_map(self.h, FIELD_ID[name], dim, edge, arr, value), "lare_set_bc_map")
 
     def set_problem_bcs(self, p):
         _check(lib().lare_problem_bcs(self.h, ctypes.byref(p)), "lare_problem_bcs")

@@ -11,7 +11,10 @@ tree; the JSON it writes is the fixture the tests use.
 Cases with the remap kinetic-energy correction (rke) run the reference on one
 OpenMP thread: its atomic adds into delta_ke (xremap.cpp:552-555 and twins)
 sum in an arbitrary order when threads share a cell, and one thread's order
-is the one the oracle and the kernels reproduce.
+is the one the oracle and the kernels reproduce.  Cases with lare_bc attach
+the LARE initial-condition package's BC_OTHER conditions to every edge
+(src/InitialConditions/LARE/lareic.h:124-146, boundary.cpp) instead of the
+problem's built-in ones.
 
     make -C oracle ref && python tests/golden/make_goldens.py
 """
@@ -44,6 +47,9 @@ CASES = [
     ("bw_rke_zerograd_24x4x6", "bw", 24, 4, 6, 15, (0, 15), {"noise": 0.01, "seed": 3, "rke": True}),
     ("bw_rke_mirror_16x5x4", "bw", 16, 5, 4, 10, (0, 10), {"bcx": "mirror", "noise": 0.02, "seed": 4,
                                                            "rke": True}),
+    ("ot_larebc_rke_12", "ot", 12, 12, 12, 10, (0, 1, 10), {"amp": 0.05, "seed": 7, "rke": True, "lare_bc": True}),
+    ("ot_larebc_10x8x12", "ot", 10, 8, 12, 12, (0, 12), {"amp": 0.02, "seed": 2, "lare_bc": True}),
+    ("bw_larebc_16x5x4", "bw", 16, 5, 4, 10, (0, 10), {"noise": 0.01, "seed": 3, "lare_bc": True}),
 ]
 
 
@@ -61,7 +67,8 @@ def main():
             meta = O.run_reference(td, problem=prob, nx=nx, ny=ny, nz=nz, steps=steps, dumps=dumps,
                                    seed=extra.get("seed", 1), amp=extra.get("amp", 0.0),
                                    noise=extra.get("noise", 0.0), bcx=extra.get("bcx"),
-                                   threads=1 if extra.get("rke") else 8, rke=extra.get("rke", False))
+                                   threads=1 if extra.get("rke") else 8, rke=extra.get("rke", False),
+                                   lare_bc=extra.get("lare_bc", False))
             snaps = {}
             for s in dumps:
                 snaps[str(s)] = {f: digest(os.path.join(td, f"s{s:05d}_{f}.f64")) for f in FIELDS}
@@ -69,6 +76,7 @@ def main():
                               "seed": extra.get("seed", 1), "amp": extra.get("amp", 0.0),
                               "noise": extra.get("noise", 0.0), "bcx": extra.get("bcx", ""),
                               "rke": bool(extra.get("rke", False)),
+                              "lare_bc": bool(extra.get("lare_bc", False)),
                               "dt": meta["dt"], "snapshots": snaps}
         print(name, "done", flush=True)
     with open(os.path.join(ROOT, "tests", "golden", "goldens.json"), "w") as f:

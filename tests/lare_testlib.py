@@ -25,6 +25,15 @@ def case_problem(c):
                      amp=c["amp"], noise=c["noise"], bcx=BCX[c["bcx"]], rke=c.get("rke", False))
 
 
+def case_bcs(c, grid):
+    """The golden case's boundary conditions beyond the problem's own (the
+    LARE initial-condition package's conditions on every edge) on a device
+    or oracle grid."""
+    if c.get("lare_bc"):
+        apply_lare_bc(grid, c["nx"], c["ny"], c["nz"])
+    return grid
+
+
 def make_oracle(p):
     """Oracle grid holding problem p at t = 0 (product initial conditions)."""
     o = O.Oracle(L.problem_desc(p))
@@ -53,3 +62,54 @@ def bit_diff(a, b):
 def rel_linf(a, b):
     scale = np.max(np.abs(b))
     return float(np.max(np.abs(a - b)) / scale) if scale > 0 else float(np.max(np.abs(a - b)))
+
+
+KEEP, VALUE = O.MAP_KEEP, O.MAP_VALUE
+
+
+def lare_bc_rules(nx, ny, nz):
+    """The BC_OTHER conditions of the LARE initial-condition package
+    (src/InitialConditions/LARE/boundary.cpp:31-1120, attached to every edge
+    by lareic.h:124-146) as map edges: {(field, dim, edge): (src[3], value)}
+    with the edge's ghost layers in order (lower -2..0 or -1..0, upper n..n+2
+    or n+1..n+2).  Restated from the source; pinned by the reference's own
+    runs (goldens *_larebc_*)."""
+    n3 = (nx, ny, nz)
+    out = {}
+    for f in O.FIELDS:
+        st = L.STAGGER[f]
+        for d in range(3):
+            n, s = n3[d], st[d]
+            if f == "dm":  # dm_x_bcs (boundary.cpp:1063-1081), x only
+                if d == 0:
+                    out[(f, d, 0)] = ((0, KEEP, KEEP), 0.0)     # dm(-1) = dm(0)
+                    out[(f, d, 1)] = ((n, KEEP, KEEP), 0.0)     # dm(n+1) = dm(n)
+                continue
+            if f in ("vx", "vy", "vz", "vx1", "vy1", "vz1") and d == 0:
+                # velocities: x walls set to 0 over -2..0 and n..n+2
+                out[(f, d, 0)] = ((VALUE, VALUE, VALUE), 0.0)
+                out[(f, d, 1)] = ((VALUE, VALUE, VALUE), 0.0)
+                continue
+            if s:  # staggered along d: -2 <- 2, -1 <- 1; 0 and n kept
+                lo = (2, 1, KEEP)
+                hi = (KEEP, n, n - 1) if f in ("by", "bz") or (f == "bx" and d != 0) else (KEEP, n - 1, n - 2)
+            else:  # centred along d: -1 <- 2, 0 <- 1
+                lo = (2, 1, KEEP)
+                hi = (n, n - 1, KEEP) if f in ("bx", "by", "bz") else (n - 1, n - 2, KEEP)
+            out[(f, d, 0)] = (lo, 0.0)
+            out[(f, d, 1)] = (hi, 0.0)
+    return out
+
+
+def apply_lare_bc(grid, nx, ny, nz):
+    """Attach lare_bc_rules to a device grid or an oracle grid (any object
+    with set_bc / set_bc_map); the dm edges along y and z get none."""
+    rules = lare_bc_rules(nx, ny, nz)
+    for f in O.FIELDS:
+        for d in range(3):
+            for e in (0, 1):
+                if (f, d, e) in rules:
+                    src, val = rules[(f, d, e)]
+                    grid.set_bc_map(f, d, e, src, val)
+                else:
+                    grid.set_bc(f, d, e, 0)

@@ -37,7 +37,7 @@ def test_dropin_reproduces_reference(name):
         meta = O.run_reference(td, problem=c["problem"], nx=c["nx"], ny=c["ny"], nz=c["nz"],
                                steps=c["steps"], dumps=dumps, seed=c["seed"], amp=c["amp"],
                                noise=c["noise"], bcx=c["bcx"] or None, threads=2,
-                               exe=O.DROPIN_DRIVER, rke=c.get("rke", False))
+                               exe=O.DROPIN_DRIVER, rke=c.get("rke", False), lare_bc=c.get("lare_bc", False))
         assert meta["dt"] == c["dt"], "dt sequence"
         for s in dumps:
             for f in FIELDS:
@@ -82,7 +82,7 @@ def test_dropin_reads_the_attached_boundary_conditions(problem, bcx):
     got = {}
     for line in r.stderr.splitlines():
         if line.startswith("LARE3D (MI355X) bc "):
-            name, dim, edge, kind, value = line.split()[3:]
+            name, dim, edge, kind, value = line.split()[3:8]
             got[(name, int(dim), int(edge))] = (int(kind), float(value))
     names = ["rho", "energy_electron", "energy_ion", "vx", "vy", "vz", "bx", "by", "bz",
              "LARE/vx1", "LARE/vy1", "LARE/vz1", "LARE/dm"]
@@ -97,3 +97,43 @@ def test_dropin_reads_the_attached_boundary_conditions(problem, bcx):
             want[(n, 1, e)] = (L.BC_PERIODIC, 0.0)
             want[(n, 2, e)] = (L.BC_PERIODIC, 0.0)
     assert got == want
+
+
+def test_dropin_reads_lare_style_conditions():
+    """The LARE initial-condition package's BC_OTHER conditions
+    (LAREBoundaryConditions over src/InitialConditions/LARE/boundary.cpp) are
+    identified layer by layer: copies of other layers and untouched layers as
+    map edges, the velocities' walls (every ghost layer 0) as a clamp to 0,
+    dm's y and z edges (its rule acts along x only) as no condition.
+    Host-only, like the test above."""
+    import subprocess
+    import sams_amd.lare as L
+    from lare_testlib import lare_bc_rules
+    if not os.path.exists(O.DROPIN_DRIVER):
+        pytest.skip("oracle/_ref/lare3d_dropin not built (needs the reference tree)")
+    nx, ny, nz = 12, 5, 4
+    args = [O.DROPIN_DRIVER, "--problem", "ot", "--nx", str(nx), "--ny", str(ny), "--nz", str(nz), "--steps", "1",
+            "--lare-bc"]
+    with tempfile.TemporaryDirectory() as td:
+        r = subprocess.run(args + ["--out", td], cwd=td, capture_output=True, text=True,
+                           env=dict(os.environ, LARE3D_HIP_REPORT_BCS="1", OMP_NUM_THREADS="2"))
+    got = {}
+    for line in r.stderr.splitlines():
+        if line.startswith("LARE3D (MI355X) bc "):
+            name, dim, edge, kind, value, s0, s1, s2 = line.split()[3:]
+            got[(name.replace("LARE/", ""), int(dim), int(edge))] = (int(kind), float(value), (int(s0), int(s1),
+                                                                                               int(s2)))
+    assert got, r.stderr[-2000:]
+    rules = lare_bc_rules(nx, ny, nz)
+    for f in O.FIELDS:
+        for d in range(3):
+            for e in (0, 1):
+                if (f, d, e) not in rules:
+                    assert got[(f, d, e)][0] == L.BC_NONE, (f, d, e)
+                    continue
+                src, val = rules[(f, d, e)]
+                nl = 3 if L.STAGGER[f][d] else 2
+                if all(s == L.MAP_VALUE for s in src[:nl]):
+                    assert got[(f, d, e)][:2] == (L.BC_CLAMP, val), (f, d, e)
+                else:
+                    assert got[(f, d, e)] == (L.BC_MAP, val, tuple(src)), (f, d, e)

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import sams_amd.lare as L
-from lare_testlib import bit_diff, case_problem, goldens, make_oracle, sha
+from lare_testlib import bit_diff, case_bcs, case_problem, goldens, make_oracle, sha
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -27,13 +27,14 @@ def _state_diffs(dev, orc, fields=L.STATE_FIELDS):
 
 
 def _skip_set0_rke(c, kernel_set):
-    if kernel_set == 0 and c.get("rke"):
-        pytest.skip("the kinetic-energy correction runs on kernel sets 1-3 only")
+    if kernel_set == 0 and (c.get("rke") or c.get("lare_bc")):
+        pytest.skip("the kinetic-energy correction and map conditions run on kernel sets 1-3 only")
 
 
 @pytest.mark.parametrize("kernel_set", [0, 1, 2])
 @pytest.mark.parametrize("name", ["ot_pert_10x8x12", "bw_zerograd_24x4x6", "bw_walls_16x5x3",
-                                  "bw_mirror_16x4x4", "ot_rke_12", "bw_rke_mirror_16x5x4"])
+                                  "bw_mirror_16x4x4", "ot_rke_12", "bw_rke_mirror_16x5x4", "ot_larebc_rke_12",
+                                  "bw_larebc_16x5x4"])
 def test_events_of_first_two_steps_match_oracle(name, kernel_set):
     """Event by event (startOfTimestep, halfTimestep, endOfTimestep) for two
     steps: dt, the half-step velocities and every state field; for the
@@ -44,7 +45,8 @@ def test_events_of_first_two_steps_match_oracle(name, kernel_set):
     p = case_problem(CASES[name])
     dev = L.Lare.from_problem(p)
     dev.set_kernel_set(kernel_set)
-    orc = make_oracle(p)
+    case_bcs(CASES[name], dev)
+    orc = case_bcs(CASES[name], make_oracle(p))
     for step in (1, 2):
         dt_d = dev.start_of_timestep()
         dt_o = orc.start_of_timestep()
@@ -69,6 +71,7 @@ def test_reference_goldens(name, kernel_set):
     _skip_set0_rke(c, kernel_set)
     dev = L.Lare.from_problem(case_problem(c))
     dev.set_kernel_set(kernel_set)
+    case_bcs(c, dev)
     snaps = c["snapshots"]
     done = 0
     for s in sorted(int(k) for k in snaps):

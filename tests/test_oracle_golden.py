@@ -4,7 +4,7 @@ in place).  Bit-exact: same dt sequence and same SHA-256 of every state field,
 ghosts included, at every recorded step."""
 import pytest
 
-from lare_testlib import case_problem, goldens, make_oracle, sha
+from lare_testlib import case_bcs, case_problem, goldens, make_oracle, sha
 from oracle import oracle as O
 
 CASES = goldens()
@@ -13,7 +13,7 @@ CASES = goldens()
 @pytest.mark.parametrize("name", [n for n in sorted(CASES) if CASES[n]["nx"] <= 32])
 def test_oracle_matches_reference_goldens(name):
     c = CASES[name]
-    o = make_oracle(case_problem(c))
+    o = case_bcs(c, make_oracle(case_problem(c)))
     snaps = c["snapshots"]
     if "0" in snaps:
         for f in O.STATE_FIELDS:
@@ -31,7 +31,7 @@ def test_oracle_config0_64cube_100_steps():
     """Orszag-Tang 64^3, 100 steps: the whole dt sequence and the final state."""
     c = CASES["ot_64_config0"]
     O.set_threads(8)
-    o = make_oracle(case_problem(c))
+    o = case_bcs(c, make_oracle(case_problem(c)))
     for s in range(1, c["steps"] + 1):
         dt = o.step()
         assert dt.hex() == float.fromhex(c["dt"][s - 1]).hex(), f"dt of step {s}"
