@@ -137,3 +137,33 @@ def test_dropin_reads_lare_style_conditions():
                     assert got[(f, d, e)][:2] == (L.BC_CLAMP, val), (f, d, e)
                 else:
                     assert got[(f, d, e)] == (L.BC_MAP, val, tuple(src)), (f, d, e)
+
+
+DROPIN_H5 = os.path.join(os.path.dirname(O.DROPIN_DRIVER), "lare3d_dropin_h5")
+REF_H5 = os.path.join(os.path.dirname(O.REF_DRIVER), "ref_driver_h5")
+H5DIFF = "/opt/conda/bin/h5diff"
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not (os.path.exists(DROPIN_H5) and os.path.exists(REF_H5) and os.path.exists(H5DIFF)),
+                    reason="HDF5 builds of the reference driver and the drop-in, or h5diff, not present")
+@pytest.mark.parametrize("name", ["ot_pert_10x8x12", "bw_rke_zerograd_24x4x6", "ot_larebc_rke_12"])
+def test_dropin_hdf5_output_matches_reference(name):
+    """The runner's output files (diagnostics_NNNNN.h5 through the reference's
+    HDF5 writer: the mesh and every LARE dataset of diagnostics.cpp) of the
+    drop-in are identical to the reference's own, at the start and after the
+    last step (h5diff, exact): the device state reaches the harness arrays
+    before every output."""
+    import subprocess
+    c = CASES[name]
+    with tempfile.TemporaryDirectory() as td:
+        for exe, sub in ((REF_H5, "ref"), (DROPIN_H5, "dropin")):
+            out = os.path.join(td, sub)
+            O.run_reference(out, problem=c["problem"], nx=c["nx"], ny=c["ny"], nz=c["nz"], steps=c["steps"],
+                            seed=c["seed"], amp=c["amp"], noise=c["noise"], bcx=c["bcx"] or None,
+                            threads=1, exe=exe, rke=c.get("rke", False), lare_bc=c.get("lare_bc", False))
+        for k in (0, 1):
+            f = f"diagnostics_{k:05d}.h5"
+            r = subprocess.run([H5DIFF, os.path.join(td, "ref", f), os.path.join(td, "dropin", f)],
+                               capture_output=True, text=True)
+            assert r.returncode == 0, f"{f}: {r.stdout[-2000:]} {r.stderr[-500:]}"
