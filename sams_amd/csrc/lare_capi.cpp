@@ -423,8 +423,8 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
 {
     if (!d || !out) return fail(-1, "lare_ctx_create: null argument");
     *out = nullptr;
-    if (d->nx < 3 || d->ny < 3 || d->nz < 3)
-        return fail(-1, "lare_ctx_create: each of nx, ny, nz must be at least 3");
+    if (d->nx < 2 || d->ny < 2 || d->nz < 2)
+        return fail(-1, "lare_ctx_create: each of nx, ny, nz must be at least 2");
     if (d->precision != LARE_FP64 && d->precision != LARE_FP32_STORAGE)
         return fail(-1, "lare_ctx_create: unknown storage precision");
     long zp = 8;  // row pitch rounded to this many elements (LARE_ZPITCH)

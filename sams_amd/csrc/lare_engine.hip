@@ -2897,8 +2897,10 @@ KERNEL void e_bc(const EDev<T> *devs, const lare::BCSetE *sets, const int *boff,
         }
     }
     // z lines (one scattered row end per lane) gain from overlapping their
-    // loads; x and y lines are coalesced across lanes already
-    if (V.src || dim != 2) {  // in the fills' own order (after the carry-over)
+    // loads; x and y lines are coalesced across lanes already.  Lines of
+    // two cells (n = 2) keep the edges' own order: there a fill can read a
+    // layer the other edge has just written.
+    if (V.src || dim != 2 || n < 3) {  // in the fills' own order (after the carry-over)
         if (V.f64) {
             bc_edge_e((double *)V.p, c0, sd, n, V.sd, 0, V.k0, V.v0, V.m0);
             bc_edge_e((double *)V.p, c0, sd, n, V.sd, 1, V.k1, V.v1, V.m1);
@@ -3448,6 +3450,7 @@ Group::BTab &btable3(Group &gp, int vg, int subset)
                 none_edge = none_edge || !edge_writes_all(g, u, d, 0) || !edge_writes_all(g, u, d, 1);
                 if (b.s[d] && b.k0[d] == 1 && b.k1[d] == 4) ok = false;
                 if (map_reads_written(g, u, d, n3[d])) ok = false;
+                if (n3[d] < 3) ok = false;  // two-cell lines: fills read layers the other edge writes
                 if (b.k0[d] || b.k1[d]) t.any = true;
             }
             if (src >= 0 && none_edge) ok = false;

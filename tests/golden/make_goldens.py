@@ -50,6 +50,11 @@ CASES = [
     ("ot_larebc_rke_12", "ot", 12, 12, 12, 10, (0, 1, 10), {"amp": 0.05, "seed": 7, "rke": True, "lare_bc": True}),
     ("ot_larebc_10x8x12", "ot", 10, 8, 12, 12, (0, 12), {"amp": 0.02, "seed": 2, "lare_bc": True}),
     ("bw_larebc_16x5x4", "bw", 16, 5, 4, 10, (0, 10), {"noise": 0.01, "seed": 3, "lare_bc": True}),
+    # two-cell dimensions, as the reference's quasi-2D initial-condition packages use (nz = 2, or ny = nz = 2)
+    ("ot_2d_14x12x2", "ot", 14, 12, 2, 12, (0, 1, 12), {"amp": 0.05, "seed": 3}),
+    ("bw_mirror_32x2x2", "bw", 32, 2, 2, 20, (0, 20), {"bcx": "mirror", "noise": 0.01, "seed": 5}),
+    ("bw_rke_2d_20x6x2", "bw", 20, 6, 2, 12, (0, 12), {"noise": 0.02, "seed": 6, "rke": True}),
+    ("ot_larebc_12x10x2", "ot", 12, 10, 2, 10, (0, 10), {"amp": 0.05, "seed": 8, "lare_bc": True}),
 ]
 
 
