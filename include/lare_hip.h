@@ -189,6 +189,13 @@ int lare_profile_get(lare_ctx *ctx, int32_t i, char *name, int32_t name_len, int
  * out[3] their sum.  Summed in a fixed order (reproducible). */
 int lare_energy(lare_ctx *ctx, double *out);
 
+/* Quotients as the kernels form them (no counterpart in the reference: a
+ * self-check of the device's FP64 division, for tests).  q[i] = a[i] / d[i]
+ * rounded to nearest even, for host arrays of n values, computed on the
+ * device by the division every kernel uses (mode 0) or by the
+ * shared-divisor quotient (mode 1).  Blocking; allocates device memory. */
+int lare_div_check(const double *a, const double *d, double *q, int64_t n, int32_t mode);
+
 /* ------------------------------------------------------------------------
  * Batches (no counterpart in the reference, whose runner steps one
  * simulationData at a time): contexts of one storage precision, of any

@@ -1049,6 +1049,17 @@ extern "C" int lare_energy(lare_ctx *c, double *out)
     return 0;
 }
 
+extern "C" int lare_div_check(const double *a, const double *d, double *q, int64_t n, int32_t mode)
+{
+    if (n < 0 || (n > 0 && (!a || !d || !q))) return fail(-1, "lare_div_check: null argument");
+    if (mode != 0 && mode != 1) return fail(-1, "lare_div_check: mode must be 0 or 1");
+    if (n > (int64_t(1) << 31)) return fail(-1, "lare_div_check: at most 2^31 values");
+    std::string err;
+    int rc = lare::div_check(a, d, q, (long)n, mode, err);
+    if (rc) return fail(rc, err);
+    return 0;
+}
+
 extern "C" int lare_autotune(lare_ctx *c, int32_t steps, lare_tune *out)
 {
     if (!c) return fail(-1, "lare_autotune: null context");

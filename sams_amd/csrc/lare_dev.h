@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "lare_div.h"
+
 namespace lare {
 
 struct Geo {
@@ -115,7 +117,7 @@ struct Ref {
     __device__ __forceinline__ const Ref &operator+=(double v) const { return *this = (double)*q + v; }
     __device__ __forceinline__ const Ref &operator-=(double v) const { return *this = (double)*q - v; }
     __device__ __forceinline__ const Ref &operator*=(double v) const { return *this = (double)*q * v; }
-    __device__ __forceinline__ const Ref &operator/=(double v) const { return *this = (double)*q / v; }
+    __device__ __forceinline__ const Ref &operator/=(double v) const { return *this = ddiv((double)*q, v); }
 };
 
 template <class T>

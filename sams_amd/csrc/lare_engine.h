@@ -92,4 +92,8 @@ inline void group_step(Group &gp)
 // domain cells and vertices in a fixed order.
 int group_energy(Group &gp, std::vector<double> &out, std::string &err);
 
+// q[i] = a[i] / d[i] for host arrays of n values, computed on the device by
+// the kernels' division (mode 0) or shared-divisor quotient (mode 1).
+int div_check(const double *a, const double *d, double *q, long n, int mode, std::string &err);
+
 }  // namespace lare

@@ -218,27 +218,27 @@ __device__ __forceinline__ void prep_out(const ArrE<T> &A, int ix, int iy, int i
     const double azp = DZAB(ix, iy), azm = DZAB(ix, iy);
     double dv;
     if (DIR == 0) {
-        double dvxdx = (vxb * axp - vxbm * axm) / vol;
-        double dvydy = ypass * (vyb * ayp - vybm * aym) / vol;
-        double dvzdz = zpass * (vzb * azp - vzbm * azm) / vol;
+        double dvxdx = ddiv(vxb * axp - vxbm * axm, vol);
+        double dvydy = ddiv(ypass * (vyb * ayp - vybm * aym), vol);
+        double dvzdz = ddiv(zpass * (vzb * azp - vzbm * azm), vol);
         dv = (dvydy + dvzdz) * dt;
         out[1] = vol * (1.0 + dv);
         dv = dv + dvxdx * dt;
         out[0] = vol * (1.0 + dv);
         out[2] = DXB(ix) + (vxb - vxbm) * dt;
     } else if (DIR == 1) {
-        double dvxdx = xpass * (vxb * axp - vxbm * axm) / vol;
-        double dvydy = (vyb * ayp - vybm * aym) / vol;
-        double dvzdz = zpass * (vzb * azp - vzbm * azm) / vol;
+        double dvxdx = ddiv(xpass * (vxb * axp - vxbm * axm), vol);
+        double dvydy = ddiv(vyb * ayp - vybm * aym, vol);
+        double dvzdz = ddiv(zpass * (vzb * azp - vzbm * azm), vol);
         dv = (dvxdx + dvzdz) * dt;
         out[1] = vol * (1.0 + dv);
         dv = dv + dvydy * dt;
         out[0] = vol * (1.0 + dv);
         out[2] = DYB(iy) + (vyb - vybm) * dt;
     } else {
-        double dvxdx = xpass * (vxb * axp - vxbm * axm) / vol;
-        double dvydy = ypass * (vyb * ayp - vybm * aym) / vol;
-        double dvzdz = (vzb * azp - vzbm * azm) / vol;
+        double dvxdx = ddiv(xpass * (vxb * axp - vxbm * axm), vol);
+        double dvydy = ddiv(ypass * (vyb * ayp - vybm * aym), vol);
+        double dvzdz = ddiv(vzb * azp - vzbm * azm, vol);
         dv = (dvxdx + dvydy) * dt;
         out[1] = vol * (1.0 + dv);
         dv = dv + dvzdz * dt;
@@ -355,7 +355,7 @@ __device__ __forceinline__ double ct_flux_v(double fm, double fi, double fp, dou
     double dxcu = cvc_m * vad_p + cvc_p * vad_m;
     double dxbu = cv1_0 * vad_p + cv1_p * vad_m;
     double dxu = dbx * vad_p + dbxp * vad_m;
-    double phi = fabs(o_v) / dxu;
+    double phi = ddiv(fabs(o_v), dxu);
     double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
     return (fu + Di * (1.0 - phi)) * o_v;
 }
@@ -382,10 +382,10 @@ __device__ __forceinline__ double ct_flux(const ArrE<T> &A, const Geo &G, int c,
     double dbxp = (db1[c + sd] + db1[c + sd + tw]) * 0.5;
     double dbxp2 = (db1[c + 2 * sd] + db1[c + 2 * sd + tw]) * 0.5;
     double dbxm = (db1[c - sd] + db1[c - sd + tw]) * 0.5;
-    double fm = B(-1) / dbxm;
-    double fi = B(0) / dbx;
-    double fp = B(1) / dbxp;
-    double fp2 = B(2) / dbxp2;
+    double fm = ddiv(B(-1), dbxm);
+    double fi = ddiv(B(0), dbx);
+    double fp = ddiv(B(1), dbxp);
+    double fp2 = ddiv(B(2), dbxp2);
     return ct_flux_v(fm, fi, fp, fp2, dbx, dbxp, v_advect, dt, A.cvc1[c - sd], A.cvc1[c], A.cvc1[c + sd], A.cv1[c],
                      A.cv1[c + sd]);
 }
@@ -465,9 +465,9 @@ KERNEL void f_ct_update(const Src<T> S)
         if (bzr) bz = bz + fa[c] - fa[c - sx];
         if (x1 && y0 && z1) by = by - fb[c] + fb[c - 1];
         if (bzr) bz = bz + fb[c] - fb[c - sy];
-        if (ebx) BX[c] = bx / (b_area<0>(A, ix, iy, iz) + DBL_EPSILON);
-        if (eby) BY[c] = by / (b_area<1>(A, ix, iy, iz) + DBL_EPSILON);
-        if (ebz) BZ[c] = bz / (b_area<2>(A, ix, iy, iz) + DBL_EPSILON);
+        if (ebx) BX[c] = ddiv(bx, b_area<0>(A, ix, iy, iz) + DBL_EPSILON);
+        if (eby) BY[c] = ddiv(by, b_area<1>(A, ix, iy, iz) + DBL_EPSILON);
+        if (ebz) BZ[c] = ddiv(bz, b_area<2>(A, ix, iy, iz) + DBL_EPSILON);
     }
     };
     ECELL(cell)
@@ -506,7 +506,7 @@ __device__ __forceinline__ double mass_flux_v(double fm, double fi, double fp, d
     double dxci = cvc_0;
     double dxcu = cvc_m * vad_p + cvc_p * vad_m;
     double dxbu = cv1_0 * vad_p + cv1_p * vad_m;
-    double phi = fabs(o_v) / dxbu;
+    double phi = ddiv(fabs(o_v), dxbu);
     double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
     return (fu + Di * (1.0 - phi)) * o_v;
 }
@@ -555,10 +555,10 @@ __device__ __forceinline__ double energy_flux_v(double fm, double fi, double fp,
     double dxci = cvc_0;
     double dxcu = cvc_m * vad_p + cvc_p * vad_m;
     double dxbu = cv1_0 * vad_p + cv1_p * vad_m;
-    double phi = fabs(o_v) / dxbu;
+    double phi = ddiv(fabs(o_v), dxbu);
     double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
     double rhou = rho_0 * vad_p + rho_p * vad_m;
-    double dmu = fabs(dm) / dxbu / rhou;
+    double dmu = ddiv(ddiv(fabs(dm), dxbu), rhou);
     return (fu + Di * (1.0 - dmu)) * dm;
 }
 
@@ -598,9 +598,9 @@ __device__ __forceinline__ double mom_fl_v(double fm, double fi, double fp, doub
     double vad_m = 1.0 - vad_p;
     double fu = fi * vad_p + fp * vad_m;
     double dfu = dfm * vad_p + dfp * vad_m;
-    double phi = fabs(o_v) / dxbu;
+    double phi = ddiv(fabs(o_v), dxbu);
     double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
-    double dmu = DIR == 2 ? fabs(dm) / (dxbu * rhou) : fabs(dm) / dxbu / rhou;
+    double dmu = DIR == 2 ? ddiv(fabs(dm), dxbu * rhou) : ddiv(ddiv(fabs(dm), dxbu), rhou);
     return fu + Di * (1.0 - dmu);
 }
 template <int DIR>
@@ -629,8 +629,8 @@ __device__ __forceinline__ double rrho_v_at(const ArrE<T> &A, const Geo &G, int 
     double s = R1[c] * C1[c] + R1[c + sx] * C1[c + sx] + R1[c + sy] * C1[c + sy] +
                R1[c + sx + sy] * C1[c + sx + sy] + R1[c + 1] * C1[c + 1] + R1[c + sx + 1] * C1[c + sx + 1] +
                R1[c + sy + 1] * C1[c + sy + 1] + R1[c + sx + sy + 1] * C1[c + sx + sy + 1];
-    if (DIR == 0) return s * 0.125 / A.cvc1[c];
-    return s * (0.125 / A.cvc1[c]);
+    if (DIR == 0) return ddiv(s * 0.125, A.cvc1[c]);
+    return s * ddiv(0.125, A.cvc1[c]);
 }
 
 // Vertex mass flux at vertex c (xremap.cpp:209-219 and twins).
@@ -704,21 +704,21 @@ KERNEL void f_update_all(const Src<T> S)
                 C2[c + sx + sy + 1];
     const double cv2v = DIR == 0 ? s2 * 0.125 : 0.125 * s2;
     const double cvc1 = A.cvc1[c];
-    const double rho_v1 = (rv * cvc1 + dmvm - dmv) / cv2v;
+    const double rho_v1 = ddiv(rv * cvc1 + dmvm - dmv, cv2v);
     const double den = cv2v * rho_v1;
-    A.vx2[c] = (rv * VX[c] * cvc1 + flm[0] - flc[0]) / den;
-    A.vy2[c] = (rv * VY[c] * cvc1 + flm[1] - flc[1]) / den;
-    A.vz2[c] = (rv * VZ[c] * cvc1 + flm[2] - flc[2]) / den;
+    A.vx2[c] = ddiv(rv * VX[c] * cvc1 + flm[0] - flc[0], den);
+    A.vy2[c] = ddiv(rv * VY[c] * cvc1 + flm[1] - flc[1], den);
+    A.vz2[c] = ddiv(rv * VZ[c] * cvc1 + flm[2] - flc[2], den);
     if (ix >= 1 && iy >= 1 && iz >= 1) {
         const double em[2] = {energy_flux_face<DIR>(A, G, c - sd, mx, my, mz, EE, dt),
                               energy_flux_face<DIR>(A, G, c - sd, mx, my, mz, EI, dt)};
         const double ec[2] = {energy_flux_face<DIR>(A, G, c, ix, iy, iz, EE, dt),
                               energy_flux_face<DIR>(A, G, c, ix, iy, iz, EI, dt)};
         const double rho1 = RHO[c], cv1 = A.cv1[c], cv2 = C2[c];
-        const double rho = (rho1 * cv1 + DM[c - sd] - DM[c]) / cv2;
+        const double rho = ddiv(rho1 * cv1 + DM[c - sd] - DM[c], cv2);
         A.rho2[c] = rho;
-        A.ee2[c] = (EE[c] * cv1 * rho1 + em[0] - ec[0]) / (cv2 * rho);
-        A.ei2[c] = (EI[c] * cv1 * rho1 + em[1] - ec[1]) / (cv2 * rho);
+        A.ee2[c] = ddiv(EE[c] * cv1 * rho1 + em[0] - ec[0], cv2 * rho);
+        A.ei2[c] = ddiv(EI[c] * cv1 * rho1 + em[1] - ec[1], cv2 * rho);
     }
     };
     ECELL(cell)
@@ -763,8 +763,8 @@ KERNEL void r_ke_dk(const Src<T> S)
                     C2[q + sx + sy + 1];
         cv2v[k] = DIR == 0 ? s2 * 0.125 : 0.125 * s2;
     }
-    const double rho_v1_0 = (rv[1] * A.cvc1[c] + dmv[0] - dmv[1]) / cv2v[0];
-    const double rho_v1_p = (rv[2] * A.cvc1[c + sd] + dmv[1] - dmv[2]) / cv2v[1];
+    const double rho_v1_0 = ddiv(rv[1] * A.cvc1[c] + dmv[0] - dmv[1], cv2v[0]);
+    const double rho_v1_p = ddiv(rv[2] * A.cvc1[c + sd] + dmv[1] - dmv[2], cv2v[1]);
     const double m = rho_v1_0 * cv2v[0];
     const double mp = rho_v1_p * cv2v[1];
     double flm[3], fl0[3], flp[3];
@@ -775,8 +775,8 @@ KERNEL void r_ke_dk(const Src<T> S)
     double *const out[3] = {A.flux_x, A.flux_y, A.flux_z};
     for (int k = 0; k < 3; ++k) {
         const double f0 = V3[k][c], fp = V3[k][c + sd];
-        double ai = (f0 - flm[k]) * dmv[0] / m - (f0 - fl0[k]) * dmv[1] / m;
-        double aip = (fp - fl0[k]) * dmv[1] / mp - (fp - flp[k]) * dmv[2] / mp;
+        double ai = ddiv((f0 - flm[k]) * dmv[0], m) - ddiv((f0 - fl0[k]) * dmv[1], m);
+        double aip = ddiv((fp - fl0[k]) * dmv[1], mp) - ddiv((fp - flp[k]) * dmv[2], mp);
         double dk = (fp - f0) * (fl0[k] - 0.5 * (fp + f0)) - 0.5 * ai * (f0 - fl0[k]) + 0.5 * aip * (fp - fl0[k]);
         dk = dk * dmv[1] * 0.5;
         lare::gbl(out[k])[c] = dk;
@@ -834,7 +834,7 @@ KERNEL void r_ke_energy(const Src<T> S)
     EGRID
     auto cell = [&](const int ix, const int iy, const int iz) {
     const int c = IX(ix, iy, iz);
-    const double dke = pmax(-lare::gbl(A.dke)[c], 0.0) / (RHO[c] * CV(ix, iy, iz));
+    const double dke = ddiv(pmax(-lare::gbl(A.dke)[c], 0.0), RHO[c] * CV(ix, iy, iz));
     EE[c] += 0.5 * dke;
     EI[c] += 0.5 * dke;
     };
@@ -915,7 +915,7 @@ KERNEL void f_update_march(const Src<T> S)
         P0[j] = R1[c + tj[j]] * C1[c + tj[j]];
         PL[j] = R1[c + sd + tj[j]] * C1[c + sd + tj[j]];
     }
-    double rv_c = DIR == 0 ? sum8<DIR>(P0, PL) * 0.125 / cvc_0 : sum8<DIR>(P0, PL) * (0.125 / cvc_0);
+    double rv_c = DIR == 0 ? ddiv(sum8<DIR>(P0, PL) * 0.125, cvc_0) : sum8<DIR>(P0, PL) * ddiv(0.125, cvc_0);
     double dmv_m = 0.0, flm[3] = {0.0, 0.0, 0.0}, em[2] = {0.0, 0.0};
     for (int k = -1; k < K; ++k) {
         const int d = d0 + k;
@@ -941,7 +941,7 @@ KERNEL void f_update_march(const Src<T> S)
         }
         // vertex density at c+sd, vertex mass flux at c
         const double sp = sum8<DIR>(PL, PH);
-        const double rv_p = DIR == 0 ? sp * 0.125 / cvc_p : sp * (0.125 / cvc_p);
+        const double rv_p = DIR == 0 ? ddiv(sp * 0.125, cvc_p) : sp * ddiv(0.125, cvc_p);
         const double q = sum8<DIR>(DML, DMH);
         const double dmv_c = DIR == 0 ? q * 0.125 : 0.125 * q;
         // momentum fluxes through vertex face c
@@ -963,11 +963,11 @@ KERNEL void f_update_march(const Src<T> S)
         if (k >= 0) {
             const double s2 = sum8<DIR>(C2L, C2H);
             const double cv2v = DIR == 0 ? s2 * 0.125 : 0.125 * s2;
-            const double rho_v1 = (rv_c * cvc_0 + dmv_m - dmv_c) / cv2v;
+            const double rho_v1 = ddiv(rv_c * cvc_0 + dmv_m - dmv_c, cv2v);
             const double den = cv2v * rho_v1;
-            A.vx2[c] = (rv_c * F[0][1] * cvc_0 + flm[0] - flc[0]) / den;
-            A.vy2[c] = (rv_c * F[1][1] * cvc_0 + flm[1] - flc[1]) / den;
-            A.vz2[c] = (rv_c * F[2][1] * cvc_0 + flm[2] - flc[2]) / den;
+            A.vx2[c] = ddiv(rv_c * F[0][1] * cvc_0 + flm[0] - flc[0], den);
+            A.vy2[c] = ddiv(rv_c * F[1][1] * cvc_0 + flm[1] - flc[1], den);
+            A.vz2[c] = ddiv(rv_c * F[2][1] * cvc_0 + flm[2] - flc[2], den);
         }
         // energy fluxes through cell face c, and the cell update
         double ec[2] = {0.0, 0.0};
@@ -978,10 +978,10 @@ KERNEL void f_update_march(const Src<T> S)
                                       rho_0, rho_p, DML[0]);
             if (k >= 0 && d >= 1) {
                 const double cv2 = C2L[0];
-                const double rho = (rho_0 * cv1_0 + dm_m - DML[0]) / cv2;
+                const double rho = ddiv(rho_0 * cv1_0 + dm_m - DML[0], cv2);
                 A.rho2[c] = rho;
-                A.ee2[c] = (E[0][1] * cv1_0 * rho_0 + em[0] - ec[0]) / (cv2 * rho);
-                A.ei2[c] = (E[1][1] * cv1_0 * rho_0 + em[1] - ec[1]) / (cv2 * rho);
+                A.ee2[c] = ddiv(E[0][1] * cv1_0 * rho_0 + em[0] - ec[0], cv2 * rho);
+                A.ei2[c] = ddiv(E[1][1] * cv1_0 * rho_0 + em[1] - ec[1], cv2 * rho);
             }
         }
         // slide the window one cell along DIR
@@ -1141,7 +1141,7 @@ KERNEL void f_update_zline(const Src<T> S)
         q = q + lane_next(D4[j]);
         s2 = s2 + lane_next(S4[j]);
     }
-    const double rv = sp * (0.125 / cvc_0);
+    const double rv = sp * ddiv(0.125, cvc_0);
     const double dmv = 0.125 * q;
     const double cv2v = 0.125 * s2;
     // neighbours along z
@@ -1181,17 +1181,17 @@ KERNEL void f_update_zline(const Src<T> S)
     const double em[2] = {lane_prev(ec[0]), lane_prev(ec[1])};
     const double dmv_m = lane_prev(dmv), dm_m = lane_prev(D4[0]);
     if (!writer) return;
-    const double rho_v1 = (rv * cvc_0 + dmv_m - dmv) / cv2v;
+    const double rho_v1 = ddiv(rv * cvc_0 + dmv_m - dmv, cv2v);
     const double den = cv2v * rho_v1;
-    A.vx2[c] = (rv * F[0] * cvc_0 + flm[0] - flc[0]) / den;
-    A.vy2[c] = (rv * F[1] * cvc_0 + flm[1] - flc[1]) / den;
-    A.vz2[c] = (rv * F[2] * cvc_0 + flm[2] - flc[2]) / den;
+    A.vx2[c] = ddiv(rv * F[0] * cvc_0 + flm[0] - flc[0], den);
+    A.vy2[c] = ddiv(rv * F[1] * cvc_0 + flm[1] - flc[1], den);
+    A.vz2[c] = ddiv(rv * F[2] * cvc_0 + flm[2] - flc[2], den);
     if (ix >= 1 && iy >= 1 && iz >= 1) {
         const double cv2 = S4[0];
-        const double rho = (rho_0 * cv1_0 + dm_m - D4[0]) / cv2;
+        const double rho = ddiv(rho_0 * cv1_0 + dm_m - D4[0], cv2);
         A.rho2[c] = rho;
-        A.ee2[c] = (E[0] * cv1_0 * rho_0 + em[0] - ec[0]) / (cv2 * rho);
-        A.ei2[c] = (E[1] * cv1_0 * rho_0 + em[1] - ec[1]) / (cv2 * rho);
+        A.ee2[c] = ddiv(E[0] * cv1_0 * rho_0 + em[0] - ec[0], cv2 * rho);
+        A.ei2[c] = ddiv(E[1] * cv1_0 * rho_0 + em[1] - ec[1], cv2 * rho);
     }
 }
 
@@ -1236,7 +1236,7 @@ KERNEL void f_ct_march(const Src<T> S)
         db = (db1[p] + db1[p + tw]) * 0.5;
         double b = Bf[p];
         if (DIR == 0) b = b * (a ? area_a(d) : area_b(d));
-        return b / db;
+        return ddiv(b, db);
     };
     int c = DIR == 0 ? IX(d0, ps, iz) : IX(ps, d0, iz);
     double fa[3], fb[3], da[2], db[2], dtmp;
@@ -1310,7 +1310,7 @@ KERNEL void f_cty_mass(const Src<T> S)
     // f = B / db at y (bx flux: tw = sx; bz flux: tw = 1)
     auto fval = [&](const Fld<T> Bf, int p, int tw, double &db) -> double {
         db = (db1[p] + db1[p + tw]) * 0.5;
-        return (double)Bf[p] / db;
+        return ddiv((double)Bf[p], db);
     };
     double fa[3], fb[3], da[2], db[2], dtmp;
     fa[0] = fval(Ba, at(d0 - 1), sx, dtmp);
@@ -1400,7 +1400,7 @@ KERNEL void f_ct_zline(const Src<T> S)
     const double d0 = db1[c];
     // flux a: bx (tw = sx, ow = -sy); flux b: by (tw = sy, ow = -sx)
     const double dba = (d0 + db1[c + sx]) * 0.5, dbb = (d0 + db1[c + sy]) * 0.5;
-    const double fa = BX[c] / dba, fb = BY[c] / dbb;
+    const double fa = ddiv(BX[c], dba), fb = ddiv(BY[c], dbb);
     const double w0 = W[c];
     const double va = (w0 + W[c - sy]) * 0.5, vb = (w0 + W[c - sx]) * 0.5;
     const double cvc_0 = A.cvc1[c], cv1_0 = A.cv1[c];
@@ -1451,10 +1451,10 @@ KERNEL OCC(3) void f_ctx_fused(const Src<T> S)
         db = (d0 + dbz) * 0.5;
         const double dxb = DXB(x);
         sa = BY[c] * (dxb * dzb);
-        fa = sa / da;
-        fm = BY[IX(x, ym, iz)] * (dxb * dzb) / dm;
+        fa = ddiv(sa, da);
+        fm = ddiv(BY[IX(x, ym, iz)] * (dxb * dzb), dm);
         sb = BZ[c] * (dxb * dyb);
-        fb = sb / db;
+        fb = ddiv(sb, db);
     };
     double fa[3], da[2], sa[2], fm[3], dmm[2], fb[3], dbb[2], sb[2], t[5];
     col(Z.ix0 - 1, fa[0], t[0], t[1], fm[0], t[2], fb[0], t[3], t[4]);  // distinct outputs: col writes them in turn
@@ -1570,7 +1570,7 @@ KERNEL OCC(4) void f_ctz_fused(const Src<T> S)
     // the z flux of f = B / db at this lane (ct_flux<2>: fm..fp2 and db,
     // dbp from the lanes at z-1..z+2, v from w and its offset column)
     auto zflux = [&](double b, double d, double w, double wo, double cvc, double cv1) -> double {
-        const double f = b / d;
+        const double f = ddiv(b, d);
         const double fp = lane_next(f);
         return ct_flux_v(lane_prev(f), f, fp, lane_next(fp), d, lane_next(d), (w + wo) * 0.5, dt, lane_prev(cvc), cvc,
                          lane_next(cvc), cv1, lane_next(cv1));
@@ -1617,9 +1617,9 @@ KERNEL OCC(4) void f_ctz_fused(const Src<T> S)
             if (bzr) bz = bz + Fa - Fa_x;
             if (x1 && y0 && z1) by = by - Fb + Fbz;
             if (bzr) bz = bz + Fb - Fbm;
-            if (ebx) BX[c] = bx / (b_area<0>(A, ix, iy, iz) + DBL_EPSILON);
-            if (eby) BY[c] = by / (b_area<1>(A, ix, iy, iz) + DBL_EPSILON);
-            if (ebz) BZ[c] = bz / (b_area<2>(A, ix, iy, iz) + DBL_EPSILON);
+            if (ebx) BX[c] = ddiv(bx, b_area<0>(A, ix, iy, iz) + DBL_EPSILON);
+            if (eby) BY[c] = ddiv(by, b_area<1>(A, ix, iy, iz) + DBL_EPSILON);
+            if (ebz) BZ[c] = ddiv(bz, b_area<2>(A, ix, iy, iz) + DBL_EPSILON);
         }
         Fa_x = Fa;
         w_xm = w0;
@@ -1677,7 +1677,7 @@ KERNEL void g_cs(const Src<T> S)
     if (P.keep_pi) A.p_i[c] = pi;
     double rmin = pmax(RHO[c], DBL_EPSILON);
     double b2 = bx1 * bx1 + by1 * by1 + bz1 * bz1;
-    A.cs[c] = sqrt((gas_gamma * p + b2) / rmin);
+    A.cs[c] = sqrt(ddiv(gas_gamma * p + b2, rmin));
     };
     ECELL(cell)
 }
@@ -1702,11 +1702,11 @@ KERNEL void g_vertex(const Src<T> S)
     double sum_rho_cv = r[c] * c000 + r[c + sx] * c100 + r[c + sy] * c010 + r[c + sx + sy] * c110 + r[c + 1] * c001 +
                         r[c + sx + 1] * c101 + r[c + sy + 1] * c011 + r[c + sx + sy + 1] * c111;
     double sum_cv = c000 + c100 + c010 + c110 + c001 + c101 + c011 + c111;
-    A.rho_v[c] = sum_rho_cv / sum_cv;
+    A.rho_v[c] = ddiv(sum_rho_cv, sum_cv);
     double cv_v = 0.125 * sum_cv;
     double sum = s[c] * c000 + s[c + sx] * c100 + s[c + sy] * c010 + s[c + sx + sy] * c110 + s[c + 1] * c001 +
                  s[c + sx + 1] * c101 + s[c + sy + 1] * c011 + s[c + sx + sy + 1] * c111;
-    A.cs_v[c] = 0.125 * sum / cv_v;
+    A.cs_v[c] = ddiv(0.125 * sum, cv_v);
     };
     ECELL(cell)
 }
@@ -1720,23 +1720,23 @@ __device__ __forceinline__ double edge_visc_v(double dt, double visc1, double vi
                                               const double *v3)
 {
     dvdots = pmin(0.0, dvdots);
-    double rho_edge = 2.0 * rv1 * rv2 / (rv1 + rv2);
+    double rho_edge = ddiv(2.0 * rv1 * rv2, rv1 + rv2);
     double dvx = v1[0] - v2[0];
     double dvy = v1[1] - v2[1];
     double dvz = v1[2] - v2[2];
     double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
     double dv = sqrt(dv2);
     double psi = 0.0;
-    const bool small = dv * dt / dx < 1.e-14;
-    dvdots = small ? 0.0 : dvdots / dv;
+    const bool small = ddiv(dv * dt, dx) < 1.e-14;
+    dvdots = small ? 0.0 : ddiv(dvdots, dv);
     double dvxm = v0[0] - v1[0];
     double dvxp = v2[0] - v3[0];
     double dvym = v0[1] - v1[1];
     double dvyp = v2[1] - v3[1];
     double dvzm = v0[2] - v1[2];
     double dvzp = v2[2] - v3[2];
-    double rl = small ? 1.0 : (dvxp * dvx + dvyp * dvy + dvzp * dvz) * dx / (dxp * dv2);
-    double rr = small ? 1.0 : (dvxm * dvx + dvym * dvy + dvzm * dvz) * dx / (dxm * dv2);
+    double rl = small ? 1.0 : ddiv((dvxp * dvx + dvyp * dvy + dvzp * dvz) * dx, dxp * dv2);
+    double rr = small ? 1.0 : ddiv((dvxm * dvx + dvym * dvy + dvzm * dvz) * dx, dxm * dv2);
     psi = pmin_list4(0.5 * (rr + rl), 2.0 * rl, 2.0 * rr, 1.0);
     psi = pmax(0.0, psi);
     double q_k_bar = rho_edge * (visc2_norm * dv + sqrt(visc2_norm * visc2_norm * dv2 + (visc1 * cs_edge) * (visc1 * cs_edge)));
@@ -1830,7 +1830,7 @@ KERNEL OCC(4) void g_visc_dt(const Src<T> S)
             - 0.25 * dx * dy * A.alpha3[c - sx - sy] * a11
             - 0.25 * dx * dy * A.alpha3[c - sy] * a12;
         const double cv = CV(ix, iy, iz);
-        A.visc_heat[c] = vh / cv;
+        A.visc_heat[c] = ddiv(vh, cv);
         if (ix <= G.nx && iy <= G.ny && iz <= G.nz) {
             // viscous forces
             double b1 = A.alpha1[c + sy + 1] * dy * dz;
@@ -1841,8 +1841,9 @@ KERNEL OCC(4) void g_visc_dt(const Src<T> S)
             double b6 = A.alpha3[c + 1] * dx * dy;
             double cvv = 0.125 * CV8SUM(ix, iy, iz);
 #define FV(V)                                                                                   \
-    ((b1 * (V[c] - V[c - sx]) + b2 * (V[c] - V[c + sx]) + b3 * (V[c] - V[c - sy]) +            \
-      b4 * (V[c] - V[c + sy]) + b5 * (V[c] - V[c - 1]) + b6 * (V[c] - V[c + 1])) / cvv)
+    ddiv(b1 * (V[c] - V[c - sx]) + b2 * (V[c] - V[c + sx]) + b3 * (V[c] - V[c - sy]) +              \
+             b4 * (V[c] - V[c + sy]) + b5 * (V[c] - V[c - 1]) + b6 * (V[c] - V[c + 1]),             \
+         cvv)
             A.fxv[c] = FV(VX);
             A.fyv[c] = FV(VY);
             A.fzv[c] = FV(VZ);
@@ -1850,11 +1851,11 @@ KERNEL OCC(4) void g_visc_dt(const Src<T> S)
             // set_dt
             double rho0 = pmax(RHO[c], DBL_EPSILON);
             double press = PE_OF(c) + PI_OF(c);
-            double cs2 = gas_gamma * press / rho0;
-            double w1 = (BX[c] * BX[c] + BY[c] * BY[c] + BZ[c] * BZ[c]) / mu0 / rho0;
-            double c_visc2 = pv / rho0;
+            double cs2 = ddiv(gas_gamma * press, rho0);
+            double w1 = ddiv(ddiv(BX[c] * BX[c] + BY[c] * BY[c] + BZ[c] * BZ[c], mu0), rho0);
+            double c_visc2 = ddiv(pv, rho0);
             double length = pmin(pmin(dx, dy), dz);
-            double dt1 = length / (sqrt(c_visc2) + sqrt(cs2 + w1 + c_visc2));
+            double dt1 = ddiv(length, sqrt(c_visc2) + sqrt(cs2 + w1 + c_visc2));
             double ax = 0.25 * DXAB(iy, iz);
             double ay = 0.25 * DYAB(ix, iz);
             double az = 0.25 * DZAB(ix, iy);
@@ -1865,9 +1866,9 @@ KERNEL OCC(4) void g_visc_dt(const Src<T> S)
             double vzbm = (VZ[c - 1] + VZ[c - sx - 1] + VZ[c - sy - 1] + VZ[c - sx - sy - 1]) * az;
             double vzbp = (VZ[c] + VZ[c - sx] + VZ[c - sy] + VZ[c - sx - sy]) * az;
             double dvx = fabs(vxbp - vxbm), dvy = fabs(vybp - vybm), dvz = fabs(vzbp - vzbm);
-            double dt2 = cv / pmax(pmax(pmax(fabs(vxbm), fabs(vxbp)), dvx), 1.0e-10 * cv);
-            double dt3 = cv / pmax(pmax(pmax(fabs(vybm), fabs(vybp)), dvy), 1.0e-10 * cv);
-            double dt4 = cv / pmax(pmax(pmax(fabs(vzbm), fabs(vzbp)), dvz), 1.0e-10 * cv);
+            double dt2 = ddiv(cv, pmax(pmax(pmax(fabs(vxbm), fabs(vxbp)), dvx), 1.0e-10 * cv));
+            double dt3 = ddiv(cv, pmax(pmax(pmax(fabs(vybm), fabs(vybp)), dvy), 1.0e-10 * cv));
+            double dt4 = ddiv(cv, pmax(pmax(pmax(fabs(vzbm), fabs(vzbp)), dvz), 1.0e-10 * cv));
             m = pmin(pmin(pmin(dt1, dt2), dt3), dt4);
         }
         return m;
@@ -1950,7 +1951,7 @@ KERNEL OCC(3) void g_visc_dt_zx(const Src<T> S)
                         0.25 * dy * dz * al1yp * a7 - 0.25 * dx * dz * al2p_m * a8 - 0.25 * dx * dy * al3 * a9 -
                         0.25 * dx * dy * al3_m * a10 - 0.25 * dx * dy * al3y_m * a11 - 0.25 * dx * dy * al3y * a12;
             const double cv = CV(ix, iy, iz);
-            A.visc_heat[c] = vh / cv;
+            A.visc_heat[c] = ddiv(vh, cv);
             if (fz && ix <= G.nx) {
                 double b1 = al1yp * dy * dz;
                 double b2 = al1ynp * dy * dz;
@@ -1961,18 +1962,18 @@ KERNEL OCC(3) void g_visc_dt_zx(const Src<T> S)
                 double cvv = 0.125 * CV8SUM(ix, iy, iz);
                 double fv[3];
                 for (int k = 0; k < 3; ++k)
-                    fv[k] = (b1 * (O[k] - M[k]) + b2 * (O[k] - N[k]) + b3 * (O[k] - Om[k]) + b4 * (O[k] - Yp[k]) +
-                             b5 * (O[k] - pO[k]) + b6 * (O[k] - fO[k])) / cvv;
+                    fv[k] = ddiv(b1 * (O[k] - M[k]) + b2 * (O[k] - N[k]) + b3 * (O[k] - Om[k]) + b4 * (O[k] - Yp[k]) +
+                             b5 * (O[k] - pO[k]) + b6 * (O[k] - fO[k]), cvv);
                 A.fxv[c] = fv[0];
                 A.fyv[c] = fv[1];
                 A.fzv[c] = fv[2];
                 double rho0 = pmax(RHO[c], DBL_EPSILON);
                 double press = PE_OF(c) + PI_OF(c);
-                double cs2 = gas_gamma * press / rho0;
-                double w1 = (BX[c] * BX[c] + BY[c] * BY[c] + BZ[c] * BZ[c]) / mu0 / rho0;
-                double c_visc2 = pv / rho0;
+                double cs2 = ddiv(gas_gamma * press, rho0);
+                double w1 = ddiv(ddiv(BX[c] * BX[c] + BY[c] * BY[c] + BZ[c] * BZ[c], mu0), rho0);
+                double c_visc2 = ddiv(pv, rho0);
                 double length = pmin(pmin(dx, dy), dz);
-                double dt1 = length / (sqrt(c_visc2) + sqrt(cs2 + w1 + c_visc2));
+                double dt1 = ddiv(length, sqrt(c_visc2) + sqrt(cs2 + w1 + c_visc2));
                 double ay = 0.25 * DYAB(ix, iz);
                 double az = 0.25 * DZAB(ix, iy);
                 double vxbm = vxbp_m;
@@ -1981,9 +1982,9 @@ KERNEL OCC(3) void g_visc_dt_zx(const Src<T> S)
                 double vzbm = (pO[2] + pM[2] + pOm[2] + pMm[2]) * az;
                 double vzbp = (O[2] + M[2] + Om[2] + Mm[2]) * az;
                 double dvx = fabs(vxbp - vxbm), dvy = fabs(vybp - vybm), dvz = fabs(vzbp - vzbm);
-                double dt2 = cv / pmax(pmax(pmax(fabs(vxbm), fabs(vxbp)), dvx), 1.0e-10 * cv);
-                double dt3 = cv / pmax(pmax(pmax(fabs(vybm), fabs(vybp)), dvy), 1.0e-10 * cv);
-                double dt4 = cv / pmax(pmax(pmax(fabs(vzbm), fabs(vzbp)), dvz), 1.0e-10 * cv);
+                double dt2 = ddiv(cv, pmax(pmax(pmax(fabs(vxbm), fabs(vxbp)), dvx), 1.0e-10 * cv));
+                double dt3 = ddiv(cv, pmax(pmax(pmax(fabs(vybm), fabs(vybp)), dvy), 1.0e-10 * cv));
+                double dt4 = ddiv(cv, pmax(pmax(pmax(fabs(vzbm), fabs(vzbp)), dvz), 1.0e-10 * cv));
                 m = pmin(m, pmin(pmin(pmin(dt1, dt2), dt3), dt4));
             }
         }
@@ -2065,9 +2066,9 @@ KERNEL void g_predict(const Src<T> S)
     double vzbm = 0.25 * (VZ[oom] + VZ[mom] + VZ[omm] + VZ[mmm]);
     const double vol = CV(ix, iy, iz);
     const double axx = DXAB(iy, iz), ayy = DYAB(ix, iz), azz = DZAB(ix, iy);
-    double dvxdx = (vxb * axx - vxbm * axx) / vol;
-    double dvydy = (vyb * ayy - vybm * ayy) / vol;
-    double dvzdz = (vzb * azz - vzbm * azz) / vol;
+    double dvxdx = ddiv(vxb * axx - vxbm * axx, vol);
+    double dvydy = ddiv(vyb * ayy - vybm * ayy, vol);
+    double dvzdz = ddiv(vzb * azz - vzbm * azz, vol);
     double dv = (dvxdx + dvydy + dvzdz) * dt / 2.0;
     const double cv1 = vol * (1.0 + dv);
     A.cv1[c] = cv1;
@@ -2075,20 +2076,20 @@ KERNEL void g_predict(const Src<T> S)
     vxbm = 0.25 * (VX[omo] + VX[mmo] + VX[omm] + VX[mmm]);
     vyb = 0.25 * (VY[ooo] + VY[omo] + VY[oom] + VY[omm]);
     vybm = 0.25 * (VY[moo] + VY[mmo] + VY[mom] + VY[mmm]);
-    double dvxdy = (vxb * ayy - vxbm * ayy) / vol;
-    double dvydx = (vyb * axx - vybm * axx) / vol;
+    double dvxdy = ddiv(vxb * ayy - vxbm * ayy, vol);
+    double dvydx = ddiv(vyb * axx - vybm * axx, vol);
     vxb = 0.25 * (VX[ooo] + VX[moo] + VX[omo] + VX[mmo]);
     vxbm = 0.25 * (VX[oom] + VX[mom] + VX[omm] + VX[mmm]);
     double vzb2 = 0.25 * (VZ[ooo] + VZ[omo] + VZ[oom] + VZ[omm]);
     double vzbm2 = 0.25 * (VZ[moo] + VZ[mmo] + VZ[mom] + VZ[mmm]);
-    double dvxdz = (vxb * azz - vxbm * azz) / vol;
-    double dvzdx = (vzb2 * axx - vzbm2 * axx) / vol;
+    double dvxdz = ddiv(vxb * azz - vxbm * azz, vol);
+    double dvzdx = ddiv(vzb2 * axx - vzbm2 * axx, vol);
     double vyb3 = 0.25 * (VY[ooo] + VY[moo] + VY[omo] + VY[mmo]);
     double vybm3 = 0.25 * (VY[oom] + VY[mom] + VY[omm] + VY[mmm]);
     double vzb3 = 0.25 * (VZ[ooo] + VZ[moo] + VZ[oom] + VZ[mom]);
     double vzbm3 = 0.25 * (VZ[omo] + VZ[mmo] + VZ[omm] + VZ[mmm]);
-    double dvydz = (vyb3 * azz - vybm3 * azz) / vol;
-    double dvzdy = (vzb3 * ayy - vzbm3 * ayy) / vol;
+    double dvydz = ddiv(vyb3 * azz - vybm3 * azz, vol);
+    double dvzdy = ddiv(vzb3 * ayy - vzbm3 * ayy, vol);
     double bx1, by1, bz1;
     if (res) {
         bx1 = A.bx1[c];
@@ -2102,19 +2103,19 @@ KERNEL void g_predict(const Src<T> S)
     double w3 = bx1 * dvxdx + by1 * dvxdy + bz1 * dvxdz;
     double w4 = bx1 * dvydx + by1 * dvydy + bz1 * dvydz;
     double w5 = bx1 * dvzdx + by1 * dvzdy + bz1 * dvzdz;
-    A.bx1[c] = (bx1 + w3 * dt / 2.0) / (1.0 + dv) * cv1;
-    A.by1[c] = (by1 + w4 * dt / 2.0) / (1.0 + dv) * cv1;
-    A.bz1[c] = (bz1 + w5 * dt / 2.0) / (1.0 + dv) * cv1;
+    A.bx1[c] = ddiv(bx1 + w3 * dt / 2.0, 1.0 + dv) * cv1;
+    A.by1[c] = ddiv(by1 + w4 * dt / 2.0, 1.0 + dv) * cv1;
+    A.bz1[c] = ddiv(bz1 + w5 * dt / 2.0, 1.0 + dv) * cv1;
     if (ix >= 0 && iy >= 0 && iz >= 0 && ix <= G.nx + 1 && iy <= G.ny + 1 && iz <= G.nz + 1) {
         double rho = RHO[c];
-        double dvp = cv1 / vol - 1.0;
+        double dvp = ddiv(cv1, vol) - 1.0;
         double pe0 = res ? (double)A.p_e[c] : PE_OF(c);
         double pi0 = P.keep_pi ? (double)A.p_i[c] : PI_OF(c);
-        double e1_e = EE[c] - pe0 * dvp / rho;
-        double e1_i = EI[c] - pi0 * dvp / rho;
-        e1_i += A.visc_heat[c] * dt / 2.0 / rho;
-        A.p_e[c] = e1_e * (gas_gamma - 1.0) * rho * vol / cv1;
-        A.p_i[c] = e1_i * (gas_gamma - 1.0) * rho * vol / cv1;
+        double e1_e = EE[c] - ddiv(pe0 * dvp, rho);
+        double e1_i = EI[c] - ddiv(pi0 * dvp, rho);
+        e1_i += ddiv(A.visc_heat[c] * dt / 2.0, rho);
+        A.p_e[c] = ddiv(e1_e * (gas_gamma - 1.0) * rho * vol, cv1);
+        A.p_i[c] = ddiv(e1_i * (gas_gamma - 1.0) * rho * vol, cv1);
     }
     };
     ECELL(cell)
@@ -2142,44 +2143,44 @@ __device__ __forceinline__ void predict_cell(const ArrE<T> &A, double gas_gamma,
     double vzbm = 0.25 * (Zv[oom] + Zv[mom] + Zv[omm] + Zv[mmm]);
     const double vol = CV(ix, iy, iz);
     const double axx = DXAB(iy, iz), ayy = DYAB(ix, iz), azz = DZAB(ix, iy);
-    double dvxdx = (vxb * axx - vxbm * axx) / vol;
-    double dvydy = (vyb * ayy - vybm * ayy) / vol;
-    double dvzdz = (vzb * azz - vzbm * azz) / vol;
+    double dvxdx = ddiv(vxb * axx - vxbm * axx, vol);
+    double dvydy = ddiv(vyb * ayy - vybm * ayy, vol);
+    double dvzdz = ddiv(vzb * azz - vzbm * azz, vol);
     double dv = (dvxdx + dvydy + dvzdz) * dt / 2.0;
     const double cv1 = vol * (1.0 + dv);
     vxb = 0.25 * (X[ooo] + X[moo] + X[oom] + X[mom]);
     vxbm = 0.25 * (X[omo] + X[mmo] + X[omm] + X[mmm]);
     vyb = 0.25 * (Y[ooo] + Y[omo] + Y[oom] + Y[omm]);
     vybm = 0.25 * (Y[moo] + Y[mmo] + Y[mom] + Y[mmm]);
-    double dvxdy = (vxb * ayy - vxbm * ayy) / vol;
-    double dvydx = (vyb * axx - vybm * axx) / vol;
+    double dvxdy = ddiv(vxb * ayy - vxbm * ayy, vol);
+    double dvydx = ddiv(vyb * axx - vybm * axx, vol);
     vxb = 0.25 * (X[ooo] + X[moo] + X[omo] + X[mmo]);
     vxbm = 0.25 * (X[oom] + X[mom] + X[omm] + X[mmm]);
     double vzb2 = 0.25 * (Zv[ooo] + Zv[omo] + Zv[oom] + Zv[omm]);
     double vzbm2 = 0.25 * (Zv[moo] + Zv[mmo] + Zv[mom] + Zv[mmm]);
-    double dvxdz = (vxb * azz - vxbm * azz) / vol;
-    double dvzdx = (vzb2 * axx - vzbm2 * axx) / vol;
+    double dvxdz = ddiv(vxb * azz - vxbm * azz, vol);
+    double dvzdx = ddiv(vzb2 * axx - vzbm2 * axx, vol);
     double vyb3 = 0.25 * (Y[ooo] + Y[moo] + Y[omo] + Y[mmo]);
     double vybm3 = 0.25 * (Y[oom] + Y[mom] + Y[omm] + Y[mmm]);
     double vzb3 = 0.25 * (Zv[ooo] + Zv[moo] + Zv[oom] + Zv[mom]);
     double vzbm3 = 0.25 * (Zv[omo] + Zv[mmo] + Zv[omm] + Zv[mmm]);
-    double dvydz = (vyb3 * azz - vybm3 * azz) / vol;
-    double dvzdy = (vzb3 * ayy - vzbm3 * ayy) / vol;
+    double dvydz = ddiv(vyb3 * azz - vybm3 * azz, vol);
+    double dvzdy = ddiv(vzb3 * ayy - vzbm3 * ayy, vol);
     double w3 = bx1 * dvxdx + by1 * dvxdy + bz1 * dvxdz;
     double w4 = bx1 * dvydx + by1 * dvydy + bz1 * dvydz;
     double w5 = bx1 * dvzdx + by1 * dvzdy + bz1 * dvzdz;
     out[0] = cv1;
-    out[1] = (bx1 + w3 * dt / 2.0) / (1.0 + dv) * cv1;
-    out[2] = (by1 + w4 * dt / 2.0) / (1.0 + dv) * cv1;
-    out[3] = (bz1 + w5 * dt / 2.0) / (1.0 + dv) * cv1;
+    out[1] = ddiv(bx1 + w3 * dt / 2.0, 1.0 + dv) * cv1;
+    out[2] = ddiv(by1 + w4 * dt / 2.0, 1.0 + dv) * cv1;
+    out[3] = ddiv(bz1 + w5 * dt / 2.0, 1.0 + dv) * cv1;
     out[4] = out[5] = 0.0;
     if (energy) {
-        double dvp = cv1 / vol - 1.0;
-        double e1_e = ee - pe0 * dvp / rho;
-        double e1_i = ei - pi0 * dvp / rho;
-        e1_i += vh * dt / 2.0 / rho;
-        out[4] = e1_e * (gas_gamma - 1.0) * rho * vol / cv1;
-        out[5] = e1_i * (gas_gamma - 1.0) * rho * vol / cv1;
+        double dvp = ddiv(cv1, vol) - 1.0;
+        double e1_e = ee - ddiv(pe0 * dvp, rho);
+        double e1_i = ei - ddiv(pi0 * dvp, rho);
+        e1_i += ddiv(vh * dt / 2.0, rho);
+        out[4] = ddiv(e1_e * (gas_gamma - 1.0) * rho * vol, cv1);
+        out[5] = ddiv(e1_i * (gas_gamma - 1.0) * rho * vol, cv1);
     }
 }
 
@@ -2295,51 +2296,51 @@ KERNEL void g_forces(const Src<T> S)
     double ppxy = PE[xy] + PI[xy], ppxz = PE[xz] + PI[xz], ppyz = PE[yz] + PI[yz], ppxyz = PE[xyz] + PI[xyz];
     double w1 = pp + ppy + ppz + ppyz;
     double w2 = ppx + ppxy + ppxz + ppxyz;
-    double fx = (w1 - w2) * 0.25 / dx;
+    double fx = ddiv((w1 - w2) * 0.25, dx);
     w1 = pp + ppx + ppz + ppxz;
     w2 = ppy + ppxy + ppyz + ppxyz;
-    double fy = (w1 - w2) * 0.25 / dhy;
+    double fy = ddiv((w1 - w2) * 0.25, dhy);
     w1 = pp + ppx + ppy + ppxy;
     w2 = ppz + ppxz + ppyz + ppxyz;
-    double fz = (w1 - w2) * 0.25 / dhz;
+    double fz = ddiv((w1 - w2) * 0.25, dhz);
     double cvx = C1[c] + C1[y_] + C1[z_] + C1[yz];
     double cvxp = C1[x_] + C1[xy] + C1[xz] + C1[xyz];
     double cvy = C1[c] + C1[x_] + C1[z_] + C1[xz];
     double cvyp = C1[y_] + C1[xy] + C1[yz] + C1[xyz];
     double cvz = C1[c] + C1[x_] + C1[y_] + C1[xy];
     double cvzp = C1[z_] + C1[xz] + C1[yz] + C1[xyz];
-    w1 = (B1z[c] + B1z[x_] + B1z[z_] + B1z[xz]) / cvy;
-    w2 = (B1z[y_] + B1z[xy] + B1z[yz] + B1z[xyz]) / cvyp;
-    double jx = (w2 - w1) / dhy;
-    w1 = (B1y[c] + B1y[x_] + B1y[y_] + B1y[xy]) / cvz;
-    w2 = (B1y[z_] + B1y[xz] + B1y[yz] + B1y[xyz]) / cvzp;
-    jx -= (w2 - w1) / dhz;
-    w1 = (B1z[c] + B1z[y_] + B1z[z_] + B1z[yz]) / cvx;
-    w2 = (B1z[x_] + B1z[xy] + B1z[xz] + B1z[xyz]) / cvxp;
-    double jy = -(w2 - w1) / dx;
-    w1 = (B1x[c] + B1x[x_] + B1x[y_] + B1x[xy]) / cvz;
-    w2 = (B1x[z_] + B1x[xz] + B1x[yz] + B1x[xyz]) / cvzp;
-    jy += (w2 - w1) / dhz;
-    w1 = (B1y[c] + B1y[y_] + B1y[z_] + B1y[yz]) / cvx;
-    w2 = (B1y[x_] + B1y[xy] + B1y[xz] + B1y[xyz]) / cvxp;
-    double jz = (w2 - w1) / dx;
-    w1 = (B1x[c] + B1x[x_] + B1x[z_] + B1x[xz]) / cvy;
-    w2 = (B1x[y_] + B1x[xy] + B1x[yz] + B1x[xyz]) / cvyp;
-    jz -= (w2 - w1) / dhy;
-    double bxv = (B1x[c] + B1x[x_] + B1x[y_] + B1x[xy] + B1x[z_] + B1x[xz] + B1x[yz] + B1x[xyz]) / (cvx + cvxp);
-    double byv = (B1y[c] + B1y[x_] + B1y[y_] + B1y[xy] + B1y[z_] + B1y[xz] + B1y[yz] + B1y[xyz]) / (cvx + cvxp);
-    double bzv = (B1z[c] + B1z[x_] + B1z[y_] + B1z[xy] + B1z[z_] + B1z[xz] + B1z[yz] + B1z[xyz]) / (cvx + cvxp);
-    fx += (jy * bzv - jz * byv) / mu0;
-    fy += (jz * bxv - jx * bzv) / mu0;
-    fz += (jx * byv - jy * bxv) / mu0;
+    w1 = ddiv(B1z[c] + B1z[x_] + B1z[z_] + B1z[xz], cvy);
+    w2 = ddiv(B1z[y_] + B1z[xy] + B1z[yz] + B1z[xyz], cvyp);
+    double jx = ddiv(w2 - w1, dhy);
+    w1 = ddiv(B1y[c] + B1y[x_] + B1y[y_] + B1y[xy], cvz);
+    w2 = ddiv(B1y[z_] + B1y[xz] + B1y[yz] + B1y[xyz], cvzp);
+    jx -= ddiv(w2 - w1, dhz);
+    w1 = ddiv(B1z[c] + B1z[y_] + B1z[z_] + B1z[yz], cvx);
+    w2 = ddiv(B1z[x_] + B1z[xy] + B1z[xz] + B1z[xyz], cvxp);
+    double jy = ddiv(-(w2 - w1), dx);
+    w1 = ddiv(B1x[c] + B1x[x_] + B1x[y_] + B1x[xy], cvz);
+    w2 = ddiv(B1x[z_] + B1x[xz] + B1x[yz] + B1x[xyz], cvzp);
+    jy += ddiv(w2 - w1, dhz);
+    w1 = ddiv(B1y[c] + B1y[y_] + B1y[z_] + B1y[yz], cvx);
+    w2 = ddiv(B1y[x_] + B1y[xy] + B1y[xz] + B1y[xyz], cvxp);
+    double jz = ddiv(w2 - w1, dx);
+    w1 = ddiv(B1x[c] + B1x[x_] + B1x[z_] + B1x[xz], cvy);
+    w2 = ddiv(B1x[y_] + B1x[xy] + B1x[yz] + B1x[xyz], cvyp);
+    jz -= ddiv(w2 - w1, dhy);
+    double bxv = ddiv(B1x[c] + B1x[x_] + B1x[y_] + B1x[xy] + B1x[z_] + B1x[xz] + B1x[yz] + B1x[xyz], cvx + cvxp);
+    double byv = ddiv(B1y[c] + B1y[x_] + B1y[y_] + B1y[xy] + B1y[z_] + B1y[xz] + B1y[yz] + B1y[xyz], cvx + cvxp);
+    double bzv = ddiv(B1z[c] + B1z[x_] + B1z[y_] + B1z[xy] + B1z[z_] + B1z[xz] + B1z[yz] + B1z[xyz], cvx + cvxp);
+    fx += ddiv(jy * bzv - jz * byv, mu0);
+    fy += ddiv(jz * bxv - jx * bzv, mu0);
+    fz += ddiv(jx * byv - jy * bxv, mu0);
     double rv = A.rho_v[c];
     double sxv = A.fxv[c] + fx, syv = A.fyv[c] + fy, szv = A.fzv[c] + fz;
     A.fxv[c] = sxv;
     A.fyv[c] = syv;
     A.fzv[c] = szv;
-    VX1[c] = VX[c] + dt / 2.0 * sxv / rv;
-    VY1[c] = VY[c] + dt / 2.0 * syv / rv;
-    VZ1[c] = VZ[c] + dt / 2.0 * szv / rv;
+    VX1[c] = VX[c] + ddiv(dt / 2.0 * sxv, rv);
+    VY1[c] = VY[c] + ddiv(dt / 2.0 * syv, rv);
+    VZ1[c] = VZ[c] + ddiv(dt / 2.0 * szv, rv);
     };
     ECELL(cell)
 }
@@ -2366,9 +2367,9 @@ KERNEL OCC(2) void g_correct(const Src<T> S)
     const int sx = G.sx, sy = G.sy;
     const double dt = lare::gbl(A.s)->dt;
     double rv = A.rho_v[c];
-    A.vx2[c] = VX[c] + dt * A.fxv[c] / rv;
-    A.vy2[c] = VY[c] + dt * A.fyv[c] / rv;
-    A.vz2[c] = VZ[c] + dt * A.fzv[c] / rv;
+    A.vx2[c] = VX[c] + ddiv(dt * A.fxv[c], rv);
+    A.vy2[c] = VY[c] + ddiv(dt * A.fyv[c], rv);
+    A.vz2[c] = VZ[c] + ddiv(dt * A.fzv[c], rv);
     if (ix < 1 || iy < 1 || iz < 1) return;
     const int ooo = c, moo = c - sx, omo = c - sy, mmo = c - sx - sy;
     const int oop = c + 1, mop = c - sx + 1, omp = c - sy + 1, mmp = c - sx - sy + 1;
@@ -2386,7 +2387,7 @@ KERNEL OCC(2) void g_correct(const Src<T> S)
              0.25 * dy * dz * A.alpha1[c + sy + 1] * a7 - 0.25 * dx * dz * A.alpha2[c - sx + 1] * a8 -
              0.25 * dy * dy * A.alpha3[c] * a9 - 0.25 * dx * dy * A.alpha3[c - sx] * a10 -
              0.25 * dy * dy * A.alpha3[c - sx - sy] * a11 - 0.25 * dx * dy * A.alpha3[c - sy] * a12);
-        visc_heat = pmax(vh / CV(ix, iy, iz), 0.0);
+        visc_heat = pmax(ddiv(vh, CV(ix, iy, iz)), 0.0);
     }
     double vxb = 0.25 * (VX1[ooo] + VX1[omo] + VX1[oom] + VX1[omm]);
     double vxbm = 0.25 * (VX1[moo] + VX1[mmo] + VX1[mom] + VX1[mmm]);
@@ -2396,14 +2397,14 @@ KERNEL OCC(2) void g_correct(const Src<T> S)
     double vzbm = 0.25 * (VZ1[oom] + VZ1[mom] + VZ1[omm] + VZ1[mmm]);
     const double vol = CV(ix, iy, iz);
     const double axx = DXAB(iy, iz), ayy = DYAB(ix, iz), azz = DZAB(ix, iy);
-    double dvxdx = (vxb * axx - vxbm * axx) / vol;
-    double dvydy = (vyb * ayy - vybm * ayy) / vol;
-    double dvzdz = (vzb * azz - vzbm * azz) / vol;
+    double dvxdx = ddiv(vxb * axx - vxbm * axx, vol);
+    double dvydy = ddiv(vyb * ayy - vybm * ayy, vol);
+    double dvzdz = ddiv(vzb * azz - vzbm * azz, vol);
     double dv = (dvxdx + dvydy + dvzdz) * dt;
     double rho = RHO[c];
-    A.ee2[c] = EE[c] - dv * A.p_e[c] / rho;
-    A.ei2[c] = EI[c] + (dt * visc_heat - dv * A.p_i[c]) / rho;
-    A.rho2[c] = rho / (1.0 + dv);
+    A.ee2[c] = EE[c] - ddiv(dv * A.p_e[c], rho);
+    A.ei2[c] = EI[c] + ddiv(dt * visc_heat - dv * A.p_i[c], rho);
+    A.rho2[c] = ddiv(rho, 1.0 + dv);
     };
     ECELL(cell)
 }
@@ -2457,9 +2458,9 @@ KERNEL OCC(4) void g_correct_zx(const Src<T> S)
         const double vxb = 0.25 * (O[3] + Om[3] + pO[3] + pOm[3]);
         if (Z.wz && ix > Z.ix0) {
             const double rv = A.rho_v[c];
-            A.vx2[c] = O[0] + dt * A.fxv[c] / rv;
-            A.vy2[c] = O[1] + dt * A.fyv[c] / rv;
-            A.vz2[c] = O[2] + dt * A.fzv[c] / rv;
+            A.vx2[c] = O[0] + ddiv(dt * A.fxv[c], rv);
+            A.vy2[c] = O[1] + ddiv(dt * A.fyv[c], rv);
+            A.vz2[c] = O[2] + ddiv(dt * A.fzv[c], rv);
             if (ix >= 1 && cyz) {
                 const double dx = DXB(ix);
                 const double vh =
@@ -2468,21 +2469,21 @@ KERNEL OCC(4) void g_correct_zx(const Src<T> S)
                      0.25 * dy * dz * al1yp * a7 - 0.25 * dx * dz * al2p_m * a8 - 0.25 * dy * dy * al3 * a9 -
                      0.25 * dx * dy * al3_m * a10 - 0.25 * dy * dy * al3y_m * a11 - 0.25 * dx * dy * al3y * a12);
                 const double vol = CV(ix, iy, iz);
-                const double visc_heat = pmax(vh / vol, 0.0);
+                const double visc_heat = pmax(ddiv(vh, vol), 0.0);
                 const double vxbm = vxb_m;
                 const double vyb = 0.25 * (O[4] + M[4] + pO[4] + pM_vy1);
                 const double vybm = 0.25 * (Om[4] + Mm[4] + pOm[4] + pMm_vy1);
                 const double vzb = 0.25 * (O[5] + M[5] + Om[5] + Mm[5]);
                 const double vzbm = 0.25 * (pO[5] + pM_vz1 + pOm[5] + pMm_vz1);
                 const double axx = DXAB(iy, iz), ayy = DYAB(ix, iz), azz = DZAB(ix, iy);
-                double dvxdx = (vxb * axx - vxbm * axx) / vol;
-                double dvydy = (vyb * ayy - vybm * ayy) / vol;
-                double dvzdz = (vzb * azz - vzbm * azz) / vol;
+                double dvxdx = ddiv(vxb * axx - vxbm * axx, vol);
+                double dvydy = ddiv(vyb * ayy - vybm * ayy, vol);
+                double dvzdz = ddiv(vzb * azz - vzbm * azz, vol);
                 double dv = (dvxdx + dvydy + dvzdz) * dt;
                 double rho = RHO[c];
-                A.ee2[c] = EE[c] - dv * A.p_e[c] / rho;
-                A.ei2[c] = EI[c] + (dt * visc_heat - dv * A.p_i[c]) / rho;
-                A.rho2[c] = rho / (1.0 + dv);
+                A.ee2[c] = EE[c] - ddiv(dv * A.p_e[c], rho);
+                A.ei2[c] = EI[c] + ddiv(dt * visc_heat - dv * A.p_i[c], rho);
+                A.rho2[c] = ddiv(rho, 1.0 + dv);
             }
         }
         for (int k = 0; k < 6; ++k) {
@@ -2577,20 +2578,20 @@ KERNEL void r_b_energy(const Src<T> S)
     const auto fx = lare::gbl(A.flux_x), fy = lare::gbl(A.flux_y), fz = lare::gbl(A.flux_z);
     if (iy >= 1 && iz >= 1) {
         double area = DYB(iy) * DZB(iz);
-        BX[c] = BX[c] + (fz[c] - fz[c - sy] + fz[c - 1] - fz[c - sy - 1] - fy[c] + fy[c - 1] - fy[c - sy] + fy[c - sy - 1]) * dt / area;
+        BX[c] = BX[c] + ddiv((fz[c] - fz[c - sy] + fz[c - 1] - fz[c - sy - 1] - fy[c] + fy[c - 1] - fy[c - sy] + fy[c - sy - 1]) * dt, area);
     }
     if (ix >= 1 && iz >= 1) {
         double area = DXB(ix) * DZB(iz);
-        BY[c] = BY[c] + (fx[c] - fx[c - 1] + fx[c - sx] - fx[c - sx - 1] - fz[c] + fz[c - sx] - fz[c - 1] + fz[c - sx - 1]) * dt / area;
+        BY[c] = BY[c] + ddiv((fx[c] - fx[c - 1] + fx[c - sx] - fx[c - sx - 1] - fz[c] + fz[c - sx] - fz[c - 1] + fz[c - sx - 1]) * dt, area);
     }
     if (ix >= 1 && iy >= 1) {
         double area = DXB(ix) * DYB(iy);
-        BZ[c] = BZ[c] + (fy[c] - fy[c - sx] + fy[c - sy] - fy[c - sx - sy] - fx[c] + fx[c - sy] - fx[c - sx] + fx[c - sx - sy]) * dt / area;
+        BZ[c] = BZ[c] + ddiv((fy[c] - fy[c - sx] + fy[c - sy] - fy[c - sx - sy] - fx[c] + fx[c - sy] - fx[c - sx] + fx[c - sx - sy]) * dt, area);
     }
     if (ix >= 1 && iy >= 1 && iz >= 1 && ix <= nx && iy <= ny && iz <= nz) {
         const Fld<T> cb = A.curlb;
         double sum_curlb = cb[c] + cb[c - sx] + cb[c - sy] + cb[c - sx - sy] + cb[c - 1] + cb[c - sx - 1] + cb[c - sy - 1] + cb[c - sx - sy - 1];
-        EE[c] += sum_curlb * dt / (8.0 * RHO[c]);
+        EE[c] += ddiv(sum_curlb * dt, 8.0 * RHO[c]);
     }
     };
     ECELL(cell)
@@ -2610,7 +2611,7 @@ __device__ __forceinline__ void rk_vertex_q(double bz00, double bz01, double bz1
         if constexpr (FAST)
             return qdiv_f(a, R, slow);
         else
-            return a / R.d;
+            return ddiv(a, R.d);
     };
     double jx1 = D(bz01 - bz00, qy) - D(by00z - by00, qz);
     double jx2 = D(bz11 - bz10, qy) - D(by10z - by10, qz);
@@ -2700,19 +2701,19 @@ KERNEL OCC(3) void r_substep_zx(const Src<T> S, int to_bx1)
             // this x (F0, Fm, pF0, pFm) and at x - 1 (F0x, Fmx, pF0x, pFmx)
             if (iy >= 1 && iz >= 1) {
                 double area = DYB(iy) * DZB(iz);
-                OX_[c] = bx_0 + (F0[2] - Fm[2] + pF0[2] - pFm[2] - F0[1] + pF0[1] - Fm[1] + pFm[1]) * dt / area;
+                OX_[c] = bx_0 + ddiv((F0[2] - Fm[2] + pF0[2] - pFm[2] - F0[1] + pF0[1] - Fm[1] + pFm[1]) * dt, area);
             }
             if (ix >= 1 && iz >= 1) {
                 double area = DXB(ix) * DZB(iz);
-                OY_[c] = by_0 + (F0[0] - pF0[0] + F0x[0] - pF0x[0] - F0[2] + F0x[2] - pF0[2] + pF0x[2]) * dt / area;
+                OY_[c] = by_0 + ddiv((F0[0] - pF0[0] + F0x[0] - pF0x[0] - F0[2] + F0x[2] - pF0[2] + pF0x[2]) * dt, area);
             }
             if (ix >= 1 && iy >= 1) {
                 double area = DXB(ix) * DYB(iy);
-                OZ_[c] = bz_0 + (F0[1] - F0x[1] + Fm[1] - Fmx[1] - F0[0] + Fm[0] - F0x[0] + Fmx[0]) * dt / area;
+                OZ_[c] = bz_0 + ddiv((F0[1] - F0x[1] + Fm[1] - Fmx[1] - F0[0] + Fm[0] - F0x[0] + Fmx[0]) * dt, area);
             }
             if (ix >= 1 && iy >= 1 && iz >= 1 && ix <= nx && iy <= ny && iz <= nz) {
                 double sum_curlb = F0[3] + F0x[3] + Fm[3] + Fmx[3] + pF0[3] + pF0x[3] + pFm[3] + pFmx[3];
-                EE[c] += sum_curlb * dt / (8.0 * RHO[c]);
+                EE[c] += ddiv(sum_curlb * dt, 8.0 * RHO[c]);
             }
         }
         for (int q = 0; q < 4; ++q) {
@@ -3090,9 +3091,9 @@ KERNEL void e_energy_part(const Src<T> S, double *part)
                      r[c + sx + 1] * c101 + r[c + sy + 1] * c011 + r[c + sx + sy + 1] * c111;
         double scv = c000 + c100 + c010 + c110 + c001 + c101 + c011 + c111;
         const double v2 = VX[c] * VX[c] + VY[c] * VY[c] + VZ[c] * VZ[c];
-        e[1] += 0.5 * (src / scv) * v2 * (0.125 * scv);
+        e[1] += 0.5 * ddiv(src, scv) * v2 * (0.125 * scv);
         const double bx = 0.5 * (BX[c] + BX[c - sx]), by = 0.5 * (BY[c] + BY[c - sy]), bz = 0.5 * (BZ[c] + BZ[c - 1]);
-        e[2] += (bx * bx + by * by + bz * bz) / (2.0 * P.mu0) * cv;
+        e[2] += ddiv(bx * bx + by * by + bz * bz, 2.0 * P.mu0) * cv;
     };
     ECELL(cell)
     __shared__ double sh[3][256];
@@ -3902,6 +3903,41 @@ int group_energy(Group &gp, std::vector<double> &out, std::string &err)
     if (res) (void)hipFree(res);
     if (e != hipSuccess) {
         err = std::string("energy diagnostic: ") + hipGetErrorString(e);
+        return -3;
+    }
+    return 0;
+}
+
+// Division self-check: q[i] = a[i] / d[i] as the kernels form it, by ddiv
+// (mode 0) or by a shared-divisor QDiv (mode 1).
+KERNEL void k_div_check(const double *a, const double *d, double *q, long n, int mode)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    q[i] = mode == 0 ? ddiv(a[i], d[i]) : qdiv(a[i], qdiv_of(d[i]));
+}
+
+int div_check(const double *a, const double *d, double *q, long n, int mode, std::string &err)
+{
+    if (n <= 0) return 0;
+    const size_t bytes = (size_t)n * sizeof(double);
+    double *da = nullptr, *dd = nullptr, *dq = nullptr;
+    hipError_t e = hipMalloc(&da, bytes);
+    if (e == hipSuccess) e = hipMalloc(&dd, bytes);
+    if (e == hipSuccess) e = hipMalloc(&dq, bytes);
+    if (e == hipSuccess) e = hipMemcpy(da, a, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dd, d, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_div_check, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, (const double *)da,
+                           (const double *)dd, dq, n, mode);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(q, dq, bytes, hipMemcpyDeviceToHost);
+    if (da) (void)hipFree(da);
+    if (dd) (void)hipFree(dd);
+    if (dq) (void)hipFree(dq);
+    if (e != hipSuccess) {
+        err = std::string("division check: ") + hipGetErrorString(e);
         return -3;
     }
     return 0;

@@ -87,7 +87,7 @@ KERNEL void k_prologue2(Dev D, R3 R)
     const int c = IX(ix, iy, iz);
     double sum_rho_cv = SUM8(RHO, A.cv, c);
     double sum_cv = PLAIN8(A.cv, c);
-    A.rho_v[c] = sum_rho_cv / sum_cv;
+    A.rho_v[c] = ddiv(sum_rho_cv, sum_cv);
     A.cv_v[c] = 0.125 * sum_cv;
 }
 
@@ -101,7 +101,7 @@ KERNEL void k_cs(Dev D, R3 R, double gas_gamma)
     double rmin = pmax(RHO[c], DBL_EPSILON);
     double b2 = A.bx1[c] * A.bx1[c] + A.by1[c] * A.by1[c] + A.bz1[c] * A.bz1[c];
     double p = A.pressure[c];
-    A.cs[c] = sqrt((gas_gamma * p + b2) / rmin);
+    A.cs[c] = sqrt(ddiv(gas_gamma * p + b2, rmin));
 }
 
 // lagran.cpp:199-216 over (-1..n+1)^3.
@@ -110,7 +110,7 @@ KERNEL void k_cs_v(Dev D, R3 R)
     KBEGIN
     const int c = IX(ix, iy, iz);
     double sum = SUM8(A.cs, A.cv, c);
-    A.cs_v[c] = 0.125 * sum / A.cv_v[c];
+    A.cs_v[c] = ddiv(0.125 * sum, A.cv_v[c]);
 }
 
 // edge_viscosity (lagran.cpp:56-93) for the edge between linear indices
@@ -122,14 +122,14 @@ __device__ __forceinline__ double edge_viscosity(const Arr &A, double dt, double
 {
     const int c0 = c1 - s, c2 = c1 + s, c3 = c2 + s;
     dvdots = pmin(0.0, dvdots);
-    double rho_edge = 2.0 * A.rho_v[c1] * A.rho_v[c2] / (A.rho_v[c1] + A.rho_v[c2]);
+    double rho_edge = ddiv(2.0 * A.rho_v[c1] * A.rho_v[c2], A.rho_v[c1] + A.rho_v[c2]);
     double dvx = VX[c1] - VX[c2];
     double dvy = VY[c1] - VY[c2];
     double dvz = VZ[c1] - VZ[c2];
     double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
     double dv = sqrt(dv2);
     double psi = 0.0;
-    dvdots = dv * dt / dx < 1.e-14 ? 0.0 : dvdots / dv;
+    dvdots = ddiv(dv * dt, dx) < 1.e-14 ? 0.0 : ddiv(dvdots, dv);
     double dvxm = VX[c0] - VX[c1];
     double dvxp = VX[c2] - VX[c3];
     double dvym = VY[c0] - VY[c1];
@@ -137,8 +137,8 @@ __device__ __forceinline__ double edge_viscosity(const Arr &A, double dt, double
     double dvzm = VZ[c0] - VZ[c1];
     double dvzp = VZ[c2] - VZ[c3];
     double rl = 1.0, rr = 1.0;
-    rl = dv * dt / dx < 1.e-14 ? 1.0 : (dvxp * dvx + dvyp * dvy + dvzp * dvz) * dx / (dxp * dv2);
-    rr = dv * dt / dx < 1.e-14 ? 1.0 : (dvxm * dvx + dvym * dvy + dvzm * dvz) * dx / (dxm * dv2);
+    rl = ddiv(dv * dt, dx) < 1.e-14 ? 1.0 : ddiv((dvxp * dvx + dvyp * dvy + dvzp * dvz) * dx, dxp * dv2);
+    rr = ddiv(dv * dt, dx) < 1.e-14 ? 1.0 : ddiv((dvxm * dvx + dvym * dvy + dvzm * dvz) * dx, dxm * dv2);
     psi = pmin_list4(0.5 * (rr + rl), 2.0 * rl, 2.0 * rr, 1.0);
     psi = pmax(0.0, psi);
     double q_k_bar = rho_edge * (visc2_norm * dv + sqrt(visc2_norm * visc2_norm * dv2 + (visc1 * cs_edge) * (visc1 * cs_edge)));
@@ -220,7 +220,7 @@ KERNEL void k_pvisc(Dev D, R3 R)
         - 0.25 * dx * dy * A.alpha3[c - sx] * a10
         - 0.25 * dx * dy * A.alpha3[c - sx - sy] * a11
         - 0.25 * dx * dy * A.alpha3[c - sy] * a12;
-    A.visc_heat[c] = vh / A.cv[c];
+    A.visc_heat[c] = ddiv(vh, A.cv[c]);
 }
 
 // Viscous forces (lagran.cpp:330-376) over (0..n)^3.
@@ -240,8 +240,9 @@ KERNEL void k_fvisc(Dev D, R3 R)
     double a6 = A.alpha3[c + 1] * dx * dy;
     double cvv = A.cv_v[c];
 #define FVISC(V)                                                                                   \
-    ((a1 * (V[c] - V[c - sx]) + a2 * (V[c] - V[c + sx]) + a3 * (V[c] - V[c - sy]) +               \
-      a4 * (V[c] - V[c + sy]) + a5 * (V[c] - V[c - 1]) + a6 * (V[c] - V[c + 1])) / cvv)
+    ddiv(a1 * (V[c] - V[c - sx]) + a2 * (V[c] - V[c + sx]) + a3 * (V[c] - V[c - sy]) +                \
+             a4 * (V[c] - V[c + sy]) + a5 * (V[c] - V[c - 1]) + a6 * (V[c] - V[c + 1]),               \
+         cvv)
     A.fxv[c] = FVISC(VX);
     A.fyv[c] = FVISC(VY);
     A.fzv[c] = FVISC(VZ);
@@ -270,11 +271,11 @@ KERNEL void k_set_dt(Dev D, R3 R, double gas_gamma, double mu0)
         double dz = DZB(iz);
         double dhx = dx, dhy = dy, dhz = dz;
         double rho0 = pmax(RHO[c], DBL_EPSILON);
-        double cs2 = gas_gamma * A.pressure[c] / rho0;
-        double w1 = (BX[c] * BX[c] + BY[c] * BY[c] + BZ[c] * BZ[c]) / mu0 / rho0;
-        double c_visc2 = A.p_visc[c] / rho0;
+        double cs2 = ddiv(gas_gamma * A.pressure[c], rho0);
+        double w1 = ddiv(ddiv(BX[c] * BX[c] + BY[c] * BY[c] + BZ[c] * BZ[c], mu0), rho0);
+        double c_visc2 = ddiv(A.p_visc[c], rho0);
         double length = pmin(pmin(dhx, dhy), dhz);
-        double dt1 = length / (sqrt(c_visc2) + sqrt(cs2 + w1 + c_visc2));
+        double dt1 = ddiv(length, sqrt(c_visc2) + sqrt(cs2 + w1 + c_visc2));
         double ax = 0.25 * A.dxab[c];
         double ay = 0.25 * A.dyab[c];
         double az = 0.25 * A.dzab[c];
@@ -291,9 +292,9 @@ KERNEL void k_set_dt(Dev D, R3 R, double gas_gamma, double mu0)
         double avym = fabs(vybm), avyp = fabs(vybp);
         double avzm = fabs(vzbm), avzp = fabs(vzbp);
         double volume = A.cv[c];
-        double dt2 = volume / pmax(pmax(pmax(avxm, avxp), dvx), 1.0e-10 * volume);
-        double dt3 = volume / pmax(pmax(pmax(avym, avyp), dvy), 1.0e-10 * volume);
-        double dt4 = volume / pmax(pmax(pmax(avzm, avzp), dvz), 1.0e-10 * volume);
+        double dt2 = ddiv(volume, pmax(pmax(pmax(avxm, avxp), dvx), 1.0e-10 * volume));
+        double dt3 = ddiv(volume, pmax(pmax(pmax(avym, avyp), dvy), 1.0e-10 * volume));
+        double dt4 = ddiv(volume, pmax(pmax(pmax(avzm, avzp), dvz), 1.0e-10 * volume));
         m = pmin(pmin(pmin(dt1, dt2), dt3), dt4);
     }
     // wavefront reduction (64 lanes)
@@ -340,12 +341,12 @@ KERNEL void k_eta_calc(Dev D, R3 R, double j_max, double eta_background, double 
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     double dxc = DXC(ix), dyc = DYC(iy), dzc = DZC(iz);
-    double jx = (BZ[c + sy] - BZ[c]) / dyc - (BY[c + 1] - BY[c]) / dzc;
-    double jxp = (BZ[c + sx + sy] - BZ[c + sx]) / dyc - (BY[c + sx + 1] - BY[c + sx]) / dzc;
-    double jy = (BX[c + 1] - BX[c]) / dzc - (BZ[c + sx] - BZ[c]) / dxc;
-    double jyp = (BX[c + sy + 1] - BX[c + sy]) / dzc - (BZ[c + sx + sy] - BZ[c + sy]) / dxc;
-    double jz = (BY[c + sx] - BY[c]) / dxc - (BX[c + sy] - BX[c]) / dyc;
-    double jzp = (BY[c + sx + 1] - BY[c + 1]) / dxc - (BX[c + sy + 1] - BX[c + 1]) / dyc;
+    double jx = ddiv(BZ[c + sy] - BZ[c], dyc) - ddiv(BY[c + 1] - BY[c], dzc);
+    double jxp = ddiv(BZ[c + sx + sy] - BZ[c + sx], dyc) - ddiv(BY[c + sx + 1] - BY[c + sx], dzc);
+    double jy = ddiv(BX[c + 1] - BX[c], dzc) - ddiv(BZ[c + sx] - BZ[c], dxc);
+    double jyp = ddiv(BX[c + sy + 1] - BX[c + sy], dzc) - ddiv(BZ[c + sx + sy] - BZ[c + sy], dxc);
+    double jz = ddiv(BY[c + sx] - BY[c], dxc) - ddiv(BX[c + sy] - BX[c], dyc);
+    double jzp = ddiv(BY[c + sx + 1] - BY[c + 1], dxc) - ddiv(BX[c + sy + 1] - BX[c + 1], dyc);
     jx = (jx + jxp) * 0.5;
     jy = (jy + jyp) * 0.5;
     jz = (jz + jzp) * 0.5;
@@ -372,12 +373,12 @@ KERNEL void k_rkstep(Dev D, R3 R)
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     double dxc = DXC(ix), dyc = DYC(iy), dzc = DZC(iz);
-    double jx1 = (BZ[c + sy] - BZ[c]) / dyc - (BY[c + 1] - BY[c]) / dzc;
-    double jx2 = (BZ[c + sx + sy] - BZ[c + sx]) / dyc - (BY[c + sx + 1] - BY[c + sx]) / dzc;
-    double jy1 = (BX[c + 1] - BX[c]) / dzc - (BZ[c + sx] - BZ[c]) / dxc;
-    double jy2 = (BX[c + sy + 1] - BX[c + sy]) / dzc - (BZ[c + sx + sy] - BZ[c + sy]) / dxc;
-    double jz1 = (BY[c + sx] - BY[c]) / dxc - (BX[c + sy] - BX[c]) / dyc;
-    double jz2 = (BY[c + sx + 1] - BY[c + 1]) / dxc - (BX[c + sy + 1] - BX[c + 1]) / dyc;
+    double jx1 = ddiv(BZ[c + sy] - BZ[c], dyc) - ddiv(BY[c + 1] - BY[c], dzc);
+    double jx2 = ddiv(BZ[c + sx + sy] - BZ[c + sx], dyc) - ddiv(BY[c + sx + 1] - BY[c + sx], dzc);
+    double jy1 = ddiv(BX[c + 1] - BX[c], dzc) - ddiv(BZ[c + sx] - BZ[c], dxc);
+    double jy2 = ddiv(BX[c + sy + 1] - BX[c + sy], dzc) - ddiv(BZ[c + sx + sy] - BZ[c + sy], dxc);
+    double jz1 = ddiv(BY[c + sx] - BY[c], dxc) - ddiv(BX[c + sy] - BX[c], dyc);
+    double jz2 = ddiv(BY[c + sx + 1] - BY[c + 1], dxc) - ddiv(BX[c + sy + 1] - BX[c + 1], dyc);
     double jx = 0.5 * (jx1 + jx2);
     double jy = 0.5 * (jy1 + jy2);
     double jz = 0.5 * (jz1 + jz2);
@@ -398,13 +399,13 @@ KERNEL void k_bstep(Dev D, R3 R, int comp)
     const double *fx = A.flux_x, *fy = A.flux_y, *fz = A.flux_z;
     if (comp == 0) {
         double area = DYB(iy) * DZB(iz);
-        BX[c] = A.bx1[c] + (fz[c] - fz[c - sy] + fz[c - 1] - fz[c - sy - 1] - fy[c] + fy[c - 1] - fy[c - sy] + fy[c - sy - 1]) * dt / area;
+        BX[c] = A.bx1[c] + ddiv((fz[c] - fz[c - sy] + fz[c - 1] - fz[c - sy - 1] - fy[c] + fy[c - 1] - fy[c - sy] + fy[c - sy - 1]) * dt, area);
     } else if (comp == 1) {
         double area = DXB(ix) * DZB(iz);
-        BY[c] = A.by1[c] + (fx[c] - fx[c - 1] + fx[c - sx] - fx[c - sx - 1] - fz[c] + fz[c - sx] - fz[c - 1] + fz[c - sx - 1]) * dt / area;
+        BY[c] = A.by1[c] + ddiv((fx[c] - fx[c - 1] + fx[c - sx] - fx[c - sx - 1] - fz[c] + fz[c - sx] - fz[c - 1] + fz[c - sx - 1]) * dt, area);
     } else {
         double area = DXB(ix) * DYB(iy);
-        BZ[c] = A.bz1[c] + (fy[c] - fy[c - sx] + fy[c - sy] - fy[c - sx - sy] - fx[c] + fx[c - sy] - fx[c - sx] + fx[c - sx - sy]) * dt / area;
+        BZ[c] = A.bz1[c] + ddiv((fy[c] - fy[c - sx] + fy[c - sy] - fy[c - sx - sy] - fx[c] + fx[c - sy] - fx[c - sx] + fx[c - sx - sy]) * dt, area);
     }
 }
 
@@ -417,7 +418,7 @@ KERNEL void k_res_energy(Dev D, R3 R)
     const double dt = A.s->dt_sub;
     const double *cb = A.curlb;
     double sum_curlb = cb[c] + cb[c - sx] + cb[c - sy] + cb[c - sx - sy] + cb[c - 1] + cb[c - sx - 1] + cb[c - sy - 1] + cb[c - sx - sy - 1];
-    EE[c] += sum_curlb * dt / (8.0 * RHO[c]);
+    EE[c] += ddiv(sum_curlb * dt, 8.0 * RHO[c]);
 }
 
 // ------------------------------------------------- predictor
@@ -438,9 +439,9 @@ KERNEL void k_bfield_cv1(Dev D, R3 R)
     double vzb = 0.25 * (VZ[ooo] + VZ[moo] + VZ[omo] + VZ[mmo]);
     double vzbm = 0.25 * (VZ[oom] + VZ[mom] + VZ[omm] + VZ[mmm]);
     double vol = A.cv[c];
-    double dvxdx = (vxb * A.dxab[c] - vxbm * A.dxab[c - sx]) / vol;
-    double dvydy = (vyb * A.dyab[c] - vybm * A.dyab[c - sy]) / vol;
-    double dvzdz = (vzb * A.dzab[c] - vzbm * A.dzab[c - 1]) / vol;
+    double dvxdx = ddiv(vxb * A.dxab[c] - vxbm * A.dxab[c - sx], vol);
+    double dvydy = ddiv(vyb * A.dyab[c] - vybm * A.dyab[c - sy], vol);
+    double dvzdz = ddiv(vzb * A.dzab[c] - vzbm * A.dzab[c - 1], vol);
     double dv = (dvxdx + dvydy + dvzdz) * dt / 2.0;
     A.cv1[c] = vol * (1.0 + dv);
 
@@ -448,22 +449,22 @@ KERNEL void k_bfield_cv1(Dev D, R3 R)
     vxbm = 0.25 * (VX[omo] + VX[mmo] + VX[omm] + VX[mmm]);
     vyb = 0.25 * (VY[ooo] + VY[omo] + VY[oom] + VY[omm]);
     vybm = 0.25 * (VY[moo] + VY[mmo] + VY[mom] + VY[mmm]);
-    double dvxdy = (vxb * A.dyab[c] - vxbm * A.dyab[c - sy]) / vol;
-    double dvydx = (vyb * A.dxab[c] - vybm * A.dxab[c - sx]) / vol;
+    double dvxdy = ddiv(vxb * A.dyab[c] - vxbm * A.dyab[c - sy], vol);
+    double dvydx = ddiv(vyb * A.dxab[c] - vybm * A.dxab[c - sx], vol);
 
     vxb = 0.25 * (VX[ooo] + VX[moo] + VX[omo] + VX[mmo]);
     vxbm = 0.25 * (VX[oom] + VX[mom] + VX[omm] + VX[mmm]);
     vzb = 0.25 * (VZ[ooo] + VZ[omo] + VZ[oom] + VZ[omm]);
     vzbm = 0.25 * (VZ[moo] + VZ[mmo] + VZ[mom] + VZ[mmm]);
-    double dvxdz = (vxb * A.dzab[c] - vxbm * A.dzab[c - 1]) / vol;
-    double dvzdx = (vzb * A.dxab[c] - vzbm * A.dxab[c - sx]) / vol;
+    double dvxdz = ddiv(vxb * A.dzab[c] - vxbm * A.dzab[c - 1], vol);
+    double dvzdx = ddiv(vzb * A.dxab[c] - vzbm * A.dxab[c - sx], vol);
 
     vyb = 0.25 * (VY[ooo] + VY[moo] + VY[omo] + VY[mmo]);
     vybm = 0.25 * (VY[oom] + VY[mom] + VY[omm] + VY[mmm]);
     vzb = 0.25 * (VZ[ooo] + VZ[moo] + VZ[oom] + VZ[mom]);
     vzbm = 0.25 * (VZ[omo] + VZ[mmo] + VZ[omm] + VZ[mmm]);
-    double dvydz = (vyb * A.dzab[c] - vybm * A.dzab[c - 1]) / vol;
-    double dvzdy = (vzb * A.dyab[c] - vzbm * A.dyab[c - sy]) / vol;
+    double dvydz = ddiv(vyb * A.dzab[c] - vybm * A.dzab[c - 1], vol);
+    double dvzdy = ddiv(vzb * A.dyab[c] - vzbm * A.dyab[c - sy], vol);
 
     double bx1 = A.bx1[c], by1 = A.by1[c], bz1 = A.bz1[c];
     double w3 = bx1 * dvxdx + by1 * dvxdy + bz1 * dvxdz;
@@ -471,9 +472,9 @@ KERNEL void k_bfield_cv1(Dev D, R3 R)
     double w5 = bx1 * dvzdx + by1 * dvzdy + bz1 * dvzdz;
     // bx1 *= cv1 of predictor_step (lagran.cpp:598-601) fused: the same
     // product of the stored value, so bit-identical.
-    double nbx = (bx1 + w3 * dt / 2.0) / (1.0 + dv);
-    double nby = (by1 + w4 * dt / 2.0) / (1.0 + dv);
-    double nbz = (bz1 + w5 * dt / 2.0) / (1.0 + dv);
+    double nbx = ddiv(bx1 + w3 * dt / 2.0, 1.0 + dv);
+    double nby = ddiv(by1 + w4 * dt / 2.0, 1.0 + dv);
+    double nbz = ddiv(bz1 + w5 * dt / 2.0, 1.0 + dv);
     double cv1 = A.cv1[c];
     A.bx1[c] = nbx * cv1;
     A.by1[c] = nby * cv1;
@@ -487,12 +488,12 @@ KERNEL void k_energy_pred(Dev D, R3 R, double gas_gamma)
     const int c = IX(ix, iy, iz);
     const double dt = A.s->dt;
     double rho = RHO[c], cv = A.cv[c], cv1 = A.cv1[c];
-    double dv = cv1 / cv - 1.0;
-    double e1_e = EE[c] - A.p_e[c] * dv / rho;
-    double e1_i = EI[c] - A.p_i[c] * dv / rho;
-    e1_i += A.visc_heat[c] * dt / 2.0 / rho;
-    double pe = e1_e * (gas_gamma - 1.0) * rho * cv / cv1;
-    double pi = e1_i * (gas_gamma - 1.0) * rho * cv / cv1;
+    double dv = ddiv(cv1, cv) - 1.0;
+    double e1_e = EE[c] - ddiv(A.p_e[c] * dv, rho);
+    double e1_i = EI[c] - ddiv(A.p_i[c] * dv, rho);
+    e1_i += ddiv(A.visc_heat[c] * dt / 2.0, rho);
+    double pe = ddiv(e1_e * (gas_gamma - 1.0) * rho * cv, cv1);
+    double pi = ddiv(e1_i * (gas_gamma - 1.0) * rho * cv, cv1);
     A.p_e[c] = pe;
     A.p_i[c] = pi;
     A.pressure[c] = pe + pi;
@@ -512,13 +513,13 @@ KERNEL void k_forces(Dev D, R3 R, double mu0)
     double pp = P[c], ppx = P[x_], ppy = P[y_], ppz = P[z_], ppxy = P[xy], ppxz = P[xz], ppyz = P[yz], ppxyz = P[xyz];
     double w1 = pp + ppy + ppz + ppyz;
     double w2 = ppx + ppxy + ppxz + ppxyz;
-    double fx = (w1 - w2) * 0.25 / dx;
+    double fx = ddiv((w1 - w2) * 0.25, dx);
     w1 = pp + ppx + ppz + ppxz;
     w2 = ppy + ppxy + ppyz + ppxyz;
-    double fy = (w1 - w2) * 0.25 / dhy;
+    double fy = ddiv((w1 - w2) * 0.25, dhy);
     w1 = pp + ppx + ppy + ppxy;
     w2 = ppz + ppxz + ppyz + ppxyz;
-    double fz = (w1 - w2) * 0.25 / dhz;
+    double fz = ddiv((w1 - w2) * 0.25, dhz);
 
     double cvx = C1[c] + C1[y_] + C1[z_] + C1[yz];
     double cvxp = C1[x_] + C1[xy] + C1[xz] + C1[xyz];
@@ -527,40 +528,40 @@ KERNEL void k_forces(Dev D, R3 R, double mu0)
     double cvz = C1[c] + C1[x_] + C1[y_] + C1[xy];
     double cvzp = C1[z_] + C1[xz] + C1[yz] + C1[xyz];
 
-    w1 = (B1z[c] + B1z[x_] + B1z[z_] + B1z[xz]) / cvy;
-    w2 = (B1z[y_] + B1z[xy] + B1z[yz] + B1z[xyz]) / cvyp;
-    double jx = (w2 - w1) / dhy;
-    w1 = (B1y[c] + B1y[x_] + B1y[y_] + B1y[xy]) / cvz;
-    w2 = (B1y[z_] + B1y[xz] + B1y[yz] + B1y[xyz]) / cvzp;
-    jx -= (w2 - w1) / dhz;
+    w1 = ddiv(B1z[c] + B1z[x_] + B1z[z_] + B1z[xz], cvy);
+    w2 = ddiv(B1z[y_] + B1z[xy] + B1z[yz] + B1z[xyz], cvyp);
+    double jx = ddiv(w2 - w1, dhy);
+    w1 = ddiv(B1y[c] + B1y[x_] + B1y[y_] + B1y[xy], cvz);
+    w2 = ddiv(B1y[z_] + B1y[xz] + B1y[yz] + B1y[xyz], cvzp);
+    jx -= ddiv(w2 - w1, dhz);
 
-    w1 = (B1z[c] + B1z[y_] + B1z[z_] + B1z[yz]) / cvx;
-    w2 = (B1z[x_] + B1z[xy] + B1z[xz] + B1z[xyz]) / cvxp;
-    double jy = -(w2 - w1) / dx;
-    w1 = (B1x[c] + B1x[x_] + B1x[y_] + B1x[xy]) / cvz;
-    w2 = (B1x[z_] + B1x[xz] + B1x[yz] + B1x[xyz]) / cvzp;
-    jy += (w2 - w1) / dhz;
+    w1 = ddiv(B1z[c] + B1z[y_] + B1z[z_] + B1z[yz], cvx);
+    w2 = ddiv(B1z[x_] + B1z[xy] + B1z[xz] + B1z[xyz], cvxp);
+    double jy = ddiv(-(w2 - w1), dx);
+    w1 = ddiv(B1x[c] + B1x[x_] + B1x[y_] + B1x[xy], cvz);
+    w2 = ddiv(B1x[z_] + B1x[xz] + B1x[yz] + B1x[xyz], cvzp);
+    jy += ddiv(w2 - w1, dhz);
 
-    w1 = (B1y[c] + B1y[y_] + B1y[z_] + B1y[yz]) / cvx;
-    w2 = (B1y[x_] + B1y[xy] + B1y[xz] + B1y[xyz]) / cvxp;
-    double jz = (w2 - w1) / dx;
-    w1 = (B1x[c] + B1x[x_] + B1x[z_] + B1x[xz]) / cvy;
-    w2 = (B1x[y_] + B1x[xy] + B1x[yz] + B1x[xyz]) / cvyp;
-    jz -= (w2 - w1) / dhy;
+    w1 = ddiv(B1y[c] + B1y[y_] + B1y[z_] + B1y[yz], cvx);
+    w2 = ddiv(B1y[x_] + B1y[xy] + B1y[xz] + B1y[xyz], cvxp);
+    double jz = ddiv(w2 - w1, dx);
+    w1 = ddiv(B1x[c] + B1x[x_] + B1x[z_] + B1x[xz], cvy);
+    w2 = ddiv(B1x[y_] + B1x[xy] + B1x[yz] + B1x[xyz], cvyp);
+    jz -= ddiv(w2 - w1, dhy);
 
-    double bxv = (B1x[c] + B1x[x_] + B1x[y_] + B1x[xy] + B1x[z_] + B1x[xz] + B1x[yz] + B1x[xyz]) / (cvx + cvxp);
-    double byv = (B1y[c] + B1y[x_] + B1y[y_] + B1y[xy] + B1y[z_] + B1y[xz] + B1y[yz] + B1y[xyz]) / (cvx + cvxp);
-    double bzv = (B1z[c] + B1z[x_] + B1z[y_] + B1z[xy] + B1z[z_] + B1z[xz] + B1z[yz] + B1z[xyz]) / (cvx + cvxp);
-    fx += (jy * bzv - jz * byv) / mu0;
-    fy += (jz * bxv - jx * bzv) / mu0;
-    fz += (jx * byv - jy * bxv) / mu0;
+    double bxv = ddiv(B1x[c] + B1x[x_] + B1x[y_] + B1x[xy] + B1x[z_] + B1x[xz] + B1x[yz] + B1x[xyz], cvx + cvxp);
+    double byv = ddiv(B1y[c] + B1y[x_] + B1y[y_] + B1y[xy] + B1y[z_] + B1y[xz] + B1y[yz] + B1y[xyz], cvx + cvxp);
+    double bzv = ddiv(B1z[c] + B1z[x_] + B1z[y_] + B1z[xy] + B1z[z_] + B1z[xz] + B1z[yz] + B1z[xyz], cvx + cvxp);
+    fx += ddiv(jy * bzv - jz * byv, mu0);
+    fy += ddiv(jz * bxv - jx * bzv, mu0);
+    fz += ddiv(jx * byv - jy * bxv, mu0);
     A.fx[c] = fx;
     A.fy[c] = fy;
     A.fz[c] = fz;
     double rv = A.rho_v[c];
-    VX1[c] = VX[c] + dt / 2.0 * (A.fxv[c] + fx) / rv;
-    VY1[c] = VY[c] + dt / 2.0 * (A.fyv[c] + fy) / rv;
-    VZ1[c] = VZ[c] + dt / 2.0 * (A.fzv[c] + fz) / rv;
+    VX1[c] = VX[c] + ddiv(dt / 2.0 * (A.fxv[c] + fx), rv);
+    VY1[c] = VY[c] + ddiv(dt / 2.0 * (A.fyv[c] + fy), rv);
+    VZ1[c] = VZ[c] + ddiv(dt / 2.0 * (A.fzv[c] + fz), rv);
 }
 
 // ------------------------------------------------- corrector
@@ -601,7 +602,7 @@ KERNEL void k_shock_heating(Dev D, R3 R)
          0.25 * dy * dz * A.alpha1[c + sy + 1] * a7 - 0.25 * dx * dz * A.alpha2[c - sx + 1] * a8 -
          0.25 * dy * dy * A.alpha3[c] * a9 - 0.25 * dx * dy * A.alpha3[c - sx] * a10 -
          0.25 * dy * dy * A.alpha3[c - sx - sy] * a11 - 0.25 * dx * dy * A.alpha3[c - sy] * a12);
-    A.visc_heat[c] = pmax(vh / A.cv[c], 0.0);
+    A.visc_heat[c] = pmax(ddiv(vh, A.cv[c]), 0.0);
 }
 
 // Velocity correction (lagran.cpp:754-758) over (0..n)^3.
@@ -611,9 +612,9 @@ KERNEL void k_vel_update(Dev D, R3 R)
     const int c = IX(ix, iy, iz);
     const double dt = A.s->dt;
     double rv = A.rho_v[c];
-    VX[c] += dt * (A.fxv[c] + A.fx[c]) / rv;
-    VY[c] += dt * (A.fyv[c] + A.fy[c]) / rv;
-    VZ[c] += dt * (A.fzv[c] + A.fz[c]) / rv;
+    VX[c] += ddiv(dt * (A.fxv[c] + A.fx[c]), rv);
+    VY[c] += ddiv(dt * (A.fyv[c] + A.fy[c]), rv);
+    VZ[c] += ddiv(dt * (A.fzv[c] + A.fz[c]), rv);
 }
 
 // Density and energy correction (lagran.cpp:762-796) over (1..n)^3.
@@ -631,15 +632,15 @@ KERNEL void k_rho_e_update(Dev D, R3 R)
     double vzb = 0.25 * (VZ1[ooo] + VZ1[moo] + VZ1[omo] + VZ1[mmo]);
     double vzbm = 0.25 * (VZ1[oom] + VZ1[mom] + VZ1[omm] + VZ1[mmm]);
     double vol = A.cv[c];
-    double dvxdx = (vxb * A.dxab[c] - vxbm * A.dxab[c - sx]) / vol;
-    double dvydy = (vyb * A.dyab[c] - vybm * A.dyab[c - sy]) / vol;
-    double dvzdz = (vzb * A.dzab[c] - vzbm * A.dzab[c - 1]) / vol;
+    double dvxdx = ddiv(vxb * A.dxab[c] - vxbm * A.dxab[c - sx], vol);
+    double dvydy = ddiv(vyb * A.dyab[c] - vybm * A.dyab[c - sy], vol);
+    double dvzdz = ddiv(vzb * A.dzab[c] - vzbm * A.dzab[c - 1], vol);
     double dv = (dvxdx + dvydy + dvzdz) * dt;
     A.cv1[c] = vol * (1.0 + dv);
     double rho = RHO[c];
-    EE[c] -= dv * A.p_e[c] / rho;
-    EI[c] += (dt * A.visc_heat[c] - dv * A.p_i[c]) / rho;
-    RHO[c] = rho / (1.0 + dv);
+    EE[c] -= ddiv(dv * A.p_e[c], rho);
+    EI[c] += ddiv(dt * A.visc_heat[c] - dv * A.p_i[c], rho);
+    RHO[c] = ddiv(rho, 1.0 + dv);
 }
 
 // ------------------------------------------------- remap
@@ -653,7 +654,7 @@ KERNEL void k_b_area(Dev D, R3 R, int comp, int divide)
     double *B = fsel(A, 6 + comp);
     const double *ar = comp == 0 ? A.dxab : comp == 1 ? A.dyab : A.dzab;
     if (divide)
-        B[c] /= (ar[c] + DBL_EPSILON);
+        B[c] = ddiv(B[c], ar[c] + DBL_EPSILON);
     else
         B[c] *= ar[c];
 }
@@ -677,27 +678,27 @@ KERNEL void k_remap_prep(Dev D, R3 R, int d, double xpass, double ypass, double 
     double vol = A.cv[c];
     double dv;
     if (d == 0) {
-        double dvxdx = (vxb * A.dxab[c] - vxbm * A.dxab[c - sx]) / vol;
-        double dvydy = ypass * (vyb * A.dyab[c] - vybm * A.dyab[c - sy]) / vol;
-        double dvzdz = zpass * (vzb * A.dzab[c] - vzbm * A.dzab[c - 1]) / vol;
+        double dvxdx = ddiv(vxb * A.dxab[c] - vxbm * A.dxab[c - sx], vol);
+        double dvydy = ddiv(ypass * (vyb * A.dyab[c] - vybm * A.dyab[c - sy]), vol);
+        double dvzdz = ddiv(zpass * (vzb * A.dzab[c] - vzbm * A.dzab[c - 1]), vol);
         dv = (dvydy + dvzdz) * dt;
         A.cv2[c] = vol * (1.0 + dv);
         dv = dv + dvxdx * dt;
         A.cv1[c] = vol * (1.0 + dv);
         A.db1[c] = DXB(ix) + (vxb - vxbm) * dt;
     } else if (d == 1) {
-        double dvxdx = xpass * (vxb * A.dxab[c] - vxbm * A.dxab[c - sx]) / vol;
-        double dvydy = (vyb * A.dyab[c] - vybm * A.dyab[c - sy]) / vol;
-        double dvzdz = zpass * (vzb * A.dzab[c] - vzbm * A.dzab[c - 1]) / vol;
+        double dvxdx = ddiv(xpass * (vxb * A.dxab[c] - vxbm * A.dxab[c - sx]), vol);
+        double dvydy = ddiv(vyb * A.dyab[c] - vybm * A.dyab[c - sy], vol);
+        double dvzdz = ddiv(zpass * (vzb * A.dzab[c] - vzbm * A.dzab[c - 1]), vol);
         dv = (dvxdx + dvzdz) * dt;
         A.cv2[c] = vol * (1.0 + dv);
         dv = dv + dvydy * dt;
         A.cv1[c] = vol * (1.0 + dv);
         A.db1[c] = DYB(iy) + (vyb - vybm) * dt;
     } else {
-        double dvxdx = xpass * (vxb * A.dxab[c] - vxbm * A.dxab[c - sx]) / vol;
-        double dvydy = ypass * (vyb * A.dyab[c] - vybm * A.dyab[c - sy]) / vol;
-        double dvzdz = (vzb * A.dzab[c] - vzbm * A.dzab[c - 1]) / vol;
+        double dvxdx = ddiv(xpass * (vxb * A.dxab[c] - vxbm * A.dxab[c - sx]), vol);
+        double dvydy = ddiv(ypass * (vyb * A.dyab[c] - vybm * A.dyab[c - sy]), vol);
+        double dvzdz = ddiv(vzb * A.dzab[c] - vzbm * A.dzab[c - 1], vol);
         dv = (dvxdx + dvydy) * dt;
         A.cv2[c] = vol * (1.0 + dv);
         dv = dv + dvzdz * dt;
@@ -734,10 +735,10 @@ KERNEL void k_b_flux(Dev D, R3 R, int bcomp, int wcomp, int sd, int ow, int tw)
     double dbxp = (db1[c + sd] + db1[c + sd + tw]) * 0.5;
     double dbxp2 = (db1[c + 2 * sd] + db1[c + 2 * sd + tw]) * 0.5;
     double dbxm = (db1[c - sd] + db1[c - sd + tw]) * 0.5;
-    double fm = B[c - sd] / dbxm;
-    double fi = B[c] / dbx;
-    double fp = B[c + sd] / dbxp;
-    double fp2 = B[c + 2 * sd] / dbxp2;
+    double fm = ddiv(B[c - sd], dbxm);
+    double fi = ddiv(B[c], dbx);
+    double fp = ddiv(B[c + sd], dbxp);
+    double fp2 = ddiv(B[c + 2 * sd], dbxp2);
     double dfm = fi - fm, dfi = fp - fi, dfp = fp2 - fp;
     double sign_v = (v_advect >= 0.0) ? 1.0 : -1.0;
     double vad_p = (sign_v + 1.0) * 0.5;
@@ -748,7 +749,7 @@ KERNEL void k_b_flux(Dev D, R3 R, int bcomp, int wcomp, int sd, int ow, int tw)
     double dxcu = A.cvc1[c - sd] * vad_p + A.cvc1[c + sd] * vad_m;
     double dxbu = A.cv1[c] * vad_p + A.cv1[c + sd] * vad_m;
     double dxu = dbx * vad_p + dbxp * vad_m;
-    double phi = fabs(o_v) / dxu;
+    double phi = ddiv(fabs(o_v), dxu);
     double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
     A.rflux[c] = (fu + Di * (1.0 - phi)) * o_v;
 }
@@ -799,7 +800,7 @@ KERNEL void k_mass_flux(Dev D, R3 R, int d)
     double dxci = A.cvc1[c];
     double dxcu = A.cvc1[c - sd] * vad_p + A.cvc1[c + sd] * vad_m;
     double dxbu = A.cv1[c] * vad_p + A.cv1[c + sd] * vad_m;
-    double phi = fabs(o_v) / dxbu;
+    double phi = ddiv(fabs(o_v), dxbu);
     double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
     DM[c] = (fu + Di * (1.0 - phi)) * o_v;
 }
@@ -809,7 +810,7 @@ KERNEL void k_rho_remap(Dev D, R3 R, int sd)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
-    RHO[c] = (A.rho1[c] * A.cv1[c] + DM[c - sd] - DM[c]) / A.cv2[c];
+    RHO[c] = ddiv(A.rho1[c] * A.cv1[c] + DM[c - sd] - DM[c], A.cv2[c]);
 }
 
 // x/y/z_energy_flux (xremap.cpp:429-483, yremap.cpp:452-506,
@@ -844,10 +845,10 @@ KERNEL void k_energy_flux(Dev D, R3 R, int d, int e)
     double dxci = A.cvc1[c];
     double dxcu = A.cvc1[c - sd] * vad_p + A.cvc1[c + sd] * vad_m;
     double dxbu = A.cv1[c] * vad_p + A.cv1[c + sd] * vad_m;
-    double phi = fabs(o_v) / dxbu;
+    double phi = ddiv(fabs(o_v), dxbu);
     double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
     double rhou = A.rho1[c] * vad_p + A.rho1[c + sd] * vad_m;
-    double dmu = fabs(DM[c]) / dxbu / rhou;
+    double dmu = ddiv(ddiv(fabs(DM[c]), dxbu), rhou);
     A.rflux[c] = (fu + Di * (1.0 - dmu)) * DM[c];
 }
 
@@ -857,7 +858,7 @@ KERNEL void k_energy_remap(Dev D, R3 R, int sd, int e)
     KBEGIN
     const int c = IX(ix, iy, iz);
     double *E = fsel(A, e);
-    E[c] = (E[c] * A.cv1[c] * A.rho1[c] + A.rflux[c - sd] - A.rflux[c]) / (A.cv2[c] * RHO[c]);
+    E[c] = ddiv(E[c] * A.cv1[c] * A.rho1[c] + A.rflux[c - sd] - A.rflux[c], A.cv2[c] * RHO[c]);
 }
 
 // Vertex density before remap (xremap.cpp:159-176, yremap.cpp:163-183,
@@ -869,9 +870,9 @@ KERNEL void k_vertex_rho(Dev D, R3 R, int d)
     const int c = IX(ix, iy, iz);
     double s = SUM8(A.rho1, A.cv1, c);
     if (d == 0)
-        A.rrho_v[c] = s * 0.125 / A.cvc1[c];
+        A.rrho_v[c] = ddiv(s * 0.125, A.cvc1[c]);
     else
-        A.rrho_v[c] = s * (0.125 / A.cvc1[c]);
+        A.rrho_v[c] = s * ddiv(0.125, A.cvc1[c]);
 }
 
 // Vertex cv2 (xremap.cpp:180-190 and twins) over (0..n)^3 into cv2v.
@@ -914,7 +915,7 @@ KERNEL void k_rho_v1(Dev D, R3 R, int sd)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
-    A.rho_v1[c] = (A.rrho_v[c] * A.cvc1[c] + A.dmv[c - sd] - A.dmv[c]) / A.cv2v[c];
+    A.rho_v1[c] = ddiv(A.rrho_v[c] * A.cvc1[c] + A.dmv[c - sd] - A.dmv[c], A.cv2v[c]);
 }
 
 // x/y/z_mom_flux (xremap.cpp:485-566, yremap.cpp:508-588, zremap.cpp:461-541)
@@ -941,11 +942,11 @@ KERNEL void k_mom_flux(Dev D, R3 R, int d, int comp)
     double dxci = A.cv1[c + sd];
     double dxcu = A.cv1[c] * vad_p + A.cv1[c + 2 * sd] * vad_m;
     double dxbu = A.cvc1[c] * vad_p + A.cvc1[c + sd] * vad_m;
-    double phi = fabs(o_v) / dxbu;
+    double phi = ddiv(fabs(o_v), dxbu);
     double Di = vl_di(sign_v, dfi, dfu, phi, dxci, dxcu, dxbu);
     double rhou = A.rrho_v[c] * vad_p + A.rrho_v[c + sd] * vad_m;
     double dm = A.dmv[c];
-    double dmu = d == 2 ? fabs(dm) / (dxbu * rhou) : fabs(dm) / dxbu / rhou;
+    double dmu = d == 2 ? ddiv(fabs(dm), dxbu * rhou) : ddiv(ddiv(fabs(dm), dxbu), rhou);
     double fl = fu + Di * (1.0 - dmu);
     A.rflux[c] = fl * dm;
 }
@@ -956,7 +957,7 @@ KERNEL void k_mom_remap(Dev D, R3 R, int sd, int comp)
     KBEGIN
     const int c = IX(ix, iy, iz);
     double *v = fsel(A, 3 + comp);
-    v[c] = (A.rrho_v[c] * v[c] * A.cvc1[c] + A.rflux[c - sd] - A.rflux[c]) / (A.cv2v[c] * A.rho_v1[c]);
+    v[c] = ddiv(A.rrho_v[c] * v[c] * A.cvc1[c] + A.rflux[c - sd] - A.rflux[c], A.cv2v[c] * A.rho_v1[c]);
 }
 
 // ------------------------------------------------- boundary conditions

@@ -30,20 +30,17 @@ __device__ __forceinline__ double pmin_list4(double a, double b, double c, doubl
 
 static constexpr double kSixth = 1.0 / 6.0;  // constants.h:46
 
-// Quotients by a shared divisor, equal bit for bit to IEEE a / d.
-// hipcc lowers a / d (FP64, gfx950) to: v_div_scale of d and of a,
-// y = v_rcp_f64(d') refined by two Newton steps (e = fma(-d', y, 1),
-// y = fma(y, e, y)), m = a' * y, r = fma(-d', m, a'), q = v_div_fmas(r, y,
-// m), then v_div_fixup.  The two scale steps return their operand unchanged
-// and v_div_fmas is a plain fma unless an operand or the quotient is near the
-// exponent limits, and v_div_fixup then only sets the sign.  QDiv keeps d and
-// its refined reciprocal y (computed once, as above), so a quotient costs
-// m = a * y and q = fma(-fma(d, m, -a), y, m): the same operations on the
-// same values (fma(d, m, -a) is exactly -r), for |a| and d in
-// [2^-300, 2^301).  For d > 0 a zero a gives the signed zero a / d gives
-// (-fma(d, m, -a) is -0 there, so the sum keeps m's sign).  Any other a, and
-// every a when d is negative or out of that range, marks the quotient slow:
-// the caller recomputes it with the full division.
+// Quotients by a shared divisor, equal bit for bit to IEEE a / d (and to
+// ddiv).  QDiv keeps d and y = 1/d rounded to nearest (computed once), so a
+// quotient costs m = a * y and q = fma(-fma(d, m, -a), y, m): ddiv's last
+// steps without its scaling, v_div_fmas and v_div_fixup (fma(d, m, -a) is
+// exactly -r).  The scaling steps return their operand unchanged and
+// v_div_fmas is a plain fma unless an operand or the quotient is near the
+// exponent limits, and v_div_fixup then only sets the sign; so for |a| and d
+// in [2^-300, 2^301) the two agree.  For d > 0 a zero a gives the signed zero
+// a / d gives (-fma(d, m, -a) is -0 there, so the sum keeps m's sign).  Any
+// other a, and every a when d is negative or out of that range, marks the
+// quotient slow: the caller recomputes it with ddiv.
 struct QDiv {
     double d, y;
     bool ok;
@@ -51,12 +48,8 @@ struct QDiv {
 __device__ __forceinline__ unsigned qdiv_expo(double v) { return ((unsigned)__double2hiint(v) >> 20) & 0x7ffu; }
 __device__ __forceinline__ QDiv qdiv_of(double d)
 {
-    double y = __builtin_amdgcn_rcp(d);
-    double e = fma(-d, y, 1.0);
-    y = fma(y, e, y);
-    e = fma(-d, y, 1.0);
-    y = fma(y, e, y);
-    return QDiv{d, y, d > 0.0 && qdiv_expo(d) - 723u <= 600u};
+    const bool ok = d > 0.0 && qdiv_expo(d) - 723u <= 600u;
+    return QDiv{d, rcp_nearest(d), ok};
 }
 // The quotient, with `slow` set where it may differ from a / d.
 __device__ __forceinline__ double qdiv_f(double a, const QDiv &R, bool &slow)
@@ -70,7 +63,7 @@ __device__ __forceinline__ double qdiv(double a, const QDiv &R)
 {
     bool slow = false;
     double q = qdiv_f(a, R, slow);
-    if (__builtin_expect(slow, 0)) q = a / R.d;
+    if (__builtin_expect(slow, 0)) q = ddiv(a, R.d);
     return q;
 }
 
@@ -78,7 +71,7 @@ __device__ __forceinline__ double qdiv(double a, const QDiv &R)
 // (xremap.cpp:304-316 and twins).
 __device__ __forceinline__ double vl_di(double sign_v, double dfi, double dfu, double phi, double dxci, double dxcu, double dxbu)
 {
-    double Da = (2.0 - phi) * fabs(dfi) / dxci + (1.0 + phi) * fabs(dfu) / dxcu;
+    double Da = ddiv((2.0 - phi) * fabs(dfi), dxci) + ddiv((1.0 + phi) * fabs(dfu), dxcu);
     Da = Da * kSixth;
     double ss = 0.5 * ((dfi >= 0.0 ? 1.0 : -1.0) + (dfu >= 0.0 ? 1.0 : -1.0));
     return sign_v * ss * pmin_list3(fabs(Da) * dxbu, fabs(dfi), fabs(dfu));

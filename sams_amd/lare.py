@@ -115,6 +115,8 @@ def lib():
             "lare_profile_get": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32,
                                                 P(ctypes.c_int32), P(ctypes.c_double), P(ctypes.c_double)]),
             "lare_energy": (ctypes.c_int, [vp, P(ctypes.c_double)]),
+            "lare_div_check": (ctypes.c_int, [P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double),
+                                              ctypes.c_int64, ctypes.c_int32]),
             "lare_batch_create": (ctypes.c_int, [P(vp), ctypes.c_int32, P(vp)]),
             "lare_batch_destroy": (ctypes.c_int, [vp]),
             "lare_batch_steps": (ctypes.c_int, [vp, ctypes.c_int32]),
@@ -164,6 +166,20 @@ def problem_fill(p, name):
                                    a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
            "lare_problem_fill")
     return a
+
+
+def div_check(a, d, mode=0):
+    """a / d computed on the device as the kernels compute quotients: by the
+    division every kernel uses (mode 0) or the shared-divisor form (mode 1)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    if a.shape != d.shape:
+        raise ValueError("div_check: a and d differ in shape")
+    q = np.empty_like(a)
+    P = ctypes.POINTER(ctypes.c_double)
+    _check(lib().lare_div_check(a.ctypes.data_as(P), d.ctypes.data_as(P), q.ctypes.data_as(P), a.size, mode),
+           "lare_div_check")
+    return q
 
 
 class Lare:

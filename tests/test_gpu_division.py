@@ -1,0 +1,107 @@
+"""The device's FP64 quotients against IEEE division (numpy on the host).
+
+Every kernel divides with one routine (and the shared-divisor form built on
+the same reciprocal); the reference divides with the host's IEEE division,
+so the two must agree bit for bit.  The cases are the ones that break a
+reciprocal left one unit away from the nearest: divisors 1 - k 2^-53 with
+small odd k, whose reciprocals lie next to a midpoint, paired with
+numerators that put the quotient next to a midpoint too; divisors with an
+all-ones significand; and random operands over the whole range.
+"""
+import numpy as np
+import pytest
+
+from sams_amd import lare
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(x):
+    return np.ascontiguousarray(x, dtype=np.float64).view(np.uint64)
+
+
+def _agree(a, d, mode):
+    q = lare.div_check(a, d, mode)
+    want = a / d
+    bad = (_bits(q) != _bits(want)) & ~(np.isnan(q) & np.isnan(want))
+    idx = np.flatnonzero(bad)
+    assert idx.size == 0, (
+        f"mode {mode}: {idx.size} of {a.size} quotients differ, first "
+        f"{a[idx[0]].hex()} / {d[idx[0]].hex()}: device {q[idx[0]].hex()} host {want[idx[0]].hex()}")
+
+
+def _near_midpoint(rng, n, kmax):
+    """a / d with d = 1 - k 2^-53 (k odd, < kmax) and a = m 2^-52 chosen so
+    that m k = 2^52 + s (mod 2^53), |s| <= 64: the exact quotient lies within
+    (|s| + k^2 2^-53) 2^-53 units of a midpoint."""
+    a, d = [], []
+    M = 1 << 53
+    while len(a) < n:
+        k = int(rng.integers(0, kmax // 2)) * 2 + 1
+        s = int(rng.integers(-64, 65))
+        m = ((1 << 52) + s) * pow(k, -1, M) % M
+        if m < (1 << 52):
+            continue
+        ea, ed = (int(v) for v in rng.integers(-20, 21, 2))
+        a.append(np.ldexp(float(m), -52 + ea) * (1 if rng.integers(2) else -1))
+        d.append(np.ldexp(1.0 - np.ldexp(float(k), -53), ed))
+    return np.array(a), np.array(d)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_quotient_found_in_a_wind_tunnel_step(mode):
+    # rho / (1 + dv) of one cell of the Emery wind tunnel's fourth step
+    a = np.array([1.3999999999999992])
+    d = np.array([1.0 - 5 * 2.0 ** -53])
+    assert (a / d)[0] == 1.4000000000000001
+    _agree(a, d, mode)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_reciprocals_of_divisors_just_below_one(mode):
+    k = np.arange(1, 1 << 16, dtype=np.float64)
+    d = 1.0 - k * 2.0 ** -53
+    for e in (0, -7, 12):
+        _agree(np.ones_like(d), np.ldexp(d, e), mode)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("kmax", [1 << 8, 1 << 12, 1 << 16])
+def test_quotients_next_to_midpoints(mode, kmax):
+    a, d = _near_midpoint(np.random.default_rng(kmax), 1 << 16, kmax)
+    _agree(a, d, mode)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_all_ones_significands(mode):
+    rng = np.random.default_rng(7)
+    n = 1 << 14
+    d = np.ldexp(np.full(n, 2.0 - 2.0 ** -52), rng.integers(-100, 101, n))
+    a = np.ldexp(np.where(np.arange(n) % 2 == 0, 1.0, rng.uniform(1.0, 2.0, n)), rng.integers(-100, 101, n))
+    d[1::4] = -d[1::4]
+    a[2::4] = -a[2::4]
+    _agree(a, d, mode)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_random_operands(mode):
+    rng = np.random.default_rng(11)
+    n = 1 << 20
+    a = np.ldexp(rng.uniform(1.0, 2.0, n), rng.integers(-60, 61, n)) * rng.choice([-1.0, 1.0], n)
+    d = np.ldexp(rng.uniform(1.0, 2.0, n), rng.integers(-60, 61, n)) * rng.choice([-1.0, 1.0], n)
+    _agree(a, d, mode)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_special_and_extreme_operands(mode):
+    rng = np.random.default_rng(13)
+    a = rng.integers(0, 1 << 63, 1 << 18, dtype=np.uint64).view(np.float64).copy()
+    d = rng.integers(0, 1 << 63, 1 << 18, dtype=np.uint64).view(np.float64).copy()
+    a[1::2] = -a[1::2]
+    special = [(0.0, 0.0), (1.0, 0.0), (-0.0, 3.0), (0.0, -3.0), (np.inf, np.inf), (np.inf, 2.0),
+               (2.0, np.inf), (np.nan, 1.0), (1e-310, 3.0), (1.0, 1e-310), (1e300, 1e-300),
+               (1e-300, 1e300), (5e-324, 0.5), (np.finfo(float).max, 0.5)]
+    for i, (x, y) in enumerate(special):
+        a[i], d[i] = x, y
+    with np.errstate(all="ignore"):
+        _agree(a, d, mode)
