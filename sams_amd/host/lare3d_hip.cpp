@@ -33,6 +33,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -60,6 +61,11 @@ namespace LARE
             double dtNew = 0.0;      // dt of the last lare_viscosity_dt
             double dtrNew = 0.0;
             double devDt = 0.0;      // previous dt held by the device
+            // host-synchronised mode (LARE3D_HIP_HOST_SYNC=1): the state is
+            // downloaded after every event and uploaded before the next, so
+            // that packages which write it on the host between LARE's events
+            // (the obstacles of KarmanVortex and EmeryWindTunnel) are seen
+            bool hostSync = false;
         };
 
         std::map<const simulationData *, deviceGrid> &grids()
@@ -290,6 +296,8 @@ namespace LARE
             check(lare_set_dt(g.ctx, d.dt), "lare_set_dt");
             g.devDt = d.dt;
             g.hostCurrent = true;
+            const char *sync = std::getenv("LARE3D_HIP_HOST_SYNC");
+            g.hostSync = sync && std::strcmp(sync, "0") != 0;
             return g;
         }
 
@@ -309,11 +317,33 @@ namespace LARE
             }
             it->second.hostCurrent = true;
         }
+
+        // Host-synchronised mode: before an event, the harness arrays (which
+        // other packages may have written since the last event) replace the
+        // device state; after it, the device state replaces them.
+        void beginEvent(deviceGrid &g, simulationData &d)
+        {
+            if (!g.hostSync || !g.hostCurrent)
+                return;
+            std::vector<double> buf;
+            for (int f = 0; f < LARE_NFIELDS; ++f)
+            {
+                toBuffer(*stateArray(d, f), buf);
+                check(lare_upload_field(g.ctx, f, buf.data()), "lare_upload_field");
+            }
+        }
+        void endEvent(deviceGrid &g, simulationData &d)
+        {
+            g.hostCurrent = false;
+            if (g.hostSync)
+                download(d);
+        }
     } // namespace
 
     void LARE3D::lagrangian_step(simulationData &data, SAMS::controlFunctions &controlFns)
     {
         deviceGrid &g = bind(harness, data);
+        beginEvent(g, data);
         // the shock viscosity reads the previous dt (lagran.cpp:72): the
         // runner's, which another package may have lowered
         if (data.dt != g.devDt)
@@ -322,7 +352,7 @@ namespace LARE
         controlFns.calculateTimestep();
         check(lare_lagrange(g.ctx, data.dt), "lare_lagrange");
         g.devDt = data.dt;
-        g.hostCurrent = false;
+        endEvent(g, data);
     }
 
     void LARE3D::set_dt(simulationData &data)
@@ -342,8 +372,9 @@ namespace LARE
     void LARE3D::corrector_step(simulationData &data)
     {
         deviceGrid &g = bind(harness, data);
+        beginEvent(g, data);
         check(lare_half_timestep(g.ctx), "lare_half_timestep");
-        g.hostCurrent = false;
+        endEvent(g, data);
     }
 
     void LARE3D::eta_calc(simulationData &)
@@ -355,8 +386,9 @@ namespace LARE
     void LARE3D::eulerian_remap(simulationData &data, remapData &)
     {
         deviceGrid &g = bind(harness, data);
+        beginEvent(g, data);
         check(lare_end_of_timestep(g.ctx), "lare_end_of_timestep");
-        g.hostCurrent = false;
+        endEvent(g, data);
     }
 
     void LARE3D::remap_x(simulationData &, remapData &)

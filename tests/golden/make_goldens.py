@@ -16,7 +16,7 @@ the LARE initial-condition package's BC_OTHER conditions to every edge
 (src/InitialConditions/LARE/lareic.h:124-146, boundary.cpp) instead of the
 problem's built-in ones.
 
-    make -C oracle ref && python tests/golden/make_goldens.py
+    make -C oracle ref ic && python tests/golden/make_goldens.py
 """
 import hashlib
 import json
@@ -58,6 +58,31 @@ CASES = [
 ]
 
 
+# The reference's own initial-condition packages (src/InitialConditions/*),
+# run by oracle/_ref/ic_driver as the reference's src/main.cpp runs them:
+# package, steps, OpenMP threads (EmeryWindTunnel turns rke on: one thread)
+IC_CASES = [
+    ("SodShockTube", 10, 8),
+    ("BrioAndWu", 10, 8),
+    ("MHDRotor", 10, 8),
+    ("OrszagTang", 10, 8),
+    ("OrszagTang3D", 3, 8),
+    ("EmeryWindTunnel", 4, 1),
+    ("KarmanVortex", 4, 8),
+]
+IC_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ic_driver")
+
+
+def run_ic(exe, out_dir, ic, steps, threads):
+    """Run an initial-condition driver build; return its meta dict."""
+    import subprocess
+    os.makedirs(out_dir, exist_ok=True)
+    subprocess.run([exe, "--ic", ic, "--steps", str(steps), "--dump", str(steps), "--out", out_dir], cwd=out_dir,
+                   env=dict(os.environ, OMP_NUM_THREADS=str(threads)), check=True, capture_output=True)
+    with open(os.path.join(out_dir, "meta.json")) as f:
+        return json.load(f)
+
+
 def digest(path):
     a = np.fromfile(path, dtype="<f8")
     return {"sha256": hashlib.sha256(a.tobytes()).hexdigest(),
@@ -84,6 +109,13 @@ def main():
                               "lare_bc": bool(extra.get("lare_bc", False)),
                               "dt": meta["dt"], "snapshots": snaps}
         print(name, "done", flush=True)
+    out["ic_cases"] = {}
+    for ic, steps, threads in IC_CASES:
+        with tempfile.TemporaryDirectory() as td:
+            meta = run_ic(IC_DRIVER, td, ic, steps, threads)
+            snaps = {str(steps): {f: digest(os.path.join(td, f"s{steps:05d}_{f}.f64")) for f in FIELDS}}
+        out["ic_cases"][ic] = {"steps": steps, "threads": threads, "dt": meta["dt"], "snapshots": snaps}
+        print(ic, "done", flush=True)
     with open(os.path.join(ROOT, "tests", "golden", "goldens.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
 

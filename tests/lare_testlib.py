@@ -19,6 +19,12 @@ def goldens():
         return json.load(f)["cases"]
 
 
+def ic_goldens():
+    """Runs of the reference's own initial-condition packages."""
+    with open(GOLDEN) as f:
+        return json.load(f)["ic_cases"]
+
+
 def case_problem(c):
     """LareProblem for a golden case."""
     return L.problem(c["problem"], n=c["nx"], ny=c["ny"], nz=c["nz"], seed=c["seed"],

@@ -38,7 +38,7 @@ def test_errors_are_reported_not_raised():
     h = ctypes.c_void_p()
     rc = L.lib().lare_ctx_create(ctypes.byref(d), ctypes.byref(h))
     assert rc != 0
-    assert b"at least 3" in L.lib().lare_last_error()
+    assert b"at least 2" in L.lib().lare_last_error()
 
 
 @pytest.mark.parametrize("name", ["ot_12", "ot_pert_10x8x12", "bw_zerograd_24x4x6", "bw_walls_16x5x3"])
