@@ -193,7 +193,9 @@ int lare_energy(lare_ctx *ctx, double *out);
  * self-check of the device's FP64 division, for tests).  q[i] = a[i] / d[i]
  * rounded to nearest even, for host arrays of n values, computed on the
  * device by the division every kernel uses (mode 0) or by the
- * shared-divisor quotient (mode 1).  Blocking; allocates device memory. */
+ * shared-divisor quotient (mode 1); mode 2 gives q[i] = sqrt(a[i]) as the
+ * kernels compute it (d is read but unused).  Blocking; allocates device
+ * memory. */
 int lare_div_check(const double *a, const double *d, double *q, int64_t n, int32_t mode);
 
 /* ------------------------------------------------------------------------

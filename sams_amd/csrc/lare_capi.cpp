@@ -1052,7 +1052,7 @@ extern "C" int lare_energy(lare_ctx *c, double *out)
 extern "C" int lare_div_check(const double *a, const double *d, double *q, int64_t n, int32_t mode)
 {
     if (n < 0 || (n > 0 && (!a || !d || !q))) return fail(-1, "lare_div_check: null argument");
-    if (mode != 0 && mode != 1) return fail(-1, "lare_div_check: mode must be 0 or 1");
+    if (mode < 0 || mode > 2) return fail(-1, "lare_div_check: mode must be 0, 1 or 2");
     if (n > (int64_t(1) << 31)) return fail(-1, "lare_div_check: at most 2^31 values");
     std::string err;
     int rc = lare::div_check(a, d, q, (long)n, mode, err);

@@ -93,7 +93,8 @@ inline void group_step(Group &gp)
 int group_energy(Group &gp, std::vector<double> &out, std::string &err);
 
 // q[i] = a[i] / d[i] for host arrays of n values, computed on the device by
-// the kernels' division (mode 0) or shared-divisor quotient (mode 1).
+// the kernels' division (mode 0) or shared-divisor quotient (mode 1); mode 2
+// gives q[i] = sqrt(a[i]) as the kernels compute it (d unused).
 int div_check(const double *a, const double *d, double *q, long n, int mode, std::string &err);
 
 }  // namespace lare

@@ -3908,13 +3908,13 @@ int group_energy(Group &gp, std::vector<double> &out, std::string &err)
     return 0;
 }
 
-// Division self-check: q[i] = a[i] / d[i] as the kernels form it, by ddiv
-// (mode 0) or by a shared-divisor QDiv (mode 1).
+// Arithmetic self-check: q[i] = a[i] / d[i] as the kernels form it, by ddiv
+// (mode 0) or by a shared-divisor QDiv (mode 1), or q[i] = sqrt(a[i]) (mode 2).
 KERNEL void k_div_check(const double *a, const double *d, double *q, long n, int mode)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    q[i] = mode == 0 ? ddiv(a[i], d[i]) : qdiv(a[i], qdiv_of(d[i]));
+    q[i] = mode == 0 ? ddiv(a[i], d[i]) : mode == 1 ? qdiv(a[i], qdiv_of(d[i])) : sqrt(a[i]);
 }
 
 int div_check(const double *a, const double *d, double *q, long n, int mode, std::string &err)

@@ -170,7 +170,8 @@ def problem_fill(p, name):
 
 def div_check(a, d, mode=0):
     """a / d computed on the device as the kernels compute quotients: by the
-    division every kernel uses (mode 0) or the shared-divisor form (mode 1)."""
+    division every kernel uses (mode 0) or the shared-divisor form (mode 1);
+    mode 2 gives sqrt(a) as the kernels compute it (d unused)."""
     a = np.ascontiguousarray(a, dtype=np.float64)
     d = np.ascontiguousarray(d, dtype=np.float64)
     if a.shape != d.shape:

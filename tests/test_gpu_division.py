@@ -1,4 +1,5 @@
-"""The device's FP64 quotients against IEEE division (numpy on the host).
+"""The device's FP64 quotients and square roots against IEEE arithmetic
+(numpy on the host).
 
 Every kernel divides with one routine (and the shared-divisor form built on
 the same reciprocal); the reference divides with the host's IEEE division,
@@ -105,3 +106,36 @@ def test_special_and_extreme_operands(mode):
         a[i], d[i] = x, y
     with np.errstate(all="ignore"):
         _agree(a, d, mode)
+
+
+def _near_root_midpoints(rng, n):
+    """x in [1, 4) (scaled by even powers of two) whose square root lies
+    within 1/256 unit of a midpoint, found with the host's 64-bit-significand
+    long double."""
+    out = []
+    while len(out) < n:
+        x = rng.uniform(1.0, 4.0, 1 << 16)
+        r = np.sqrt(x.astype(np.longdouble))
+        lo = r.astype(np.float64)
+        up = np.nextafter(lo, np.where(r > lo, np.inf, -np.inf))
+        mid = (lo.astype(np.longdouble) + up.astype(np.longdouble)) / 2
+        dist = np.abs(r - mid) / np.abs(up.astype(np.longdouble) - lo.astype(np.longdouble))
+        out.extend(x[dist < 1.0 / 256].tolist())
+    x = np.array(out[:n])
+    return np.ldexp(x, 2 * rng.integers(-100, 101, n))
+
+
+def test_square_roots():
+    """sqrt as the kernels compute it (hipcc's sequence) against the host's."""
+    rng = np.random.default_rng(17)
+    assert np.finfo(np.longdouble).nmant >= 63, "needs the x86 long double"
+    x = np.concatenate([
+        _near_root_midpoints(rng, 1 << 15),
+        np.ldexp(rng.uniform(1.0, 4.0, 1 << 18), 2 * rng.integers(-200, 201, 1 << 18)),
+        rng.integers(0, 0x7ff0000000000000, 1 << 18, dtype=np.uint64).view(np.float64),
+        np.array([0.0, -0.0, np.inf, 5e-324, 1e-310, np.finfo(float).max, 2.0, 3.0]),
+    ])
+    q = lare.div_check(x, np.ones_like(x), 2)
+    want = np.sqrt(x)
+    bad = np.flatnonzero(_bits(q) != _bits(want))
+    assert bad.size == 0, f"{bad.size} of {x.size} roots differ, first sqrt({x[bad[0]].hex()}): device {q[bad[0]].hex()} host {want[bad[0]].hex()}"
