@@ -202,3 +202,47 @@ def test_dropin_runs_the_reference_initial_conditions(ic):
         for f in FIELDS:
             a = np.fromfile(os.path.join(td, f"s{steps:05d}_{f}.f64"), dtype="<f8")
             assert hashlib.sha256(a.tobytes()).hexdigest() == c["snapshots"][str(steps)][f]["sha256"], f
+
+
+IC_DRIVER = os.path.join(os.path.dirname(O.DROPIN_DRIVER), "ic_driver")
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(os.environ.get("LARE_LONG_RUNS") != "1", reason="long runs (minutes): set LARE_LONG_RUNS=1")
+@pytest.mark.skipif(not (os.path.exists(IC_DROPIN) and os.path.exists(IC_DRIVER)),
+                    reason="oracle/_ref/ic_driver and ic_dropin not built (needs the reference tree)")
+@pytest.mark.parametrize("ic,steps", [("SodShockTube", 500), ("BrioAndWu", 500), ("MHDRotor", 300),
+                                      ("OrszagTang", 300), ("OrszagTang3D", 60), ("KarmanVortex", 200),
+                                      ("EmeryWindTunnel", 120)])
+def test_dropin_long_runs_of_the_reference_initial_conditions(ic, steps):
+    """The reference built from its own sources and the drop-in, both under
+    the reference runner, over many steps (or to the package's end time):
+    identical dt sequences and identical state, ghosts included, after the
+    last step.  Emery runs the reference on one OpenMP thread (its rke
+    atomics make more threads nondeterministic)."""
+    import json
+    import subprocess
+    import numpy as np
+    sync = "1" if ic in ("KarmanVortex", "EmeryWindTunnel") else "0"
+    threads = "1" if ic == "EmeryWindTunnel" else "8"
+
+    def run(exe, out, n):
+        os.makedirs(out, exist_ok=True)
+        subprocess.run([exe, "--ic", ic, "--steps", str(n), "--dump", str(n), "--out", out], cwd=out, check=True,
+                       capture_output=True,
+                       env=dict(os.environ, OMP_NUM_THREADS=threads, LARE3D_HIP_HOST_SYNC=sync))
+        with open(os.path.join(out, "meta.json")) as f:
+            return json.load(f)["dt"]
+
+    with tempfile.TemporaryDirectory() as td:
+        ref = os.path.join(td, "ref")
+        dts = run(IC_DRIVER, ref, steps)
+        if len(dts) < steps:  # the package's own end time came first
+            steps = len(dts)
+            dts = run(IC_DRIVER, ref, steps)
+        dev = os.path.join(td, "dev")
+        assert run(IC_DROPIN, dev, steps) == dts, "dt sequence"
+        for f in FIELDS:
+            a = np.fromfile(os.path.join(ref, f"s{steps:05d}_{f}.f64"), dtype="<f8")
+            b = np.fromfile(os.path.join(dev, f"s{steps:05d}_{f}.f64"), dtype="<f8")
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), f
