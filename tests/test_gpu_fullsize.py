@@ -1,0 +1,40 @@
+"""The step at the benchmark's full size against the reference itself.
+
+Brio-Wu 256^3 with 1% density noise (seed 1) is the benchmark workload
+without its resistivity: the reference's resistive path throws
+(lagran.cpp:482), so its ideal-MHD twin is the largest run it can make.
+At this size the lane-walking kernels run (the small golden cases use the
+one-cell forms), so this pins them against the reference's own output:
+the reference built from its sources (oracle/_ref/ref_driver, the
+-ffp-contract=off build: the -march=x86-64-v3 timing build contracts
+multiply-adds, already in the noisy initial density) runs on the host cores, the device through the C-ABI, and every state field must
+be bit-identical, ghosts included, after each step.
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import sams_amd.lare as L
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.skipif(not os.path.exists(O.REF_DRIVER), reason="oracle/_ref/ref_driver not built")
+def test_brio_wu_256_cubed_matches_reference():
+    n, steps = 256, 8
+    with tempfile.TemporaryDirectory() as td:
+        meta = O.run_reference(td, problem="bw", nx=n, steps=steps, dumps=tuple(range(1, steps + 1)), noise=0.01,
+                               seed=1, threads=min(16, os.cpu_count() or 1))
+        dev = L.Lare.from_problem(L.problem("bw", n=n, seed=1, noise=0.01))
+        for s in range(1, steps + 1):
+            dt = dev.steps(1)[0]
+            assert dt.hex() == float.fromhex(meta["dt"][s - 1]).hex(), f"dt of step {s}"
+            for f in L.STATE_FIELDS:
+                ref = np.fromfile(os.path.join(td, f"s{s:05d}_{f}.f64"), dtype="<f8")
+                got = np.ascontiguousarray(dev.download(f), dtype="<f8").ravel()
+                assert ref.size == got.size, f
+                bad = np.flatnonzero(ref.view(np.uint64) != got.view(np.uint64))
+                assert bad.size == 0, f"{f} after step {s}: {bad.size} values differ"
