@@ -283,9 +283,12 @@ typedef struct lare_problem {
     double amp;         /* Orszag-Tang: Gaussian velocity perturbation amplitude */
     double beta;        /* Orszag-Tang: plasma beta; <= 0 keeps p = 5/(12 pi) */
     double noise;       /* Brio-Wu: relative uniform density noise */
-    double eta;         /* Brio-Wu: background resistivity; > 0 enables resistive MHD */
+    double eta;         /* background resistivity (eta_background) */
     int32_t precision;  /* enum lare_precision */
     int32_t rke;        /* remap kinetic-energy correction on */
+    double eta0;        /* resistivity added where |j| > j_max (lagran.cpp:473) */
+    double j_max;       /* current threshold of eta0; <= 0 means 1e30 (never reached) */
+    /* eta > 0 or eta0 > 0 turns resistive MHD on (shared_data.h:113-117) */
 } lare_problem;
 
 /* Grid and physics scalars of the problem. */

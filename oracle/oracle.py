@@ -15,6 +15,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblare_oracle.so")
 REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
 REF_DRIVER_FAST = os.path.join(HERE, "_ref", "ref_driver_fast")
+# The reference with its resistive path made runnable (the three assigns at
+# lagran.cpp:482-484 copy element by element; oracle/Makefile patches a copy
+# of lagran.cpp at build time): resistive runs only.
+REF_DRIVER_RES = os.path.join(HERE, "_ref", "ref_driver_res")
+REF_DRIVER_RES_FAST = os.path.join(HERE, "_ref", "ref_driver_res_fast")
 # The same driver and reference runner with the MI355X LARE3D package in place
 # of the reference's step sources (needs a GPU to run).
 DROPIN_DRIVER = os.path.join(HERE, "_ref", "lare3d_dropin")
@@ -185,11 +190,16 @@ def problem_bcs(o, kind, bcx=0):
 
 def run_reference(out_dir, problem="ot", nx=16, ny=None, nz=None, steps=10, dumps=(), seed=1,
                   amp=0.0, noise=0.0, bcx=None, threads=4, fast=False, beta=-1.0, exe=None, rke=False,
-                  lare_bc=False):
+                  lare_bc=False, eta=0.0, eta0=0.0, j_max=None):
     """Run the reference driver (oracle/_ref, built from the reference's own
     sources; exe selects another build of it, such as DROPIN_DRIVER) and
-    return its meta dict.  Raises if it has not been built."""
+    return its meta dict.  Resistive runs (eta or eta0 > 0) use the build
+    whose resistive path runs (REF_DRIVER_RES).  Raises if it has not been
+    built."""
     import json
+    resistive = eta > 0.0 or eta0 > 0.0
+    if resistive:
+        exe = exe or (REF_DRIVER_RES_FAST if fast else REF_DRIVER_RES)
     exe = exe or (REF_DRIVER_FAST if fast else REF_DRIVER)
     if not os.path.exists(exe):
         raise FileNotFoundError(exe)
@@ -205,6 +215,10 @@ def run_reference(out_dir, problem="ot", nx=16, ny=None, nz=None, steps=10, dump
         args += ["--rke"]
     if lare_bc:
         args += ["--lare-bc"]
+    if resistive:
+        args += ["--eta", repr(float(eta)), "--eta0", repr(float(eta0))]
+        if j_max is not None:
+            args += ["--j-max", repr(float(j_max))]
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     subprocess.run(args, cwd=out_dir, env=env, check=True, capture_output=True)
     with open(os.path.join(out_dir, "meta.json")) as f:

@@ -18,8 +18,12 @@
 // Usage:
 //   ref_driver --problem ot|bw --nx N [--ny N --nz N] --steps K
 //              [--dump s1,s2,...] [--out DIR] [--seed S] [--amp A] [--beta B]
-//              [--noise F] [--eta E] [--bcx periodic|zerograd|mirror|clamp]
-//              [--time] [--rke] [--lare-bc]
+//              [--noise F] [--eta E] [--eta0 E0] [--j-max J]
+//              [--bcx periodic|zerograd|mirror|clamp] [--time] [--rke] [--lare-bc]
+// --eta / --eta0 (either > 0) turn resistive MHD on (shared_data.h:113-117):
+// eta_background = E, and eta0 is added where the current exceeds j_max
+// (lagran.cpp:473).  The reference's resistive path runs only in the
+// ref_driver_res build (oracle/Makefile).
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -65,7 +69,9 @@ namespace refdrv
         double amp = 0.0;   // OT: Gaussian velocity perturbation amplitude
         double beta = -1.0; // OT: plasma beta (<=0: classic 5/(12 pi) pressure)
         double noise = 0.0; // BW: relative uniform density noise
-        double eta = 0.0;   // BW: background resistivity (>0 turns resistive MHD on)
+        double eta = 0.0;   // background resistivity (>0 turns resistive MHD on)
+        double eta0 = 0.0;  // resistivity added where |j| > j_max (>0 turns resistive MHD on)
+        double jMax = 1.0e30;
         bcKind bcx = bcKind::periodic;
         bool bcxSet = false;
         bool time = false;
@@ -229,11 +235,13 @@ namespace refdrv
                 data.z_min = 0.0;
                 data.z_max = (data.x_max - data.x_min) * static_cast<double>(c.nz) / static_cast<double>(c.nx);
                 data.gas_gamma = 2.0;
-                if (c.eta > 0.0)
-                {
-                    data.resistiveMHD = true;
-                    data.eta_background = c.eta;
-                }
+            }
+            if (c.eta > 0.0 || c.eta0 > 0.0)
+            {
+                data.resistiveMHD = true;
+                data.eta_background = c.eta;
+                data.eta0 = c.eta0;
+                data.j_max = c.jMax;
             }
         }
 
@@ -437,6 +445,10 @@ int main(int argc, char *argv[])
             c.noise = std::stod(next());
         else if (a == "--eta")
             c.eta = std::stod(next());
+        else if (a == "--eta0")
+            c.eta0 = std::stod(next());
+        else if (a == "--j-max")
+            c.jMax = std::stod(next());
         else if (a == "--time")
             c.time = true;
         else if (a == "--rke")

@@ -82,12 +82,15 @@ extern "C" int lare_problem_desc(const lare_problem *p, lare_desc *d)
         d->z_min = 0.0;
         d->z_max = (d->x_max - d->x_min) * (double)p->nz / (double)p->nx;
         d->gas_gamma = 2.0;
-        if (p->eta > 0.0) {
-            d->resistive = 1;
-            d->eta_background = p->eta;
-        }
     } else {
         return -1;
+    }
+    if (p->eta < 0.0 || p->eta0 < 0.0) return -1;
+    if (p->eta > 0.0 || p->eta0 > 0.0) {
+        d->resistive = 1;
+        d->eta_background = p->eta;
+        d->eta0 = p->eta0;
+        if (p->j_max > 0.0) d->j_max = p->j_max;
     }
     return 0;
 }

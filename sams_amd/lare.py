@@ -58,16 +58,18 @@ class LareProblem(ctypes.Structure):
                 ("nx", ctypes.c_int64), ("ny", ctypes.c_int64), ("nz", ctypes.c_int64),
                 ("seed", ctypes.c_uint64), ("amp", ctypes.c_double), ("beta", ctypes.c_double),
                 ("noise", ctypes.c_double), ("eta", ctypes.c_double),
-                ("precision", ctypes.c_int32), ("rke", ctypes.c_int32)]
+                ("precision", ctypes.c_int32), ("rke", ctypes.c_int32),
+                ("eta0", ctypes.c_double), ("j_max", ctypes.c_double)]
 
 
 def problem(kind="ot", n=16, ny=None, nz=None, seed=1, amp=0.0, beta=-1.0, noise=0.0, eta=0.0,
-            bcx=0, precision=FP64, rke=False):
+            bcx=0, precision=FP64, rke=False, eta0=0.0, j_max=0.0):
     """A LareProblem: kind 'ot' (Orszag-Tang) or 'bw' (Brio-Wu); rke turns the
-    remap kinetic-energy correction on."""
+    remap kinetic-energy correction on; eta (background) or eta0 (added where
+    |j| > j_max; j_max <= 0 means never) turn resistive MHD on."""
     k = {"ot": ORSZAG_TANG, "bw": BRIO_WU}[kind] if isinstance(kind, str) else int(kind)
     return LareProblem(k, bcx, n, ny if ny is not None else n, nz if nz is not None else n,
-                       seed, amp, beta, noise, eta, precision, int(bool(rke)))
+                       seed, amp, beta, noise, eta, precision, int(bool(rke)), eta0, j_max)
 
 
 _lib = None

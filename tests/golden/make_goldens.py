@@ -16,7 +16,12 @@ the LARE initial-condition package's BC_OTHER conditions to every edge
 (src/InitialConditions/LARE/lareic.h:124-146, boundary.cpp) instead of the
 problem's built-in ones.
 
+Resistive cases run oracle/_ref/ref_driver_res, the reference with the three
+element-wise copies at lagran.cpp:482-484 that let its resistive path run
+(oracle/Makefile writes that copy of lagran.cpp at build time).
+
     make -C oracle ref ic && python tests/golden/make_goldens.py
+    python tests/golden/make_goldens.py --fullsize   (the 256^3 benchmark workload)
 """
 import hashlib
 import json
@@ -55,6 +60,28 @@ CASES = [
     ("bw_mirror_32x2x2", "bw", 32, 2, 2, 20, (0, 20), {"bcx": "mirror", "noise": 0.01, "seed": 5}),
     ("bw_rke_2d_20x6x2", "bw", 20, 6, 2, 12, (0, 12), {"noise": 0.02, "seed": 6, "rke": True}),
     ("ot_larebc_12x10x2", "ot", 12, 10, 2, 10, (0, 10), {"amp": 0.05, "seed": 8, "lare_bc": True}),
+    # resistive MHD (oracle/_ref/ref_driver_res: the reference with the element-wise copies at
+    # lagran.cpp:482-484 that let its resistive path run); eta0 with a small j_max makes
+    # eta_calc's j > j_max branch (lagran.cpp:473) fire in part of the domain (about 20-50%
+    # of the cells for the Orszag-Tang case, the shock front for Brio-Wu)
+    ("bw_res_24x5x4", "bw", 24, 5, 4, 12, (0, 1, 12), {"noise": 0.01, "seed": 3, "eta": 1e-3}),
+    ("bw_res_strong_20x6x5", "bw", 20, 6, 5, 10, (0, 10), {"noise": 0.02, "seed": 5, "eta": 0.05}),
+    ("ot_res_16x12x10", "ot", 16, 12, 10, 8, (0, 1, 8), {"amp": 0.05, "seed": 7, "eta": 0.05}),
+    ("ot_res_jmax_16x12x10", "ot", 16, 12, 10, 10, (1, 10), {"amp": 0.05, "seed": 7, "eta": 0.01, "eta0": 0.05,
+                                                             "j_max": 2.0}),
+    ("bw_res_jmax_24x5x4", "bw", 24, 5, 4, 12, (1, 12), {"noise": 0.01, "seed": 9, "eta": 1e-3, "eta0": 0.02,
+                                                         "j_max": 5.0}),
+    ("ot_res_rke_12x10x8", "ot", 12, 10, 8, 10, (0, 10), {"amp": 0.05, "seed": 2, "eta": 0.02, "rke": True}),
+    ("ot_res_larebc_12x10x8", "ot", 12, 10, 8, 10, (0, 10), {"amp": 0.05, "seed": 4, "eta": 0.02,
+                                                             "lare_bc": True}),
+    ("bw_res_mirror_2d_20x6x2", "bw", 20, 6, 2, 12, (0, 12), {"bcx": "mirror", "noise": 0.02, "seed": 6,
+                                                              "eta": 0.01, "eta0": 0.02, "j_max": 3.0}),
+]
+
+# The benchmark workload at its full size and step count: resistive Brio-Wu
+# 256^3, 1% density noise (seed 1), eta = 1e-3, 200 steps (--fullsize).
+FULL_CASES = [
+    ("bw_res_256_200", "bw", 256, 256, 256, 200, (1, 50, 100, 200), {"noise": 0.01, "seed": 1, "eta": 1e-3}),
 ]
 
 
@@ -89,36 +116,64 @@ def digest(path):
             "linf": float(np.max(np.abs(a))), "l2": float(np.sqrt(np.sum(a * a)))}
 
 
-def main():
-    out = {"about": "Reference LARE3D (oracle/_ref/ref_driver) dt sequences and state-field "
-                    "SHA-256 digests; see tests/golden/make_goldens.py", "cases": {}}
-    for name, prob, nx, ny, nz, steps, dumps, extra in CASES:
-        with tempfile.TemporaryDirectory() as td:
-            meta = O.run_reference(td, problem=prob, nx=nx, ny=ny, nz=nz, steps=steps, dumps=dumps,
-                                   seed=extra.get("seed", 1), amp=extra.get("amp", 0.0),
-                                   noise=extra.get("noise", 0.0), bcx=extra.get("bcx"),
-                                   threads=1 if extra.get("rke") else 8, rke=extra.get("rke", False),
-                                   lare_bc=extra.get("lare_bc", False))
-            snaps = {}
-            for s in dumps:
-                snaps[str(s)] = {f: digest(os.path.join(td, f"s{s:05d}_{f}.f64")) for f in FIELDS}
-        out["cases"][name] = {"problem": prob, "nx": nx, "ny": ny, "nz": nz, "steps": steps,
-                              "seed": extra.get("seed", 1), "amp": extra.get("amp", 0.0),
-                              "noise": extra.get("noise", 0.0), "bcx": extra.get("bcx", ""),
-                              "rke": bool(extra.get("rke", False)),
-                              "lare_bc": bool(extra.get("lare_bc", False)),
-                              "dt": meta["dt"], "snapshots": snaps}
-        print(name, "done", flush=True)
-    out["ic_cases"] = {}
-    for ic, steps, threads in IC_CASES:
-        with tempfile.TemporaryDirectory() as td:
-            meta = run_ic(IC_DRIVER, td, ic, steps, threads)
-            snaps = {str(steps): {f: digest(os.path.join(td, f"s{steps:05d}_{f}.f64")) for f in FIELDS}}
-        out["ic_cases"][ic] = {"steps": steps, "threads": threads, "dt": meta["dt"], "snapshots": snaps}
-        print(ic, "done", flush=True)
-    with open(os.path.join(ROOT, "tests", "golden", "goldens.json"), "w") as f:
+def run_case(name, prob, nx, ny, nz, steps, dumps, extra, threads=8):
+    """One reference run: the case's golden record."""
+    with tempfile.TemporaryDirectory() as td:
+        meta = O.run_reference(td, problem=prob, nx=nx, ny=ny, nz=nz, steps=steps, dumps=dumps,
+                               seed=extra.get("seed", 1), amp=extra.get("amp", 0.0),
+                               noise=extra.get("noise", 0.0), bcx=extra.get("bcx"),
+                               threads=1 if extra.get("rke") else threads, rke=extra.get("rke", False),
+                               lare_bc=extra.get("lare_bc", False), eta=extra.get("eta", 0.0),
+                               eta0=extra.get("eta0", 0.0), j_max=extra.get("j_max"))
+        snaps = {}
+        for s in dumps:
+            snaps[str(s)] = {f: digest(os.path.join(td, f"s{s:05d}_{f}.f64")) for f in FIELDS}
+    print(name, "done", flush=True)
+    return {"problem": prob, "nx": nx, "ny": ny, "nz": nz, "steps": steps,
+            "seed": extra.get("seed", 1), "amp": extra.get("amp", 0.0),
+            "noise": extra.get("noise", 0.0), "bcx": extra.get("bcx", ""),
+            "rke": bool(extra.get("rke", False)),
+            "lare_bc": bool(extra.get("lare_bc", False)),
+            "eta": extra.get("eta", 0.0), "eta0": extra.get("eta0", 0.0), "j_max": extra.get("j_max", 0.0),
+            "dt": meta["dt"], "snapshots": snaps}
+
+
+def main(argv):
+    """No arguments: every small case and initial-condition case.
+    --only a,b: those cases, merged into the existing file.
+    --fullsize: the full-size cases (long: the 256^3 case takes about half an
+    hour on 8 cores), merged into the existing file."""
+    path = os.path.join(ROOT, "tests", "golden", "goldens.json")
+    if argv and argv[0] in ("--only", "--fullsize"):
+        with open(path) as f:
+            out = json.load(f)
+        if argv[0] == "--only":
+            want = set(argv[1].split(","))
+            for case in CASES:
+                if case[0] in want:
+                    out["cases"][case[0]] = run_case(*case)
+        else:
+            out.setdefault("full_cases", {})
+            for case in FULL_CASES:
+                out["full_cases"][case[0]] = run_case(*case)
+    else:
+        out = {"about": "Reference LARE3D (oracle/_ref/ref_driver, ref_driver_res) dt sequences and "
+                        "state-field SHA-256 digests; see tests/golden/make_goldens.py", "cases": {}}
+        if os.path.exists(path):
+            with open(path) as f:
+                out["full_cases"] = json.load(f).get("full_cases", {})
+        for case in CASES:
+            out["cases"][case[0]] = run_case(*case)
+        out["ic_cases"] = {}
+        for ic, steps, threads in IC_CASES:
+            with tempfile.TemporaryDirectory() as td:
+                meta = run_ic(IC_DRIVER, td, ic, steps, threads)
+                snaps = {str(steps): {f: digest(os.path.join(td, f"s{steps:05d}_{f}.f64")) for f in FIELDS}}
+            out["ic_cases"][ic] = {"steps": steps, "threads": threads, "dt": meta["dt"], "snapshots": snaps}
+            print(ic, "done", flush=True)
+    with open(path, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -19,6 +19,12 @@ def goldens():
         return json.load(f)["cases"]
 
 
+def full_goldens():
+    """Reference runs at a benchmark's full size and step count."""
+    with open(GOLDEN) as f:
+        return json.load(f).get("full_cases", {})
+
+
 def ic_goldens():
     """Runs of the reference's own initial-condition packages."""
     with open(GOLDEN) as f:
@@ -28,7 +34,8 @@ def ic_goldens():
 def case_problem(c):
     """LareProblem for a golden case."""
     return L.problem(c["problem"], n=c["nx"], ny=c["ny"], nz=c["nz"], seed=c["seed"],
-                     amp=c["amp"], noise=c["noise"], bcx=BCX[c["bcx"]], rke=c.get("rke", False))
+                     amp=c["amp"], noise=c["noise"], bcx=BCX[c["bcx"]], rke=c.get("rke", False),
+                     eta=c.get("eta", 0.0), eta0=c.get("eta0", 0.0), j_max=c.get("j_max") or 0.0)
 
 
 def case_bcs(c, grid):
