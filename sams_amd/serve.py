@@ -53,7 +53,9 @@ def device_bytes(p):
     """HBM held by one grid of problem p: 39 state/work arrays in the storage
     type and 6 FP64 flux arrays over the padded box (7 with the kinetic-energy
     correction), each rounded up to 256 bytes, plus axes and scalars (the
-    pool of lare_ctx_create with its default layout)."""
+    pool of lare_ctx_create with its default layout), the field-transfer
+    staging buffer ((nx+5)(ny+5)(nz+5) doubles, held from the first upload
+    until the grid is destroyed) and 64 KiB for the launch tables."""
     pz = ((int(p.nz) + 5) + 7) // 8 * 8
     box = (int(p.nx) + 5) * (int(p.ny) + 5) * pz
     t = 4 if p.precision == L.FP32_STORAGE else 8
@@ -62,7 +64,8 @@ def device_bytes(p):
     def arr(nbytes):
         return (nbytes + 255) // 256 * 256
 
-    return 39 * arr(box * t) + (7 if p.rke else 6) * arr(box * 8) + 6 * lmax * 8 + 256
+    stage = arr((int(p.nx) + 5) * (int(p.ny) + 5) * (int(p.nz) + 5) * 8)
+    return 39 * arr(box * t) + (7 if p.rke else 6) * arr(box * 8) + 6 * lmax * 8 + 256 + stage + 65536
 
 
 def poisson_requests(qps, duration, seed, nmin=32, nmax=192, smin=50, smax=500):
@@ -82,11 +85,10 @@ def poisson_requests(qps, duration, seed, nmin=32, nmax=192, smin=50, smax=500):
             p = L.problem("ot", n=n, seed=int(rng.integers(1, 2**31)), amp=float(rng.uniform(0.0, 0.05)),
                           beta=float(rng.uniform(0.1, 10.0)), precision=prec)
         else:
-            # x-directed shock tube on an n x m x m grid (m = max(4, n/8)),
-            # resistive with probability 1/2
-            m = max(4, n // 8)
+            # x-directed shock tube on the n^3 grid, resistive with
+            # probability 1/2
             eta = 1e-3 if rng.random() < 0.5 else 0.0
-            p = L.problem("bw", n=n, ny=m, nz=m, seed=int(rng.integers(1, 2**31)),
+            p = L.problem("bw", n=n, seed=int(rng.integers(1, 2**31)),
                           noise=float(rng.uniform(0.0, 0.02)), eta=eta, precision=prec)
         out.append(Request(len(out), p, steps, t))
     return out
@@ -109,6 +111,7 @@ class Server:
         self.cell_updates = 0.0
         self.peak_active = 0
         self.peak_used = 0.0
+        self.busy = 0.0  # seconds from a tick's first enqueue to its last sync
         self._batches = {}
         self._members = {}
         # host initial conditions of the next waiting requests are computed
@@ -190,6 +193,7 @@ class Server:
             groups = {}
             for r in running:
                 groups.setdefault(int(r.problem.precision), []).append(r)
+            tick0 = self.clock()
             for prec, mem in groups.items():
                 self._batch(prec, mem).steps(k)
             # overlap: admissions while the GPU works
@@ -198,6 +202,7 @@ class Server:
                 self._batches[prec].sync()
             self.ticks += 1
             now = self.clock() - t0
+            self.busy += self.clock() - tick0
             finished = []
             for r in running:
                 r.done_steps += k

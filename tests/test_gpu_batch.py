@@ -67,6 +67,66 @@ def test_batch_matches_oracle_and_continues():
         assert not _diffs(mem[i], orc[i]), f"member {i} after a solo step"
 
 
+# A batch above the lane-walking threshold (300000 cells per launch): with the
+# default launch configuration every group launch of this batch, including
+# the resistive and the kinetic-energy-correction subsets (about 340000 and
+# 306000 cells), takes the lane-walking forms, while each grid alone (at most
+# 118000 cells) runs the one-cell forms.
+BIG = [
+    L.problem("ot", n=40, seed=21, amp=0.03),
+    L.problem("bw", n=72, ny=40, nz=36, seed=22, noise=0.01, eta=1e-3),
+    L.problem("ot", n=48, ny=56, nz=44, seed=23, amp=0.02, beta=0.5, eta=0.01, eta0=0.05, j_max=2.0),
+    L.problem("bw", n=64, ny=40, nz=40, seed=24, noise=0.02, bcx=L.BC_MIRROR, rke=True),
+    L.problem("ot", n=44, seed=25, amp=0.05, rke=True),
+    L.problem("bw", n=56, ny=44, nz=48, seed=26, noise=0.01, eta=1e-3, rke=True),
+]
+
+
+def _check_batch_against_alone_and_oracle(probs, steps, tune=None, oracle_members=None):
+    from oracle import oracle as O
+    mem = [L.Lare.from_problem(p) for p in probs]
+    b = L.Batch(mem)
+    if tune:
+        b.set_tune(**tune)
+    b.steps(steps)
+    b.sync()
+    print("batch tune:", b.get_tune(), flush=True)
+    for i, p in enumerate(probs):
+        alone = L.Lare.from_problem(p)
+        dts = alone.steps(steps)
+        assert mem[i].get_dt().hex() == dts[-1].hex(), f"member {i} dt against its solo run"
+        d = _diffs(mem[i], alone)
+        assert not d, f"member {i} against its solo run: {d}"
+        alone.close()
+    O.set_threads(16)
+    for i in (oracle_members if oracle_members is not None else range(len(probs))):
+        o = make_oracle(probs[i])
+        for _ in range(steps):
+            o.step()
+        assert mem[i].get_dt().hex() == o.get_dt().hex(), f"member {i} dt against the oracle"
+        d = _diffs(mem[i], o)
+        assert not d, f"member {i} against the oracle: {d}"
+        print(f"member {i}: equal to its solo run and to the oracle", flush=True)
+
+
+def test_large_batch_lane_walking_group_kernels_match_alone_and_oracle():
+    """Default configuration above the threshold: the lane-walking group
+    kernels over six uneven grids (ideal, resistive with and without the
+    eta0/j_max branch, rke, mirror edges) against each grid's one-cell solo
+    run and the oracle."""
+    _check_batch_against_alone_and_oracle(BIG, 4)
+
+
+def test_small_batch_forced_lane_walking_matches_alone_and_oracle():
+    """walk=1 forces the lane-walking forms on the small mixed batch."""
+    _check_batch_against_alone_and_oracle(MIX, 7, tune={"walk": 1}, oracle_members=[0, 1, 3, 4, 6, 7])
+
+
+def test_large_batch_one_cell_forms_match_alone():
+    """walk=2 keeps the one-cell forms above the threshold too."""
+    _check_batch_against_alone_and_oracle(BIG[:3], 3, tune={"walk": 2}, oracle_members=[])
+
+
 def test_energy_diagnostic_matches_host_sum():
     """lare_energy against the same sums on the host in numpy (FP64, a
     different summation order: relative 1e-12)."""

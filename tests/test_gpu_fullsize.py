@@ -1,14 +1,19 @@
 """The step at the benchmark's full size against the reference itself.
 
-Brio-Wu 256^3 with 1% density noise (seed 1) is the benchmark workload
-without its resistivity: the reference's resistive path throws
-(lagran.cpp:482), so its ideal-MHD twin is the largest run it can make.
+The benchmark workload (resistive Brio-Wu 256^3, 1% density noise, seed 1,
+eta = 1e-3) for its 200 steps against the reference's own run of it
+(oracle/_ref/ref_driver_res: the reference with the three element-wise
+copies at lagran.cpp:482-484 that let its resistive path run): the dt of
+every step and SHA-256 digests of every state field, ghosts included,
+after steps 1, 50, 100 and 200 (tests/golden/goldens.json "full_cases",
+written by tests/golden/make_goldens.py --fullsize).
+
+The ideal-MHD twin (the same grid with eta = 0) runs the reference itself
+on the host cores (oracle/_ref/ref_driver, the -ffp-contract=off build: the
+-march=x86-64-v3 timing build contracts multiply-adds, already in the noisy
+initial density) for 8 steps, every state field compared after each step.
 At this size the lane-walking kernels run (the small golden cases use the
-one-cell forms), so this pins them against the reference's own output:
-the reference built from its sources (oracle/_ref/ref_driver, the
--ffp-contract=off build: the -march=x86-64-v3 timing build contracts
-multiply-adds, already in the noisy initial density) runs on the host cores, the device through the C-ABI, and every state field must
-be bit-identical, ghosts included, after each step.
+one-cell forms), so these tests pin them against the reference's output.
 """
 import os
 import tempfile
@@ -20,6 +25,53 @@ import sams_amd.lare as L
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
+
+
+def _run_against_digests(c, dev, upto=None):
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    from lare_testlib import sha
+    done = 0
+    for s in sorted(int(k) for k in c["snapshots"]):
+        if upto is not None and s > upto:
+            break
+        dts = dev.steps(s - done)
+        for i, dt in enumerate(dts):
+            assert dt.hex() == float.fromhex(c["dt"][done + i]).hex(), f"dt of step {done + i + 1}"
+        done = s
+        for f in L.STATE_FIELDS:
+            assert sha(dev.download(f)) == c["snapshots"][str(s)][f]["sha256"], f"{f} after step {s}"
+        print(f"step {s}: dt and every state field equal to the reference", flush=True)
+
+
+def _full_case(name):
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    from lare_testlib import case_problem, full_goldens
+    c = full_goldens().get(name)
+    if c is None:
+        pytest.skip(f"no full-size golden {name}")
+    return c, case_problem(c)
+
+
+def test_benchmark_workload_200_steps_matches_reference():
+    """The benchmark workload, every one of its 200 steps, against the
+    reference: the default launch configuration (lane-walking kernels at
+    this size)."""
+    c, p = _full_case("bw_res_256_200")
+    dev = L.Lare.from_problem(p)
+    _run_against_digests(c, dev)
+
+
+def test_benchmark_workload_other_launch_configurations_match_reference():
+    """The other launch configurations the autotuner can choose (8 cells per
+    lane in the marches, plane-major workgroup order, the z sweep without
+    the lane exchange, CT flux and update as separate kernels) give the
+    same bits for the first 50 steps."""
+    c, p = _full_case("bw_res_256_200")
+    dev = L.Lare.from_problem(p)
+    dev.set_tune(walk=1, march_k=8, zx_k=8, zline=0, order=0, ct_fuse=0)
+    _run_against_digests(c, dev, upto=50)
 
 
 @pytest.mark.skipif(not os.path.exists(O.REF_DRIVER), reason="oracle/_ref/ref_driver not built")

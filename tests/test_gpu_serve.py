@@ -6,7 +6,7 @@ import pytest
 
 import sams_amd.lare as L
 from sams_amd import serve as S
-from lare_testlib import bit_diff
+from lare_testlib import bit_diff, make_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -51,3 +51,38 @@ def test_budget_limits_residency():
     assert len(done) == 6
     assert srv.peak_used <= budget
     _check_alone(done)
+
+
+def test_served_results_at_lane_walking_sizes_equal_oracle():
+    """Requests of 32^3 to 56^3 (the serving workload's lower sizes): the
+    first three FP64 requests arrive together, so their batch holds 349 568
+    cells and runs the lane-walking group kernels (300 000 cells and up);
+    later ticks mix other sizes and precisions.  Every FP64 result is
+    bit-identical to the oracle (dt and every state field), and every
+    result to the same solve run alone."""
+    from oracle import oracle as O
+    specs = [
+        (L.problem("ot", n=48, seed=31, amp=0.03), 12, 0.00),
+        (L.problem("bw", n=56, seed=32, noise=0.01, eta=1e-3), 10, 0.00),
+        (L.problem("ot", n=44, ny=40, nz=36, seed=33, amp=0.02, beta=0.4, eta=0.01, eta0=0.05, j_max=2.0), 9, 0.00),
+        (L.problem("bw", n=40, seed=34, noise=0.02, bcx=L.BC_MIRROR, rke=True), 14, 0.02),
+        (L.problem("ot", n=36, seed=35, amp=0.05, precision=L.FP32_STORAGE), 8, 0.02),
+        (L.problem("bw", n=32, seed=36, noise=0.01, eta=1e-3, precision=L.FP32_STORAGE), 11, 0.04),
+    ]
+    assert sum(int(p.nx * p.ny * p.nz) for p, _, _ in specs[:3]) >= 300000
+    reqs = [S.Request(i, p, st, t) for i, (p, st, t) in enumerate(specs)]
+    srv = S.Server(budget_bytes=50e9, steps_per_tick=3, keep_results=True)
+    done = srv.run(reqs)
+    assert len(done) == len(specs)
+    _check_alone(done)
+    O.set_threads(16)
+    for r in done:
+        if r.problem.precision != L.FP64:
+            continue
+        o = make_oracle(r.problem)
+        for _ in range(r.steps):
+            o.step()
+        assert o.get_dt().hex() == r.dt_last.hex(), f"request {r.rid} dt against the oracle"
+        for f in L.STATE_FIELDS:
+            assert bit_diff(r.grid.download(f), o.download(f)) is None, (r.rid, f)
+        print(f"request {r.rid}: equal to the oracle", flush=True)
