@@ -73,6 +73,56 @@ def test_fp32_storage_error_per_field(name, p, steps, l2max, linfmax):
     assert abs(eb[3] - ea[3]) / abs(ea[3]) < 1e-5
 
 
+def test_config2_fp32_storage_512_cubed_100_steps():
+    """The FP32-storage workload at its size: Orszag-Tang 512^3 with seeded
+    N(0, 1e-3^2) velocity perturbations, 100 steps, FP32 storage against
+    FP64.  Per field (domain cells) the relative L2 and L-inf error,
+    normalised by the FP64 vector's norm for velocity and B components, must
+    stay below the bounds below; the dt sequences within 1e-4 and the total
+    energy (FP64 diagnostic) within 1e-6."""
+    n, steps = 512, 100
+    a = L.Lare.from_problem(L.problem("ot", n=n, seed=7, amp=1e-3, precision=L.FP64))
+    da = np.array(a.steps(steps))
+    b = L.Lare.from_problem(L.problem("ot", n=n, seed=7, amp=1e-3, precision=L.FP32_STORAGE))
+    db = np.array(b.steps(steps))
+    rel_dt = float(np.max(np.abs(db - da) / da))
+    print(f"dt: max rel {rel_dt:.3e}", flush=True)
+    ea, eb = a.energy(), b.energy()
+    rel_e = abs(eb[3] - ea[3]) / abs(ea[3])
+    print(f"total energy: rel {rel_e:.3e}", flush=True)
+    # field by field, so that at most two full fields are on the host at once
+    norms = {}
+    errs = {}
+    for f in L.STATE_FIELDS:
+        x = a.download(f)[2:-2, 2:-2, 2:-2]
+        y = b.download(f)[2:-2, 2:-2, 2:-2]
+        d = x - y
+        norms[f] = (float(np.sum(x * x)), float(np.max(np.abs(x))))
+        errs[f] = (float(np.sqrt(np.sum(d * d))), float(np.max(np.abs(d))))
+        del x, y, d
+    # measured on MI355X (vector-normalised L2 / Linf): rho 3.0e-7 / 7.0e-6,
+    # e_e 2.4e-7 / 7.8e-6, e_i 3.6e-7 / 1.7e-5, vx 4.1e-7 / 1.9e-5,
+    # vy 5.7e-7 / 5.8e-5, vz 1.9e-6 / 7.3e-5, bx 8.7e-7 / 1.7e-5,
+    # by 9.7e-7 / 2.1e-5, bz 2.9e-6 / 6.3e-5; dt 1.5e-5, total energy 4.7e-8
+    bounds = {"rho": (2e-6, 1e-4), "energy_electron": (2e-6, 1e-4), "energy_ion": (2e-6, 1e-4),
+              "vx": (1e-5, 3e-4), "vy": (1e-5, 3e-4), "vz": (1e-5, 3e-4),
+              "bx": (1e-5, 3e-4), "by": (1e-5, 3e-4), "bz": (1e-5, 3e-4)}
+    bad = []
+    for f in L.STATE_FIELDS:
+        comps = VECTOR.get(f, (f,))
+        n2 = np.sqrt(sum(norms[c][0] for c in comps) / len(comps))
+        ni = max(norms[c][1] for c in comps)
+        e2, ei = errs[f][0] / n2, errs[f][1] / ni
+        own2 = errs[f][0] / np.sqrt(norms[f][0]) if norms[f][0] > 0 else errs[f][0]
+        owni = errs[f][1] / norms[f][1] if norms[f][1] > 0 else errs[f][1]
+        print(f"{f}: rel L2 {own2:.3e} Linf {owni:.3e}; vector-normalised L2 {e2:.3e} Linf {ei:.3e}", flush=True)
+        if e2 >= bounds[f][0] or ei >= bounds[f][1]:
+            bad.append(f"{f}: rel L2 {e2:.3e}, Linf {ei:.3e}")
+    assert rel_dt < 1e-4, f"dt rel {rel_dt:.3e}"
+    assert rel_e < 1e-6, f"total energy rel {rel_e:.3e}"
+    assert not bad, bad
+
+
 def test_fp32_storage_batch_equals_alone():
     ps = [L.problem("ot", n=10, ny=12, nz=9, seed=2, amp=0.03, precision=L.FP32_STORAGE),
           L.problem("bw", n=20, ny=4, nz=5, seed=1, noise=0.01, eta=1e-3, precision=L.FP32_STORAGE)]
