@@ -189,6 +189,17 @@ int lare_profile_get(lare_ctx *ctx, int32_t i, char *name, int32_t name_len, int
  * out[3] their sum.  Summed in a fixed order (reproducible). */
 int lare_energy(lare_ctx *ctx, double *out);
 
+/* lare_energy without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Same sums, same order, same bits as lare_energy.  The partial sums live in
+ * a fixed 16 MiB device array of the library, one per GPU (looked up when a
+ * context is created), which limits the grid to about 560^3 cells: a larger
+ * grid fails with a clear error and nothing allocated.  Calls that overlap
+ * on one GPU (contexts on different streams, other host threads) race for
+ * that array; serialise them.  Blocking. */
+int lare_energy_noalloc(lare_ctx *ctx, double *out);
+
 /* Quotients as the kernels form them (no counterpart in the reference: a
  * self-check of the device's FP64 division, for tests).  q[i] = a[i] / d[i]
  * rounded to nearest even, for host arrays of n values, computed on the

@@ -50,6 +50,8 @@ struct lare_ctx {
     bool solo_ready = false;
     // arrays only the one-kernel-per-reference-kernel set reads
     void *pool0 = nullptr;
+    // this device's copy of the engine's energy scratch (lare_energy_noalloc)
+    double *scratch = nullptr;
     // device staging of field transfers (allocated on first use)
     void *stage = nullptr;
     size_t stage_bytes = 0;
@@ -553,6 +555,19 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
         delete c;
         return rc;
     }
+    // set-up of the allocation-free energy diagnostic: the grid as a group
+    // of one with its energy launch table, and this device's scratch address
+    rc = ensure_solo(c);
+    if (rc == 0) {
+        lare::group_energy_prepare(c->solo);
+        std::string err;
+        const int erc = lare::energy_scratch_address(&c->scratch, err);
+        if (erc) rc = fail(erc, err);
+    }
+    if (rc) {
+        lare_ctx_destroy(c);
+        return rc;
+    }
     *out = c;
     return 0;
 }
@@ -1046,6 +1061,16 @@ extern "C" int lare_energy(lare_ctx *c, double *out)
     rc = lare::group_energy(c->solo, e, err);
     if (rc) return fail(rc, err);
     std::memcpy(out, e.data(), 4 * sizeof(double));
+    return 0;
+}
+
+extern "C" int lare_energy_noalloc(lare_ctx *c, double *out)
+{
+    if (!c || !out) return fail(-1, "lare_energy_noalloc: null argument");
+    if (!c->solo_ready) return fail(-1, "lare_energy_noalloc: context not set up");
+    std::string err;
+    const int rc = lare::group_energy_noalloc(c->solo, c->scratch, out, err);
+    if (rc) return fail(rc, "lare_energy_noalloc: " + err);
     return 0;
 }
 

@@ -92,6 +92,20 @@ inline void group_step(Group &gp)
 // domain cells and vertices in a fixed order.
 int group_energy(Group &gp, std::vector<double> &out, std::string &err);
 
+// The same diagnostic without any device allocation, for callers that cannot
+// allocate: partial sums and totals go to the file-scope device array of
+// lare_engine.hip (one copy per GPU, energy_scratch_bytes() long; calls that
+// overlap on one GPU race for it).  energy_scratch_address looks the array
+// up on the current device (hipGetSymbolAddress; once per device, at set-up),
+// group_energy_prepare builds the group's launch table (allocates: set-up
+// only); group_energy_noalloc then launches, copies the four totals of each
+// member to out and waits.  It fails, allocating nothing, when the table was
+// not prepared or the group needs more scratch than the array holds.
+size_t energy_scratch_bytes();
+int energy_scratch_address(double **out, std::string &err);
+void group_energy_prepare(Group &gp);
+int group_energy_noalloc(Group &gp, double *scratch_dev, double *out, std::string &err);
+
 // q[i] = a[i] / d[i] for host arrays of n values, computed on the device by
 // the kernels' division (mode 0) or shared-divisor quotient (mode 1); mode 2
 // gives q[i] = sqrt(a[i]) as the kernels compute it (d unused).

@@ -33,6 +33,7 @@
 #include <float.h>
 
 #include <algorithm>
+#include <mutex>
 #include <cstdlib>
 #include <cstring>
 
@@ -3123,6 +3124,13 @@ KERNEL void e_energy_sum(const double *part, const int *boff, int ngrid, int nbl
     out[4 * g + 3] = e[0] + e[1] + e[2];
 }
 
+// Scratch of the allocation-free energy diagnostic (group_energy_noalloc):
+// 3 block partial sums per workgroup of the launch, then 4 totals per member.
+// 16 MiB covers one grid of up to about 560^3 cells.  Each GPU the library
+// runs on holds its own copy until the process ends; host code reaches it
+// only through the address hipGetSymbolAddress returns on that device.
+static __device__ __align__(256) double scratch[(16u << 20) / sizeof(double)];
+
 // ================================================================== host
 
 namespace lare {
@@ -3876,27 +3884,107 @@ void group_prepare(Group &gp)
     gp.planning = false;
 }
 
+namespace {
+
+// cells 1..n of every member: the range of the energy sums
+const RSpec kEnergyRange{1, 0, 1, 0, 1, 0};
+
+// The two energy kernels of a group over partial sums `part` (3 per block of
+// table t) into `res` (4 per member).
+hipError_t launch_energy(Group &gp, const Group::Tab &t, double *part, double *res)
+{
+    if (gp.fp32) {
+        Src<float> s{(const EDev<float> *)gp.devs[0], t.boff, gp.n[0], t.nblk, kEnergyRange, 0, 0};
+        hipLaunchKernelGGL(e_energy_part<float>, dim3((unsigned)t.nblk), dim3(256), 0, gp.stream, s, part);
+    } else {
+        Src<double> s{(const EDev<double> *)gp.devs[0], t.boff, gp.n[0], t.nblk, kEnergyRange, 0, 0};
+        hipLaunchKernelGGL(e_energy_part<double>, dim3((unsigned)t.nblk), dim3(256), 0, gp.stream, s, part);
+    }
+    hipLaunchKernelGGL(e_energy_sum, dim3((unsigned)gp.n[0]), dim3(64), 0, gp.stream, (const double *)part,
+                       (const int *)t.boff, gp.n[0], t.nblk, res);
+    return hipGetLastError();
+}
+
+// per-device address of `scratch` (hipGetSymbolAddress on that device)
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_dev;
+
+}  // namespace
+
+size_t energy_scratch_bytes() { return sizeof(scratch); }
+
+int energy_scratch_address(double **out, std::string &err)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0) {
+        err = std::string("energy scratch: no current HIP device: ") + hipGetErrorString(e);
+        return -3;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((size_t)dev >= g_scratch_dev.size()) g_scratch_dev.resize((size_t)dev + 1, nullptr);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            err = std::string("energy scratch: hipGetSymbolAddress: ") + hipGetErrorString(e);
+            return -3;
+        }
+        g_scratch_dev[dev] = (double *)p;
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
+void group_energy_prepare(Group &gp)
+{
+    if (!gp.mem.empty()) (void)table(gp, 0, kEnergyRange, 0);
+}
+
+int group_energy_noalloc(Group &gp, double *scratch_dev, double *out, std::string &err)
+{
+    if (gp.mem.empty()) return 0;
+    if (!scratch_dev) {
+        err = "energy diagnostic without allocation: no scratch address for this device";
+        return -1;
+    }
+    const RSpec &r = kEnergyRange;
+    auto it = gp.tabs.find({0, std::array<int, 7>{r.x0, r.dx1, r.y0, r.dy1, r.z0, r.dz1, 0}});
+    if (it == gp.tabs.end()) {
+        err = "energy diagnostic without allocation: the launch table was not prepared at set-up";
+        return -1;
+    }
+    const Group::Tab &t = it->second;
+    // partial sums, then the totals on a 256-byte boundary
+    const size_t npart = ((size_t)std::max(t.nblk, 1) * 3 + 31) / 32 * 32;
+    const size_t need = (npart + 4 * gp.mem.size()) * sizeof(double);
+    if (need > energy_scratch_bytes()) {
+        err = "energy diagnostic without allocation: the grid needs " + std::to_string(need) +
+              " bytes of scratch, more than the " + std::to_string(energy_scratch_bytes()) +
+              " bytes it has (at most about 560^3 cells)";
+        return -1;
+    }
+    double *part = scratch_dev, *res = scratch_dev + npart;
+    hipError_t e = launch_energy(gp, t, part, res);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, res, 4 * gp.mem.size() * sizeof(double), hipMemcpyDeviceToHost, gp.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(gp.stream);
+    if (e != hipSuccess) {
+        err = std::string("energy diagnostic: ") + hipGetErrorString(e);
+        return -3;
+    }
+    return 0;
+}
+
 int group_energy(Group &gp, std::vector<double> &out, std::string &err)
 {
     out.assign(4 * gp.mem.size(), 0.0);
     if (gp.mem.empty()) return 0;
-    const RSpec in1 = rs(1, 0, 1, 0, 1, 0);
-    Group::Tab &t = table(gp, 0, in1, 0);
+    Group::Tab &t = table(gp, 0, kEnergyRange, 0);
     double *part = nullptr, *res = nullptr;
     hipError_t e = hipMalloc(&part, (size_t)std::max(t.nblk, 1) * 3 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&res, out.size() * sizeof(double));
-    if (e == hipSuccess) {
-        if (gp.fp32) {
-            Src<float> s{(const EDev<float> *)gp.devs[0], t.boff, gp.n[0], t.nblk, in1, 0, 0};
-            hipLaunchKernelGGL(e_energy_part<float>, dim3((unsigned)t.nblk), dim3(256), 0, gp.stream, s, part);
-        } else {
-            Src<double> s{(const EDev<double> *)gp.devs[0], t.boff, gp.n[0], t.nblk, in1, 0, 0};
-            hipLaunchKernelGGL(e_energy_part<double>, dim3((unsigned)t.nblk), dim3(256), 0, gp.stream, s, part);
-        }
-        hipLaunchKernelGGL(e_energy_sum, dim3((unsigned)gp.n[0]), dim3(64), 0, gp.stream, (const double *)part,
-                           (const int *)t.boff, gp.n[0], t.nblk, res);
-        e = hipGetLastError();
-    }
+    if (e == hipSuccess) e = launch_energy(gp, t, part, res);
     if (e == hipSuccess) e = hipMemcpyAsync(out.data(), res, out.size() * sizeof(double), hipMemcpyDeviceToHost, gp.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(gp.stream);
     if (part) (void)hipFree(part);

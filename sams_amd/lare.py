@@ -117,6 +117,7 @@ def lib():
             "lare_profile_get": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32,
                                                 P(ctypes.c_int32), P(ctypes.c_double), P(ctypes.c_double)]),
             "lare_energy": (ctypes.c_int, [vp, P(ctypes.c_double)]),
+            "lare_energy_noalloc": (ctypes.c_int, [vp, P(ctypes.c_double)]),
             "lare_div_check": (ctypes.c_int, [P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double),
                                               ctypes.c_int64, ctypes.c_int32]),
             "lare_batch_create": (ctypes.c_int, [P(vp), ctypes.c_int32, P(vp)]),
@@ -345,6 +346,14 @@ class Lare:
         """FP64 energy diagnostic: (internal, kinetic, magnetic, total)."""
         out = (ctypes.c_double * 4)()
         _check(lib().lare_energy(self.h, out), "lare_energy")
+        return tuple(out)
+
+    def energy_noalloc(self):
+        """The same diagnostic through lare_energy_noalloc, which allocates no
+        device memory (a fixed per-GPU scratch array; grids up to about
+        560^3 cells)."""
+        out = (ctypes.c_double * 4)()
+        _check(lib().lare_energy_noalloc(self.h, out), "lare_energy_noalloc")
         return tuple(out)
 
 
