@@ -174,8 +174,9 @@ int lare_set_graph(lare_ctx *ctx, int32_t on);
  * per-event host clocks, runnerMacros.h:113-150): between begin and end
  * every kernel launch on the context is bracketed by HIP events on its
  * stream; end aggregates them by kernel name with the kernel's algorithmic
- * HBM bytes (distinct arrays read and written, 8 bytes each, per cell of its
- * range). */
+ * HBM bytes: distinct arrays read and written (8 bytes each in FP64, 4 for
+ * FP32-storage arrays) per cell-update, times the nx ny nz cells of the
+ * grid the launch advances. */
 int lare_profile_begin(lare_ctx *ctx);
 int lare_profile_end(lare_ctx *ctx);
 int lare_profile_count(lare_ctx *ctx, int32_t *n);

@@ -32,7 +32,8 @@ struct Group {
         int *boff = nullptr;
         int nblk = 0;
         int order = 0;
-        double cells = 0.0;
+        double cells = 0.0;  // cells of the launch range
+        double units = 0.0;  // domain cells of the members (nx ny nz each): the cell-updates it serves
     };
     // (subset, range spec, order)
     std::map<std::pair<int, std::array<int, 7>>, Tab> tabs;

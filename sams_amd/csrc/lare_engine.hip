@@ -3215,6 +3215,7 @@ Group::Tab &table(Group &gp, int subset, const RSpec &r, int order)
         const long nxr = (long)G.nx + r.dx1 - r.x0 + 1, nyr = (long)G.ny + r.dy1 - r.y0 + 1,
                    nzr = (long)G.nz + r.dz1 - r.z0 + 1;
         boff.push_back((int)total);
+        t.units += (double)G.nx * G.ny * G.nz;
         if (nxr > 0 && nyr > 0 && nzr > 0) {
             if (order >= kZX) {
                 const int H = (order - kZX) / 100, K = (order - kZX) % 100;
@@ -3266,7 +3267,8 @@ void elaunch_(void (*k)(Src<T>, KA...), const char *name, double bpc, Group &gp,
     if (gp.planning || t.nblk == 0) return;
     Src<T> s{(const EDev<T> *)gp.devs[subset], t.boff, gp.n[subset], t.nblk, r, t.order, st};
     Prof *pf = gp.prof;
-    Prof::Rec rec{kname(name), bpc * t.cells, nullptr, nullptr};
+    // algorithmic bytes: per cell-update (bpc) times the members' domain cells
+    Prof::Rec rec{kname(name), bpc * t.units, nullptr, nullptr};
     if (pf && pf->on) {
         if (pf->dir >= 0) {
             const size_t at = rec.name.find("<DIR>");
