@@ -61,10 +61,18 @@ enum lare_precision {
     LARE_FP32_STORAGE = 1
 };
 
+/* Coordinate systems, geometryType of shared_data.h:38-43: x, y, z are
+ * (x, y, z), (r, phi, z) or (r, theta, phi). */
+enum lare_geometry {
+    LARE_CARTESIAN = 0,
+    LARE_CYLINDRICAL = 1,
+    LARE_SPHERICAL = 2
+};
+
 /* Grid and physics scalars of LARE::simulationData
  * (src/PhysicsPackages/LARE/shared_data.h:60-226) that the step reads.
- * Geometry is Cartesian; the axes are the reference's linear axes built
- * from (n, min, max) (include/harness/axisRegistry.h:973-984). */
+ * The axes are the reference's linear axes built from (n, min, max)
+ * (include/harness/axisRegistry.h:973-984). */
 typedef struct lare_desc {
     int64_t nx, ny, nz;
     double x_min, x_max, y_min, y_max, z_min, z_max;
@@ -76,7 +84,8 @@ typedef struct lare_desc {
     int32_t precision;      /* enum lare_precision */
     double eta_background, eta0, j_max;
     int32_t rke;            /* remap kinetic-energy correction (shared_data.h:118) */
-    int32_t reserved;
+    int32_t geometry;       /* enum lare_geometry (shared_data.h:109); cylindrical and
+                               spherical grids run kernel set 0 only */
 } lare_desc;
 
 typedef struct lare_ctx lare_ctx;
@@ -280,7 +289,7 @@ enum lare_problem_kind {
      * e_e = e_i = p / ((gamma-1) rho) / 2. */
     LARE_PROBLEM_ORSZAG_TANG = 0,
     /* x-directed Brio-Wu on x in [-1,1], y,z in [0, 2 ny/nx], [0, 2 nz/nx],
-     * gamma 2: rho,p,By = 1,1,1 for xc < 0 and 0.125,0.1,-1 otherwise,
+     * gamma 2: rho,p,By = 1,1,1 for xc below the x midpoint and 0.125,0.1,-1 otherwise,
      * Bx = 0.75, rho *= 1 + noise*(2U-1); x boundaries bcx, y/z periodic. */
     LARE_PROBLEM_BRIO_WU = 1
 };
@@ -301,6 +310,11 @@ typedef struct lare_problem {
     double eta0;        /* resistivity added where |j| > j_max (lagran.cpp:473) */
     double j_max;       /* current threshold of eta0; <= 0 means 1e30 (never reached) */
     /* eta > 0 or eta0 > 0 turns resistive MHD on (shared_data.h:113-117) */
+    int32_t geometry;   /* enum lare_geometry */
+    int32_t reserved;
+    double x0, y0;      /* shifts of the domain's x and y extents (a curvilinear grid
+                           needs r > 0, and theta away from the poles); Brio-Wu's
+                           discontinuity stays at the x midpoint */
 } lare_problem;
 
 /* Grid and physics scalars of the problem. */

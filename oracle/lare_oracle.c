@@ -7,9 +7,13 @@
  * array extents, the order of kernels and boundary fills, and the
  * floating-point association of every expression follow the reference
  * source; the library is compiled with -ffp-contract=off so that no
- * multiply-add is fused.  Geometry is Cartesian: the metric factors hy, hyc,
- * hz, hzc, hz1, hz2 of setup.cpp:279-303 are 1.0 and every multiplication or
- * division by them is exact, so they are omitted.  Positions x,y,z
+ * multiply-add is fused.  Geometry (shared_data.h:38-43) is Cartesian,
+ * cylindrical or spherical: the metric factors hy, hyc, hz, hzc, hz1, hz2 of
+ * setup.cpp:276-335 (1.0 in Cartesian geometry, where every product with
+ * them is exact) enter the metric, the edge viscosities, the viscous heating
+ * and forces, the CFL lengths, the predictor's forces (with the geometric
+ * terms of lagran.cpp:711-721) and the y and z sweeps' widths exactly where
+ * the reference multiplies or divides by them.  Positions x,y,z
  * (lagran.cpp:724-726, remap.cpp:100-103) are dead and omitted.
  *
  * min/max follow include/pp/utils.h:277-335: two-argument forms are the
@@ -82,7 +86,7 @@ struct lo_ctx {
     long nx, ny, nz;
     double gas_gamma, visc1, visc2, visc2_norm, mu0, dt_multiplier;
     double none_zero, largest_number;
-    int resistive, rke;
+    int resistive, rke, geometry;
     double eta_background, eta0, j_max;
     double dt, dtr;
     double xpass, ypass, zpass;
@@ -91,6 +95,11 @@ struct lo_ctx {
     int bcm[LO_NVAR][3][2][3]; /* layer tables of LO_BC_MAP edges */
 
     a1 xb, yb, zb, xc, yc, zc, dxb, dyb, dzb, dxc, dyc, dzc;
+    /* metric factors (setup.cpp:276-335): hy(-2..nx+2), hyc(-1..nx+2);
+     * hz(-2..nx+2, -2..ny+2), hzc(-1..nx+2, -1..ny+2), hz1(-2..nx+2,
+     * -1..ny+2), hz2(-1..nx+2, -2..ny+2), 2D arrays with a unit z extent */
+    a1 hy, hyc;
+    a3 hz, hzc, hz1, hz2;
     /* registered variables (setup.cpp:38-70) */
     a3 var[LO_NVAR];
     /* package arrays (setup.cpp:170-212) */
@@ -170,28 +179,68 @@ static void make_axis(long n, double mn, double mx, a1 *xb, a1 *xc, a1 *dxb, a1 
     L(*dxc, -1) = L(*dxc, 0);
 }
 
-/* LARE3D::grid (setup.cpp:337-390), Cartesian. */
+#define H2(a, i, j) A(a, i, j, 0)
+
+/* Metric factors of LARE3D::grid (setup.cpp:276-335).  Cylindrical and
+ * spherical: hy = |xb|, hyc = |xc| (the reference's loop also writes
+ * hyc(-2), outside hyc's extent, from xc(-2), outside xc's; nothing reads
+ * it, so it is left out); spherical: hz = hy |sin yb|, hz2 = hyc |sin yb|
+ * over y -2..ny+2 and hz1 = hy |sin yc|, hzc = hyc |sin yc| over y
+ * -1..ny+2; 1 elsewhere.  The node volume factor hyv only enters cvc, which
+ * nothing reads. */
+static void make_geometry(lo_ctx *c)
+{
+    const long nx = c->nx, ny = c->ny;
+    c->hy = a1_new(-2, nx + 2);
+    c->hyc = a1_new(-1, nx + 2);
+    c->hz = a3_new(-2, nx + 2, -2, ny + 2, 0, 0, 1.0);
+    c->hzc = a3_new(-1, nx + 2, -1, ny + 2, 0, 0, 1.0);
+    c->hz1 = a3_new(-2, nx + 2, -1, ny + 2, 0, 0, 1.0);
+    c->hz2 = a3_new(-1, nx + 2, -2, ny + 2, 0, 0, 1.0);
+    for (long ix = -2; ix <= nx + 2; ++ix) {
+        L(c->hy, ix) = c->geometry == 0 ? 1.0 : fabs(L(c->xb, ix));
+        if (ix >= -1) L(c->hyc, ix) = c->geometry == 0 ? 1.0 : fabs(L(c->xc, ix));
+    }
+    if (c->geometry != 2) return;
+    for (long iy = -2; iy <= ny + 2; ++iy) {
+        double s = fabs(sin(L(c->yb, iy)));
+        for (long ix = -2; ix <= nx + 2; ++ix) H2(c->hz, ix, iy) = L(c->hy, ix) * s;
+        for (long ix = -1; ix <= nx + 2; ++ix) H2(c->hz2, ix, iy) = L(c->hyc, ix) * s;
+    }
+    for (long iy = -1; iy <= ny + 2; ++iy) {
+        double sc = fabs(sin(L(c->yc, iy)));
+        for (long ix = -2; ix <= nx + 2; ++ix) H2(c->hz1, ix, iy) = L(c->hy, ix) * sc;
+        for (long ix = -1; ix <= nx + 2; ++ix) H2(c->hzc, ix, iy) = L(c->hyc, ix) * sc;
+    }
+}
+
+/* LARE3D::grid (setup.cpp:337-390): areas and volumes with the metric
+ * factors, including the reference's separate y = -2 and z = -2 layers
+ * (setup.cpp:356-373: dzab's z = -2 layer takes hyc, not hy). */
 static void make_metric(lo_ctx *c)
 {
     const long nx = c->nx, ny = c->ny, nz = c->nz;
     FOR3(ix, -2, nx + 2, iy, -1, ny + 2, iz, -1, nz + 2)
     {
         double dydz = L(c->dyb, iy) * L(c->dzb, iz);
-        A(c->dxab, ix, iy, iz) = dydz;
+        A(c->dxab, ix, iy, iz) = dydz * L(c->hy, ix) * H2(c->hz1, ix, iy);
     }
     FOR3(ix, -1, nx + 2, iy, -2, ny + 2, iz, -1, nz + 2)
     {
-        A(c->dyab, ix, iy, iz) = L(c->dxb, ix) * L(c->dzb, iz);
+        A(c->dyab, ix, iy, iz) = L(c->dxb, ix) * L(c->dzb, iz) * H2(c->hz2, ix, iy);
     }
     FOR3(ix, -1, nx + 2, iy, -1, ny + 2, iz, -2, nz + 2)
     {
-        A(c->dzab, ix, iy, iz) = L(c->dxb, ix) * L(c->dyb, iy);
+        if (iz == -2)
+            A(c->dzab, ix, iy, iz) = L(c->dxb, ix) * L(c->dyb, iy) * L(c->hyc, ix);
+        else
+            A(c->dzab, ix, iy, iz) = L(c->dxb, ix) * L(c->dyb, iy) * L(c->hy, ix);
     }
     FOR3(ix, -1, nx + 2, iy, -1, ny + 2, iz, -1, nz + 2)
     {
         double dx = L(c->dxb, ix);
         double dydz = L(c->dyb, iy) * L(c->dzb, iz);
-        A(c->cv, ix, iy, iz) = dx * dydz;
+        A(c->cv, ix, iy, iz) = dx * dydz * L(c->hyc, ix) * H2(c->hzc, ix, iy);
     }
     FOR3(ix, -1, nx + 2, iy, -1, ny + 2, iz, -1, nz + 2)
     {
@@ -199,9 +248,9 @@ static void make_metric(lo_ctx *c)
         double dy = L(c->dyb, iy);
         double dz = L(c->dzc, iz);
         double dydz = dy * dz;
-        A(c->dxac, ix, iy, iz) = dydz;
-        A(c->dyac, ix, iy, iz) = dx * dz;
-        A(c->dzac, ix, iy, iz) = dx * dy;
+        A(c->dxac, ix, iy, iz) = dydz * L(c->hyc, ix) * H2(c->hz2, ix, iy);
+        A(c->dyac, ix, iy, iz) = dx * dz * H2(c->hz1, ix, iy);
+        A(c->dzac, ix, iy, iz) = dx * dy * L(c->hy, ix);
         A(c->cvc, ix, iy, iz) = dx * dydz;
     }
 }
@@ -231,6 +280,7 @@ lo_ctx *lo_create(const lo_desc *d)
     c->largest_number = DBL_MAX;  /* shared_data.h:93 */
     c->resistive = d->resistive;
     c->rke = d->rke;
+    c->geometry = d->geometry;
     c->eta_background = d->eta_background;
     c->eta0 = d->eta0;
     c->j_max = d->j_max;
@@ -300,6 +350,7 @@ lo_ctx *lo_create(const lo_desc *d)
     if (c->rke) c->delta_ke = a3_new(CCC, 0.0);
 #undef CCC
 #undef D0N
+    make_geometry(c);
     make_metric(c);
     return c;
 }
@@ -317,8 +368,12 @@ void lo_destroy(lo_ctx *c)
     if (c->rke) free(c->delta_ke.p);
     for (int v = 0; v < LO_NVAR; ++v) free(c->var[v].p);
     a1 *ax[] = {&c->xb, &c->yb, &c->zb, &c->xc, &c->yc, &c->zc, &c->dxb, &c->dyb, &c->dzb,
-                &c->dxc, &c->dyc, &c->dzc};
+                &c->dxc, &c->dyc, &c->dzc, &c->hy, &c->hyc};
     for (size_t i = 0; i < sizeof(ax) / sizeof(ax[0]); ++i) free(ax[i]->p);
+    free(c->hz.p);
+    free(c->hzc.p);
+    free(c->hz1.p);
+    free(c->hz2.p);
     free(c);
 }
 
@@ -595,7 +650,8 @@ static void shock_viscosity(lo_ctx *c)
         long i2 = ix, j2 = iy, k2 = izm;
         long i0 = i1, j0 = j1 - 1, k0 = k1;
         long i3 = i2, j3 = j2 + 1, k3 = k2;
-        double dx = L(c->dyb, iy), dxp = L(c->dyb, iyp), dxm = L(c->dyb, iym);
+        double dx = L(c->dyb, iy) * L(c->hy, ix), dxp = L(c->dyb, iyp) * L(c->hy, ix),
+               dxm = L(c->dyb, iym) * L(c->hy, ix);
         double dvdots = -(A(vy, i1, j1, k1) - A(vy, i2, j2, k2));
         double cs_edge = pmin(A(cs_v, i1, j1, k1), A(cs_v, i2, j2, k2));
         A(c->alpha2, ix, iy, iz) =
@@ -609,7 +665,8 @@ static void shock_viscosity(lo_ctx *c)
         long i2 = ix, j2 = iy, k2 = iz;
         long i0 = i1, j0 = j1, k0 = k1 - 1;
         long i3 = i2, j3 = j2, k3 = k2 + 1;
-        double dx = L(c->dzb, iz), dxp = L(c->dzb, izp), dxm = L(c->dzb, izm);
+        double dx = L(c->dzb, iz) * H2(c->hz, ix, iy), dxp = L(c->dzb, izp) * H2(c->hz, ix, iy),
+               dxm = L(c->dzb, izm) * H2(c->hz, ix, iy);
         double dvdots = -(A(vz, i1, j1, k1) - A(vz, i2, j2, k2));
         double cs_edge = pmin(A(cs_v, i1, j1, k1), A(cs_v, i2, j2, k2));
         A(c->alpha3, ix, iy, iz) =
@@ -640,8 +697,8 @@ static void shock_viscosity(lo_ctx *c)
         A(c->p_visc, ix, iy, iz) = pmax(A(c->p_visc, ix, iy, iz), -A(c->alpha2, ix, iy, iz) * sqrt(a2));
         A(c->p_visc, ix, iy, iz) = pmax(A(c->p_visc, ix, iy, iz), -A(c->alpha3, ix, iy, iz) * sqrt(a9));
         double dx = L(c->dxb, ix);
-        double dy = L(c->dyb, iy);
-        double dz = L(c->dzb, iz);
+        double dy = L(c->dyb, iy) * L(c->hyc, ix);
+        double dz = L(c->dzb, iz) * H2(c->hz2, ix, iy);
         A(c->visc_heat, ix, iy, iz) =
             -0.25 * dy * dz * A(c->alpha1, ix, iy, iz) * a1
             - 0.25 * dx * dz * A(c->alpha2, ix, iy, iz) * a2
@@ -665,8 +722,8 @@ static void shock_viscosity(lo_ctx *c)
     {
         long izm = iz - 1, izp = iz + 1, iym = iy - 1, iyp = iy + 1, ixm = ix - 1, ixp = ix + 1;
         double dx = L(c->dxb, ix);
-        double dy = L(c->dyb, iy);
-        double dz = L(c->dzb, iz);
+        double dy = L(c->dyb, iy) * L(c->hyc, ix);
+        double dz = L(c->dzb, iz) * H2(c->hz2, ix, iy);
         double a1 = A(c->alpha1, ix, iyp, izp) * dy * dz;
         double a2 = A(c->alpha1, ixp, iyp, izp) * dy * dz;
         double a3v = A(c->alpha2, ix, iy, izp) * dx * dz;
@@ -692,15 +749,17 @@ static void set_dt(lo_ctx *c)
     const long nx = c->nx, ny = c->ny, nz = c->nz;
     const a3 rho = RHO(c), vx = VX(c), vy = VY(c), vz = VZ(c), bx = BX(c), by = BY(c), bz = BZ(c);
     double red = c->largest_number;
+    /* lagran.cpp:384: the x range starts at 1 outside Cartesian geometry */
+    const long i0 = c->geometry == 0 ? 0 : 1;
 #pragma omp parallel for schedule(static) reduction(min : red)
-    for (long ix = 0; ix <= nx; ++ix)
+    for (long ix = i0; ix <= nx; ++ix)
         for (long iy = 0; iy <= ny; ++iy)
             for (long iz = 0; iz <= nz; ++iz) {
                 long izm = iz - 1, iym = iy - 1, ixm = ix - 1;
                 double dx = L(c->dxb, ix);
                 double dy = L(c->dyb, iy);
                 double dz = L(c->dzb, iz);
-                double dhx = dx, dhy = dy, dhz = dz;
+                double dhx = dx, dhy = dy * L(c->hyc, ix), dhz = dz * H2(c->hzc, ix, iy);
                 double rho0 = pmax(A(rho, ix, iy, iz), c->none_zero);
                 double cs2 = c->gas_gamma * A(c->pressure, ix, iy, iz) / rho0;
                 double w1 = (A(bx, ix, iy, iz) * A(bx, ix, iy, iz) + A(by, ix, iy, iz) * A(by, ix, iy, iz) +
@@ -934,7 +993,10 @@ static void predictor_step(lo_ctx *c)
     {
         long izp = iz + 1, iyp = iy + 1, ixp = ix + 1;
         double dx = L(c->dxc, ix), dy = L(c->dyc, iy), dz = L(c->dzc, iz);
-        double dhy = dy, dhz = dz;
+        double h2 = L(c->hy, ix), h2x = L(c->hy, ixp);
+        double h3 = H2(c->hz, ix, iy), h3x = H2(c->hz, ixp, iy), h3y = H2(c->hz, ix, iyp);
+        double h2c = L(c->hyc, ix), h3c = H2(c->hzc, ix, iy);
+        double dhy = h2c * dy, dhz = h3c * dz;
         double pp = A(P, ix, iy, iz), ppx = A(P, ixp, iy, iz), ppy = A(P, ix, iyp, iz), ppz = A(P, ix, iy, izp);
         double ppxy = A(P, ixp, iyp, iz), ppxz = A(P, ixp, iy, izp), ppyz = A(P, ix, iyp, izp);
         double ppxyz = A(P, ixp, iyp, izp);
@@ -955,23 +1017,23 @@ static void predictor_step(lo_ctx *c)
         double cvz = A(C1, ix, iy, iz) + A(C1, ixp, iy, iz) + A(C1, ix, iyp, iz) + A(C1, ixp, iyp, iz);
         double cvzp = A(C1, ix, iy, izp) + A(C1, ixp, iy, izp) + A(C1, ix, iyp, izp) + A(C1, ixp, iyp, izp);
 
-        w1 = (A(B1z, ix, iy, iz) + A(B1z, ixp, iy, iz) + A(B1z, ix, iy, izp) + A(B1z, ixp, iy, izp)) / cvy;
-        w2 = (A(B1z, ix, iyp, iz) + A(B1z, ixp, iyp, iz) + A(B1z, ix, iyp, izp) + A(B1z, ixp, iyp, izp)) / cvyp;
-        double jx = (w2 - w1) / dhy;
+        w1 = (A(B1z, ix, iy, iz) + A(B1z, ixp, iy, iz) + A(B1z, ix, iy, izp) + A(B1z, ixp, iy, izp)) * h3 / cvy;
+        w2 = (A(B1z, ix, iyp, iz) + A(B1z, ixp, iyp, iz) + A(B1z, ix, iyp, izp) + A(B1z, ixp, iyp, izp)) * h3y / cvyp;
+        double jx = (w2 - w1) / dhy / h3c;
         w1 = (A(B1y, ix, iy, iz) + A(B1y, ixp, iy, iz) + A(B1y, ix, iyp, iz) + A(B1y, ixp, iyp, iz)) / cvz;
         w2 = (A(B1y, ix, iy, izp) + A(B1y, ixp, iy, izp) + A(B1y, ix, iyp, izp) + A(B1y, ixp, iyp, izp)) / cvzp;
         jx -= (w2 - w1) / dhz;
 
-        w1 = (A(B1z, ix, iy, iz) + A(B1z, ix, iyp, iz) + A(B1z, ix, iy, izp) + A(B1z, ix, iyp, izp)) / cvx;
-        w2 = (A(B1z, ixp, iy, iz) + A(B1z, ixp, iyp, iz) + A(B1z, ixp, iy, izp) + A(B1z, ixp, iyp, izp)) / cvxp;
-        double jy = -(w2 - w1) / dx;
+        w1 = (A(B1z, ix, iy, iz) + A(B1z, ix, iyp, iz) + A(B1z, ix, iy, izp) + A(B1z, ix, iyp, izp)) * h3 / cvx;
+        w2 = (A(B1z, ixp, iy, iz) + A(B1z, ixp, iyp, iz) + A(B1z, ixp, iy, izp) + A(B1z, ixp, iyp, izp)) * h3x / cvxp;
+        double jy = -(w2 - w1) / dx / h3c;
         w1 = (A(B1x, ix, iy, iz) + A(B1x, ixp, iy, iz) + A(B1x, ix, iyp, iz) + A(B1x, ixp, iyp, iz)) / cvz;
         w2 = (A(B1x, ix, iy, izp) + A(B1x, ixp, iy, izp) + A(B1x, ix, iyp, izp) + A(B1x, ixp, iyp, izp)) / cvzp;
         jy += (w2 - w1) / dhz;
 
-        w1 = (A(B1y, ix, iy, iz) + A(B1y, ix, iyp, iz) + A(B1y, ix, iy, izp) + A(B1y, ix, iyp, izp)) / cvx;
-        w2 = (A(B1y, ixp, iy, iz) + A(B1y, ixp, iyp, iz) + A(B1y, ixp, iy, izp) + A(B1y, ixp, iyp, izp)) / cvxp;
-        double jz = (w2 - w1) / dx;
+        w1 = (A(B1y, ix, iy, iz) + A(B1y, ix, iyp, iz) + A(B1y, ix, iy, izp) + A(B1y, ix, iyp, izp)) * h2 / cvx;
+        w2 = (A(B1y, ixp, iy, iz) + A(B1y, ixp, iyp, iz) + A(B1y, ixp, iy, izp) + A(B1y, ixp, iyp, izp)) * h2x / cvxp;
+        double jz = (w2 - w1) / dx / h2c;
         w1 = (A(B1x, ix, iy, iz) + A(B1x, ixp, iy, iz) + A(B1x, ix, iy, izp) + A(B1x, ixp, iy, izp)) / cvy;
         w2 = (A(B1x, ix, iyp, iz) + A(B1x, ixp, iyp, iz) + A(B1x, ix, iyp, izp) + A(B1x, ixp, iyp, izp)) / cvyp;
         jz -= (w2 - w1) / dhy;
@@ -986,6 +1048,20 @@ static void predictor_step(lo_ctx *c)
         fx += (jy * bzv - jz * byv) / c->mu0;
         fy += (jz * bxv - jx * bzv) / c->mu0;
         fz += (jx * byv - jy * bxv) / c->mu0;
+        /* geometry corrections (lagran.cpp:711-721) */
+        if (c->geometry == 2) {
+            double cotantheta = 1.0 / tan(L(c->yb, iy));
+            double rv = A(c->rho_v, ix, iy, iz), ux = A(VX(c), ix, iy, iz), uy = A(VY(c), ix, iy, iz),
+                   uz = A(VZ(c), ix, iy, iz), r = L(c->xb, ix);
+            fx += rv * (uy * uy + uz * uz) / r;
+            fy -= rv * (uy * ux - cotantheta * uz * uz) / r;
+            fz -= rv * (uz * ux + cotantheta * uz * uy) / r;
+        } else if (c->geometry == 1) {
+            double rv = A(c->rho_v, ix, iy, iz), ux = A(VX(c), ix, iy, iz), uy = A(VY(c), ix, iy, iz),
+                   r = L(c->xb, ix);
+            fx += rv * uy * uy / r;
+            fy -= rv * uy * ux / r;
+        }
         A(c->fx, ix, iy, iz) = fx;
         A(c->fy, ix, iy, iz) = fy;
         A(c->fz, ix, iy, iz) = fz;
@@ -1083,8 +1159,8 @@ static void shock_heating(lo_ctx *c)
         double a12 = DOT(ix, iym, izm, ix, iym, iz);
 #undef DOT
         double dx = L(c->dxb, ix);
-        double dy = L(c->dyb, iy);
-        double dz = L(c->dzb, iz);
+        double dy = L(c->dyb, iy) * L(c->hyc, ix);
+        double dz = L(c->dzb, iz) * H2(c->hz2, ix, iy);
         A(c->visc_heat, ix, iy, iz) =
             (-0.25 * dy * dz * A(c->alpha1, ix, iy, iz) * a1 - 0.25 * dx * dz * A(c->alpha2, ix, iy, iz) * a2 -
              0.25 * dy * dz * A(c->alpha1, ix, iyp, iz) * a3v - 0.25 * dx * dz * A(c->alpha2, ixm, iy, iz) * a4 -
@@ -1190,7 +1266,7 @@ static void remap_prep(lo_ctx *c, int dir)
             A(c->cv2, ix, iy, iz) = vol * (1.0 + dv);
             dv = dv + dvydy * dt;
             A(c->cv1, ix, iy, iz) = vol * (1.0 + dv);
-            A(c->db1, ix, iy, iz) = L(c->dyb, iy) + (vyb - vybm) * dt;
+            A(c->db1, ix, iy, iz) = L(c->hyc, ix) * L(c->dyb, iy) + (vyb - vybm) * dt;
         } else {
             double dvxdx = xpass * (vxb * A(c->dxab, ix, iy, iz) - vxbm * A(c->dxab, ixm, iy, iz)) / vol;
             double dvydy = ypass * (vyb * A(c->dyab, ix, iy, iz) - vybm * A(c->dyab, ix, iym, iz)) / vol;
@@ -1199,7 +1275,7 @@ static void remap_prep(lo_ctx *c, int dir)
             A(c->cv2, ix, iy, iz) = vol * (1.0 + dv);
             dv = dv + dvzdz * dt;
             A(c->cv1, ix, iy, iz) = vol * (1.0 + dv);
-            A(c->db1, ix, iy, iz) = L(c->dzb, iz) + (vzb - vzbm) * dt;
+            A(c->db1, ix, iy, iz) = H2(c->hzc, ix, iy) * L(c->dzb, iz) + (vzb - vzbm) * dt;
         }
     }
     /* cvc1: vertex control volume before remap */

@@ -42,6 +42,7 @@ typedef struct lo_desc {
     int resistive;              /* resistiveMHD */
     double eta_background, eta0, j_max;
     int rke;                    /* remap kinetic-energy correction */
+    int geometry;               /* geometryType (shared_data.h:38-43): 0 Cartesian, 1 cylindrical, 2 spherical */
 } lo_desc;
 
 typedef struct lo_ctx lo_ctx;

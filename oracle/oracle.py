@@ -39,7 +39,7 @@ class LoDesc(ctypes.Structure):
                 ("visc2", ctypes.c_double), ("mu0", ctypes.c_double),
                 ("dt_multiplier", ctypes.c_double), ("resistive", ctypes.c_int),
                 ("eta_background", ctypes.c_double), ("eta0", ctypes.c_double),
-                ("j_max", ctypes.c_double), ("rke", ctypes.c_int)]
+                ("j_max", ctypes.c_double), ("rke", ctypes.c_int), ("geometry", ctypes.c_int)]
 
 
 _lib = None
@@ -94,7 +94,7 @@ class Oracle:
         d = LoDesc(desc.nx, desc.ny, desc.nz, desc.x_min, desc.x_max, desc.y_min, desc.y_max,
                    desc.z_min, desc.z_max, desc.gas_gamma, desc.visc1, desc.visc2, desc.mu0,
                    desc.dt_multiplier, int(desc.resistive), desc.eta_background, desc.eta0,
-                   desc.j_max, int(getattr(desc, "rke", 0)))
+                   desc.j_max, int(getattr(desc, "rke", 0)), int(getattr(desc, "geometry", 0)))
         self.h = lib().lo_create(ctypes.byref(d))
         self.nx, self.ny, self.nz = desc.nx, desc.ny, desc.nz
 
@@ -190,7 +190,7 @@ def problem_bcs(o, kind, bcx=0):
 
 def run_reference(out_dir, problem="ot", nx=16, ny=None, nz=None, steps=10, dumps=(), seed=1,
                   amp=0.0, noise=0.0, bcx=None, threads=4, fast=False, beta=-1.0, exe=None, rke=False,
-                  lare_bc=False, eta=0.0, eta0=0.0, j_max=None):
+                  lare_bc=False, eta=0.0, eta0=0.0, j_max=None, geometry=None, x0=0.0, y0=0.0):
     """Run the reference driver (oracle/_ref, built from the reference's own
     sources; exe selects another build of it, such as DROPIN_DRIVER) and
     return its meta dict.  Resistive runs (eta or eta0 > 0) use the build
@@ -215,6 +215,10 @@ def run_reference(out_dir, problem="ot", nx=16, ny=None, nz=None, steps=10, dump
         args += ["--rke"]
     if lare_bc:
         args += ["--lare-bc"]
+    if geometry and geometry != "cartesian":
+        args += ["--geometry", geometry]
+    if x0 or y0:
+        args += ["--x0", repr(float(x0)), "--y0", repr(float(y0))]
     if resistive:
         args += ["--eta", repr(float(eta)), "--eta0", repr(float(eta0))]
         if j_max is not None:

@@ -20,10 +20,15 @@
 //              [--dump s1,s2,...] [--out DIR] [--seed S] [--amp A] [--beta B]
 //              [--noise F] [--eta E] [--eta0 E0] [--j-max J]
 //              [--bcx periodic|zerograd|mirror|clamp] [--time] [--rke] [--lare-bc]
+//              [--geometry cartesian|cylindrical|spherical] [--x0 X] [--y0 Y]
 // --eta / --eta0 (either > 0) turn resistive MHD on (shared_data.h:113-117):
 // eta_background = E, and eta0 is added where the current exceeds j_max
 // (lagran.cpp:473).  The reference's resistive path runs only in the
-// ref_driver_res build (oracle/Makefile).
+// ref_driver_res build (oracle/Makefile).  --geometry selects the
+// coordinate system (shared_data.h:38-43); --x0 / --y0 shift the x and y
+// extents of the problem's domain (a curvilinear grid needs r > 0, and
+// theta away from the poles); Brio-Wu's discontinuity stays at the x
+// midpoint.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -78,6 +83,8 @@ namespace refdrv
         bool phases = false;
         bool rke = false;   // remap kinetic-energy correction
         bool lareBC = false; // LARE3DInitialConditions' BC_OTHER conditions on every edge
+        LARE::geometryType geometry = LARE::geometryType::Cartesian;
+        double x0 = 0.0, y0 = 0.0; // shifts of the domain's x and y extents
         std::vector<double> dts;
         double elapsed = 0.0;
         std::chrono::steady_clock::time_point t0, tLast;
@@ -200,7 +207,7 @@ namespace refdrv
             data.t_end = 1.0e30;
             data.dt_snapshots = 1.0e30;
             data.dt_multiplier = 0.8;
-            data.geometry = LARE::geometryType::Cartesian;
+            data.geometry = c.geometry;
             data.visc1 = 0.1;
             data.visc2 = 1.0;
             data.mf = 1.2;
@@ -236,6 +243,10 @@ namespace refdrv
                 data.z_max = (data.x_max - data.x_min) * static_cast<double>(c.nz) / static_cast<double>(c.nx);
                 data.gas_gamma = 2.0;
             }
+            data.x_min += c.x0;
+            data.x_max += c.x0;
+            data.y_min += c.y0;
+            data.y_max += c.y0;
             if (c.eta > 0.0 || c.eta0 > 0.0)
             {
                 data.resistiveMHD = true;
@@ -339,7 +350,7 @@ namespace refdrv
                         else
                         {
                             double r, p, b;
-                            if (xc(ix) < 0.0)
+                            if (xc(ix) < 0.5 * (data.x_min + data.x_max))
                             {
                                 r = 1.0;
                                 b = 1.0;
@@ -445,6 +456,25 @@ int main(int argc, char *argv[])
             c.noise = std::stod(next());
         else if (a == "--eta")
             c.eta = std::stod(next());
+        else if (a == "--geometry")
+        {
+            std::string gname = next();
+            if (gname == "cartesian")
+                c.geometry = LARE::geometryType::Cartesian;
+            else if (gname == "cylindrical")
+                c.geometry = LARE::geometryType::Cylindrical;
+            else if (gname == "spherical")
+                c.geometry = LARE::geometryType::Spherical;
+            else
+            {
+                std::fprintf(stderr, "unknown geometry %s\n", gname.c_str());
+                return 2;
+            }
+        }
+        else if (a == "--x0")
+            c.x0 = std::stod(next());
+        else if (a == "--y0")
+            c.y0 = std::stod(next());
         else if (a == "--eta0")
             c.eta0 = std::stod(next());
         else if (a == "--j-max")

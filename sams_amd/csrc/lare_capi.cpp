@@ -34,6 +34,45 @@ int fail(int code, const std::string &msg)
             return fail(-3, std::string(#call) + ": " + hipGetErrorString(e_));                   \
     } while (0)
 
+// Metric factors of LARE3D::grid (setup.cpp:276-335) and the inputs of the
+// geometric forces (lagran.cpp:711-721), on the host (glibc sin and tan, as
+// the reference calls them): h1 = hy, hyc, xb, cot(yb) at [i + 2]; h2 = hz,
+// hzc, hz1, hz2 at [(ix + 2) (ny + 5) + iy + 2].  Cartesian: every factor 1.
+// Cylindrical and spherical: hy = |xb|, hyc = |xc| (the reference's loop also
+// writes hyc(-2) from xc(-2), both outside their extents; nothing reads it);
+// spherical: hz = hy |sin yb|, hz2 = hyc |sin yb| over y -2..ny+2 and
+// hz1 = hy |sin yc|, hzc = hyc |sin yc| over y -1..ny+2.
+void make_geometry(const lare_desc &d, const std::vector<double> &xb, const std::vector<double> &xc,
+                   const std::vector<double> &yb, const std::vector<double> &yc, std::vector<double> h1[4],
+                   std::vector<double> h2[4])
+{
+    const long nx = d.nx, ny = d.ny;
+    const size_t lmax = (size_t)std::max(d.nx, std::max(d.ny, d.nz)) + 5;
+    for (int k = 0; k < 4; ++k) {
+        h1[k].assign(lmax, 1.0);
+        h2[k].assign((size_t)(nx + 5) * (size_t)(ny + 5), 1.0);
+    }
+    auto P = [&](long ix, long iy) { return (size_t)(ix + 2) * (size_t)(ny + 5) + (size_t)(iy + 2); };
+    for (long ix = -2; ix <= nx + 2; ++ix) h1[2][ix + 2] = xb[ix + 2];
+    for (long iy = -2; iy <= ny + 2; ++iy) h1[3][iy + 2] = 1.0 / std::tan(yb[iy + 2]);
+    if (d.geometry == LARE_CARTESIAN) return;
+    for (long ix = -2; ix <= nx + 2; ++ix) {
+        h1[0][ix + 2] = std::abs(xb[ix + 2]);
+        if (ix >= -1) h1[1][ix + 2] = std::abs(xc[ix + 2]);
+    }
+    if (d.geometry != LARE_SPHERICAL) return;
+    for (long iy = -2; iy <= ny + 2; ++iy) {
+        const double s = std::abs(std::sin(yb[iy + 2]));
+        for (long ix = -2; ix <= nx + 2; ++ix) h2[0][P(ix, iy)] = h1[0][ix + 2] * s;
+        for (long ix = -1; ix <= nx + 2; ++ix) h2[3][P(ix, iy)] = h1[1][ix + 2] * s;
+    }
+    for (long iy = -1; iy <= ny + 2; ++iy) {
+        const double sc = std::abs(std::sin(yc[iy + 2]));
+        for (long ix = -2; ix <= nx + 2; ++ix) h2[2][P(ix, iy)] = h1[0][ix + 2] * sc;
+        for (long ix = -1; ix <= nx + 2; ++ix) h2[1][P(ix, iy)] = h1[1][ix + 2] * sc;
+    }
+}
+
 // Stagger per dimension of each registered variable (setup.cpp:45-69).
 const int kStag[LARE_NFIELDS][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 1, 1}, {1, 1, 1},
                                     {1, 1, 1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 1},
@@ -301,9 +340,10 @@ namespace {
 // 393-431, 176-201): xb(i) = min + ((max-min)/n) * i on -2..n+2, centres
 // 0.5*(xb(i)+xb(i-1)) on -1..n+2, backward differences with the first
 // delta copied from its neighbour.  Stored at [i+2].
-void make_axis(long n, double mn, double mx, std::vector<double> &db, std::vector<double> &dc)
+void make_axis(long n, double mn, double mx, std::vector<double> &db, std::vector<double> &dc,
+               std::vector<double> *pb = nullptr, std::vector<double> *pc = nullptr)
 {
-    std::vector<double> xb(n + 5), xc(n + 5);
+    std::vector<double> xb(n + 5), xc(n + 5, 0.0);
     db.assign(n + 5, 0.0);
     dc.assign(n + 5, 0.0);
     const double delta = (mx - mn) / (double)(n + 1 - 1);
@@ -313,6 +353,8 @@ void make_axis(long n, double mn, double mx, std::vector<double> &db, std::vecto
     for (long i = -1; i <= n + 2; ++i) xc[i + 2] = 0.5 * (xb[i + 2] + xb[i + 1]);
     for (long i = 0; i <= n + 2; ++i) dc[i + 2] = xc[i + 2] - xc[i + 1];
     dc[1] = dc[2];
+    if (pb) *pb = xb;  // positions at [i + 2]: xb over -2..n+2, xc over -1..n+2
+    if (pc) *pc = xc;
 }
 
 }  // namespace
@@ -429,6 +471,11 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
         return fail(-1, "lare_ctx_create: each of nx, ny, nz must be at least 2");
     if (d->precision != LARE_FP64 && d->precision != LARE_FP32_STORAGE)
         return fail(-1, "lare_ctx_create: unknown storage precision");
+    if (d->geometry < LARE_CARTESIAN || d->geometry > LARE_SPHERICAL)
+        return fail(-1, "lare_ctx_create: unknown geometry");
+    if (d->geometry != LARE_CARTESIAN && (d->precision != LARE_FP64 || d->rke))
+        return fail(-1, "lare_ctx_create: cylindrical and spherical grids run the one-kernel-per-reference-kernel "
+                        "set, which has FP64 storage and no kinetic-energy correction");
     long zp = 8;  // row pitch rounded to this many elements (LARE_ZPITCH)
     if (const char *v = std::getenv("LARE_ZPITCH")) zp = std::max(1L, std::atol(v));
     const long pz = ((d->nz + 5) + zp - 1) / zp * zp;
@@ -463,6 +510,7 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
     g.prm.eta0 = d->eta0;
     g.prm.j_max = d->j_max;
     g.prm.rke = d->rke != 0;
+    g.prm.geometry = d->geometry;
     g.kernel_set = 1;
     std::memset(g.bck, 0, sizeof(g.bck));
     std::memset(g.bcv, 0, sizeof(g.bcv));
@@ -471,8 +519,9 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
     const size_t tsz = g.fp32 ? sizeof(float) : sizeof(double);
     const size_t lmax = (size_t)std::max(d->nx, std::max(d->ny, d->nz)) + 5;
     const PoolLayout lay = pool_layout();
+    const size_t plane = (size_t)(d->nx + 5) * (size_t)(d->ny + 5);
     g.pool_bytes = kEngineT * lay.stride(box * tsz) + (kEngineD + (g.prm.rke ? 1 : 0)) * lay.stride(box * sizeof(double)) +
-                   6 * lmax * sizeof(double) + 256;
+                   (6 + 4) * lmax * sizeof(double) + 4 * plane * sizeof(double) + 256;
     hipError_t e = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -502,6 +551,24 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
         d1[k] = (double *)p;
         p += lmax * sizeof(double);
     }
+    // metric factors of the one-kernel-per-reference-kernel set
+    double *geo1[4], *geo2[4];
+    for (int k = 0; k < 4; ++k) {
+        geo1[k] = (double *)p;
+        p += lmax * sizeof(double);
+    }
+    for (int k = 0; k < 4; ++k) {
+        geo2[k] = (double *)p;
+        p += plane * sizeof(double);
+    }
+    a.hy = geo1[0];
+    a.hyc = geo1[1];
+    a.xbv = geo1[2];
+    a.cotv = geo1[3];
+    a.hz = geo2[0];
+    a.hzc = geo2[1];
+    a.hz1 = geo2[2];
+    a.hz2 = geo2[3];
     a.dxb = d1[0];
     a.dyb = d1[1];
     a.dzb = d1[2];
@@ -536,9 +603,9 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
         e = hipMemsetAsync(g.pool, 0, g.pool_bytes, g.stream);
         if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
         if (e != hipSuccess) { rc = fail(-3, std::string("pool clear: ") + hipGetErrorString(e)); break; }
-        std::vector<double> db[3], dc[3];
-        make_axis(d->nx, d->x_min, d->x_max, db[0], dc[0]);
-        make_axis(d->ny, d->y_min, d->y_max, db[1], dc[1]);
+        std::vector<double> db[3], dc[3], pxb, pxc, pyb, pyc;
+        make_axis(d->nx, d->x_min, d->x_max, db[0], dc[0], &pxb, &pxc);
+        make_axis(d->ny, d->y_min, d->y_max, db[1], dc[1], &pyb, &pyc);
         make_axis(d->nz, d->z_min, d->z_max, db[2], dc[2]);
         double *dstb[3] = {a.dxb, a.dyb, a.dzb}, *dstc[3] = {a.dxc, a.dyc, a.dzc};
         for (int k = 0; k < 3 && rc == 0; ++k) {
@@ -548,12 +615,33 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
             if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
             if (e != hipSuccess) rc = fail(-3, std::string("axis upload: ") + hipGetErrorString(e));
         }
+        if (rc) break;
+        std::vector<double> h1[4], h2[4];
+        make_geometry(*d, pxb, pxc, pyb, pyc, h1, h2);
+        for (int k = 0; k < 4 && e == hipSuccess; ++k) {
+            e = hipMemcpyAsync(geo1[k], h1[k].data(), h1[k].size() * sizeof(double), hipMemcpyHostToDevice, g.stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(geo2[k], h2[k].data(), h2[k].size() * sizeof(double), hipMemcpyHostToDevice,
+                                   g.stream);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+        if (e != hipSuccess) rc = fail(-3, std::string("metric factor upload: ") + hipGetErrorString(e));
     } while (0);
     if (rc) {
         (void)hipFree(g.pool);
         (void)hipStreamDestroy(g.stream);
         delete c;
         return rc;
+    }
+    // cylindrical and spherical grids run the one-kernel-per-reference-kernel
+    // set, whose stored metric carries the metric factors
+    if (d->geometry != LARE_CARTESIAN) {
+        rc = ensure_set0(c);
+        if (rc) {
+            lare_ctx_destroy(c);
+            return rc;
+        }
+        g.kernel_set = 0;
     }
     // set-up of the allocation-free energy diagnostic: the grid as a group
     // of one with its energy launch table, and this device's scratch address
@@ -876,6 +964,8 @@ extern "C" int lare_set_kernel_set(lare_ctx *c, int32_t kernel_set)
 {
     if (!c) return fail(-1, "lare_set_kernel_set: null context");
     if (kernel_set < 0 || kernel_set > 3) return fail(-1, "lare_set_kernel_set: unknown kernel set");
+    if (kernel_set != 0 && c->desc.geometry != LARE_CARTESIAN)
+        return fail(-1, "lare_set_kernel_set: cylindrical and spherical grids run kernel set 0 only");
     HIPCHK(hipStreamSynchronize(c->g.stream));
     if (c->g.st != 0) return fail(-1, "lare_set_kernel_set: only between steps");
     if (kernel_set == 0) {
@@ -957,6 +1047,11 @@ extern "C" int lare_batch_create(lare_ctx *const *members, int32_t n, lare_batch
                 delete b;
                 return fail(-1, "lare_batch_create: a context appears twice");
             }
+        if (members[i]->desc.geometry != LARE_CARTESIAN) {
+            delete b;
+            return fail(-1, "lare_batch_create: batches hold Cartesian grids (cylindrical and spherical grids "
+                            "run kernel set 0 alone)");
+        }
         b->members.push_back(members[i]);
         grids.push_back(&members[i]->g);
     }
@@ -1054,6 +1149,7 @@ extern "C" int lare_batch_energy(lare_batch *b, double *out)
 extern "C" int lare_energy(lare_ctx *c, double *out)
 {
     if (!c || !out) return fail(-1, "lare_energy: null argument");
+    if (c->desc.geometry != LARE_CARTESIAN) return fail(-1, "lare_energy: Cartesian grids only");
     int rc = ensure_solo(c);
     if (rc) return rc;
     std::vector<double> e;
@@ -1068,6 +1164,7 @@ extern "C" int lare_energy_noalloc(lare_ctx *c, double *out)
 {
     if (!c || !out) return fail(-1, "lare_energy_noalloc: null argument");
     if (!c->solo_ready) return fail(-1, "lare_energy_noalloc: context not set up");
+    if (c->desc.geometry != LARE_CARTESIAN) return fail(-1, "lare_energy_noalloc: Cartesian grids only");
     std::string err;
     const int rc = lare::group_energy_noalloc(c->solo, c->scratch, out, err);
     if (rc) return fail(rc, "lare_energy_noalloc: " + err);

@@ -55,6 +55,11 @@ struct Arr {
     double *fl[5];
     // 1D axis deltas, index i stored at [i + 2]
     double *dxb, *dyb, *dzb, *dxc, *dyc, *dzc;
+    // metric factors of LARE3D::grid (setup.cpp:276-335), computed on the
+    // host: hy, hyc at [ix + 2]; hz, hzc, hz1, hz2 at [(ix + 2) (ny + 5) + iy + 2];
+    // the x positions xb and cot(yb) of the geometric forces (lagran.cpp:711-721)
+    // at [i + 2] (1 and unused in Cartesian geometry)
+    double *hy, *hyc, *hz, *hzc, *hz1, *hz2, *xbv, *cotv;
     Scal *s;
 };
 
@@ -216,3 +221,10 @@ struct BCSet3 {
 #define DXC(i) (lare::gbl(A.dxc)[(i) + 2])
 #define DYC(i) (lare::gbl(A.dyc)[(i) + 2])
 #define DZC(i) (lare::gbl(A.dzc)[(i) + 2])
+// metric factors (kernel set 0)
+#define HY(i) (A.hy[(i) + 2])
+#define HYC(i) (A.hyc[(i) + 2])
+#define HZ(i, j) (A.hz[((i) + 2) * (G.ny + 5) + (j) + 2])
+#define HZC(i, j) (A.hzc[((i) + 2) * (G.ny + 5) + (j) + 2])
+#define HZ1(i, j) (A.hz1[((i) + 2) * (G.ny + 5) + (j) + 2])
+#define HZ2(i, j) (A.hz2[((i) + 2) * (G.ny + 5) + (j) + 2])

@@ -28,6 +28,7 @@ struct Params {
     int resistive;
     double eta_background, eta0, j_max;
     int rke;  // remap kinetic-energy correction
+    int geometry;  // enum lare_geometry: 0 Cartesian, 1 cylindrical, 2 spherical
 };
 
 // Optional per-kernel timing with HIP events on the grid's stream.

@@ -159,13 +159,13 @@ KERNEL void k_alpha(Dev D, R3 R, int dir, double visc1, double visc2_norm)
         A.alpha1[IX(ix, iy, iz)] = edge_viscosity(A, dt, visc1, visc2_norm, dvdots, dx, dxm, dxp, cs_edge, c1, G.sx);
     } else if (dir == 1) {
         const int c1 = IX(ix, iy - 1, iz - 1);
-        double dx = DYB(iy), dxp = DYB(iy + 1), dxm = DYB(iy - 1);
+        double dx = DYB(iy) * HY(ix), dxp = DYB(iy + 1) * HY(ix), dxm = DYB(iy - 1) * HY(ix);
         double dvdots = -(VY[c1] - VY[c1 + G.sy]);
         double cs_edge = pmin(A.cs_v[c1], A.cs_v[c1 + G.sy]);
         A.alpha2[IX(ix, iy, iz)] = edge_viscosity(A, dt, visc1, visc2_norm, dvdots, dx, dxm, dxp, cs_edge, c1, G.sy);
     } else {
         const int c1 = IX(ix, iy, iz - 1);
-        double dx = DZB(iz), dxp = DZB(iz + 1), dxm = DZB(iz - 1);
+        double dx = DZB(iz) * HZ(ix, iy), dxp = DZB(iz + 1) * HZ(ix, iy), dxm = DZB(iz - 1) * HZ(ix, iy);
         double dvdots = -(VZ[c1] - VZ[c1 + 1]);
         double cs_edge = pmin(A.cs_v[c1], A.cs_v[c1 + 1]);
         A.alpha3[IX(ix, iy, iz)] = edge_viscosity(A, dt, visc1, visc2_norm, dvdots, dx, dxm, dxp, cs_edge, c1, 1);
@@ -205,8 +205,8 @@ KERNEL void k_pvisc(Dev D, R3 R)
     pv = pmax(pv, -A.alpha3[c] * sqrt(a9));
     A.p_visc[c] = pv;
     double dx = DXB(ix);
-    double dy = DYB(iy);
-    double dz = DZB(iz);
+    double dy = DYB(iy) * HYC(ix);
+    double dz = DZB(iz) * HZ2(ix, iy);
     double vh =
         -0.25 * dy * dz * A.alpha1[c] * a1
         - 0.25 * dx * dz * A.alpha2[c] * a2
@@ -230,8 +230,8 @@ KERNEL void k_fvisc(Dev D, R3 R)
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     double dx = DXB(ix);
-    double dy = DYB(iy);
-    double dz = DZB(iz);
+    double dy = DYB(iy) * HYC(ix);
+    double dz = DZB(iz) * HZ2(ix, iy);
     double a1 = A.alpha1[c + sy + 1] * dy * dz;
     double a2 = A.alpha1[c + sx + sy + 1] * dy * dz;
     double a3 = A.alpha2[c + 1] * dx * dz;
@@ -269,7 +269,7 @@ KERNEL void k_set_dt(Dev D, R3 R, double gas_gamma, double mu0)
         double dx = DXB(ix);
         double dy = DYB(iy);
         double dz = DZB(iz);
-        double dhx = dx, dhy = dy, dhz = dz;
+        double dhx = dx, dhy = dy * HYC(ix), dhz = dz * HZC(ix, iy);
         double rho0 = pmax(RHO[c], DBL_EPSILON);
         double cs2 = ddiv(gas_gamma * A.pressure[c], rho0);
         double w1 = ddiv(ddiv(BX[c] * BX[c] + BY[c] * BY[c] + BZ[c] * BZ[c], mu0), rho0);
@@ -500,7 +500,7 @@ KERNEL void k_energy_pred(Dev D, R3 R, double gas_gamma)
 }
 
 // Forces and half-step velocities (lagran.cpp:619-731) over (0..n)^3.
-KERNEL void k_forces(Dev D, R3 R, double mu0)
+KERNEL void k_forces(Dev D, R3 R, double mu0, int geometry)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
@@ -508,7 +508,10 @@ KERNEL void k_forces(Dev D, R3 R, double mu0)
     const double dt = A.s->dt;
     const double *P = A.pressure, *C1 = A.cv1, *B1x = A.bx1, *B1y = A.by1, *B1z = A.bz1;
     double dx = DXC(ix), dy = DYC(iy), dz = DZC(iz);
-    double dhy = dy, dhz = dz;
+    const double h2 = HY(ix), h2x = HY(ix + 1);
+    const double h3 = HZ(ix, iy), h3x = HZ(ix + 1, iy), h3y = HZ(ix, iy + 1);
+    const double h2c = HYC(ix), h3c = HZC(ix, iy);
+    double dhy = h2c * dy, dhz = h3c * dz;
     const int x_ = c + sx, y_ = c + sy, z_ = c + 1, xy = c + sx + sy, xz = c + sx + 1, yz = c + sy + 1, xyz = c + sx + sy + 1;
     double pp = P[c], ppx = P[x_], ppy = P[y_], ppz = P[z_], ppxy = P[xy], ppxz = P[xz], ppyz = P[yz], ppxyz = P[xyz];
     double w1 = pp + ppy + ppz + ppyz;
@@ -528,23 +531,23 @@ KERNEL void k_forces(Dev D, R3 R, double mu0)
     double cvz = C1[c] + C1[x_] + C1[y_] + C1[xy];
     double cvzp = C1[z_] + C1[xz] + C1[yz] + C1[xyz];
 
-    w1 = ddiv(B1z[c] + B1z[x_] + B1z[z_] + B1z[xz], cvy);
-    w2 = ddiv(B1z[y_] + B1z[xy] + B1z[yz] + B1z[xyz], cvyp);
-    double jx = ddiv(w2 - w1, dhy);
+    w1 = ddiv((B1z[c] + B1z[x_] + B1z[z_] + B1z[xz]) * h3, cvy);
+    w2 = ddiv((B1z[y_] + B1z[xy] + B1z[yz] + B1z[xyz]) * h3y, cvyp);
+    double jx = ddiv(ddiv(w2 - w1, dhy), h3c);
     w1 = ddiv(B1y[c] + B1y[x_] + B1y[y_] + B1y[xy], cvz);
     w2 = ddiv(B1y[z_] + B1y[xz] + B1y[yz] + B1y[xyz], cvzp);
     jx -= ddiv(w2 - w1, dhz);
 
-    w1 = ddiv(B1z[c] + B1z[y_] + B1z[z_] + B1z[yz], cvx);
-    w2 = ddiv(B1z[x_] + B1z[xy] + B1z[xz] + B1z[xyz], cvxp);
-    double jy = ddiv(-(w2 - w1), dx);
+    w1 = ddiv((B1z[c] + B1z[y_] + B1z[z_] + B1z[yz]) * h3, cvx);
+    w2 = ddiv((B1z[x_] + B1z[xy] + B1z[xz] + B1z[xyz]) * h3x, cvxp);
+    double jy = ddiv(ddiv(-(w2 - w1), dx), h3c);
     w1 = ddiv(B1x[c] + B1x[x_] + B1x[y_] + B1x[xy], cvz);
     w2 = ddiv(B1x[z_] + B1x[xz] + B1x[yz] + B1x[xyz], cvzp);
     jy += ddiv(w2 - w1, dhz);
 
-    w1 = ddiv(B1y[c] + B1y[y_] + B1y[z_] + B1y[yz], cvx);
-    w2 = ddiv(B1y[x_] + B1y[xy] + B1y[xz] + B1y[xyz], cvxp);
-    double jz = ddiv(w2 - w1, dx);
+    w1 = ddiv((B1y[c] + B1y[y_] + B1y[z_] + B1y[yz]) * h2, cvx);
+    w2 = ddiv((B1y[x_] + B1y[xy] + B1y[xz] + B1y[xyz]) * h2x, cvxp);
+    double jz = ddiv(ddiv(w2 - w1, dx), h2c);
     w1 = ddiv(B1x[c] + B1x[x_] + B1x[z_] + B1x[xz], cvy);
     w2 = ddiv(B1x[y_] + B1x[xy] + B1x[yz] + B1x[xyz], cvyp);
     jz -= ddiv(w2 - w1, dhy);
@@ -555,6 +558,19 @@ KERNEL void k_forces(Dev D, R3 R, double mu0)
     fx += ddiv(jy * bzv - jz * byv, mu0);
     fy += ddiv(jz * bxv - jx * bzv, mu0);
     fz += ddiv(jx * byv - jy * bxv, mu0);
+    // geometry corrections (lagran.cpp:711-721); cot(yb) and xb from the host
+    if (geometry == 2) {
+        const double cotantheta = A.cotv[iy + 2], r = A.xbv[ix + 2], rv = A.rho_v[c];
+        const double ux = VX[c], uy = VY[c], uz = VZ[c];
+        fx += ddiv(rv * (uy * uy + uz * uz), r);
+        fy -= ddiv(rv * (uy * ux - cotantheta * uz * uz), r);
+        fz -= ddiv(rv * (uz * ux + cotantheta * uz * uy), r);
+    } else if (geometry == 1) {
+        const double r = A.xbv[ix + 2], rv = A.rho_v[c];
+        const double ux = VX[c], uy = VY[c];
+        fx += ddiv(rv * uy * uy, r);
+        fy -= ddiv(rv * uy * ux, r);
+    }
     A.fx[c] = fx;
     A.fy[c] = fy;
     A.fz[c] = fz;
@@ -593,8 +609,8 @@ KERNEL void k_shock_heating(Dev D, R3 R)
     double a11 = DOT(mmm, mmo);
     double a12 = DOT(omm, omo);
     double dx = DXB(ix);
-    double dy = DYB(iy);
-    double dz = DZB(iz);
+    double dy = DYB(iy) * HYC(ix);
+    double dz = DZB(iz) * HZ2(ix, iy);
     double vh =
         (-0.25 * dy * dz * A.alpha1[c] * a1 - 0.25 * dx * dz * A.alpha2[c] * a2 -
          0.25 * dy * dz * A.alpha1[c + sy] * a3 - 0.25 * dx * dz * A.alpha2[c - sx] * a4 -
@@ -694,7 +710,7 @@ KERNEL void k_remap_prep(Dev D, R3 R, int d, double xpass, double ypass, double 
         A.cv2[c] = vol * (1.0 + dv);
         dv = dv + dvydy * dt;
         A.cv1[c] = vol * (1.0 + dv);
-        A.db1[c] = DYB(iy) + (vyb - vybm) * dt;
+        A.db1[c] = HYC(ix) * DYB(iy) + (vyb - vybm) * dt;
     } else {
         double dvxdx = ddiv(xpass * (vxb * A.dxab[c] - vxbm * A.dxab[c - sx]), vol);
         double dvydy = ddiv(ypass * (vyb * A.dyab[c] - vybm * A.dyab[c - sy]), vol);
@@ -703,7 +719,7 @@ KERNEL void k_remap_prep(Dev D, R3 R, int d, double xpass, double ypass, double 
         A.cv2[c] = vol * (1.0 + dv);
         dv = dv + dvzdz * dt;
         A.cv1[c] = vol * (1.0 + dv);
-        A.db1[c] = DZB(iz) + (vzb - vzbm) * dt;
+        A.db1[c] = HZC(ix, iy) * DZB(iz) + (vzb - vzbm) * dt;
     }
     DM[c] = 0.0;
     A.rho1[c] = RHO[c];
@@ -1251,7 +1267,8 @@ void step_start_a(Grid &g)
     launch3(k_pvisc, 72, g, rng(0, nx + 1, 0, ny + 1, 0, nz + 1));
     launch3(k_fvisc, 80, g, rng(0, nx, 0, ny, 0, nz));
     // controlFunctions::calculateTimestep -> set_dt (lagran.cpp:156, 380-443)
-    launch3(k_set_dt, 104, g, rng(0, nx, 0, ny, 0, nz), p.gas_gamma, p.mu0);
+    // the x range starts at 1 outside Cartesian geometry (lagran.cpp:384)
+    launch3(k_set_dt, 104, g, rng(p.geometry == 0 ? 0 : 1, nx, 0, ny, 0, nz), p.gas_gamma, p.mu0);
     hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, g.stream, g.dev, p.dt_multiplier);
 }
 
@@ -1264,7 +1281,7 @@ void step_start_b(Grid &g, int substeps)
     if (p.resistive) resistive_substeps(g, substeps);
     launch3(k_bfield_cv1, 112, g, C);
     launch3(k_energy_pred, 88, g, rng(0, nx + 1, 0, ny + 1, 0, nz + 1), p.gas_gamma);
-    launch3(k_forces, 144, g, rng(0, nx, 0, ny, 0, nz), p.mu0);
+    launch3(k_forces, 144, g, rng(0, nx, 0, ny, 0, nz), p.mu0, p.geometry);
     remap_v_bcs(g);
 }
 
@@ -1334,21 +1351,24 @@ void box_copy(hipStream_t st, void *box, bool f32, double *lin, const Geo &G, co
 // Metric of LARE3D::grid (setup.cpp:337-390), Cartesian, over the extents
 // the reference fills: dxab/dyab/dzab over their staggered extents, cv and
 // the vertex areas over the centred extent.
+// Areas and volumes of LARE3D::grid (setup.cpp:337-390) with the metric
+// factors, including its separate y = -2 and z = -2 layers (setup.cpp:356-373:
+// dzab's z = -2 layer takes hyc where the others take hy).
 KERNEL void k_metric(Dev D, R3 R)
 {
     KBEGIN
     const int c = IX(ix, iy, iz);
     const bool cx = ix >= -1, cy = iy >= -1, cz = iz >= -1;
-    if (cy && cz) A.dxab[c] = DYB(iy) * DZB(iz);
-    if (cx && cz) A.dyab[c] = DXB(ix) * DZB(iz);
-    if (cx && cy) A.dzab[c] = DXB(ix) * DYB(iy);
+    if (cy && cz) A.dxab[c] = DYB(iy) * DZB(iz) * HY(ix) * HZ1(ix, iy);
+    if (cx && cz) A.dyab[c] = DXB(ix) * DZB(iz) * HZ2(ix, iy);
+    if (cx && cy) A.dzab[c] = DXB(ix) * DYB(iy) * (iz == -2 ? HYC(ix) : HY(ix));
     if (cx && cy && cz) {
         double dydz = DYB(iy) * DZB(iz);
-        A.cv[c] = DXB(ix) * dydz;
+        A.cv[c] = DXB(ix) * dydz * HYC(ix) * HZC(ix, iy);
         double dz = DZC(iz);
-        A.dxac[c] = DYB(iy) * dz;
-        A.dyac[c] = DXB(ix) * dz;
-        A.dzac[c] = DXB(ix) * DYB(iy);
+        A.dxac[c] = DYB(iy) * dz * HYC(ix) * HZ2(ix, iy);
+        A.dyac[c] = DXB(ix) * dz * HZ1(ix, iy);
+        A.dzac[c] = DXB(ix) * DYB(iy) * HY(ix);
     }
 }
 

@@ -85,6 +85,12 @@ extern "C" int lare_problem_desc(const lare_problem *p, lare_desc *d)
     } else {
         return -1;
     }
+    d->x_min += p->x0;
+    d->x_max += p->x0;
+    d->y_min += p->y0;
+    d->y_max += p->y0;
+    if (p->geometry < LARE_CARTESIAN || p->geometry > LARE_SPHERICAL) return -1;
+    d->geometry = p->geometry;
     if (p->eta < 0.0 || p->eta0 < 0.0) return -1;
     if (p->eta > 0.0 || p->eta0 > 0.0) {
         d->resistive = 1;
@@ -149,7 +155,7 @@ extern "C" int lare_problem_fill(const lare_problem *p, int field, double *host)
                     }
                 } else {
                     double r, pr, b;
-                    if (xc[ix + 2] < 0.0) {
+                    if (xc[ix + 2] < 0.5 * (d.x_min + d.x_max)) {
                         r = 1.0;
                         b = 1.0;
                         pr = 1.0;

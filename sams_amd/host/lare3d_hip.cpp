@@ -240,8 +240,6 @@ namespace LARE
             deviceGrid &g = grids()[&d];
             if (g.ctx)
                 return g;
-            if (d.geometry != geometryType::Cartesian)
-                throw std::runtime_error("LARE3D (MI355X): only Cartesian geometry is implemented");
             lare_desc desc{};
             desc.nx = d.nx;
             desc.ny = d.ny;
@@ -263,6 +261,9 @@ namespace LARE
             desc.eta0 = d.eta0;
             desc.j_max = d.j_max;
             desc.rke = d.rke ? 1 : 0;
+            desc.geometry = d.geometry == geometryType::Spherical     ? LARE_SPHERICAL
+                            : d.geometry == geometryType::Cylindrical ? LARE_CYLINDRICAL
+                                                                      : LARE_CARTESIAN;
             // boundary conditions first (host only), so that a condition the
             // device does not implement is reported before any device work
             std::vector<edgeBC> bcs;

@@ -28,6 +28,7 @@ BC_NONE, BC_PERIODIC, BC_ZERO_GRADIENT, BC_MIRROR, BC_CLAMP, BC_MAP = 0, 1, 2, 3
 MAP_KEEP, MAP_VALUE = -1000, -1001  # map-edge layer entries besides a source layer
 FP64, FP32_STORAGE = 0, 1
 ORSZAG_TANG, BRIO_WU = 0, 1
+CARTESIAN, CYLINDRICAL, SPHERICAL = 0, 1, 2
 
 
 class LareDesc(ctypes.Structure):
@@ -40,7 +41,7 @@ class LareDesc(ctypes.Structure):
                 ("dt_multiplier", ctypes.c_double), ("resistive", ctypes.c_int32),
                 ("precision", ctypes.c_int32), ("eta_background", ctypes.c_double),
                 ("eta0", ctypes.c_double), ("j_max", ctypes.c_double),
-                ("rke", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("rke", ctypes.c_int32), ("geometry", ctypes.c_int32)]
 
 
 class LareTune(ctypes.Structure):
@@ -59,17 +60,25 @@ class LareProblem(ctypes.Structure):
                 ("seed", ctypes.c_uint64), ("amp", ctypes.c_double), ("beta", ctypes.c_double),
                 ("noise", ctypes.c_double), ("eta", ctypes.c_double),
                 ("precision", ctypes.c_int32), ("rke", ctypes.c_int32),
-                ("eta0", ctypes.c_double), ("j_max", ctypes.c_double)]
+                ("eta0", ctypes.c_double), ("j_max", ctypes.c_double),
+                ("geometry", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("x0", ctypes.c_double), ("y0", ctypes.c_double)]
+
+
+GEOMETRY = {"cartesian": CARTESIAN, "cylindrical": CYLINDRICAL, "spherical": SPHERICAL}
 
 
 def problem(kind="ot", n=16, ny=None, nz=None, seed=1, amp=0.0, beta=-1.0, noise=0.0, eta=0.0,
-            bcx=0, precision=FP64, rke=False, eta0=0.0, j_max=0.0):
+            bcx=0, precision=FP64, rke=False, eta0=0.0, j_max=0.0, geometry=CARTESIAN, x0=0.0, y0=0.0):
     """A LareProblem: kind 'ot' (Orszag-Tang) or 'bw' (Brio-Wu); rke turns the
     remap kinetic-energy correction on; eta (background) or eta0 (added where
-    |j| > j_max; j_max <= 0 means never) turn resistive MHD on."""
+    |j| > j_max; j_max <= 0 means never) turn resistive MHD on; geometry
+    (0/'cartesian', 1/'cylindrical', 2/'spherical') with the domain's x and y
+    extents shifted by x0, y0."""
     k = {"ot": ORSZAG_TANG, "bw": BRIO_WU}[kind] if isinstance(kind, str) else int(kind)
+    gm = GEOMETRY[geometry] if isinstance(geometry, str) else int(geometry)
     return LareProblem(k, bcx, n, ny if ny is not None else n, nz if nz is not None else n,
-                       seed, amp, beta, noise, eta, precision, int(bool(rke)), eta0, j_max)
+                       seed, amp, beta, noise, eta, precision, int(bool(rke)), eta0, j_max, gm, 0, x0, y0)
 
 
 _lib = None

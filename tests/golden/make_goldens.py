@@ -76,6 +76,21 @@ CASES = [
                                                              "lare_bc": True}),
     ("bw_res_mirror_2d_20x6x2", "bw", 20, 6, 2, 12, (0, 12), {"bcx": "mirror", "noise": 0.02, "seed": 6,
                                                               "eta": 0.01, "eta0": 0.02, "j_max": 3.0}),
+    # cylindrical (x, y, z = r, phi, z) and spherical (r, theta, phi) grids: the domain shifted to
+    # r > 0 (and theta inside (0, pi)); metric factors of setup.cpp:276-335, geometric forces of
+    # lagran.cpp:711-721
+    ("bw_cyl_16x6x5", "bw", 16, 6, 5, 10, (0, 1, 10), {"noise": 0.01, "seed": 3, "geometry": "cylindrical",
+                                                       "x0": 2.0}),
+    ("ot_cyl_12x10x8", "ot", 12, 10, 8, 10, (1, 10), {"amp": 0.05, "seed": 3, "geometry": "cylindrical",
+                                                      "x0": 1.0}),
+    ("bw_walls_cyl_16x5x4", "bw", 16, 5, 4, 8, (8,), {"bcx": "clamp", "noise": 0.02, "seed": 4,
+                                                      "geometry": "cylindrical", "x0": 1.5}),
+    ("bw_sph_16x6x5", "bw", 16, 6, 5, 10, (0, 1, 10), {"noise": 0.01, "seed": 3, "geometry": "spherical",
+                                                       "x0": 2.0, "y0": 0.7}),
+    ("ot_sph_12x10x8", "ot", 12, 10, 8, 8, (1, 8), {"amp": 0.05, "seed": 3, "geometry": "spherical",
+                                                    "x0": 1.0, "y0": 0.3}),
+    ("bw_res_sph_16x6x5", "bw", 16, 6, 5, 8, (8,), {"noise": 0.01, "seed": 5, "eta": 0.01,
+                                                    "geometry": "spherical", "x0": 2.0, "y0": 0.7}),
 ]
 
 # The benchmark workload at its full size and step count: resistive Brio-Wu
@@ -124,7 +139,8 @@ def run_case(name, prob, nx, ny, nz, steps, dumps, extra, threads=8):
                                noise=extra.get("noise", 0.0), bcx=extra.get("bcx"),
                                threads=1 if extra.get("rke") else threads, rke=extra.get("rke", False),
                                lare_bc=extra.get("lare_bc", False), eta=extra.get("eta", 0.0),
-                               eta0=extra.get("eta0", 0.0), j_max=extra.get("j_max"))
+                               eta0=extra.get("eta0", 0.0), j_max=extra.get("j_max"),
+                               geometry=extra.get("geometry"), x0=extra.get("x0", 0.0), y0=extra.get("y0", 0.0))
         snaps = {}
         for s in dumps:
             snaps[str(s)] = {f: digest(os.path.join(td, f"s{s:05d}_{f}.f64")) for f in FIELDS}
@@ -135,6 +151,7 @@ def run_case(name, prob, nx, ny, nz, steps, dumps, extra, threads=8):
             "rke": bool(extra.get("rke", False)),
             "lare_bc": bool(extra.get("lare_bc", False)),
             "eta": extra.get("eta", 0.0), "eta0": extra.get("eta0", 0.0), "j_max": extra.get("j_max", 0.0),
+            "geometry": extra.get("geometry", "cartesian"), "x0": extra.get("x0", 0.0), "y0": extra.get("y0", 0.0),
             "dt": meta["dt"], "snapshots": snaps}
 
 

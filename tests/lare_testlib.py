@@ -35,7 +35,8 @@ def case_problem(c):
     """LareProblem for a golden case."""
     return L.problem(c["problem"], n=c["nx"], ny=c["ny"], nz=c["nz"], seed=c["seed"],
                      amp=c["amp"], noise=c["noise"], bcx=BCX[c["bcx"]], rke=c.get("rke", False),
-                     eta=c.get("eta", 0.0), eta0=c.get("eta0", 0.0), j_max=c.get("j_max") or 0.0)
+                     eta=c.get("eta", 0.0), eta0=c.get("eta0", 0.0), j_max=c.get("j_max") or 0.0,
+                     geometry=c.get("geometry", "cartesian"), x0=c.get("x0", 0.0), y0=c.get("y0", 0.0))
 
 
 def case_bcs(c, grid):

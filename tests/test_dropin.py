@@ -38,7 +38,8 @@ def test_dropin_reproduces_reference(name):
                                steps=c["steps"], dumps=dumps, seed=c["seed"], amp=c["amp"],
                                noise=c["noise"], bcx=c["bcx"] or None, threads=2,
                                exe=O.DROPIN_DRIVER, rke=c.get("rke", False), lare_bc=c.get("lare_bc", False),
-                               eta=c.get("eta", 0.0), eta0=c.get("eta0", 0.0), j_max=c.get("j_max") or None)
+                               eta=c.get("eta", 0.0), eta0=c.get("eta0", 0.0), j_max=c.get("j_max") or None,
+                               geometry=c.get("geometry"), x0=c.get("x0", 0.0), y0=c.get("y0", 0.0))
         assert meta["dt"] == c["dt"], "dt sequence"
         for s in dumps:
             for f in FIELDS:

@@ -29,6 +29,8 @@ def _state_diffs(dev, orc, fields=L.STATE_FIELDS):
 def _skip_set0_rke(c, kernel_set):
     if kernel_set == 0 and (c.get("rke") or c.get("lare_bc")):
         pytest.skip("the kinetic-energy correction and map conditions run on kernel sets 1-3 only")
+    if kernel_set != 0 and c.get("geometry", "cartesian") != "cartesian":
+        pytest.skip("cylindrical and spherical grids run kernel set 0 only")
 
 
 @pytest.mark.parametrize("kernel_set", [0, 1, 2])
@@ -307,3 +309,21 @@ def test_edges_of_every_kind_on_all_dims_match_oracle(kernel_set, kinds):
     for s in range(2):
         assert dev.step().hex() == orc.step().hex(), f"dt of step {s + 1}"
         assert not _state_diffs(dev, orc), f"step {s + 1}"
+
+
+def test_curvilinear_grids_run_kernel_set_0_only():
+    """Cylindrical and spherical grids run the one-kernel-per-reference-kernel
+    set (its stored metric carries the metric factors); the fused sets,
+    batches and the Cartesian energy diagnostic refuse them with an error."""
+    p = L.problem("bw", n=16, ny=6, nz=5, seed=3, noise=0.01, geometry="spherical", x0=2.0, y0=0.7)
+    g = L.Lare.from_problem(p)
+    with pytest.raises(RuntimeError, match="kernel set 0 only"):
+        g.set_kernel_set(1)
+    with pytest.raises(RuntimeError, match="Cartesian"):
+        L.Batch([g])
+    with pytest.raises(RuntimeError, match="Cartesian"):
+        g.energy()
+    orc = make_oracle(p)
+    for s in range(3):
+        assert g.step().hex() == orc.step().hex(), f"dt of step {s + 1}"
+    assert not _state_diffs(g, orc)

@@ -38,3 +38,20 @@ def test_oracle_config0_64cube_100_steps():
         if str(s) in c["snapshots"]:
             for f in O.STATE_FIELDS:
                 assert sha(o.download(f)) == c["snapshots"][str(s)][f]["sha256"], f"{f} after step {s}"
+
+
+@pytest.mark.parametrize("name", [n for n in sorted(CASES) if CASES[n].get("geometry", "cartesian") != "cartesian"])
+def test_geometry_goldens_depend_on_the_metric_factors(name):
+    """The curvilinear goldens pin the metric factors: the same problem on a
+    Cartesian grid (every factor 1) misses the reference's first dt or its
+    first recorded state."""
+    import copy
+    c = copy.deepcopy(CASES[name])
+    c["geometry"] = "cartesian"
+    o = case_bcs(c, make_oracle(case_problem(c)))
+    first = min(int(s) for s in c["snapshots"] if int(s) > 0)
+    same = True
+    for s in range(1, first + 1):
+        same = same and o.step().hex() == float.fromhex(c["dt"][s - 1]).hex()
+    same = same and all(sha(o.download(f)) == c["snapshots"][str(first)][f]["sha256"] for f in O.STATE_FIELDS)
+    assert not same
