@@ -21,6 +21,9 @@
 //              [--noise F] [--eta E] [--eta0 E0] [--j-max J]
 //              [--bcx periodic|zerograd|mirror|clamp] [--time] [--rke] [--lare-bc]
 //              [--geometry cartesian|cylindrical|spherical] [--x0 X] [--y0 Y]
+//              [--face-varying] [--then-n N]
+// --then-n runs a second simulation (the same problem on an N^3 grid) in the
+// same process after the first, into OUT/second.
 // --eta / --eta0 (either > 0) turn resistive MHD on (shared_data.h:113-117):
 // eta_background = E, and eta0 is added where the current exceeds j_max
 // (lagran.cpp:473).  The reference's resistive path runs only in the
@@ -35,6 +38,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <chrono>
+#include <filesystem>
 #include <fstream>
 #include <set>
 #include <sstream>
@@ -83,6 +87,8 @@ namespace refdrv
         bool phases = false;
         bool rke = false;   // remap kinetic-energy correction
         bool lareBC = false; // LARE3DInitialConditions' BC_OTHER conditions on every edge
+        bool faceVarying = false; // rho's x edges take halfFaceBC
+        long thenN = 0;           // > 0: a second simulation of thenN^3 cells in the same process
         LARE::geometryType geometry = LARE::geometryType::Cartesian;
         double x0 = 0.0, y0 = 0.0; // shifts of the domain's x and y extents
         std::vector<double> dts;
@@ -169,6 +175,28 @@ namespace refdrv
         dumpArray(p + "by.f64", d.by);
         dumpArray(p + "bz.f64", d.bz);
     }
+
+    // A condition that acts differently on different lines of a face (for
+    // the drop-in's check that rejects such conditions): ghost cells of the
+    // lower half of the next dimension set to 1, the rest to 2.
+    class halfFaceBC : public SAMS::singleVariableBC<SAMS::T_dataType, 3>
+    {
+    public:
+        halfFaceBC(const SAMS::variableDef &vd) : SAMS::singleVariableBC<SAMS::T_dataType, 3>(vd) {}
+        void apply(int dimension, SAMS::domain::edges edge) override
+        {
+            const int e = edge == SAMS::domain::edges::lower ? 0 : 1;
+            auto r = this->boundaryRanges[dimension * 2 + e];
+            auto r0 = std::get<0>(r), r1 = std::get<1>(r), r2 = std::get<2>(r);
+            const int other = dimension == 1 ? 0 : 1;
+            auto ro = other == 0 ? r0 : r1;
+            const long mid = (ro.lower_bound + ro.upper_bound) / 2;
+            for (long i = r0.lower_bound; i <= r0.upper_bound; ++i)
+                for (long j = r1.lower_bound; j <= r1.upper_bound; ++j)
+                    for (long k = r2.lower_bound; k <= r2.upper_bound; ++k)
+                        this->variable(i, j, k) = ((other == 0 ? i : j) <= mid) ? 1.0 : 2.0;
+        }
+    };
 
     class RefProblem
     {
@@ -285,7 +313,13 @@ namespace refdrv
                 bcKind kx = cfg().bcx;
                 if (kx == bcKind::clamp && v[0] != 'v' && std::strncmp(v, "LARE/v", 6) != 0)
                     kx = bcKind::zerograd;
-                attach(v, harness, 0, kx);
+                if (cfg().faceVarying && std::strcmp(v, "rho") == 0)
+                {
+                    SAMS::variableDef &vd = harness.variableRegistry.getVariable(v);
+                    vd.addBoundaryCondition(0, std::make_shared<halfFaceBC>(vd));
+                }
+                else
+                    attach(v, harness, 0, kx);
                 attach(v, harness, 1, bcKind::periodic);
                 attach(v, harness, 2, bcKind::periodic);
             }
@@ -485,6 +519,10 @@ int main(int argc, char *argv[])
             c.rke = true;
         else if (a == "--lare-bc")
             c.lareBC = true;
+        else if (a == "--face-varying")
+            c.faceVarying = true;
+        else if (a == "--then-n")
+            c.thenN = std::stol(next());
         else if (a == "--phases")
         {
 #if defined(LARE_DROPIN)
@@ -537,30 +575,42 @@ int main(int argc, char *argv[])
     char **rargv = rargvBuf;
     SAMS::MPI::initialize(rargc, rargv);
     portableWrapper::initialize(rargc, rargv);
-    SAMS::runner<LARE::LARE3D, refdrv::RefProblem> runner;
-    runner.initialize(rargc, rargv);
-    runner.activatePackage("LARE3D");
-    runner.activatePackage("RefProblem");
-    runner.initializePackages();
-    // Wall time of the step loop only: from entry into runPackages()
-    // (runner.h:564-586) to the end of the last endOfTimestep, so that
-    // initialisation and the final writeOutput are excluded.
-    c.t0 = std::chrono::steady_clock::now();
-    runner.runPackages();
-    c.elapsed = std::chrono::duration<double>(c.tLast - c.t0).count();
-    runner.finalizePackages();
-    runner.finalize();
-    portableWrapper::finalize();
+    auto runOnce = [&]() {
+        SAMS::runner<LARE::LARE3D, refdrv::RefProblem> runner;
+        runner.initialize(rargc, rargv);
+        runner.activatePackage("LARE3D");
+        runner.activatePackage("RefProblem");
+        runner.initializePackages();
+        // Wall time of the step loop only: from entry into runPackages()
+        // (runner.h:564-586) to the end of the last endOfTimestep, so that
+        // initialisation and the final writeOutput are excluded.
+        c.t0 = std::chrono::steady_clock::now();
+        runner.runPackages();
+        c.elapsed = std::chrono::duration<double>(c.tLast - c.t0).count();
+        runner.finalizePackages();
+        runner.finalize();
 
-    std::ofstream m(c.out + "/meta.json");
-    m << "{\"problem\": \"" << c.problem << "\", \"nx\": " << c.nx << ", \"ny\": " << c.ny << ", \"nz\": " << c.nz
-      << ", \"steps\": " << c.steps << ", \"elapsed_s\": " << c.elapsed << ", \"dt\": [";
-    for (size_t i = 0; i < c.dts.size(); ++i)
+        std::ofstream m(c.out + "/meta.json");
+        m << "{\"problem\": \"" << c.problem << "\", \"nx\": " << c.nx << ", \"ny\": " << c.ny << ", \"nz\": " << c.nz
+          << ", \"steps\": " << c.steps << ", \"elapsed_s\": " << c.elapsed << ", \"dt\": [";
+        for (size_t i = 0; i < c.dts.size(); ++i)
+        {
+            char buf[64];
+            std::snprintf(buf, sizeof(buf), "\"%a\"", c.dts[i]);
+            m << (i ? ", " : "") << buf;
+        }
+        m << "]}\n";
+    };
+    runOnce();
+    if (c.thenN > 0)
     {
-        char buf[64];
-        std::snprintf(buf, sizeof(buf), "\"%a\"", c.dts[i]);
-        m << (i ? ", " : "") << buf;
+        // a second simulation in the same process, n^3, into OUT/second
+        c.nx = c.ny = c.nz = c.thenN;
+        c.out += "/second";
+        std::filesystem::create_directories(c.out);
+        c.dts.clear();
+        runOnce();
     }
-    m << "]}\n";
+    portableWrapper::finalize();
     return 0;
 }

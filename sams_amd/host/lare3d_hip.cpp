@@ -19,6 +19,26 @@
 //       a no-op); eta_calc is recomputed on the device inside the resistive
 //       substeps
 //
+// The device context of a simulationData is rebuilt when the simulationData
+// at that address has other dimensions or other harness arrays than the
+// context was built from (a new simulation reusing the address), and every
+// context is released when the process exits (or by lare3d_hip_release).
+// Once the device holds the state, the package-private host arrays of
+// setup.cpp that only the step reads (metric, Lagrangian and remap work
+// arrays) are released (LARE3D_HIP_KEEP_HOST_ARRAYS=1 keeps them); the
+// registered variables and the axes stay.  LARE3D_HIP_PRECISION=fp32 selects
+// FP32 storage (Cartesian grids).
+//
+// Host writes between events: after the first upload the harness arrays of
+// the 13 registered variables are mapped read-only (every whole page inside
+// each array).  A write from another package faults once, is recorded and
+// the array is made writable again, so the write completes.  At the next
+// event the written arrays are uploaded when the harness held the current
+// state at the time (after an output or a download), and otherwise the step
+// stops with an error naming the array: the device then held a newer state
+// than the one the package modified (LARE3D_HIP_HOST_SYNC=1 keeps the
+// harness current after every event for such packages).
+//
 // Boundary conditions: the built-in BC objects attached to each variable are
 // private to variableDef (variableDef.h:56).  They are identified once, at
 // the first step, by applying each variable's conditions along one dimension
@@ -28,7 +48,9 @@
 // (simplePeriodicBC, simpleZeroGradientBC, simpleMirrorBC, simpleClamp) are
 // identified layer by layer as copies of other layers of the line, constants
 // or untouched layers (the LARE-style conditions of the LARE initial-condition
-// package); any other effect is reported as an error.
+// package); any other effect, and a condition whose effect differs from one
+// line of the face to another, is reported as an error.
+#include <csignal>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -38,6 +60,9 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+
+#include <sys/mman.h>
+#include <unistd.h>
 
 #include "shared_data.h"
 #include "lare_hip.h"
@@ -53,10 +78,62 @@ namespace LARE
 
     namespace
     {
+        // ---- host-write guard (see the file comment)
+        struct guardRegion
+        {
+            char *lo, *hi;              // whole pages inside one harness array
+            volatile sig_atomic_t hit;  // written since armed
+            volatile sig_atomic_t used;
+        };
+        constexpr int kMaxRegions = 1024;
+        guardRegion g_regions[kMaxRegions];
+        struct sigaction g_prevSegv;
+        bool g_handlerInstalled = false;
+
+        void segvHandler(int sig, siginfo_t *si, void *uc)
+        {
+            char *a = static_cast<char *>(si->si_addr);
+            for (int i = 0; i < kMaxRegions; ++i)
+            {
+                guardRegion &r = g_regions[i];
+                if (r.used && a >= r.lo && a < r.hi)
+                {
+                    r.hit = 1;
+                    mprotect(r.lo, static_cast<size_t>(r.hi - r.lo), PROT_READ | PROT_WRITE);
+                    return;
+                }
+            }
+            // not a guarded array: restore the previous disposition and let
+            // the faulting instruction run again under it
+            sigaction(SIGSEGV, &g_prevSegv, nullptr);
+            (void)sig;
+            (void)uc;
+        }
+
+        void installGuardHandler()
+        {
+            if (g_handlerInstalled)
+                return;
+            struct sigaction sa;
+            std::memset(&sa, 0, sizeof(sa));
+            sa.sa_sigaction = segvHandler;
+            sa.sa_flags = SA_SIGINFO;
+            sigemptyset(&sa.sa_mask);
+            if (sigaction(SIGSEGV, &sa, &g_prevSegv) != 0)
+                throw std::runtime_error("LARE3D (MI355X): cannot install the host-write guard");
+            g_handlerInstalled = true;
+        }
+
         // Device grid bound to one simulationData.
         struct deviceGrid
         {
             lare_ctx *ctx = nullptr;
+            // what the context was built from: a simulationData at the same
+            // address with other sizes or other harness arrays is a new one
+            long nx = 0, ny = 0, nz = 0;
+            const void *arrays[LARE_NFIELDS] = {};
+            int region[LARE_NFIELDS] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};  // guard slot per field
+            bool guarded = false;
             bool hostCurrent = true; // harness arrays hold the latest state
             double dtNew = 0.0;      // dt of the last lare_viscosity_dt
             double dtrNew = 0.0;
@@ -72,6 +149,62 @@ namespace LARE
         {
             static std::map<const simulationData *, deviceGrid> g;
             return g;
+        }
+
+        void unguard(deviceGrid &g)
+        {
+            if (!g.guarded)
+                return;
+            for (int f = 0; f < LARE_NFIELDS; ++f)
+            {
+                const int i = g.region[f];
+                if (i < 0)
+                    continue;
+                mprotect(g_regions[i].lo, static_cast<size_t>(g_regions[i].hi - g_regions[i].lo),
+                         PROT_READ | PROT_WRITE);
+            }
+        }
+        void reguard(deviceGrid &g)
+        {
+            if (!g.guarded)
+                return;
+            for (int f = 0; f < LARE_NFIELDS; ++f)
+            {
+                const int i = g.region[f];
+                if (i < 0)
+                    continue;
+                g_regions[i].hit = 0;
+                if (mprotect(g_regions[i].lo, static_cast<size_t>(g_regions[i].hi - g_regions[i].lo), PROT_READ) != 0)
+                    throw std::runtime_error("LARE3D (MI355X): mprotect of a harness array failed");
+            }
+        }
+        void releaseGuard(deviceGrid &g)
+        {
+            unguard(g);
+            for (int f = 0; f < LARE_NFIELDS; ++f)
+            {
+                if (g.region[f] >= 0)
+                    g_regions[g.region[f]].used = 0;
+                g.region[f] = -1;
+            }
+            g.guarded = false;
+        }
+
+        void releaseGrid(deviceGrid &g)
+        {
+            releaseGuard(g);
+            if (g.ctx)
+                lare_ctx_destroy(g.ctx);
+            g.ctx = nullptr;
+        }
+
+        // every context, at process exit (registered at the first bind, after
+        // the HIP runtime is initialised, so this runs before its teardown)
+        void releaseAll()
+        {
+            for (auto &kv : grids())
+                releaseGrid(kv.second);
+            grids().clear();
         }
 
         void check(int rc, const char *what)
@@ -185,6 +318,30 @@ namespace LARE
                         return true;
                 return false;
             };
+            // the condition must act alike on every line of the face: the
+            // marks do not vary across it, so every line's ghost layers must
+            // equal the middle line's
+            {
+                const int o1 = dim == 0 ? 1 : 0, o2 = dim == 2 ? 1 : 2;
+                const auto ra = a.getRange(o1), rb = a.getRange(o2);
+                for (long p = ra.lower_bound; p <= ra.upper_bound; ++p)
+                    for (long q = rb.lower_bound; q <= rb.upper_bound; ++q)
+                        for (long g = g0; g <= g1; ++g)
+                        {
+                            long idx[3];
+                            idx[dim] = g;
+                            idx[o1] = p;
+                            idx[o2] = q;
+                            const double v = a(idx[0], idx[1], idx[2]), w = at(g);
+                            if (!(v == w || (v != v && w != w)))
+                                throw std::runtime_error(
+                                    std::string("LARE3D (MI355X): the boundary condition of ") + kVarNames[f] +
+                                    " on the " + (edge ? "upper" : "lower") + " edge of dimension " +
+                                    std::to_string(dim) +
+                                    " varies across the face; conditions that act alike on every line of a face "
+                                    "are implemented");
+                        }
+            }
             int kind;
             double value = 0.0;
             if (all([&](long g) { return mark(g); }))
@@ -235,11 +392,105 @@ namespace LARE
             return edgeBC{f, dim, edge, kind, value, {0, 0, 0}};
         }
 
-        deviceGrid &bind(SAMS::harness &h, simulationData &d)
+        bool sameSimulation(const deviceGrid &g, simulationData &d)
+        {
+            if (g.nx != d.nx || g.ny != d.ny || g.nz != d.nz)
+                return false;
+            for (int f = 0; f < LARE_NFIELDS; ++f)
+                if (g.arrays[f] != static_cast<const void *>(stateArray(d, f)->data()))
+                    return false;
+            return true;
+        }
+
+        // The package-private host arrays of setup.cpp:170-252 that only the
+        // step reads: the device computes its own, so their host copies go.
+        void releaseHostWorkArrays(LARE3D &pkg, simulationData &d)
+        {
+            auto &m = pkg.manager;
+            m.deallocate(d.p_visc);
+            m.deallocate(d.eta);
+            m.deallocate(d.dxab);
+            m.deallocate(d.dyab);
+            m.deallocate(d.dzab);
+            m.deallocate(d.dxac);
+            m.deallocate(d.dyac);
+            m.deallocate(d.dzac);
+            m.deallocate(d.cv);
+            m.deallocate(d.cv1);
+            m.deallocate(d.cvc);
+            m.deallocate(d.bx1);
+            m.deallocate(d.by1);
+            m.deallocate(d.bz1);
+            m.deallocate(d.alpha1);
+            m.deallocate(d.alpha2);
+            m.deallocate(d.alpha3);
+            m.deallocate(d.visc_heat);
+            m.deallocate(d.pressure);
+            m.deallocate(d.p_e);
+            m.deallocate(d.p_i);
+            m.deallocate(d.rho_v);
+            m.deallocate(d.cv_v);
+            m.deallocate(d.fx);
+            m.deallocate(d.fy);
+            m.deallocate(d.fz);
+            m.deallocate(d.fx_visc);
+            m.deallocate(d.fy_visc);
+            m.deallocate(d.fz_visc);
+            m.deallocate(d.flux_x);
+            m.deallocate(d.flux_y);
+            m.deallocate(d.flux_z);
+            m.deallocate(d.curlb);
+            m.deallocate(d.x);
+            m.deallocate(d.y);
+            m.deallocate(d.z);
+            m.deallocate(d.xp);
+            m.deallocate(d.yp);
+            m.deallocate(d.zp);
+            if (d.rke)
+                m.deallocate(d.delta_ke);
+        }
+
+        // Arm the host-write guard over the registered variables' arrays.
+        void armGuard(deviceGrid &g, simulationData &d)
+        {
+            installGuardHandler();
+            const uintptr_t page = static_cast<uintptr_t>(sysconf(_SC_PAGESIZE));
+            for (int f = 0; f < LARE_NFIELDS; ++f)
+            {
+                g.region[f] = -1;
+                volumeArray &a = *stateArray(d, f);
+                const uintptr_t b = reinterpret_cast<uintptr_t>(a.data());
+                const uintptr_t e = b + a.getElements() * sizeof(T_dataType);
+                const uintptr_t lo = (b + page - 1) / page * page, hi = e / page * page;
+                if (hi <= lo)
+                    continue;  // an array within one page is not guarded
+                for (int i = 0; i < kMaxRegions; ++i)
+                    if (!g_regions[i].used)
+                    {
+                        g_regions[i].lo = reinterpret_cast<char *>(lo);
+                        g_regions[i].hi = reinterpret_cast<char *>(hi);
+                        g_regions[i].hit = 0;
+                        g_regions[i].used = 1;
+                        g.region[f] = i;
+                        break;
+                    }
+            }
+            g.guarded = true;
+            reguard(g);
+        }
+
+        deviceGrid &bind(LARE3D &pkg, SAMS::harness &h, simulationData &d)
         {
             deviceGrid &g = grids()[&d];
-            if (g.ctx)
+            if (g.ctx && sameSimulation(g, d))
                 return g;
+            releaseGrid(g);  // a new simulation at this address
+            static bool atexitRegistered = false;
+            if (!atexitRegistered)
+            {
+                std::atexit(releaseAll);
+                atexitRegistered = true;
+            }
             lare_desc desc{};
             desc.nx = d.nx;
             desc.ny = d.ny;
@@ -257,6 +508,13 @@ namespace LARE
             desc.dt_multiplier = d.dt_multiplier;
             desc.resistive = d.resistiveMHD ? 1 : 0;
             desc.precision = LARE_FP64;
+            if (const char *pr = std::getenv("LARE3D_HIP_PRECISION"))
+            {
+                if (std::strcmp(pr, "fp32") == 0)
+                    desc.precision = LARE_FP32_STORAGE;
+                else if (std::strcmp(pr, "fp64") != 0)
+                    throw std::runtime_error("LARE3D (MI355X): LARE3D_HIP_PRECISION must be fp64 or fp32");
+            }
             desc.eta_background = d.eta_background;
             desc.eta0 = d.eta0;
             desc.j_max = d.j_max;
@@ -297,8 +555,19 @@ namespace LARE
             check(lare_set_dt(g.ctx, d.dt), "lare_set_dt");
             g.devDt = d.dt;
             g.hostCurrent = true;
+            g.nx = d.nx;
+            g.ny = d.ny;
+            g.nz = d.nz;
+            for (int f = 0; f < LARE_NFIELDS; ++f)
+                g.arrays[f] = stateArray(d, f)->data();
             const char *sync = std::getenv("LARE3D_HIP_HOST_SYNC");
             g.hostSync = sync && std::strcmp(sync, "0") != 0;
+            const char *keep = std::getenv("LARE3D_HIP_KEEP_HOST_ARRAYS");
+            if (!(keep && std::strcmp(keep, "0") != 0))
+                releaseHostWorkArrays(pkg, d);
+            const char *guard = std::getenv("LARE3D_HIP_WRITE_GUARD");
+            if (!g.hostSync && !(guard && std::strcmp(guard, "0") == 0))
+                armGuard(g, d);
             return g;
         }
 
@@ -309,6 +578,7 @@ namespace LARE
             if (it == grids().end() || it->second.hostCurrent)
                 return;
             std::vector<double> buf;
+            unguard(it->second);
             for (int f = 0; f < LARE_NFIELDS; ++f)
             {
                 volumeArray &a = *stateArray(d, f);
@@ -316,6 +586,7 @@ namespace LARE
                 check(lare_download_field(it->second.ctx, f, buf.data()), "lare_download_field");
                 fromBuffer(a, buf);
             }
+            reguard(it->second);
             it->second.hostCurrent = true;
         }
 
@@ -324,9 +595,28 @@ namespace LARE
         // device state; after it, the device state replaces them.
         void beginEvent(deviceGrid &g, simulationData &d)
         {
+            std::vector<double> buf;
+            if (g.guarded)
+            {
+                // arrays written on the host since the last event
+                for (int f = 0; f < LARE_NFIELDS; ++f)
+                {
+                    const int i = g.region[f];
+                    if (i < 0 || !g_regions[i].hit)
+                        continue;
+                    if (!g.hostCurrent)
+                        throw std::runtime_error(
+                            std::string("LARE3D (MI355X): ") + kVarNames[f] +
+                            " was written on the host between LARE3D events while the device held a newer "
+                            "state; run with LARE3D_HIP_HOST_SYNC=1 for packages that write the state between "
+                            "events");
+                    toBuffer(*stateArray(d, f), buf);
+                    check(lare_upload_field(g.ctx, f, buf.data()), "lare_upload_field");
+                }
+                reguard(g);
+            }
             if (!g.hostSync || !g.hostCurrent)
                 return;
-            std::vector<double> buf;
             for (int f = 0; f < LARE_NFIELDS; ++f)
             {
                 toBuffer(*stateArray(d, f), buf);
@@ -343,7 +633,7 @@ namespace LARE
 
     void LARE3D::lagrangian_step(simulationData &data, SAMS::controlFunctions &controlFns)
     {
-        deviceGrid &g = bind(harness, data);
+        deviceGrid &g = bind(*this, harness, data);
         beginEvent(g, data);
         // the shock viscosity reads the previous dt (lagran.cpp:72): the
         // runner's, which another package may have lowered
@@ -372,7 +662,7 @@ namespace LARE
 
     void LARE3D::corrector_step(simulationData &data)
     {
-        deviceGrid &g = bind(harness, data);
+        deviceGrid &g = bind(*this, harness, data);
         beginEvent(g, data);
         check(lare_half_timestep(g.ctx), "lare_half_timestep");
         endEvent(g, data);
@@ -386,7 +676,7 @@ namespace LARE
 
     void LARE3D::eulerian_remap(simulationData &data, remapData &)
     {
-        deviceGrid &g = bind(harness, data);
+        deviceGrid &g = bind(*this, harness, data);
         beginEvent(g, data);
         check(lare_end_of_timestep(g.ctx), "lare_end_of_timestep");
         endEvent(g, data);
@@ -467,4 +757,16 @@ namespace LARE
     // Access for drivers that need the harness arrays current outside an
     // output event (e.g. to compare states).
     void lare3d_hip_download(simulationData &data) { download(data); }
+
+    // Release the device context of a simulationData (its state is lost;
+    // download first to keep it).  Contexts are also released at process
+    // exit and when a new simulation reuses the address.
+    void lare3d_hip_release(simulationData &data)
+    {
+        auto it = grids().find(&data);
+        if (it == grids().end())
+            return;
+        releaseGrid(it->second);
+        grids().erase(it);
+    }
 } // namespace LARE

@@ -22,6 +22,7 @@ element-wise copies at lagran.cpp:482-484 that let its resistive path run
 
     make -C oracle ref ic && python tests/golden/make_goldens.py
     python tests/golden/make_goldens.py --fullsize   (the 256^3 benchmark workload)
+    python tests/golden/make_goldens.py --ic-long    (the initial-condition packages over many steps)
 """
 import hashlib
 import json
@@ -114,6 +115,18 @@ IC_CASES = [
 ]
 IC_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ic_driver")
 
+# The same packages over many steps (or to the package's own end time, when
+# it comes first): package, steps, OpenMP threads (--ic-long)
+IC_LONG_CASES = [
+    ("SodShockTube", 500, 8),
+    ("BrioAndWu", 500, 8),
+    ("MHDRotor", 300, 8),
+    ("OrszagTang", 300, 8),
+    ("OrszagTang3D", 60, 8),
+    ("KarmanVortex", 200, 8),
+    ("EmeryWindTunnel", 120, 1),
+]
+
 
 def run_ic(exe, out_dir, ic, steps, threads):
     """Run an initial-condition driver build; return its meta dict."""
@@ -161,6 +174,22 @@ def main(argv):
     --fullsize: the full-size cases (long: the 256^3 case takes about half an
     hour on 8 cores), merged into the existing file."""
     path = os.path.join(ROOT, "tests", "golden", "goldens.json")
+    if argv and argv[0] == "--ic-long":
+        with open(path) as f:
+            out = json.load(f)
+        out["ic_long_cases"] = {}
+        for ic, steps, threads in IC_LONG_CASES:
+            with tempfile.TemporaryDirectory() as td:
+                meta = run_ic(IC_DRIVER, td, ic, steps, threads)
+                n = len(meta["dt"])
+                if n < steps:  # the package's end time came first
+                    meta = run_ic(IC_DRIVER, td, ic, n, threads)
+                snaps = {str(n): {f: digest(os.path.join(td, f"s{n:05d}_{f}.f64")) for f in FIELDS}}
+            out["ic_long_cases"][ic] = {"steps": n, "threads": threads, "dt": meta["dt"], "snapshots": snaps}
+            print(ic, n, "done", flush=True)
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1, sort_keys=True)
+        return
     if argv and argv[0] in ("--only", "--fullsize"):
         with open(path) as f:
             out = json.load(f)

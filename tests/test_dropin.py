@@ -206,45 +206,129 @@ def test_dropin_runs_the_reference_initial_conditions(ic):
             assert hashlib.sha256(a.tobytes()).hexdigest() == c["snapshots"][str(steps)][f]["sha256"], f
 
 
-IC_DRIVER = os.path.join(os.path.dirname(O.DROPIN_DRIVER), "ic_driver")
+def _ic_long_goldens():
+    import json
+    from lare_testlib import GOLDEN
+    with open(GOLDEN) as f:
+        return json.load(f).get("ic_long_cases", {})
+
+
+IC_LONG = _ic_long_goldens()
 
 
 @pytest.mark.gpu
-@pytest.mark.skipif(os.environ.get("LARE_LONG_RUNS") != "1", reason="long runs (minutes): set LARE_LONG_RUNS=1")
-@pytest.mark.skipif(not (os.path.exists(IC_DROPIN) and os.path.exists(IC_DRIVER)),
-                    reason="oracle/_ref/ic_driver and ic_dropin not built (needs the reference tree)")
-@pytest.mark.parametrize("ic,steps", [("SodShockTube", 500), ("BrioAndWu", 500), ("MHDRotor", 300),
-                                      ("OrszagTang", 300), ("OrszagTang3D", 60), ("KarmanVortex", 200),
-                                      ("EmeryWindTunnel", 120)])
-def test_dropin_long_runs_of_the_reference_initial_conditions(ic, steps):
-    """The reference built from its own sources and the drop-in, both under
-    the reference runner, over many steps (or to the package's end time):
-    identical dt sequences and identical state, ghosts included, after the
-    last step.  Emery runs the reference on one OpenMP thread (its rke
-    atomics make more threads nondeterministic)."""
+@pytest.mark.skipif(not os.path.exists(IC_DROPIN), reason="oracle/_ref/ic_dropin not built (needs the reference tree)")
+@pytest.mark.parametrize("ic", sorted(IC_LONG))
+def test_dropin_long_runs_of_the_reference_initial_conditions(ic):
+    """The reference's initial-condition packages over many steps (60 to 500,
+    or to the package's own end time): the drop-in under the reference runner
+    against the reference's own runs (tests/golden/goldens.json
+    "ic_long_cases", make_goldens.py --ic-long): the dt of every step and the
+    state, ghosts included, after the last step.  The packages that write the
+    state between events run host-synchronised."""
+    import hashlib
     import json
     import subprocess
     import numpy as np
+    c = IC_LONG[ic]
+    steps = c["steps"]
     sync = "1" if ic in ("KarmanVortex", "EmeryWindTunnel") else "0"
-    threads = "1" if ic == "EmeryWindTunnel" else "8"
-
-    def run(exe, out, n):
-        os.makedirs(out, exist_ok=True)
-        subprocess.run([exe, "--ic", ic, "--steps", str(n), "--dump", str(n), "--out", out], cwd=out, check=True,
-                       capture_output=True,
-                       env=dict(os.environ, OMP_NUM_THREADS=threads, LARE3D_HIP_HOST_SYNC=sync))
-        with open(os.path.join(out, "meta.json")) as f:
-            return json.load(f)["dt"]
-
     with tempfile.TemporaryDirectory() as td:
-        ref = os.path.join(td, "ref")
-        dts = run(IC_DRIVER, ref, steps)
-        if len(dts) < steps:  # the package's own end time came first
-            steps = len(dts)
-            dts = run(IC_DRIVER, ref, steps)
-        dev = os.path.join(td, "dev")
-        assert run(IC_DROPIN, dev, steps) == dts, "dt sequence"
+        subprocess.run([IC_DROPIN, "--ic", ic, "--steps", str(steps), "--dump", str(steps), "--out", td], cwd=td,
+                       check=True, capture_output=True,
+                       env=dict(os.environ, OMP_NUM_THREADS="2", LARE3D_HIP_HOST_SYNC=sync))
+        with open(os.path.join(td, "meta.json")) as f:
+            assert json.load(f)["dt"] == c["dt"], "dt sequence"
         for f in FIELDS:
-            a = np.fromfile(os.path.join(ref, f"s{steps:05d}_{f}.f64"), dtype="<f8")
-            b = np.fromfile(os.path.join(dev, f"s{steps:05d}_{f}.f64"), dtype="<f8")
-            assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), f
+            a = np.fromfile(os.path.join(td, f"s{steps:05d}_{f}.f64"), dtype="<f8")
+            assert hashlib.sha256(a.tobytes()).hexdigest() == c["snapshots"][str(steps)][f]["sha256"], f
+
+
+def test_dropin_rejects_a_condition_that_varies_across_the_face():
+    """The condition probe checks every line of each face: a condition that
+    sets the ghost cells of half the face to one value and of the other half
+    to another (ref_driver --face-varying, on rho's x edges) is rejected
+    before any device work, with the variable, edge and dimension named."""
+    import subprocess
+    if not os.path.exists(O.DROPIN_DRIVER):
+        pytest.skip("oracle/_ref/lare3d_dropin not built (needs the reference tree)")
+    with tempfile.TemporaryDirectory() as td:
+        r = subprocess.run([O.DROPIN_DRIVER, "--problem", "bw", "--nx", "12", "--ny", "4", "--nz", "4", "--steps", "1",
+                            "--out", td, "--face-varying"], cwd=td, capture_output=True, text=True)
+    assert r.returncode != 0
+    assert "boundary condition of rho on the lower edge of dimension 0 varies across the face" in r.stderr
+
+
+@pytest.mark.gpu
+@needs_dropin
+def test_dropin_second_simulation_in_one_process():
+    """Two simulations of different sizes in one process (ref_driver
+    --then-n): the second reuses the first's simulationData address with
+    other dimensions and arrays, so the drop-in rebuilds its device context;
+    the second run equals the reference's own run of that size."""
+    with tempfile.TemporaryDirectory() as td:
+        d1, d2 = os.path.join(td, "dropin"), os.path.join(td, "ref")
+        import subprocess
+        os.makedirs(d1)
+        os.makedirs(d2)
+        subprocess.run([O.DROPIN_DRIVER, "--problem", "ot", "--nx", "16", "--ny", "12", "--nz", "10", "--steps", "4",
+                        "--dump", "4", "--amp", "0.05", "--seed", "3", "--then-n", "10", "--out", d1], cwd=d1,
+                       check=True, capture_output=True)
+        meta = O.run_reference(d2, problem="ot", nx=10, ny=10, nz=10, steps=4, dumps=(4,), amp=0.05, seed=3)
+        import json
+        with open(os.path.join(d1, "second", "meta.json")) as f:
+            got = json.load(f)
+        assert got["dt"] == meta["dt"]
+        for f in FIELDS:
+            assert _sha_file(os.path.join(d1, "second", f"s00004_{f}.f64")) == \
+                _sha_file(os.path.join(d2, f"s00004_{f}.f64")), f
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(O.DROPIN_DRIVER), "ic_dropin")),
+                    reason="oracle/_ref/ic_dropin not built (needs the reference tree)")
+def test_dropin_detects_host_writes_between_events():
+    """KarmanVortex writes the obstacle's velocities on the host between
+    LARE3D's events.  Without LARE3D_HIP_HOST_SYNC the harness arrays are
+    stale then, and the drop-in's write guard stops the run with an error
+    naming the array instead of ignoring the write."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(O.DROPIN_DRIVER), "ic_dropin")
+    with tempfile.TemporaryDirectory() as td:
+        env = {k: v for k, v in os.environ.items() if k != "LARE3D_HIP_HOST_SYNC"}
+        env["OMP_NUM_THREADS"] = "2"
+        r = subprocess.run([exe, "--ic", "KarmanVortex", "--steps", "3", "--out", td], cwd=td, env=env,
+                           capture_output=True, text=True)
+    assert r.returncode != 0
+    assert "was written on the host between LARE3D events" in r.stderr, r.stderr[-2000:]
+
+
+@pytest.mark.gpu
+@needs_dropin
+def test_dropin_fp32_storage_against_the_reference():
+    """LARE3D_HIP_PRECISION=fp32 gives the drop-in FP32 storage (FP64
+    arithmetic): after 12 steps of the perturbed Orszag-Tang case each state
+    field stays within 1e-5 (relative L2) and 1e-4 (relative L-inf) of the
+    reference's FP64 run, normalised by the field's vector (velocity and B
+    components) as in the FP32-storage tests."""
+    import subprocess
+    import numpy as np
+    with tempfile.TemporaryDirectory() as td:
+        d1, d2 = os.path.join(td, "fp32"), os.path.join(td, "ref")
+        os.makedirs(d1)
+        os.makedirs(d2)
+        subprocess.run([O.DROPIN_DRIVER, "--problem", "ot", "--nx", "10", "--ny", "8", "--nz", "12", "--steps", "12",
+                        "--dump", "12", "--amp", "0.05", "--seed", "7", "--out", d1], cwd=d1, check=True,
+                       capture_output=True, env=dict(os.environ, LARE3D_HIP_PRECISION="fp32"))
+        O.run_reference(d2, problem="ot", nx=10, ny=8, nz=12, steps=12, dumps=(12,), amp=0.05, seed=7)
+        x = {f: np.fromfile(os.path.join(d2, f"s00012_{f}.f64"), dtype="<f8") for f in FIELDS}
+        y = {f: np.fromfile(os.path.join(d1, f"s00012_{f}.f64"), dtype="<f8") for f in FIELDS}
+    vec = {"vx": ("vx", "vy", "vz"), "vy": ("vx", "vy", "vz"), "vz": ("vx", "vy", "vz"),
+           "bx": ("bx", "by", "bz"), "by": ("bx", "by", "bz"), "bz": ("bx", "by", "bz")}
+    for f in FIELDS:
+        comps = vec.get(f, (f,))
+        n2 = np.sqrt(sum(np.sum(x[c] ** 2) for c in comps) / len(comps))
+        ni = max(np.max(np.abs(x[c])) for c in comps)
+        d = x[f] - y[f]
+        assert 0 < np.sqrt(np.sum(d * d)) / n2 < 1e-5, f
+        assert np.max(np.abs(d)) / ni < 1e-4, f
