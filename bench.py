@@ -10,7 +10,9 @@ Also measured, in the same JSON line:
   "batched": 256 Orszag-Tang grids, sizes log-uniform in [32^3, 128^3],
              plasma beta U[0.1, 10], perturbation amplitude U[0, 0.05], FP64,
              stepped as one batch (one launch per kernel, graph-captured
-             step) against the same grids solved one after another;
+             step) against the same grids solved one after another, each
+             with its own captured step graph and on the plain path (every
+             kernel launched from the host);
   "serving": seeded Poisson streams of solves (sizes log-uniform in
              [32^3, 192^3], 50-500 steps, Orszag-Tang or Brio-Wu, FP64 or
              FP32 storage) served with continuous batching under a 200 GB
@@ -108,16 +110,25 @@ def bench_batched(steps, seq_steps, ngrids=256, seed=2024):
     for g in grids:
         g.steps(seq_steps, record_dt=False)
     ts = time.perf_counter() - t0
+    # the plain path: each grid alone, every kernel launched from the host
+    for g in grids:
+        g.set_graph(False)
+    t0 = time.perf_counter()
+    for g in grids:
+        g.steps(seq_steps, record_dt=False)
+    te = time.perf_counter() - t0
     for g in grids:
         g.close()
-    rb, rs = cells * steps / tb, cells * seq_steps / ts
+    rb, rs, re_ = cells * steps / tb, cells * seq_steps / ts, cells * seq_steps / te
     log(f"bench: batched {ngrids} grids ({cells / 1e6:.1f} M cells): {rb / 1e9:.3f} G cell-updates/s batched, "
-        f"{rs / 1e9:.3f} sequential, x{rb / rs:.2f} (set-up {setup:.1f} s, autotune {tune_s:.1f} s: {tune})")
+        f"{rs / 1e9:.3f} sequential with a captured graph per grid (x{rb / rs:.2f}), {re_ / 1e9:.3f} sequential "
+        f"plain path (x{rb / re_:.2f}) (set-up {setup:.1f} s, autotune {tune_s:.1f} s: {tune})")
     return {"workload": f"{ngrids} Orszag-Tang grids, n log-uniform in [32, 128], beta U[0.1, 10], "
                         f"amp U[0, 0.05], FP64, seed {seed}",
             "cells": cells, "value": rb, "unit": "cell-updates/s", "steps": steps,
             "ms_per_step": 1e3 * tb / steps, "sequential_value": rs, "sequential_steps_per_grid": seq_steps,
-            "speedup_vs_sequential": rb / rs, "launches_per_step": launches,
+            "speedup_vs_sequential": rb / rs, "sequential_plain_value": re_,
+            "speedup_vs_sequential_plain": rb / re_, "launches_per_step": launches,
             "kernel_ms_per_step_eager": prof_ms, "tune": tune, "autotune_s": tune_s}
 
 
@@ -245,6 +256,9 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch configuration instead of tuning it before the timed steps")
+    ap.add_argument("--tune", default=None,
+                    help="a fixed launch configuration instead of tuning, as key=value pairs "
+                         "(walk=1,march_k=16,zx_k=16,zline=1,order=1,ct_fuse=1)")
     ap.add_argument("--eager", action="store_true",
                     help="launch every kernel from the host instead of replaying the captured step graph")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -271,7 +285,11 @@ def main():
     log(f"bench: Brio-Wu {n}^3 resistive FP64, warmup {a.warmup}, steps {a.steps}")
     dev = L.Lare.from_problem(p)
     tune = None
-    if not a.no_autotune:
+    if a.tune:
+        dev.set_tune(**{k: int(v) for k, v in (kv.split("=") for kv in a.tune.split(","))})
+        tune = dev.get_tune()
+        log(f"bench: launch configuration {tune}")
+    elif not a.no_autotune:
         tune = dev.autotune(steps=3)  # state saved and restored around the candidates
         log(f"bench: autotuned launch configuration {tune}")
     if a.eager:
