@@ -2277,8 +2277,11 @@ KERNEL OCC(4) void g_predict_zx(const Src<T> S)
     }
 }
 
-// L7 over (0..n)^3: forces and half-step velocities (lagran.cpp:619-731);
-// fx_visc + fx is stored back into fx_visc for the corrector.
+// L7 over (0..n)^3: forces and half-step velocities (lagran.cpp:619-731),
+// and the corrector's final velocities (lagran.cpp:754-758: v + dt (fx_visc
+// + fx) / rho_v, into the second state copy), which read nothing the
+// corrector changes before them: the same values, without storing fx_visc +
+// fx for the corrector to read back.  fx_visc..fz_visc are dead after this.
 template <class T>
 KERNEL void g_forces(const Src<T> S)
 {
@@ -2336,12 +2339,17 @@ KERNEL void g_forces(const Src<T> S)
     fz += ddiv(jx * byv - jy * bxv, mu0);
     double rv = A.rho_v[c];
     double sxv = A.fxv[c] + fx, syv = A.fyv[c] + fy, szv = A.fzv[c] + fz;
-    A.fxv[c] = sxv;
-    A.fyv[c] = syv;
-    A.fzv[c] = szv;
-    VX1[c] = VX[c] + ddiv(dt / 2.0 * sxv, rv);
-    VY1[c] = VY[c] + ddiv(dt / 2.0 * syv, rv);
-    VZ1[c] = VZ[c] + ddiv(dt / 2.0 * szv, rv);
+    const double vx = VX[c], vy = VY[c], vz = VZ[c];
+    VX1[c] = vx + ddiv(dt / 2.0 * sxv, rv);
+    VY1[c] = vy + ddiv(dt / 2.0 * syv, rv);
+    VZ1[c] = vz + ddiv(dt / 2.0 * szv, rv);
+    // the sums as the storage type holds them (the corrector's operands)
+    sxv = (double)(T)sxv;
+    syv = (double)(T)syv;
+    szv = (double)(T)szv;
+    A.vx2[c] = vx + ddiv(dt * sxv, rv);
+    A.vy2[c] = vy + ddiv(dt * syv, rv);
+    A.vz2[c] = vz + ddiv(dt * szv, rv);
     };
     ECELL(cell)
 }
@@ -2351,10 +2359,9 @@ KERNEL void g_forces(const Src<T> S)
           (VZ[p] - VZ[q]) * (VZ1[p] - VZ1[q])))
 
 // C over (0..n)^3: corrector_step (lagran.cpp:739-804) into the second state
-// copy: the velocity correction (754-758, with the sum fx_visc + fx stored by
-// g_forces) on vertices 0..n; on cells 1..n shock_heating (884-993, with the
-// dy*dy factors of a9 and a11 at 987) and the density and energy correction
-// (762-796).  The corrector's visc_heat is read only by the energy
+// copy, on cells 1..n: shock_heating (884-993, with the dy*dy factors of a9
+// and a11 at 987) and the density and energy correction (762-796); the
+// velocity correction (754-758) is g_forces'.  The corrector's visc_heat is read only by the energy
 // correction of the same cell, so it stays in a register; over the rest of
 // shock_heating's range (0..n+1) nothing reads it before the next step's
 // viscosity recomputes it.  The cv1 the reference also writes here is
@@ -2367,10 +2374,6 @@ KERNEL OCC(2) void g_correct(const Src<T> S)
     const int c = IX(ix, iy, iz);
     const int sx = G.sx, sy = G.sy;
     const double dt = lare::gbl(A.s)->dt;
-    double rv = A.rho_v[c];
-    A.vx2[c] = VX[c] + ddiv(dt * A.fxv[c], rv);
-    A.vy2[c] = VY[c] + ddiv(dt * A.fyv[c], rv);
-    A.vz2[c] = VZ[c] + ddiv(dt * A.fzv[c], rv);
     if (ix < 1 || iy < 1 || iz < 1) return;
     const int ooo = c, moo = c - sx, omo = c - sy, mmo = c - sx - sy;
     const int oop = c + 1, mop = c - sx + 1, omp = c - sy + 1, mmp = c - sx - sy + 1;
@@ -2458,10 +2461,6 @@ KERNEL OCC(4) void g_correct_zx(const Src<T> S)
         const double a5 = lane_next(a1), a6 = lane_next(a2), a7 = lane_next(a3), a8 = lane_next(a4);
         const double vxb = 0.25 * (O[3] + Om[3] + pO[3] + pOm[3]);
         if (Z.wz && ix > Z.ix0) {
-            const double rv = A.rho_v[c];
-            A.vx2[c] = O[0] + ddiv(dt * A.fxv[c], rv);
-            A.vy2[c] = O[1] + ddiv(dt * A.fyv[c], rv);
-            A.vz2[c] = O[2] + ddiv(dt * A.fzv[c], rv);
             if (ix >= 1 && cyz) {
                 const double dx = DXB(ix);
                 const double vh =
@@ -3629,13 +3628,12 @@ template <class T>
 void half_t(Group &gp)
 {
     const double w = (double)sizeof(T);
-    // reads v v1 alpha1..3 rho_v fx_visc..fz_visc rho e_e e_i p_e p_i;
-    // writes v rho e_e e_i
+    // reads v v1 alpha1..3 rho e_e e_i p_e p_i; writes rho e_e e_i
     if (use_zx(gp, 0))
-        elaunch_(gp.zx_k >= 16 ? g_correct_zx<T, 16> : g_correct_zx<T, 8>, "g_correct", 22 * w, gp, 0,
+        elaunch_(gp.zx_k >= 16 ? g_correct_zx<T, 16> : g_correct_zx<T, 8>, "g_correct", 17 * w, gp, 0,
                  rs(0, 0, 0, 0, 0, 0), 0, kZX + 100 + zxk(gp));
     else
-        EL(g_correct<T>, 22 * w, gp, 0, rs(0, 0, 0, 0, 0, 0));
+        EL(g_correct<T>, 17 * w, gp, 0, rs(0, 0, 0, 0, 0, 0));
     // velocity_bcs, energy_bcs, density_bcs (independent variables).  The
     // reference fills the velocities twice (lagran.cpp:759 and 803); the
     // second fill repeats the first except on a staggered line with a
