@@ -1018,19 +1018,20 @@ KERNEL void f_update_march(const Src<T> S)
 }
 
 // Value of v held by the previous (lane - 1) or next (lane + 1) lane of the
-// wavefront (DPP wave_shr:1 / wave_shl:1 on both halves); the end lanes get
-// their own value.
+// wavefront (DPP wave_shr:1 / wave_shl:1 on both halves); the end lanes
+// (0 for lane_prev, 63 for lane_next) get 0.  Those are halo lanes: no
+// lane that writes reads a value derived from their exchange.
 __device__ __forceinline__ double lane_prev(double v)
 {
     const int2 h = __builtin_bit_cast(int2, v);
-    return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_update_dpp(h.x, h.x, 0x138, 0xf, 0xf, false),
-                                                __builtin_amdgcn_update_dpp(h.y, h.y, 0x138, 0xf, 0xf, false)));
+    return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_mov_dpp(h.x, 0x138, 0xf, 0xf, true),
+                                                __builtin_amdgcn_mov_dpp(h.y, 0x138, 0xf, 0xf, true)));
 }
 __device__ __forceinline__ double lane_next(double v)
 {
     const int2 h = __builtin_bit_cast(int2, v);
-    return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_update_dpp(h.x, h.x, 0x130, 0xf, 0xf, false),
-                                                __builtin_amdgcn_update_dpp(h.y, h.y, 0x130, 0xf, 0xf, false)));
+    return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_mov_dpp(h.x, 0x130, 0xf, 0xf, true),
+                                                __builtin_amdgcn_mov_dpp(h.y, 0x130, 0xf, 0xf, true)));
 }
 
 // Lanes of one wavefront per z-row segment: a wave covers 64 consecutive
@@ -1193,6 +1194,117 @@ KERNEL void f_update_zline(const Src<T> S)
         A.rho2[c] = rho;
         A.ee2[c] = ddiv(E[0] * cv1_0 * rho_0 + em[0] - ec[0], cv2 * rho);
         A.ei2[c] = ddiv(E[1] * cv1_0 * rho_0 + em[1] - ec[1], cv2 * rho);
+    }
+}
+
+// R6' for the z sweep as a z-line x-march (H = 2): f_update_zline's
+// arithmetic at each position, with each lane walking K positions along x.
+// The columns (x+1, y) and (x+1, y+1) of rho, cv1, dm and cv2 loaded at one
+// position are the columns (x, y) and (x, y+1) of the next, and w at (x, y)
+// and (x, y-1) is w at (x-1, ...) there, so each value is loaded once per
+// lane instead of re-read from the neighbouring plane.
+template <class T, int K>
+KERNEL void f_update_zx(const Src<T> S)
+{
+    const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
+    const int g_ = grid_of(S.boff, S.ngrid, b_);
+    const EDev<T> &D_ = S.devs[g_];
+    const Geo G = D_.g;
+    const ArrE<T> A = state_view(D_.a, S.st);
+    const ZXPos Z = zx_map<2, K>(S, G, g_, b_);
+    const int sx = G.sx, sy = G.sy;
+    const int iy = Z.iy, iz = Z.iz;
+    const double dt = lare::gbl(A.s)->dt;
+    const Fld<T> R1 = RHO, C1 = A.cv1, C2 = A.cv2, W = VZ1;
+    const auto M = DM;
+    // column x (carried): rho and cv1 of row y, rho cv1 of rows y and y+1,
+    // dm and cv2 of rows y and y+1; w of column x - 1, rows y and y-1
+    double rho_x = 0.0, cv1_x = 0.0, p_x[2] = {0.0, 0.0}, d_x[2] = {0.0, 0.0}, s_x[2] = {0.0, 0.0};
+    double w_xm = 0.0, w_xmm = 0.0;
+    int c = IX(Z.ix0, iy, iz);
+    for (int ix = Z.ix0; ix <= Z.ix1; ++ix, c += sx) {
+        // column x + 1, rows y and y + 1; w at (x, y) and (x, y - 1)
+        const double rho_a = R1[c + sx], cv1_a = C1[c + sx];
+        const double p_a[2] = {rho_a * cv1_a, R1[c + sx + sy] * C1[c + sx + sy]};
+        const double d_a[2] = {M[c + sx], M[c + sx + sy]};
+        const double s_a[2] = {C2[c + sx], C2[c + sx + sy]};
+        const double w_0 = W[c], w_0m = W[c - sy];
+        if (ix > Z.ix0) {
+            const double P4[4] = {p_x[0], p_a[0], p_x[1], p_a[1]};
+            const double D4[4] = {d_x[0], d_a[0], d_x[1], d_a[1]};
+            const double S4[4] = {s_x[0], s_a[0], s_x[1], s_a[1]};
+            const double rho_0 = rho_x, cv1_0 = cv1_x, cvc_0 = A.cvc1[c];
+            const double fv = (w_xm + w_0 + w_xmm + w_0m) * 0.25;
+            const double F[3] = {VX[c], VY[c], VZ[c]};
+            const double E[2] = {EE[c], EI[c]};
+            const double area_m = DYAC(ix, iz), area_e = DZAB(ix, iy);
+            double sp = P4[0] + P4[1] + P4[2] + P4[3];
+            double q = D4[0] + D4[1] + D4[2] + D4[3];
+            double s2 = S4[0] + S4[1] + S4[2] + S4[3];
+            for (int j = 0; j < 4; ++j) {
+                sp = sp + lane_next(P4[j]);
+                q = q + lane_next(D4[j]);
+                s2 = s2 + lane_next(S4[j]);
+            }
+            const double rv = sp * ddiv(0.125, cvc_0);
+            const double dmv = 0.125 * q;
+            const double cv2v = 0.125 * s2;
+            const double rv_p = lane_next(rv), w_p = lane_next(w_0), rho_p = lane_next(rho_0);
+            const double cv1_p = lane_next(cv1_0), cv1_pp = lane_next(cv1_p);
+            const double cvc_p = lane_next(cvc_0), cvc_m = lane_prev(cvc_0);
+            double flc[3];
+            {
+                double v_advect = (w_0 + w_p) * 0.5;
+                double o_v = v_advect * dt * area_m;
+                double sign_v = (o_v >= 0.0) ? 1.0 : -1.0;
+                double vad_p = (sign_v + 1.0) * 0.5;
+                double vad_m = 1.0 - vad_p;
+                double dxci = cv1_p;
+                double dxcu = cv1_0 * vad_p + cv1_pp * vad_m;
+                double dxbu = cvc_0 * vad_p + cvc_p * vad_m;
+                double rhou = rv * vad_p + rv_p * vad_m;
+                for (int f = 0; f < 3; ++f) {
+                    const double fp = lane_next(F[f]);
+                    flc[f] = mom_flux_v<2>(lane_prev(F[f]), F[f], fp, lane_next(fp), o_v, sign_v, dxci, dxcu, dxbu,
+                                           rhou, dmv);
+                }
+            }
+            double ec[2];
+            {
+                const double o_v = fv * dt * area_e;
+                for (int e = 0; e < 2; ++e) {
+                    const double ep = lane_next(E[e]);
+                    ec[e] = energy_flux_v(lane_prev(E[e]), E[e], ep, lane_next(ep), fv, o_v, cvc_m, cvc_0, cvc_p,
+                                          cv1_0, cv1_p, rho_0, rho_p, D4[0]);
+                }
+            }
+            const double flm[3] = {lane_prev(flc[0]), lane_prev(flc[1]), lane_prev(flc[2])};
+            const double em[2] = {lane_prev(ec[0]), lane_prev(ec[1])};
+            const double dmv_m = lane_prev(dmv), dm_m = lane_prev(D4[0]);
+            if (Z.wz) {
+                const double rho_v1 = ddiv(rv * cvc_0 + dmv_m - dmv, cv2v);
+                const double den = cv2v * rho_v1;
+                A.vx2[c] = ddiv(rv * F[0] * cvc_0 + flm[0] - flc[0], den);
+                A.vy2[c] = ddiv(rv * F[1] * cvc_0 + flm[1] - flc[1], den);
+                A.vz2[c] = ddiv(rv * F[2] * cvc_0 + flm[2] - flc[2], den);
+                if (ix >= 1 && iy >= 1 && iz >= 1) {
+                    const double cv2 = S4[0];
+                    const double rho = ddiv(rho_0 * cv1_0 + dm_m - D4[0], cv2);
+                    A.rho2[c] = rho;
+                    A.ee2[c] = ddiv(E[0] * cv1_0 * rho_0 + em[0] - ec[0], cv2 * rho);
+                    A.ei2[c] = ddiv(E[1] * cv1_0 * rho_0 + em[1] - ec[1], cv2 * rho);
+                }
+            }
+        }
+        rho_x = rho_a;
+        cv1_x = cv1_a;
+        for (int j = 0; j < 2; ++j) {
+            p_x[j] = p_a[j];
+            d_x[j] = d_a[j];
+            s_x[j] = s_a[j];
+        }
+        w_xm = w_0;
+        w_xmm = w_0m;
     }
 }
 
@@ -3711,7 +3823,11 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
     ebcs<T>(gp, VDM, DIR);
     // reads rho e_e e_i v cv1 cv2 cvc1 VX1..VZ1 (sweep component) dm;
     // writes rho e_e e_i v
-    if (DIR == 2 && gp.zline && use_walk(gp)) {
+    if (DIR == 2 && gp.zline && use_zx(gp, 0)) {
+        const int K = zxk(gp);
+        elaunch_(K == 16 ? f_update_zx<T, 16> : f_update_zx<T, 8>, "f_update_zline", 16 * w + 8, gp, 0, all0,
+                 st | pv, kZX + 200 + K);
+    } else if (DIR == 2 && gp.zline && use_walk(gp)) {
         elaunch_(f_update_zline<T>, "f_update_zline", 16 * w + 8, gp, 0, all0, st | pv, kZLine);
     } else if (DIR == 2 || gp.march_k <= 0 || !use_walk(gp)) {
         ELS((f_update_all<T, DIR>), 16 * w + 8, gp, 0, all0, st | pv);
