@@ -1308,6 +1308,34 @@ KERNEL void f_update_zx(const Src<T> S)
     }
 }
 
+// R2 (f_cvc1) as a z-line x-march (H = 1): cv1 of the columns (x+1, y) and
+// (x+1, y+1) is loaded once and carried as the columns at x of the next
+// position; the values at z + 1 come from the next lane.  The eight terms
+// are summed in f_cvc1's order.
+template <class T, int K>
+KERNEL void f_cvc1_zx(const Src<T> S)
+{
+    const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
+    const int g_ = grid_of(S.boff, S.ngrid, b_);
+    const EDev<T> &D_ = S.devs[g_];
+    const Geo G = D_.g;
+    const ArrE<T> A = state_view(D_.a, S.st);
+    const ZXPos Z = zx_map<1, K>(S, G, g_, b_);
+    const int sx = G.sx, sy = G.sy;
+    const Fld<T> C1 = A.cv1;
+    double o0 = 0.0, o1 = 0.0, o0z = 0.0, o1z = 0.0;  // column x, rows y and y+1, at z and z+1
+    int c = IX(Z.ix0, Z.iy, Z.iz);
+    for (int ix = Z.ix0; ix <= Z.ix1; ++ix, c += sx) {
+        const double n0 = C1[c + sx], n1 = C1[c + sx + sy];
+        const double n0z = lane_next(n0), n1z = lane_next(n1);
+        if (Z.wz && ix > Z.ix0) A.cvc1[c] = 0.125 * (o0 + n0 + o1 + n1 + o0z + n0z + o1z + n1z);
+        o0 = n0;
+        o1 = n1;
+        o0z = n0z;
+        o1z = n1z;
+    }
+}
+
 // R3 for the x and y sweeps with each lane walking K vertices along DIR (the
 // mapping of f_update_march): B / db along DIR, cvc1 and cv1 are carried as
 // a window, so each is loaded once; the same arithmetic as f_ct_fluxes.
@@ -3774,7 +3802,11 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
         EL((f_prep<T, DIR>), 6 * w, gp, 0, rs(-1, 2, -1, 2, -1, 2), xpass, ypass, zpass);
     else if (DIR == 0)
         elaunch_(f_prep_all<T>, "f_prep_all", 12 * w, gp, 0, rs(-1, 2, -1, 2, -1, 2), 0, gp.order);
-    elaunch_(f_cvc1<T>, "f_cvc1", 2 * w, gp, 0, rs(-1, 1, -1, 1, -1, 1), pv, gp.order);
+    if (use_zx(gp, 0))
+        elaunch_(zxk(gp) == 16 ? f_cvc1_zx<T, 16> : f_cvc1_zx<T, 8>, "f_cvc1", 2 * w, gp, 0, rs(-1, 1, -1, 1, -1, 1),
+                 pv, kZX + 100 + zxk(gp));
+    else
+        elaunch_(f_cvc1<T>, "f_cvc1", 2 * w, gp, 0, rs(-1, 1, -1, 1, -1, 1), pv, gp.order);
     // fused CT (flux and update in one kernel): B moves to bx1..bz1 in the x
     // sweep, is updated there in place by the y sweep (B view bit 2) and
     // comes back in the z sweep
