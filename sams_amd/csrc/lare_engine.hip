@@ -1085,120 +1085,15 @@ __device__ __forceinline__ ZXPos zx_map(const Src<T> &S, const Geo &G, int g_, i
     return z;
 }
 
-// R6' for the z sweep: the transport of f_update_all with every lane
-// loading only its own z (the values at z-1, z+1, z+2 and the fluxes of the
-// face below come from the neighbouring lanes), so each face flux, vertex
-// density and vertex mass flux is computed once.  The arithmetic of each
-// flux and update is the same expression on the same operands as in
-// f_update_all.  Every lane of the wave stays active through the lane
-// exchanges; lanes outside the range work on a clamped position and write
-// nothing.
-template <class T>
-KERNEL void f_update_zline(const Src<T> S)
-{
-    const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
-    const int g_ = grid_of(S.boff, S.ngrid, b_);
-    const EDev<T> &D_ = S.devs[g_];
-    const Geo G = D_.g;
-    const ArrE<T> A = state_view(D_.a, S.st);
-    const R3 R = {S.r.x0, G.nx + S.r.dx1, S.r.y0, G.ny + S.r.dy1, S.r.z0, G.nz + S.r.dz1};
-    const int nyr = R.y1 - R.y0 + 1, nzr = R.z1 - R.z0 + 1;
-    const int L = nzr + 4, P = nyr * L;
-    const int bpp = (P + 239) / 240;
-    const int lb = b_ - S.boff[g_];
-    const int xp = lb / bpp;
-    const int lane = (int)threadIdx.x & 63;
-    const int w = (lb - xp * bpp) * 4 + ((int)threadIdx.x >> 6);
-    int u = 60 * w + lane - 2;
-    const bool inside = u >= 0 && u < P;
-    u = u < 0 ? 0 : (u >= P ? P - 1 : u);
-    const int row = u / L;
-    const int ix = R.x0 + xp, iy = R.y0 + row, iz = R.z0 - 2 + (u - row * L);
-    const bool writer = inside && lane >= 2 && lane < 62 && iz >= R.z0 && iz <= R.z1;
-    const int sx = G.sx, sy = G.sy;
-    const int c = IX(ix, iy, iz);
-    const int oj[4] = {0, sx, sy, sx + sy};
-    const double dt = lare::gbl(A.s)->dt;
-    const Fld<T> R1 = RHO, C1 = A.cv1, C2 = A.cv2, W = VZ1;
-    const auto M = DM;
-    // own loads
-    double P4[4], D4[4], S4[4];
-    for (int j = 0; j < 4; ++j) {
-        P4[j] = R1[c + oj[j]] * C1[c + oj[j]];
-        D4[j] = M[c + oj[j]];
-        S4[j] = C2[c + oj[j]];
-    }
-    const double rho_0 = R1[c], cv1_0 = C1[c], cvc_0 = A.cvc1[c], w_0 = W[c];
-    const double fv = (W[c - sx] + w_0 + W[c - sx - sy] + W[c - sy]) * 0.25;
-    const double F[3] = {VX[c], VY[c], VZ[c]};
-    const double E[2] = {EE[c], EI[c]};
-    const double area_m = DYAC(ix, iz), area_e = DZAB(ix, iy);
-    // vertex density, vertex mass flux and vertex cv2 at c (corners at z+1
-    // from the next lane), in the reference's corner order
-    double sp = P4[0] + P4[1] + P4[2] + P4[3];
-    double q = D4[0] + D4[1] + D4[2] + D4[3];
-    double s2 = S4[0] + S4[1] + S4[2] + S4[3];
-    for (int j = 0; j < 4; ++j) {
-        sp = sp + lane_next(P4[j]);
-        q = q + lane_next(D4[j]);
-        s2 = s2 + lane_next(S4[j]);
-    }
-    const double rv = sp * ddiv(0.125, cvc_0);
-    const double dmv = 0.125 * q;
-    const double cv2v = 0.125 * s2;
-    // neighbours along z
-    const double rv_p = lane_next(rv), w_p = lane_next(w_0), rho_p = lane_next(rho_0);
-    const double cv1_p = lane_next(cv1_0), cv1_pp = lane_next(cv1_p);
-    const double cvc_p = lane_next(cvc_0), cvc_m = lane_prev(cvc_0);
-    // momentum fluxes through vertex face c
-    double flc[3];
-    {
-        double v_advect = (w_0 + w_p) * 0.5;
-        double o_v = v_advect * dt * area_m;
-        double sign_v = (o_v >= 0.0) ? 1.0 : -1.0;
-        double vad_p = (sign_v + 1.0) * 0.5;
-        double vad_m = 1.0 - vad_p;
-        double dxci = cv1_p;
-        double dxcu = cv1_0 * vad_p + cv1_pp * vad_m;
-        double dxbu = cvc_0 * vad_p + cvc_p * vad_m;
-        double rhou = rv * vad_p + rv_p * vad_m;
-        for (int f = 0; f < 3; ++f) {
-            const double fp = lane_next(F[f]);
-            flc[f] = mom_flux_v<2>(lane_prev(F[f]), F[f], fp, lane_next(fp), o_v, sign_v, dxci, dxcu, dxbu, rhou,
-                                   dmv);
-        }
-    }
-    // energy fluxes through cell face c
-    double ec[2];
-    {
-        const double o_v = fv * dt * area_e;
-        for (int e = 0; e < 2; ++e) {
-            const double ep = lane_next(E[e]);
-            ec[e] = energy_flux_v(lane_prev(E[e]), E[e], ep, lane_next(ep), fv, o_v, cvc_m, cvc_0, cvc_p, cv1_0,
-                                  cv1_p, rho_0, rho_p, D4[0]);
-        }
-    }
-    // fluxes of the face below from the previous lane
-    const double flm[3] = {lane_prev(flc[0]), lane_prev(flc[1]), lane_prev(flc[2])};
-    const double em[2] = {lane_prev(ec[0]), lane_prev(ec[1])};
-    const double dmv_m = lane_prev(dmv), dm_m = lane_prev(D4[0]);
-    if (!writer) return;
-    const double rho_v1 = ddiv(rv * cvc_0 + dmv_m - dmv, cv2v);
-    const double den = cv2v * rho_v1;
-    A.vx2[c] = ddiv(rv * F[0] * cvc_0 + flm[0] - flc[0], den);
-    A.vy2[c] = ddiv(rv * F[1] * cvc_0 + flm[1] - flc[1], den);
-    A.vz2[c] = ddiv(rv * F[2] * cvc_0 + flm[2] - flc[2], den);
-    if (ix >= 1 && iy >= 1 && iz >= 1) {
-        const double cv2 = S4[0];
-        const double rho = ddiv(rho_0 * cv1_0 + dm_m - D4[0], cv2);
-        A.rho2[c] = rho;
-        A.ee2[c] = ddiv(E[0] * cv1_0 * rho_0 + em[0] - ec[0], cv2 * rho);
-        A.ei2[c] = ddiv(E[1] * cv1_0 * rho_0 + em[1] - ec[1], cv2 * rho);
-    }
-}
-
-// R6' for the z sweep as a z-line x-march (H = 2): f_update_zline's
-// arithmetic at each position, with each lane walking K positions along x.
+// R6' for the z sweep: the transport of f_update_all with lanes along z,
+// each loading only its own z (the values at z-1, z+1, z+2 and the fluxes of
+// the face below come from the neighbouring lanes, two halo lanes at each
+// end of a wavefront's z-row segment, so each face flux, vertex density and
+// vertex mass flux is computed once), and each lane walking K positions
+// along x.  The arithmetic of each flux and update is the same expression
+// on the same operands as in f_update_all; every lane stays active through
+// the exchanges, and lanes outside the range work on a clamped position and
+// write nothing.
 // The columns (x+1, y) and (x+1, y+1) of rho, cv1, dm and cv2 loaded at one
 // position are the columns (x, y) and (x, y+1) of the next, and w at (x, y)
 // and (x, y-1) is w at (x-1, ...) there, so each value is loaded once per
@@ -1425,7 +1320,7 @@ KERNEL void f_ct_march(const Src<T> S)
 // y too.  Positions beyond the box along y are clamped; they feed only
 // fluxes that are not stored.
 template <class T, int K>
-KERNEL void f_cty_mass(const Src<T> S)
+KERNEL OCC(5) void f_cty_mass(const Src<T> S)
 {
     const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
     const int g_ = grid_of(S.boff, S.ngrid, b_);
@@ -1509,7 +1404,7 @@ KERNEL void f_cty_mass(const Src<T> S)
     }
 }
 
-// R3 for the z sweep with the z-line lane mapping of f_update_zline: each
+// R3 for the z sweep with the z-line lane mapping (table code kZLine): each
 // lane loads its own z and takes B / db, db, cvc1 and cv1 at z-1, z+1, z+2
 // from its neighbour lanes; the same arithmetic as f_ct_fluxes<2>.
 template <class T>
@@ -3859,8 +3754,6 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
         const int K = zxk(gp);
         elaunch_(K == 16 ? f_update_zx<T, 16> : f_update_zx<T, 8>, "f_update_zline", 16 * w + 8, gp, 0, all0,
                  st | pv, kZX + 200 + K);
-    } else if (DIR == 2 && gp.zline && use_walk(gp)) {
-        elaunch_(f_update_zline<T>, "f_update_zline", 16 * w + 8, gp, 0, all0, st | pv, kZLine);
     } else if (DIR == 2 || gp.march_k <= 0 || !use_walk(gp)) {
         ELS((f_update_all<T, DIR>), 16 * w + 8, gp, 0, all0, st | pv);
     } else {
