@@ -38,9 +38,12 @@ __device__ __forceinline__ double rcp_nearest(double d)
     y = fma(y, e, y);
     e = fma(-d, y, 1.0);
     y = fma(y, e, y);
-    const unsigned lo = (unsigned)__double2loint(d), hi = (unsigned)__double2hiint(d);
-    const bool ones = lo == 0xffffffffu && (hi | 0xfff00000u) == 0xffffffffu;
-    return __hiloint2double(__double2hiint(y), __double2loint(y) | (ones ? 1 : 0));
+    // all-ones significand: |mantissa| (frexp, in [1/2, 1)) is 1 - 2^-53.
+    // There y is 2^(-e-1) (the tie) or already 2^(-e-1) (1 + 2^-52) (when
+    // the second step had reached it): both have the high word of 2^(-e-1)
+    // and a low word of 0 or 1, and the nearest has the low word 1.
+    const bool ones = __builtin_fabs(__builtin_amdgcn_frexp_mant(d)) == 0x1.fffffffffffffp-1;
+    return __hiloint2double(__double2hiint(y), ones ? 1 : __double2loint(y));
 }
 
 // a / d, correctly rounded (IEEE 754 round to nearest even), for all inputs

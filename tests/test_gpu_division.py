@@ -85,6 +85,28 @@ def test_all_ones_significands(mode):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
+def test_all_ones_significands_at_the_exponent_limits(mode):
+    """Divisors with an all-ones significand at the top and the bottom of
+    the exponent range, subnormal ones included (whose significand field is
+    all ones, or all ones after normalisation), of both signs, against
+    numerators that keep the quotient normal."""
+    rng = np.random.default_rng(19)
+    sub = np.array([0x000FFFFFFFFFFFFF, 0x0007FFFFFFFFFFFF, 0x0003FFFFFFFFFFFF, 0x00000000FFFFFFFF,
+                    0x00000000000001FF, 0x0000000000000001], dtype=np.uint64).view(np.float64)
+    ones = 2.0 - 2.0 ** -52
+    d = np.concatenate([sub, np.ldexp(ones, np.arange(990, 1024)), np.ldexp(ones, np.arange(-1022, -990))])
+    d = np.concatenate([d, -d])
+    m = 256
+    ed = np.frexp(d)[1]
+    ea = np.clip(ed[:, None] + rng.integers(-60, 61, (d.size, m)), -1060, 1020)
+    a = np.ldexp(np.where(rng.integers(0, 2, (d.size, m)) == 0, 1.0, rng.uniform(1.0, 2.0, (d.size, m))), ea)
+    a[:, 1::2] = -a[:, 1::2]
+    dd = np.repeat(d, m)
+    with np.errstate(all="ignore"):
+        _agree(a.ravel(), dd, mode)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
 def test_random_operands(mode):
     rng = np.random.default_rng(11)
     n = 1 << 20
