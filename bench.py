@@ -230,7 +230,14 @@ def kernel_roofline(dev, nsteps, cells, ms_step):
                 "avg_launch_us": round(avg_ms * 1e3, 2),
                 "step_algorithmic_GBs": round(step_bytes / (ms_step * 1e-3) / 1e9, 1),
                 "step_frac": round(step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-
+    # every kernel of the step: launches per step, average launch, its
+    # algorithmic GB/s and the measured HBM bytes per launch where profiled
+    n = round(cells ** (1 / 3))
+    roofline["kernels"] = [
+        {"kernel": k, "launches_per_step": round(c / nsteps, 2), "avg_us": round(t / c * 1e3, 1),
+         "algorithmic_GBs": round(b / c / (t / c * 1e-3) / 1e9, 1) if t > 0 and b > 0 else None,
+         "traffic_bytes": measured_traffic(k, n)}
+        for k, (c, t, b) in sorted(prof.items(), key=lambda kv: -kv[1][1])]
     return roofline
 
 
