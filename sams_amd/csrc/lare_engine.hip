@@ -3798,7 +3798,13 @@ void end_t(Group &gp)
     sweep_t<T, 0>(gp, 1.0, 1.0, 1.0, 1);
     sweep_t<T, 1>(gp, 0.0, 1.0, 1.0, 0);
     sweep_t<T, 2>(gp, 0.0, 0.0, 1.0, 1);
-    ebcs<T>(gp, VB);
+    // bfield_bcs after the B quotient (remap.cpp:91-95): the z sweep's fill of
+    // its nine variables has just filled B from the same values, and the
+    // all-dimension fill is idempotent (it reads only cells that no fill
+    // changes), so the repeat is needed only where the per-dimension fills ran
+    // (a staggered line with a periodic lower and a clamped upper edge, map
+    // edges that read written layers, two-cell dimensions)
+    if (!gp.bc_all || btable3(gp, VALL_1, 0).nblk < 0) ebcs<T>(gp, VB);
     // energy_correction after the remap's last fill (shared_data.h:415-417);
     // no fill follows it in the reference either
     if (gp.n[2]) EL(r_ke_energy<T>, 4 * w + 8, gp, 2, rs(1, 0, 1, 0, 1, 0));
