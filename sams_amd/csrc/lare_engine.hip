@@ -1034,6 +1034,24 @@ __device__ __forceinline__ double lane_next(double v)
                                                 __builtin_amdgcn_mov_dpp(h.y, 0x130, 0xf, 0xf, true)));
 }
 
+// A 1D delta along x for the positions a wavefront marches over: lane l
+// holds d[i0 + l] (indices clamped to the array's -2..imax), and
+// x_at(v, i - i0) returns the value of position i to every lane.  The x
+// deltas of a march are loaded once, before its loop, instead of once per
+// position (a uniform load per position would be waited for at once, to move
+// it to a scalar register, before the loads issued ahead of it complete).
+__device__ __forceinline__ double x_row(const double *d, int i0, int imax)
+{
+    const int i = i0 + ((int)threadIdx.x & 63);
+    return lare::gbl(d)[(i < imax ? i : imax) + 2];
+}
+__device__ __forceinline__ double x_at(double v, int l)
+{
+    const int2 h = __builtin_bit_cast(int2, v);
+    return __builtin_bit_cast(double,
+                              make_int2(__builtin_amdgcn_readlane(h.x, l), __builtin_amdgcn_readlane(h.y, l)));
+}
+
 // Lanes of one wavefront per z-row segment: a wave covers 64 consecutive
 // positions of a z-row flattened over (y, z), with two halo positions
 // beyond each end of the row (z0-2 .. z1+2), and successive waves overlap by
@@ -1936,7 +1954,7 @@ KERNEL OCC(3) void g_visc_dt_zx(const Src<T> S)
     const EDev<T> &D_ = S.devs[g_];
     const Geo G = D_.g;
     const ArrE<T> A = state_view(D_.a, S.st);
-    const Phys &P = D_.p;
+    const Phys P = D_.p;
     const double gas_gamma = P.gas_gamma, mu0 = P.mu0;
     const ZXPos Z = zx_map<1, K>(S, G, g_, b_);
     const int sx = G.sx, sy = G.sy;
@@ -2158,80 +2176,89 @@ KERNEL void g_predict(const Src<T> S)
 }
 
 // b_field_and_cv1_update and the energy predictor of g_predict at one cell,
-// from the eight v corners W[k][p] (p: ooo, moo, omo, oom, mmo, mom, omm,
-// mmm), the cell's face-averaged B (bx1..bz1: the resistive copy or the
-// average of its faces) and, when `energy`, its rho, e_e, e_i, the
-// predictor's p_e input pe0 and visc_heat.  out = cv1, bx1..bz1 (times
-// cv1), p_e, p_i.
-template <class T>
-__device__ __forceinline__ void predict_cell(const ArrE<T> &A, double gas_gamma, int ix, int iy, int iz,
-                                             const double W[3][8], double bx1, double by1, double bz1, bool energy,
-                                             double rho, double ee, double ei, double pe0, double pi0, double vh,
-                                             double dt, double out[6])
+// in two parts: predict_grad, the velocity gradients from the eight v
+// corners W[k][p] (p: ooo, moo, omo, oom, mmo, mom, omm, mmm), dv and cv1;
+// predict_rest, the B and energy predictor from them, the cell's
+// face-averaged B (bx1..bz1: the resistive copy or the average of its faces),
+// its rho, e_e, e_i, the predictor's p_e and p_i inputs pe0, pi0 and
+// visc_heat.  vol is the cell volume and axx, ayy, azz its face areas (CV,
+// DXAB, DYAB, DZAB).  out = cv1, bx1..bz1 (times cv1), p_e, p_i.
+struct PredGrad {
+    double dvxdx, dvydy, dvzdz, dvxdy, dvydx, dvxdz, dvzdx, dvydz, dvzdy, dv, cv1;
+};
+__device__ __forceinline__ PredGrad predict_grad(double vol, double axx, double ayy, double azz,
+                                                 const double W[3][8], double dt)
 {
     enum { ooo, moo, omo, oom, mmo, mom, omm, mmm };
     const double *X = W[0], *Y = W[1], *Zv = W[2];
+    PredGrad g;
     double vxb = 0.25 * (X[ooo] + X[omo] + X[oom] + X[omm]);
     double vxbm = 0.25 * (X[moo] + X[mmo] + X[mom] + X[mmm]);
     double vyb = 0.25 * (Y[ooo] + Y[moo] + Y[oom] + Y[mom]);
     double vybm = 0.25 * (Y[omo] + Y[mmo] + Y[omm] + Y[mmm]);
     double vzb = 0.25 * (Zv[ooo] + Zv[moo] + Zv[omo] + Zv[mmo]);
     double vzbm = 0.25 * (Zv[oom] + Zv[mom] + Zv[omm] + Zv[mmm]);
-    const double vol = CV(ix, iy, iz);
-    const double axx = DXAB(iy, iz), ayy = DYAB(ix, iz), azz = DZAB(ix, iy);
-    double dvxdx = ddiv(vxb * axx - vxbm * axx, vol);
-    double dvydy = ddiv(vyb * ayy - vybm * ayy, vol);
-    double dvzdz = ddiv(vzb * azz - vzbm * azz, vol);
-    double dv = (dvxdx + dvydy + dvzdz) * dt / 2.0;
-    const double cv1 = vol * (1.0 + dv);
+    g.dvxdx = ddiv(vxb * axx - vxbm * axx, vol);
+    g.dvydy = ddiv(vyb * ayy - vybm * ayy, vol);
+    g.dvzdz = ddiv(vzb * azz - vzbm * azz, vol);
+    g.dv = (g.dvxdx + g.dvydy + g.dvzdz) * dt / 2.0;
+    g.cv1 = vol * (1.0 + g.dv);
     vxb = 0.25 * (X[ooo] + X[moo] + X[oom] + X[mom]);
     vxbm = 0.25 * (X[omo] + X[mmo] + X[omm] + X[mmm]);
     vyb = 0.25 * (Y[ooo] + Y[omo] + Y[oom] + Y[omm]);
     vybm = 0.25 * (Y[moo] + Y[mmo] + Y[mom] + Y[mmm]);
-    double dvxdy = ddiv(vxb * ayy - vxbm * ayy, vol);
-    double dvydx = ddiv(vyb * axx - vybm * axx, vol);
+    g.dvxdy = ddiv(vxb * ayy - vxbm * ayy, vol);
+    g.dvydx = ddiv(vyb * axx - vybm * axx, vol);
     vxb = 0.25 * (X[ooo] + X[moo] + X[omo] + X[mmo]);
     vxbm = 0.25 * (X[oom] + X[mom] + X[omm] + X[mmm]);
     double vzb2 = 0.25 * (Zv[ooo] + Zv[omo] + Zv[oom] + Zv[omm]);
     double vzbm2 = 0.25 * (Zv[moo] + Zv[mmo] + Zv[mom] + Zv[mmm]);
-    double dvxdz = ddiv(vxb * azz - vxbm * azz, vol);
-    double dvzdx = ddiv(vzb2 * axx - vzbm2 * axx, vol);
+    g.dvxdz = ddiv(vxb * azz - vxbm * azz, vol);
+    g.dvzdx = ddiv(vzb2 * axx - vzbm2 * axx, vol);
     double vyb3 = 0.25 * (Y[ooo] + Y[moo] + Y[omo] + Y[mmo]);
     double vybm3 = 0.25 * (Y[oom] + Y[mom] + Y[omm] + Y[mmm]);
     double vzb3 = 0.25 * (Zv[ooo] + Zv[moo] + Zv[oom] + Zv[mom]);
     double vzbm3 = 0.25 * (Zv[omo] + Zv[mmo] + Zv[omm] + Zv[mmm]);
-    double dvydz = ddiv(vyb3 * azz - vybm3 * azz, vol);
-    double dvzdy = ddiv(vzb3 * ayy - vzbm3 * ayy, vol);
-    double w3 = bx1 * dvxdx + by1 * dvxdy + bz1 * dvxdz;
-    double w4 = bx1 * dvydx + by1 * dvydy + bz1 * dvydz;
-    double w5 = bx1 * dvzdx + by1 * dvzdy + bz1 * dvzdz;
+    g.dvydz = ddiv(vyb3 * azz - vybm3 * azz, vol);
+    g.dvzdy = ddiv(vzb3 * ayy - vzbm3 * ayy, vol);
+    return g;
+}
+__device__ __forceinline__ void predict_rest(double gas_gamma, double vol, const PredGrad &g, double bx1,
+                                             double by1, double bz1, double rho, double ee, double ei, double pe0,
+                                             double pi0, double vh, double dt, double out[6])
+{
+    double w3 = bx1 * g.dvxdx + by1 * g.dvxdy + bz1 * g.dvxdz;
+    double w4 = bx1 * g.dvydx + by1 * g.dvydy + bz1 * g.dvydz;
+    double w5 = bx1 * g.dvzdx + by1 * g.dvzdy + bz1 * g.dvzdz;
+    const double cv1 = g.cv1, dv = g.dv;
     out[0] = cv1;
     out[1] = ddiv(bx1 + w3 * dt / 2.0, 1.0 + dv) * cv1;
     out[2] = ddiv(by1 + w4 * dt / 2.0, 1.0 + dv) * cv1;
     out[3] = ddiv(bz1 + w5 * dt / 2.0, 1.0 + dv) * cv1;
-    out[4] = out[5] = 0.0;
-    if (energy) {
-        double dvp = ddiv(cv1, vol) - 1.0;
-        double e1_e = ee - ddiv(pe0 * dvp, rho);
-        double e1_i = ei - ddiv(pi0 * dvp, rho);
-        e1_i += ddiv(vh * dt / 2.0, rho);
-        out[4] = ddiv(e1_e * (gas_gamma - 1.0) * rho * vol, cv1);
-        out[5] = ddiv(e1_i * (gas_gamma - 1.0) * rho * vol, cv1);
-    }
+    double dvp = ddiv(cv1, vol) - 1.0;
+    double e1_e = ee - ddiv(pe0 * dvp, rho);
+    double e1_i = ei - ddiv(pi0 * dvp, rho);
+    e1_i += ddiv(vh * dt / 2.0, rho);
+    out[4] = ddiv(e1_e * (gas_gamma - 1.0) * rho * vol, cv1);
+    out[5] = ddiv(e1_i * (gas_gamma - 1.0) * rho * vol, cv1);
 }
 
 // L6 as a z-line x-march (H = 1): the same arithmetic as g_predict.  The
 // eight corners (x-1..x, y-1..y, z-1..z) of the velocities are this lane's
 // columns (x, y) and (x, y-1), the previous position's, and the lane below.
+// The velocities of a position are loaded one position ahead (the last
+// position reloads its own), so their latency overlaps the previous
+// position's arithmetic; the cell's other operands are loaded at the start
+// of its position and first used after the velocity gradients.
 template <class T, int K>
-KERNEL OCC(4) void g_predict_zx(const Src<T> S)
+KERNEL OCC(3) void g_predict_zx(const Src<T> S)
 {
     const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
     const int g_ = grid_of(S.boff, S.ngrid, b_);
     const EDev<T> &D_ = S.devs[g_];
     const Geo G = D_.g;
     const ArrE<T> A = state_view(D_.a, S.st);
-    const Phys &P = D_.p;
+    const Phys P = D_.p;
     const double gas_gamma = P.gas_gamma;
     const bool res = P.resistive != 0;
     const ZXPos Z = zx_map<1, K>(S, G, g_, b_);
@@ -2240,26 +2267,42 @@ KERNEL OCC(4) void g_predict_zx(const Src<T> S)
     const double dt = lare::gbl(A.s)->dt;
     const Fld<T> V3[3] = {VX, VY, VZ};
     enum { ooo, moo, omo, oom, mmo, mom, omm, mmm };
+    // face-averaged B: the resistive copy bx1..bz1, or the average of B's
+    // faces (x - 1 from the previous position)
+    const Fld<T> B1x = res ? A.bx1 : BX, B1y = res ? A.by1 : BY, B1z = res ? A.bz1 : BZ;
+    const int zm = res ? 0 : 1;
+    const double dyb = DYB(iy), dzb = DZB(iz), axx = dyb * dzb;
+    const double dxb_row = x_row(A.dxb, Z.ix0, G.nx + 2);
     double M[3] = {0, 0, 0}, Mm[3] = {0, 0, 0}, pM[3] = {0, 0, 0}, pMm[3] = {0, 0, 0};
     double bx_m = 0.0;
     int c = IX(Z.ix0, iy, iz);
+    double nO[3], nOm[3];
+    for (int k = 0; k < 3; ++k) {
+        nO[k] = V3[k][c];
+        nOm[k] = V3[k][c - sy];
+    }
     for (int ix = Z.ix0; ix <= Z.ix1; ++ix, c += sx) {
         double O[3], Om[3], pO[3], pOm[3];
         for (int k = 0; k < 3; ++k) {
-            O[k] = V3[k][c];
-            Om[k] = V3[k][c - sy];
+            O[k] = nO[k];
+            Om[k] = nOm[k];
         }
+        const double x1 = B1x[c], y1 = B1y[c], y1m = B1y[c - sy], z1 = B1z[c], z1m = B1z[c - zm];
+        const double rho = RHO[c], ee = EE[c], ei = EI[c], pe_a = A.p_e[c], pi_a = A.p_i[c], vh = A.visc_heat[c];
+        {
+            const int n = ix < Z.ix1 ? c + sx : c;
+            for (int k = 0; k < 3; ++k) {
+                nO[k] = V3[k][n];
+                nOm[k] = V3[k][n - sy];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
         for (int k = 0; k < 3; ++k) {
             pO[k] = lane_prev(O[k]);
             pOm[k] = lane_prev(Om[k]);
         }
-        double bz_own = 0.0, bx_own = 0.0;
-        if (!res) {
-            bx_own = BX[c];
-            bz_own = BZ[c];
-        }
-        const double bz_m = lane_prev(bz_own);
-        if (Z.wz && ix > Z.ix0) {
+        const double dxb = x_at(dxb_row, ix - Z.ix0);
+        if (ix > Z.ix0 && Z.wz) {  // the warm-up position only loads
             double W[3][8];
             for (int k = 0; k < 3; ++k) {
                 W[k][ooo] = O[k];
@@ -2271,36 +2314,34 @@ KERNEL OCC(4) void g_predict_zx(const Src<T> S)
                 W[k][omm] = pOm[k];
                 W[k][mmm] = pMm[k];
             }
+            const double vol = dxb * axx, ayy = dxb * dzb, azz = dxb * dyb;
+            const PredGrad gr = predict_grad(vol, axx, ayy, azz, W, dt);
+            // the cell's own operands are first used here: their loads have
+            // had the gradients' time to land
+            __builtin_amdgcn_sched_barrier(0);
             double bx1, by1, bz1;
             if (res) {
-                bx1 = A.bx1[c];
-                by1 = A.by1[c];
-                bz1 = A.bz1[c];
+                bx1 = x1;
+                by1 = y1;
+                bz1 = z1;
             } else {
-                bx1 = 0.5 * (bx_own + bx_m);
-                by1 = 0.5 * (BY[c] + BY[c - sy]);
-                bz1 = 0.5 * (bz_own + bz_m);
+                bx1 = 0.5 * (x1 + bx_m);
+                by1 = 0.5 * (y1 + y1m);
+                bz1 = 0.5 * (z1 + z1m);
             }
+            // the energy predictor covers (0..n+1)^3; elsewhere p_e and p_i
+            // are stored back unchanged
             const bool energy = ix >= 0 && iy >= 0 && iz >= 0 && ix <= G.nx + 1 && iy <= G.ny + 1 && iz <= G.nz + 1;
+            const double pe0 = res ? pe_a : (gas_gamma - 1.0) * rho * ee;
+            const double pi0 = P.keep_pi ? pi_a : (gas_gamma - 1.0) * rho * ei;
             double o[6];
-            if (energy) {
-                const double rho = RHO[c], ee = EE[c], ei = EI[c];
-                const double pe0 = res ? (double)A.p_e[c] : (gas_gamma - 1.0) * rho * ee;
-                const double pi0 = P.keep_pi ? (double)A.p_i[c] : (gas_gamma - 1.0) * rho * ei;
-                predict_cell(A, gas_gamma, ix, iy, iz, W, bx1, by1, bz1, true, rho, ee, ei, pe0, pi0,
-                             (double)A.visc_heat[c], dt, o);
-            } else {
-                predict_cell(A, gas_gamma, ix, iy, iz, W, bx1, by1, bz1, false, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, dt,
-                             o);
-            }
+            predict_rest(gas_gamma, vol, gr, bx1, by1, bz1, rho, ee, ei, pe0, pi0, vh, dt, o);
             A.cv1[c] = o[0];
             A.bx1[c] = o[1];
             A.by1[c] = o[2];
             A.bz1[c] = o[3];
-            if (energy) {
-                A.p_e[c] = o[4];
-                A.p_i[c] = o[5];
-            }
+            A.p_e[c] = energy ? o[4] : pe_a;
+            A.p_i[c] = energy ? o[5] : pi_a;
         }
         for (int k = 0; k < 3; ++k) {
             M[k] = O[k];
@@ -2308,7 +2349,7 @@ KERNEL OCC(4) void g_predict_zx(const Src<T> S)
             pM[k] = pO[k];
             pMm[k] = pOm[k];
         }
-        bx_m = bx_own;
+        bx_m = x1;
     }
 }
 
@@ -2454,7 +2495,7 @@ KERNEL OCC(2) void g_correct(const Src<T> S)
 // is formed once and reused by the cells that share it (a DOT of two
 // points is the same value in either order: both differences change sign).
 template <class T, int K>
-KERNEL OCC(4) void g_correct_zx(const Src<T> S)
+KERNEL OCC(3) void g_correct_zx(const Src<T> S)
 {
     const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
     const int g_ = grid_of(S.boff, S.ngrid, b_);
@@ -2466,7 +2507,8 @@ KERNEL OCC(4) void g_correct_zx(const Src<T> S)
     const int iy = Z.iy, iz = Z.iz;
     const double dt = lare::gbl(A.s)->dt;
     const Fld<T> V6[6] = {VX, VY, VZ, VX1, VY1, VZ1};
-    const double dy = DYB(iy), dz = DZB(iz);
+    const double dy = DYB(iy), dz = DZB(iz), dyz = dy * dz;
+    const double dx_row = x_row(A.dxb, Z.ix0, G.nx + 2);
     const bool cyz = iy >= 1 && iz >= 1;
     // plane x - 1: column (x-1, y) = M, (x-1, y-1) = Mm at z; their z-1
     // values of VY1, VZ1; the carried products and coefficients
@@ -2475,18 +2517,37 @@ KERNEL OCC(4) void g_correct_zx(const Src<T> S)
     double a2_m = 0.0, a9_m = 0.0, a12_m = 0.0, vxb_m = 0.0;
     double al2_m = 0.0, al2p_m = 0.0, al3_m = 0.0, al3y_m = 0.0;
     int c = IX(Z.ix0, iy, iz);
+    // the velocities and alphas of a position are loaded one position
+    // ahead (the last position reloads its own); the cell's energies are
+    // loaded at the start of its position and first used after its
+    // viscous heating and velocity gradients
+    double nO[6], nOm[6], nal[5];
+    auto fetch = [&](int n) {
+        for (int k = 0; k < 6; ++k) {
+            nO[k] = V6[k][n];
+            nOm[k] = V6[k][n - sy];
+        }
+        nal[0] = A.alpha1[n];
+        nal[1] = A.alpha1[n + sy];
+        nal[2] = A.alpha2[n];
+        nal[3] = A.alpha3[n];
+        nal[4] = A.alpha3[n - sy];
+    };
+    fetch(c);
     for (int ix = Z.ix0; ix <= Z.ix1; ++ix, c += sx) {
         double O[6], Om[6], pO[6], pOm[6];
         for (int k = 0; k < 6; ++k) {
-            O[k] = V6[k][c];
-            Om[k] = V6[k][c - sy];
+            O[k] = nO[k];
+            Om[k] = nOm[k];
         }
+        const double al1 = nal[0], al1y = nal[1], al2 = nal[2], al3 = nal[3], al3y = nal[4];
+        const double rho = RHO[c], ee = EE[c], ei = EI[c], pe = A.p_e[c], pi = A.p_i[c];
+        fetch(ix < Z.ix1 ? c + sx : c);
+        __builtin_amdgcn_sched_barrier(0);
         for (int k = 0; k < 6; ++k) {
             pO[k] = lane_prev(O[k]);
             pOm[k] = lane_prev(Om[k]);
         }
-        const double al1 = A.alpha1[c], al1y = A.alpha1[c + sy], al2 = A.alpha2[c], al3 = A.alpha3[c],
-                     al3y = A.alpha3[c - sy];
         const double al1p = lane_next(al1), al1yp = lane_next(al1y), al2p = lane_next(al2);
 #define ZDOT(p, q) \
     (1 * (((p)[0] - (q)[0]) * ((p)[3] - (q)[3]) + ((p)[1] - (q)[1]) * ((p)[4] - (q)[4]) + ((p)[2] - (q)[2]) * ((p)[5] - (q)[5])))
@@ -2495,31 +2556,31 @@ KERNEL OCC(4) void g_correct_zx(const Src<T> S)
 #undef ZDOT
         const double a5 = lane_next(a1), a6 = lane_next(a2), a7 = lane_next(a3), a8 = lane_next(a4);
         const double vxb = 0.25 * (O[3] + Om[3] + pO[3] + pOm[3]);
-        if (Z.wz && ix > Z.ix0) {
-            if (ix >= 1 && cyz) {
-                const double dx = DXB(ix);
-                const double vh =
-                    (-0.25 * dy * dz * al1 * a1 - 0.25 * dx * dz * al2 * a2 - 0.25 * dy * dz * al1y * a3 -
-                     0.25 * dx * dz * al2_m * a4 - 0.25 * dy * dz * al1p * a5 - 0.25 * dx * dz * al2p * a6 -
-                     0.25 * dy * dz * al1yp * a7 - 0.25 * dx * dz * al2p_m * a8 - 0.25 * dy * dy * al3 * a9 -
-                     0.25 * dx * dy * al3_m * a10 - 0.25 * dy * dy * al3y_m * a11 - 0.25 * dx * dy * al3y * a12);
-                const double vol = CV(ix, iy, iz);
-                const double visc_heat = pmax(ddiv(vh, vol), 0.0);
-                const double vxbm = vxb_m;
-                const double vyb = 0.25 * (O[4] + M[4] + pO[4] + pM_vy1);
-                const double vybm = 0.25 * (Om[4] + Mm[4] + pOm[4] + pMm_vy1);
-                const double vzb = 0.25 * (O[5] + M[5] + Om[5] + Mm[5]);
-                const double vzbm = 0.25 * (pO[5] + pM_vz1 + pOm[5] + pMm_vz1);
-                const double axx = DXAB(iy, iz), ayy = DYAB(ix, iz), azz = DZAB(ix, iy);
-                double dvxdx = ddiv(vxb * axx - vxbm * axx, vol);
-                double dvydy = ddiv(vyb * ayy - vybm * ayy, vol);
-                double dvzdz = ddiv(vzb * azz - vzbm * azz, vol);
-                double dv = (dvxdx + dvydy + dvzdz) * dt;
-                double rho = RHO[c];
-                A.ee2[c] = EE[c] - ddiv(dv * A.p_e[c], rho);
-                A.ei2[c] = EI[c] + ddiv(dt * visc_heat - dv * A.p_i[c], rho);
-                A.rho2[c] = ddiv(rho, 1.0 + dv);
-            }
+        const double dx = x_at(dx_row, ix - Z.ix0);
+        if (Z.wz && ix > Z.ix0 && ix >= 1 && cyz) {
+            const double vh =
+                (-0.25 * dy * dz * al1 * a1 - 0.25 * dx * dz * al2 * a2 - 0.25 * dy * dz * al1y * a3 -
+                 0.25 * dx * dz * al2_m * a4 - 0.25 * dy * dz * al1p * a5 - 0.25 * dx * dz * al2p * a6 -
+                 0.25 * dy * dz * al1yp * a7 - 0.25 * dx * dz * al2p_m * a8 - 0.25 * dy * dy * al3 * a9 -
+                 0.25 * dx * dy * al3_m * a10 - 0.25 * dy * dy * al3y_m * a11 - 0.25 * dx * dy * al3y * a12);
+            // CV, DXAB, DYAB, DZAB from the 1D deltas
+            const double vol = dx * dyz;
+            const double visc_heat = pmax(ddiv(vh, vol), 0.0);
+            const double vxbm = vxb_m;
+            const double vyb = 0.25 * (O[4] + M[4] + pO[4] + pM_vy1);
+            const double vybm = 0.25 * (Om[4] + Mm[4] + pOm[4] + pMm_vy1);
+            const double vzb = 0.25 * (O[5] + M[5] + Om[5] + Mm[5]);
+            const double vzbm = 0.25 * (pO[5] + pM_vz1 + pOm[5] + pMm_vz1);
+            const double axx = dyz, ayy = dx * dz, azz = dx * dy;
+            double dvxdx = ddiv(vxb * axx - vxbm * axx, vol);
+            double dvydy = ddiv(vyb * ayy - vybm * ayy, vol);
+            double dvzdz = ddiv(vzb * azz - vzbm * azz, vol);
+            double dv = (dvxdx + dvydy + dvzdz) * dt;
+            // the energies are first used here
+            __builtin_amdgcn_sched_barrier(0);
+            A.ee2[c] = ee - ddiv(dv * pe, rho);
+            A.ei2[c] = ei + ddiv(dt * visc_heat - dv * pi, rho);
+            A.rho2[c] = ddiv(rho, 1.0 + dv);
         }
         for (int k = 0; k < 6; ++k) {
             M[k] = O[k];
@@ -2697,7 +2758,7 @@ KERNEL OCC(3) void r_substep_zx(const Src<T> S, int to_bx1)
     const EDev<T> &D_ = S.devs[g_];
     const Geo G = D_.g;
     const ArrE<T> A = state_view(D_.a, S.st);
-    const Phys &P = D_.p;
+    const Phys P = D_.p;
     const ZXPos Z = zx_map<1, K>(S, G, g_, b_);
     const int sx = G.sx, sy = G.sy;
     const int nx = G.nx, ny = G.ny, nz = G.nz;
@@ -2706,6 +2767,7 @@ KERNEL OCC(3) void r_substep_zx(const Src<T> S, int to_bx1)
     const Fld<T> IX_ = to_bx1 ? BX : A.bx1, IY_ = to_bx1 ? BY : A.by1, IZ_ = to_bx1 ? BZ : A.bz1;
     const Fld<T> OX_ = to_bx1 ? A.bx1 : BX, OY_ = to_bx1 ? A.by1 : BY, OZ_ = to_bx1 ? A.bz1 : BZ;
     const QDiv qy0 = qdiv_of(DYC(iy)), qym = qdiv_of(DYC(iy - 1)), qz = qdiv_of(DZC(iz));
+    const double dyb = DYB(iy), dzb = DZB(iz), area_x = dyb * dzb;
     int c = IX(Z.ix0, iy, iz);
     // plane x: BY at (x, y-1), (x, y) and their z+1; BZ at (x, y-1..y+1)
     double by_m = IY_[c - sy], by_0 = IY_[c];
@@ -2714,13 +2776,33 @@ KERNEL OCC(3) void r_substep_zx(const Src<T> S, int to_bx1)
     // rkstep terms (fx, fy, fz, curlb) of plane x - 1 at (y) and (y-1), and
     // of the lane below
     double F0x[4] = {0, 0, 0, 0}, Fmx[4] = {0, 0, 0, 0}, pF0x[4] = {0, 0, 0, 0}, pFmx[4] = {0, 0, 0, 0};
+    // The loads of a position are issued one position ahead (n*; the last
+    // position reloads its own), so their latency overlaps the previous
+    // position's arithmetic; the x deltas of the march are lane values.
+    double nbx_m = IX_[c - sy], nbx_0 = IX_[c], nbx_p = IX_[c + sy];
+    double nby_m = IY_[c + sx - sy], nby_0 = IY_[c + sx];
+    double nbz_m = IZ_[c + sx - sy], nbz_0 = IZ_[c + sx], nbz_p = IZ_[c + sx + sy];
+    const double dxc_row = x_row(A.dxc, Z.ix0, nx + 2), dxb_row = x_row(A.dxb, Z.ix0, nx + 2);
     for (int ix = Z.ix0; ix <= Z.ix1; ++ix, c += sx) {
-        const double bx_m = IX_[c - sy], bx_0 = IX_[c], bx_p = IX_[c + sy];
+        const double bx_m = nbx_m, bx_0 = nbx_0, bx_p = nbx_p;
+        const double byN_m = nby_m, byN_0 = nby_0;
+        const double bzN_m = nbz_m, bzN_0 = nbz_0, bzN_p = nbz_p;
+        const double dxc = x_at(dxc_row, ix - Z.ix0), dxb = x_at(dxb_row, ix - Z.ix0);
+        const double rho = RHO[c], ee = EE[c];
+        {
+            const int n = ix < Z.ix1 ? c + sx : c;
+            nbx_m = IX_[n - sy];
+            nbx_0 = IX_[n];
+            nbx_p = IX_[n + sy];
+            nby_m = IY_[n + sx - sy];
+            nby_0 = IY_[n + sx];
+            nbz_m = IZ_[n + sx - sy];
+            nbz_0 = IZ_[n + sx];
+            nbz_p = IZ_[n + sx + sy];
+        }
         const double bx_mz = lane_next(bx_m), bx_0z = lane_next(bx_0), bx_pz = lane_next(bx_p);
-        const double byN_m = IY_[c + sx - sy], byN_0 = IY_[c + sx];
         const double byN_mz = lane_next(byN_m), byN_0z = lane_next(byN_0);
-        const double bzN_m = IZ_[c + sx - sy], bzN_0 = IZ_[c + sx], bzN_p = IZ_[c + sx + sy];
-        const QDiv qx = qdiv_of(DXC(ix));
+        const QDiv qx = qdiv_of(dxc);
         double F0[4], Fm[4];
         rk_vertex<T>(bz_0, bz_p, bzN_0, bzN_p, by_0, by_0z, byN_0, byN_0z, bx_0, bx_0z, bx_p, bx_pz, qx, qy0, qz, P,
                      F0);
@@ -2734,21 +2816,19 @@ KERNEL OCC(3) void r_substep_zx(const Src<T> S, int to_bx1)
         if (Z.wz && ix > Z.ix0) {
             // fx = [0], fy = [1], fz = [2], curlb = [3]; c, c-sy, c-1, c-sy-1 at
             // this x (F0, Fm, pF0, pFm) and at x - 1 (F0x, Fmx, pF0x, pFmx)
-            if (iy >= 1 && iz >= 1) {
-                double area = DYB(iy) * DZB(iz);
-                OX_[c] = bx_0 + ddiv((F0[2] - Fm[2] + pF0[2] - pFm[2] - F0[1] + pF0[1] - Fm[1] + pFm[1]) * dt, area);
-            }
+            if (iy >= 1 && iz >= 1)
+                OX_[c] = bx_0 + ddiv((F0[2] - Fm[2] + pF0[2] - pFm[2] - F0[1] + pF0[1] - Fm[1] + pFm[1]) * dt, area_x);
             if (ix >= 1 && iz >= 1) {
-                double area = DXB(ix) * DZB(iz);
+                double area = dxb * dzb;
                 OY_[c] = by_0 + ddiv((F0[0] - pF0[0] + F0x[0] - pF0x[0] - F0[2] + F0x[2] - pF0[2] + pF0x[2]) * dt, area);
             }
             if (ix >= 1 && iy >= 1) {
-                double area = DXB(ix) * DYB(iy);
+                double area = dxb * dyb;
                 OZ_[c] = bz_0 + ddiv((F0[1] - F0x[1] + Fm[1] - Fmx[1] - F0[0] + Fm[0] - F0x[0] + Fmx[0]) * dt, area);
             }
             if (ix >= 1 && iy >= 1 && iz >= 1 && ix <= nx && iy <= ny && iz <= nz) {
                 double sum_curlb = F0[3] + F0x[3] + Fm[3] + Fmx[3] + pF0[3] + pF0x[3] + pFm[3] + pFmx[3];
-                EE[c] += ddiv(sum_curlb * dt, 8.0 * RHO[c]);
+                EE[c] = ee + ddiv(sum_curlb * dt, 8.0 * rho);
             }
         }
         for (int q = 0; q < 4; ++q) {
