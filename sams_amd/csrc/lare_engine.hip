@@ -1383,21 +1383,26 @@ KERNEL OCC(5) void f_cty_mass(const Src<T> S)
         const int d = d0 + k;
         if (d > R.y1) break;
         const int c = at(d);
-        double da2, db2;
-        const double fa3 = fval(Ba, at(d + 2), sx, da2);
-        const double fb3 = fval(Bb, at(d + 2), 1, db2);
-        const double cvc_p = A.cvc1[at(d + 1)], cv1_p = A.cv1[at(d + 1)];
-        const double ro3 = RHO[at(d + 2)];
-        const double w0 = W[c];
+        // every load of the position, issued before any is used (ix >= 0
+        // and iz >= 0 over this kernel's range, so c - sx - 1 is in the box)
+        const int p2 = at(d + 2), p1 = at(d + 1);
+        const double ba2 = Ba[p2], dba2 = db1[p2], dba2t = db1[p2 + sx], bb2 = Bb[p2], dbb2t = db1[p2 + 1];
+        const double cvc_p = A.cvc1[p1], cv1_p = A.cv1[p1];
+        const double ro3 = RHO[p2];
+        const double w0 = W[c], w_z = W[c - 1], w_x = W[c - sx], w_xz = W[c - sx - 1];
+        __builtin_amdgcn_sched_barrier(0);
+        // fval at d + 2
+        const double da2 = (dba2 + dba2t) * 0.5, db2 = (dba2 + dbb2t) * 0.5;
+        const double fa3 = ddiv(ba2, da2), fb3 = ddiv(bb2, db2);
         if (ct_xz && d >= 0 && d <= ny) {
-            const double va = (w0 + W[c - 1]) * 0.5, vb = (w0 + W[c - sx]) * 0.5;
+            const double va = (w0 + w_z) * 0.5, vb = (w0 + w_x) * 0.5;
             FA[c] = ct_flux_v(fa[0], fa[1], fa[2], fa3, da[0], da[1], va, dt, cvc_m, cvc_0, cvc_p, cv1_0, cv1_p);
             FB[c] = ct_flux_v(fb[0], fb[1], fb[2], fb3, db[0], db[1], vb, dt, cvc_m, cvc_0, cvc_p, cv1_0, cv1_p);
         }
         if (mf_xz) {
             double dmf = 0.0;
             if (d >= 0 && d <= ny) {
-                const double v_advect = (W[c - sx] + w0 + W[c - sx - 1] + W[c - 1]) * 0.25;  // face_v<1>
+                const double v_advect = (w_x + w0 + w_xz + w_z) * 0.25;  // face_v<1>
                 const double o_v = v_advect * dt * area_m;
                 dmf = mass_flux_v(ro[0], ro[1], ro[2], ro3, v_advect, o_v, cvc_m, cvc_0, cvc_p, cv1_0, cv1_p);
             }
@@ -1490,30 +1495,44 @@ KERNEL OCC(3) void f_ctx_fused(const Src<T> S)
     const double dt = lare::gbl(A.s)->dt;
     auto clx = [&](int x) { return x < -2 ? -2 : (x > nx + 2 ? nx + 2 : x); };
     const int ym = iy - 1 < -2 ? -2 : iy - 1, yp = iy + 1 > ny + 2 ? ny + 2 : iy + 1;
+    const int sy = G.sy, dyp = (yp - iy) * sy, dym = (ym - iy) * sy;
     const Fld<T> db1 = A.db1;
-    const double dzb = DZB(iz), dyb = DYB(iy);
+    const double dzb = DZB(iz), dyb = DYB(iy), axx = dyb * dzb;  // DXAB(iy, iz), b_area<0>
+    // DXB along the march: lane l holds DXB(clx(ix0 - 1 + l))
+    const int xr0 = Z.ix0 - 1;
+    const double dxb_row = lare::gbl(A.dxb)[clx(xr0 + ((int)threadIdx.x & 63)) + 2];
     // f = B * area / db and db along x at x+k, for the by flux on rows y
     // (a) and y-1 (m) and the bz flux on row y (b); area: dyab = dxb * dzb,
-    // dzab = dxb * dyb (remap.cpp:46-50)
-    auto col = [&](int xq, double &fa, double &da, double &sa, double &fm, double &dm, double &fb, double &db,
-                   double &sb) {
-        const int x = clx(xq);
-        const int c = IX(x, iy, iz);
-        const double d0 = db1[c], dbz = lane_next(d0);
-        da = (d0 + db1[IX(x, yp, iz)]) * 0.5;
-        dm = (db1[IX(x, ym, iz)] + d0) * 0.5;
+    // dzab = dxb * dyb (remap.cpp:46-50).  The loads of a column (colv) are
+    // issued together with the position's other loads, before any of them
+    // is used.
+    struct ColV {
+        double d0, dp, dm, by, bym, bz;
+    };
+    auto col_load = [&](int xq) {
+        const int c = IX(clx(xq), iy, iz);
+        return ColV{db1[c], db1[c + dyp], db1[c + dym], BY[c], BY[c + dym], BZ[c]};
+    };
+    auto col = [&](const ColV &v, int xq, double &fa, double &da, double &sa, double &fm, double &dm, double &fb,
+                   double &db, double &sb) {
+        const double d0 = v.d0, dbz = lane_next(d0);
+        da = (d0 + v.dp) * 0.5;
+        dm = (v.dm + d0) * 0.5;
         db = (d0 + dbz) * 0.5;
-        const double dxb = DXB(x);
-        sa = BY[c] * (dxb * dzb);
+        const double dxb = x_at(dxb_row, xq - xr0);
+        sa = v.by * (dxb * dzb);
         fa = ddiv(sa, da);
-        fm = ddiv(BY[IX(x, ym, iz)] * (dxb * dzb), dm);
-        sb = BZ[c] * (dxb * dyb);
+        fm = ddiv(v.bym * (dxb * dzb), dm);
+        sb = v.bz * (dxb * dyb);
         fb = ddiv(sb, db);
     };
     double fa[3], da[2], sa[2], fm[3], dmm[2], fb[3], dbb[2], sb[2], t[5];
-    col(Z.ix0 - 1, fa[0], t[0], t[1], fm[0], t[2], fb[0], t[3], t[4]);  // distinct outputs: col writes them in turn
-    col(Z.ix0, fa[1], da[0], sa[0], fm[1], dmm[0], fb[1], dbb[0], sb[0]);
-    col(Z.ix0 + 1, fa[2], da[1], sa[1], fm[2], dmm[1], fb[2], dbb[1], sb[1]);
+    {
+        const ColV v0 = col_load(Z.ix0 - 1), v1 = col_load(Z.ix0), v2 = col_load(Z.ix0 + 1);
+        col(v0, Z.ix0 - 1, fa[0], t[0], t[1], fm[0], t[2], fb[0], t[3], t[4]);  // distinct outputs: col writes them in turn
+        col(v1, Z.ix0, fa[1], da[0], sa[0], fm[1], dmm[0], fb[1], dbb[0], sb[0]);
+        col(v2, Z.ix0 + 1, fa[2], da[1], sa[1], fm[2], dmm[1], fb[2], dbb[1], sb[1]);
+    }
     auto ca = [&](int x, int y) { return A.cvc1[IX(clx(x), y, iz)]; };
     auto cv = [&](int x, int y) { return A.cv1[IX(clx(x), y, iz)]; };
     double cva_m = ca(Z.ix0 - 1, iy), cva_0 = ca(Z.ix0, iy), cvm_m = ca(Z.ix0 - 1, ym), cvm_0 = ca(Z.ix0, ym);
@@ -1523,11 +1542,16 @@ KERNEL OCC(3) void f_ctx_fused(const Src<T> S)
     double ro[3] = {RHO[IX(clx(Z.ix0 - 1), iy, iz)], RHO[IX(clx(Z.ix0), iy, iz)], RHO[IX(clx(Z.ix0 + 1), iy, iz)]};
     for (int ix = Z.ix0; ix <= Z.ix1; ++ix) {
         const int c = IX(ix, iy, iz);
-        double fa3, da2, sa2, fm3, dm2, fb3, db2, sb2;
-        col(ix + 2, fa3, da2, sa2, fm3, dm2, fb3, db2, sb2);
+        // every load of the position, issued before any is used
+        const ColV v3 = col_load(ix + 2);
         const double cva_p = ca(ix + 1, iy), cvm_p = ca(ix + 1, ym), c1a_p = cv(ix + 1, iy), c1m_p = cv(ix + 1, ym);
         const int xc = clx(ix);
         const double w0 = VX1[IX(xc, iy, iz)], wm = VX1[IX(xc, ym, iz)];
+        const double ro3 = RHO[IX(clx(ix + 2), iy, iz)];
+        const double bxc = BX[c];
+        __builtin_amdgcn_sched_barrier(0);
+        double fa3, da2, sa2, fm3, dm2, fb3, db2, sb2;
+        col(v3, ix + 2, fa3, da2, sa2, fm3, dm2, fb3, db2, sb2);
         const double w0z = lane_prev(w0), wmz = lane_prev(wm);
         // by flux (x_by: v = (vx1[c] + vx1[c-1]) / 2) at (y) and (y-1);
         // bz flux (v = (vx1[c] + vx1[c-sy]) / 2) at (y)
@@ -1538,14 +1562,13 @@ KERNEL OCC(3) void f_ctx_fused(const Src<T> S)
         const double Fb = ct_flux_v(fb[0], fb[1], fb[2], fb3, dbb[0], dbb[1], (w0 + wm) * 0.5, dt, cva_m, cva_0,
                                     cva_p, c1a_0, c1a_p);
         const double Fbz = lane_prev(Fb);
-        const double ro3 = RHO[IX(clx(ix + 2), iy, iz)];
         if (Z.wz && ix > Z.ix0) {
             // R5: the x mass flux over its range (dm = 0 on the ghost faces)
             if (ix >= -1 && ix <= nx + 2 && iy >= 0 && iy <= ny + 1 && iz >= 0 && iz <= nz + 1) {
                 double dmf = 0.0;
                 if (ix >= 0 && ix <= nx) {
                     const double v_advect = (w0 + wm + w0z + wmz) * 0.25;  // face_v<0>
-                    const double o_v = v_advect * dt * DXAB(iy, iz);
+                    const double o_v = v_advect * dt * axx;
                     dmf = mass_flux_v(ro[0], ro[1], ro[2], ro3, v_advect, o_v, cva_m, cva_0, cva_p, c1a_0, c1a_p);
                 }
                 DM[c] = dmf;
@@ -1554,7 +1577,7 @@ KERNEL OCC(3) void f_ctx_fused(const Src<T> S)
             const bool y1 = iy >= 1 && iy <= ny, y0 = iy >= 0 && iy <= ny;
             const bool z1 = iz >= 1 && iz <= nz, z0 = iz >= 0 && iz <= nz;
             const bool ebx = iy >= -1 && iz >= -1, eby = ix >= -1 && iz >= -1, ebz = ix >= -1 && iy >= -1;
-            double bx = ebx ? BX[c] * b_area<0>(A, ix, iy, iz) : 0.0;
+            double bx = ebx ? bxc * axx : 0.0;
             double by = eby ? sa[0] : 0.0;
             double bz = ebz ? sb[0] : 0.0;
             if (x1 && y0 && z1) by = by - Fa + Fa_x;
@@ -1629,32 +1652,38 @@ KERNEL OCC(4) void f_ctz_fused(const Src<T> S)
         return ct_flux_v(lane_prev(f), f, fp, lane_next(fp), d, lane_next(d), (w + wo) * 0.5, dt, lane_prev(cvc), cvc,
                          lane_next(cvc), cv1, lane_next(cv1));
     };
+    const double dyb = DYB(iy), dzb = DZB(iz), axx = dyb * dzb;  // b_area<0>
+    const double dxb_row = lare::gbl(A.dxb)[clx(Z.ix0 + ((int)threadIdx.x & 63)) + 2];  // DXB(clx(ix0 + lane))
     double w_xm = W[IX(clx(Z.ix0 - 1), iy, iz)], wm_xm = W[IX(clx(Z.ix0 - 1), ym, iz)];
     double Fa_x = 0.0;
     for (int ix = Z.ix0; ix <= Z.ix1; ++ix) {
         const int x = clx(ix);
         const int c = IX(x, iy, iz);
-        const double d0 = db1[c];
-        const double w0 = W[c], wm = W[IX(x, ym, iz)];
-        const double cvc0 = A.cvc1[c], cv10 = A.cv1[c];
-        const double bx0 = B1x[c], by0 = B1y[c];
-        // bx flux (tw = sx, ow = -sy); by flux (tw = sy, ow = -sx) at (y)
-        const double Fa = zflux(bx0, (d0 + db1[IX(clx(ix + 1), iy, iz)]) * 0.5, w0, wm, cvc0, cv10);
-        const double Fb = zflux(by0, (d0 + db1[IX(x, yp, iz)]) * 0.5, w0, w_xm, cvc0, cv10);
-        // by flux at (y-1)
         const int cm = IX(x, ym, iz);
-        const double Fbm = zflux(B1y[cm], (db1[cm] + d0) * 0.5, wm, wm_xm, A.cvc1[cm], A.cv1[cm]);
+        // every load of the position, issued before any is used
+        const double d0 = db1[c], d_xp = db1[IX(clx(ix + 1), iy, iz)], d_yp = db1[IX(x, yp, iz)], d_m = db1[cm];
+        const double w0 = W[c], wm = W[cm];
+        const double cvc0 = A.cvc1[c], cv10 = A.cv1[c], cvcm = A.cvc1[cm], cv1m = A.cv1[cm];
+        const double bx0 = B1x[c], by0 = B1y[c], bym = B1y[cm], bz0 = B1z[c];
+        const double r0 = RHO[c];
+        __builtin_amdgcn_sched_barrier(0);
+        // bx flux (tw = sx, ow = -sy); by flux (tw = sy, ow = -sx) at (y)
+        const double Fa = zflux(bx0, (d0 + d_xp) * 0.5, w0, wm, cvc0, cv10);
+        const double Fb = zflux(by0, (d0 + d_yp) * 0.5, w0, w_xm, cvc0, cv10);
+        // by flux at (y-1)
+        const double Fbm = zflux(bym, (d_m + d0) * 0.5, wm, wm_xm, cvcm, cv1m);
         const double Faz = lane_prev(Fa), Fbz = lane_prev(Fb);
         // R5: the z mass flux from rho (current state copy) at z-1..z+2
-        const double r0 = RHO[c], rp = lane_next(r0);
+        const double rp = lane_next(r0);
         const double rm = lane_prev(r0), rp2 = lane_next(rp);
         const double cvc_m = lane_prev(cvc0), cvc_p = lane_next(cvc0), cv1_p = lane_next(cv10);
+        const double dxb = x_at(dxb_row, ix - Z.ix0);  // DXB(ix) wherever a lane writes
         if (Z.wz && ix > Z.ix0) {
             if (ix >= 0 && ix <= nx + 1 && iy >= 0 && iy <= ny + 1 && iz >= -1 && iz <= nz + 2) {
                 double dmf = 0.0;
                 if (iz >= 0 && iz <= nz) {
                     const double v_advect = (w_xm + w0 + wm_xm + wm) * 0.25;  // face_v<2>
-                    const double o_v = v_advect * dt * DZAB(ix, iy);
+                    const double o_v = v_advect * dt * (dxb * dyb);  // DZAB(ix, iy)
                     dmf = mass_flux_v(rm, r0, rp, rp2, v_advect, o_v, cvc_m, cvc0, cvc_p, cv10, cv1_p);
                 }
                 DM[c] = dmf;
@@ -1665,15 +1694,16 @@ KERNEL OCC(4) void f_ctz_fused(const Src<T> S)
             const bool ebx = iy >= -1 && iz >= -1, eby = ix >= -1 && iz >= -1, ebz = ix >= -1 && iy >= -1;
             double bx = ebx ? bx0 : 0.0;
             double by = eby ? by0 : 0.0;
-            double bz = ebz ? (double)B1z[c] : 0.0;
+            double bz = ebz ? bz0 : 0.0;
             if (x0 && y1 && z1) bx = bx - Fa + Faz;
             const bool bzr = x1 && y1 && z0;
             if (bzr) bz = bz + Fa - Fa_x;
             if (x1 && y0 && z1) by = by - Fb + Fbz;
             if (bzr) bz = bz + Fb - Fbm;
-            if (ebx) BX[c] = ddiv(bx, b_area<0>(A, ix, iy, iz) + DBL_EPSILON);
-            if (eby) BY[c] = ddiv(by, b_area<1>(A, ix, iy, iz) + DBL_EPSILON);
-            if (ebz) BZ[c] = ddiv(bz, b_area<2>(A, ix, iy, iz) + DBL_EPSILON);
+            // b_area<0..2>: DXAB(iy, iz), DYAB(ix, iz), DZAB(ix, iy)
+            if (ebx) BX[c] = ddiv(bx, axx + DBL_EPSILON);
+            if (eby) BY[c] = ddiv(by, dxb * dzb + DBL_EPSILON);
+            if (ebz) BZ[c] = ddiv(bz, dxb * dyb + DBL_EPSILON);
         }
         Fa_x = Fa;
         w_xm = w0;
@@ -1960,9 +1990,11 @@ KERNEL OCC(3) void g_visc_dt_zx(const Src<T> S)
     const int sx = G.sx, sy = G.sy;
     const int iy = Z.iy, iz = Z.iz;
     const Fld<T> V3[3] = {VX, VY, VZ};
-    const double dy = DYB(iy), dz = DZB(iz);
+    const double dy = DYB(iy), dz = DZB(iz), dyp = DYB(iy + 1), dzp = DZB(iz + 1);
     const bool fz = iy <= G.ny && iz <= G.nz;  // force and dt range, y and z
-    const double ax = 0.25 * DXAB(iy, iz);
+    const double ax = 0.25 * (dy * dz);  // DXAB(iy, iz)
+    // DXB along the march: lane l holds DXB(ix0 + l)
+    const double dx_row = x_row(A.dxb, Z.ix0, G.nx + 2);
     int c = IX(Z.ix0, iy, iz);
     // columns at own z: O = (x, y), Om = (x, y-1), M = (x-1, y), Mm = (x-1, y-1)
     double M[3] = {0, 0, 0}, Mm[3] = {0, 0, 0};
@@ -1984,6 +2016,7 @@ KERNEL OCC(3) void g_visc_dt_zx(const Src<T> S)
         const double al1 = A.alpha1[c], al1y = al1y_n, al2 = A.alpha2[c], al2y = A.alpha2[c + sy];
         const double al3 = A.alpha3[c], al3y = A.alpha3[c - sy];
         al1y_n = A.alpha1[c + sx + sy];
+        __builtin_amdgcn_sched_barrier(0);
         const double al1p = lane_next(al1), al1yp = lane_next(al1y), al2p = lane_next(al2);
         const double al1ynp = lane_next(al1y_n), al2yp = lane_next(al2y), al3p = lane_next(al3);
 #define ZSQD(p, q) \
@@ -1994,26 +2027,32 @@ KERNEL OCC(3) void g_visc_dt_zx(const Src<T> S)
         const double a1 = lane_prev(a5), a2 = lane_prev(a6), a3 = lane_prev(a7), a4 = lane_prev(a8);
         const double vxbp = (O[0] + Om[0] + pO[0] + pOm[0]) * ax;
         const double fO[3] = {lane_next(O[0]), lane_next(O[1]), lane_next(O[2])};
+        const double dx = x_at(dx_row, ix - Z.ix0), dxp = x_at(dx_row, ix + 1 - Z.ix0);
         if (Z.wz && ix > Z.ix0) {
             double pv = 0.0;
             pv = pmax(pv, -al1 * sqrt(a1));
             pv = pmax(pv, -al2 * sqrt(a2));
             pv = pmax(pv, -al3 * sqrt(a9));
-            const double dx = DXB(ix);
             double vh = -0.25 * dy * dz * al1 * a1 - 0.25 * dx * dz * al2 * a2 - 0.25 * dy * dz * al1y * a3 -
                         0.25 * dx * dz * al2_m * a4 - 0.25 * dy * dz * al1p * a5 - 0.25 * dx * dz * al2p * a6 -
                         0.25 * dy * dz * al1yp * a7 - 0.25 * dx * dz * al2p_m * a8 - 0.25 * dx * dy * al3 * a9 -
                         0.25 * dx * dy * al3_m * a10 - 0.25 * dx * dy * al3y_m * a11 - 0.25 * dx * dy * al3y * a12;
-            const double cv = CV(ix, iy, iz);
+            const double cv = dx * (dy * dz);  // CV(ix, iy, iz)
             A.visc_heat[c] = ddiv(vh, cv);
             if (fz && ix <= G.nx) {
+                // the cell's state for the time-step bound, loaded before the
+                // viscous forces and first used after them
+                const double rho_c = RHO[c], ee_c = EE[c], ei_c = EI[c], bx_c = BX[c], by_c = BY[c], bz_c = BZ[c];
+                __builtin_amdgcn_sched_barrier(0);
                 double b1 = al1yp * dy * dz;
                 double b2 = al1ynp * dy * dz;
                 double b3 = al2p * dx * dz;
                 double b4 = al2yp * dx * dz;
                 double b5 = al3 * dx * dy;
                 double b6 = al3p * dx * dy;
-                double cvv = 0.125 * CV8SUM(ix, iy, iz);
+                // CV8SUM(ix, iy, iz)
+                double cvv = 0.125 * (dx * (dy * dz) + dxp * (dy * dz) + dx * (dyp * dz) + dxp * (dyp * dz) +
+                                      dx * (dy * dzp) + dxp * (dy * dzp) + dx * (dyp * dzp) + dxp * (dyp * dzp));
                 double fv[3];
                 for (int k = 0; k < 3; ++k)
                     fv[k] = ddiv(b1 * (O[k] - M[k]) + b2 * (O[k] - N[k]) + b3 * (O[k] - Om[k]) + b4 * (O[k] - Yp[k]) +
@@ -2021,15 +2060,16 @@ KERNEL OCC(3) void g_visc_dt_zx(const Src<T> S)
                 A.fxv[c] = fv[0];
                 A.fyv[c] = fv[1];
                 A.fzv[c] = fv[2];
-                double rho0 = pmax(RHO[c], DBL_EPSILON);
-                double press = PE_OF(c) + PI_OF(c);
+                __builtin_amdgcn_sched_barrier(0);
+                double rho0 = pmax(rho_c, DBL_EPSILON);
+                double press = (gas_gamma - 1.0) * rho_c * ee_c + (gas_gamma - 1.0) * rho_c * ei_c;  // PE_OF + PI_OF
                 double cs2 = ddiv(gas_gamma * press, rho0);
-                double w1 = ddiv(ddiv(BX[c] * BX[c] + BY[c] * BY[c] + BZ[c] * BZ[c], mu0), rho0);
+                double w1 = ddiv(ddiv(bx_c * bx_c + by_c * by_c + bz_c * bz_c, mu0), rho0);
                 double c_visc2 = ddiv(pv, rho0);
                 double length = pmin(pmin(dx, dy), dz);
                 double dt1 = ddiv(length, sqrt(c_visc2) + sqrt(cs2 + w1 + c_visc2));
-                double ay = 0.25 * DYAB(ix, iz);
-                double az = 0.25 * DZAB(ix, iy);
+                double ay = 0.25 * (dx * dz);  // DYAB(ix, iz)
+                double az = 0.25 * (dx * dy);  // DZAB(ix, iy)
                 double vxbm = vxbp_m;
                 double vybm = (Om[1] + Mm[1] + pOm[1] + pMm[1]) * ay;
                 double vybp = (O[1] + M[1] + pO[1] + pM[1]) * ay;
@@ -2517,21 +2557,16 @@ KERNEL OCC(3) void g_correct_zx(const Src<T> S)
     double a2_m = 0.0, a9_m = 0.0, a12_m = 0.0, vxb_m = 0.0;
     double al2_m = 0.0, al2p_m = 0.0, al3_m = 0.0, al3y_m = 0.0;
     int c = IX(Z.ix0, iy, iz);
-    // the velocities and alphas of a position are loaded one position
-    // ahead (the last position reloads its own); the cell's energies are
-    // loaded at the start of its position and first used after its
-    // viscous heating and velocity gradients
-    double nO[6], nOm[6], nal[5];
+    // the velocities of a position are loaded one position ahead (the last
+    // position reloads its own); the alphas and the cell's energies are
+    // loaded at the start of its position, the energies first used after
+    // its viscous heating and velocity gradients
+    double nO[6], nOm[6];
     auto fetch = [&](int n) {
         for (int k = 0; k < 6; ++k) {
             nO[k] = V6[k][n];
             nOm[k] = V6[k][n - sy];
         }
-        nal[0] = A.alpha1[n];
-        nal[1] = A.alpha1[n + sy];
-        nal[2] = A.alpha2[n];
-        nal[3] = A.alpha3[n];
-        nal[4] = A.alpha3[n - sy];
     };
     fetch(c);
     for (int ix = Z.ix0; ix <= Z.ix1; ++ix, c += sx) {
@@ -2540,7 +2575,8 @@ KERNEL OCC(3) void g_correct_zx(const Src<T> S)
             O[k] = nO[k];
             Om[k] = nOm[k];
         }
-        const double al1 = nal[0], al1y = nal[1], al2 = nal[2], al3 = nal[3], al3y = nal[4];
+        const double al1 = A.alpha1[c], al1y = A.alpha1[c + sy], al2 = A.alpha2[c], al3 = A.alpha3[c],
+                     al3y = A.alpha3[c - sy];
         const double rho = RHO[c], ee = EE[c], ei = EI[c], pe = A.p_e[c], pi = A.p_i[c];
         fetch(ix < Z.ix1 ? c + sx : c);
         __builtin_amdgcn_sched_barrier(0);
