@@ -1134,23 +1134,33 @@ KERNEL void f_update_zx(const Src<T> S)
     // dm and cv2 of rows y and y+1; w of column x - 1, rows y and y-1
     double rho_x = 0.0, cv1_x = 0.0, p_x[2] = {0.0, 0.0}, d_x[2] = {0.0, 0.0}, s_x[2] = {0.0, 0.0};
     double w_xm = 0.0, w_xmm = 0.0;
+    // face areas: DYAC(ix, iz) = DXB(ix) DZC(iz), DZAB(ix, iy) = DXB(ix) DYB(iy),
+    // with DXB along the march in lane registers (lane l: DXB(ix0 + l))
+    const double dzc = DZC(iz), dyb = DYB(iy);
+    const double dxb_row = x_row(A.dxb, Z.ix0, G.nx + 2);
     int c = IX(Z.ix0, iy, iz);
     for (int ix = Z.ix0; ix <= Z.ix1; ++ix, c += sx) {
-        // column x + 1, rows y and y + 1; w at (x, y) and (x, y - 1)
+        // every load of the position, issued before any is used:
+        // column x + 1, rows y and y + 1; w at (x, y) and (x, y - 1); the
+        // cell's own cvc1, v and energies
         const double rho_a = R1[c + sx], cv1_a = C1[c + sx];
-        const double p_a[2] = {rho_a * cv1_a, R1[c + sx + sy] * C1[c + sx + sy]};
+        const double r_b = R1[c + sx + sy], c_b = C1[c + sx + sy];
         const double d_a[2] = {M[c + sx], M[c + sx + sy]};
         const double s_a[2] = {C2[c + sx], C2[c + sx + sy]};
         const double w_0 = W[c], w_0m = W[c - sy];
+        const double cvc_c = A.cvc1[c];
+        const double F[3] = {VX[c], VY[c], VZ[c]};
+        const double E[2] = {EE[c], EI[c]};
+        __builtin_amdgcn_sched_barrier(0);
+        const double p_a[2] = {rho_a * cv1_a, r_b * c_b};
+        const double dxb = x_at(dxb_row, ix - Z.ix0);
         if (ix > Z.ix0) {
             const double P4[4] = {p_x[0], p_a[0], p_x[1], p_a[1]};
             const double D4[4] = {d_x[0], d_a[0], d_x[1], d_a[1]};
             const double S4[4] = {s_x[0], s_a[0], s_x[1], s_a[1]};
-            const double rho_0 = rho_x, cv1_0 = cv1_x, cvc_0 = A.cvc1[c];
+            const double rho_0 = rho_x, cv1_0 = cv1_x, cvc_0 = cvc_c;
             const double fv = (w_xm + w_0 + w_xmm + w_0m) * 0.25;
-            const double F[3] = {VX[c], VY[c], VZ[c]};
-            const double E[2] = {EE[c], EI[c]};
-            const double area_m = DYAC(ix, iz), area_e = DZAB(ix, iy);
+            const double area_m = dxb * dzc, area_e = dxb * dyb;
             double sp = P4[0] + P4[1] + P4[2] + P4[3];
             double q = D4[0] + D4[1] + D4[2] + D4[3];
             double s2 = S4[0] + S4[1] + S4[2] + S4[3];

@@ -21,10 +21,8 @@
 //
 // The device context of a simulationData is rebuilt when the simulationData
 // at that address has other dimensions or other harness arrays than the
-// context was built from (a new simulation reusing the address); a context
-// is released by lare3d_hip_release or when its address gets a new
-// simulation, and otherwise goes with the process (at exit only the write
-// guard is removed, without HIP calls).
+// context was built from (a new simulation reusing the address), and every
+// context is released when the process exits (or by lare3d_hip_release).
 // Once the device holds the state, the package-private host arrays of
 // setup.cpp that only the step reads (metric, Lagrangian and remap work
 // arrays) are released (LARE3D_HIP_KEEP_HOST_ARRAYS=1 keeps them); the
@@ -200,15 +198,13 @@ namespace LARE
             g.ctx = nullptr;
         }
 
-        // At process exit only the write guard is taken down: the device
-        // contexts go with the process.  No HIP call is made from the exit
-        // handler, which is registered at the first bind before that bind's
-        // first HIP call, so it may run after teardown the HIP runtime
-        // registers when it initialises.
-        void releaseAtExit()
+        // every context, at process exit (registered at the first bind, after
+        // the HIP runtime is initialised, so this runs before its teardown)
+        void releaseAll()
         {
             for (auto &kv : grids())
-                releaseGuard(kv.second);
+                releaseGrid(kv.second);
+            grids().clear();
         }
 
         void check(int rc, const char *what)
@@ -492,7 +488,7 @@ namespace LARE
             static bool atexitRegistered = false;
             if (!atexitRegistered)
             {
-                std::atexit(releaseAtExit);
+                std::atexit(releaseAll);
                 atexitRegistered = true;
             }
             lare_desc desc{};
@@ -763,9 +759,8 @@ namespace LARE
     void lare3d_hip_download(simulationData &data) { download(data); }
 
     // Release the device context of a simulationData (its state is lost;
-    // download first to keep it).  A context is also released when a new
-    // simulation reuses the address; at process exit the device memory goes
-    // with the process.
+    // download first to keep it).  Contexts are also released at process
+    // exit and when a new simulation reuses the address.
     void lare3d_hip_release(simulationData &data)
     {
         auto it = grids().find(&data);
