@@ -2743,7 +2743,10 @@ KERNEL void r_b_energy(const Src<T> S)
 // one vertex from the B values around it: bzAB / byAB / bxAB is the
 // component at (x + A, y + B), the z suffix its value at z + 1.  The curlb
 // returned is rounded to the storage type, as stored by r_rkstep.
-template <class T, bool FAST>
+// ETA_VAR: some grid of the launch has eta_calc's current-dependent branch
+// (eta0 changes eta); otherwise eta is eta_background everywhere and the
+// current's magnitude is not formed.
+template <class T, bool FAST, bool ETA_VAR>
 __device__ __forceinline__ void rk_vertex_q(double bz00, double bz01, double bz10, double bz11, double by00,
                                             double by00z, double by10, double by10z, double bx00, double bx00z,
                                             double bx01, double bx01z, const QDiv &qx, const QDiv &qy,
@@ -2765,7 +2768,7 @@ __device__ __forceinline__ void rk_vertex_q(double bz00, double bz01, double bz1
     double jy = 0.5 * (jy1 + jy2);
     double jz = 0.5 * (jz1 + jz2);
     double eta = P.eta_background;
-    if (!P.eta_fixed) {
+    if (ETA_VAR && !P.eta_fixed) {
         double j_local = sqrt(jx * jx + jy * jy + jz * jz);
         eta = j_local > P.j_max ? P.eta_background + P.eta0 : P.eta_background;
     }
@@ -2774,18 +2777,18 @@ __device__ __forceinline__ void rk_vertex_q(double bz00, double bz01, double bz1
     f[2] = -jz * eta * qz.d * 0.5;
     f[3] = (double)(T)(eta * (jx * jx + jy * jy + jz * jz));
 }
-template <class T>
+template <class T, bool ETA_VAR>
 __device__ __forceinline__ void rk_vertex(double bz00, double bz01, double bz10, double bz11, double by00,
                                           double by00z, double by10, double by10z, double bx00, double bx00z,
                                           double bx01, double bx01z, const QDiv &qx, const QDiv &qy,
                                           const QDiv &qz, const Phys &P, double f[4])
 {
     bool slow = false;
-    rk_vertex_q<T, true>(bz00, bz01, bz10, bz11, by00, by00z, by10, by10z, bx00, bx00z, bx01, bx01z, qx, qy, qz, P,
-                         f, slow);
+    rk_vertex_q<T, true, ETA_VAR>(bz00, bz01, bz10, bz11, by00, by00z, by10, by10z, bx00, bx00z, bx01, bx01z, qx, qy,
+                                  qz, P, f, slow);
     if (__builtin_expect(slow, 0))
-        rk_vertex_q<T, false>(bz00, bz01, bz10, bz11, by00, by00z, by10, by10z, bx00, bx00z, bx01, bx01z, qx, qy,
-                              qz, P, f, slow);
+        rk_vertex_q<T, false, ETA_VAR>(bz00, bz01, bz10, bz11, by00, by00z, by10, by10z, bx00, bx00z, bx01, bx01z,
+                                       qx, qy, qz, P, f, slow);
 }
 
 // One resistive substep (r_rkstep then r_b_energy, lagran.cpp:490-590) as a
@@ -2796,7 +2799,7 @@ __device__ __forceinline__ void rk_vertex(double bz00, double bz01, double bz10,
 // are carried and those at z - 1 come from the lane below.  The same
 // arithmetic as r_rkstep and r_b_energy; B outside each component's update
 // range is left to the boundary fill that follows.
-template <class T, int K>
+template <class T, int K, bool ETA_VAR>
 KERNEL OCC(3) void r_substep_zx(const Src<T> S, int to_bx1)
 {
     const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
@@ -2850,10 +2853,10 @@ KERNEL OCC(3) void r_substep_zx(const Src<T> S, int to_bx1)
         const double byN_mz = lane_next(byN_m), byN_0z = lane_next(byN_0);
         const QDiv qx = qdiv_of(dxc);
         double F0[4], Fm[4];
-        rk_vertex<T>(bz_0, bz_p, bzN_0, bzN_p, by_0, by_0z, byN_0, byN_0z, bx_0, bx_0z, bx_p, bx_pz, qx, qy0, qz, P,
-                     F0);
-        rk_vertex<T>(bz_m, bz_0, bzN_m, bzN_0, by_m, by_mz, byN_m, byN_mz, bx_m, bx_mz, bx_0, bx_0z, qx, qym, qz, P,
-                     Fm);
+        rk_vertex<T, ETA_VAR>(bz_0, bz_p, bzN_0, bzN_p, by_0, by_0z, byN_0, byN_0z, bx_0, bx_0z, bx_p, bx_pz, qx, qy0,
+                              qz, P, F0);
+        rk_vertex<T, ETA_VAR>(bz_m, bz_0, bzN_m, bzN_0, by_m, by_mz, byN_m, byN_mz, bx_m, bx_mz, bx_0, bx_0z, qx, qym,
+                              qz, P, Fm);
         double pF0[4], pFm[4];
         for (int q = 0; q < 4; ++q) {
             pF0[q] = lane_prev(F0[q]);
@@ -3760,10 +3763,17 @@ void start_b_t(Group &gp, int substeps)
         // the last substep reads bx1..bz1, which then hold B at its start
         // (the copy of lagran.cpp:482).  An odd count starts from that copy.
         if (substeps % 2) EL(r_copy<T>, 6 * w, gp, 1, C);
+        // eta_calc's current-dependent branch only where some grid has it
+        bool eta_var = false;
+        for (int i : gp.res) {
+            const Grid &g = *gp.mem[i];
+            eta_var = eta_var || !(g.fp32 ? g.e32.p.eta_fixed : g.e64.p.eta_fixed);
+        }
         for (int i = 0; i < substeps; ++i) {
             const int to_bx1 = (substeps - 1 - i) % 2;
-            elaunch_(gp.zx_k >= 16 ? r_substep_zx<T, 16> : r_substep_zx<T, 8>, "r_substep", 9 * w, gp, 1,
-                     rs(0, 0, 0, 0, 0, 0), 0, kZX + 100 + zxk(gp), to_bx1);
+            void (*k)(Src<T>, int) = eta_var ? (gp.zx_k >= 16 ? r_substep_zx<T, 16, true> : r_substep_zx<T, 8, true>)
+                                             : (gp.zx_k >= 16 ? r_substep_zx<T, 16, false> : r_substep_zx<T, 8, false>);
+            elaunch_(k, "r_substep", 9 * w, gp, 1, rs(0, 0, 0, 0, 0, 0), 0, kZX + 100 + zxk(gp), to_bx1);
             ebcs<T>(gp, to_bx1 ? VBE_1 : VBE_0, -1, 1);
         }
     } else if (gp.n[1] > 0) {

@@ -45,10 +45,17 @@ struct QDiv {
     double d, y;
     bool ok;
 };
-__device__ __forceinline__ unsigned qdiv_expo(double v) { return ((unsigned)__double2hiint(v) >> 20) & 0x7ffu; }
+// |v| in [2^-300, 2^301): the biased exponent field in 723..1323.  Compared
+// as doubles (a NaN compares false, so it is out of the range like an
+// infinity; zeros and subnormals are below it).
+__device__ __forceinline__ bool qdiv_in_range(double v)
+{
+    const double a = __builtin_fabs(v);
+    return a >= 0x1p-300 && a < 0x1p301;
+}
 __device__ __forceinline__ QDiv qdiv_of(double d)
 {
-    const bool ok = d > 0.0 && qdiv_expo(d) - 723u <= 600u;
+    const bool ok = d > 0.0 && qdiv_in_range(d);
     return QDiv{d, rcp_nearest(d), ok};
 }
 // The quotient, with `slow` set where it may differ from a / d.
@@ -56,7 +63,7 @@ __device__ __forceinline__ double qdiv_f(double a, const QDiv &R, bool &slow)
 {
     const double m = a * R.y;
     const double q = fma(-fma(R.d, m, -a), R.y, m);
-    slow = slow || !R.ok || !(qdiv_expo(a) - 723u <= 600u || a == 0.0);
+    slow = slow || !R.ok || !(qdiv_in_range(a) || a == 0.0);
     return q;
 }
 __device__ __forceinline__ double qdiv(double a, const QDiv &R)
