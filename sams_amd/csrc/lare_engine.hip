@@ -3164,8 +3164,8 @@ __device__ __forceinline__ double bc3_val(const lare::BCVar3 &V, const E *p, con
 }
 
 // Workgroups per grid: each variable's ghost cells (the box lo..n+2 of
-// every dim less the cells 1..n-s of all three), 256 per workgroup, one
-// cell per lane: first the x slabs (all y, z), then the y slabs (x
+// every dim less the cells 1..n-s of all three), 256 kBC3Cells per
+// workgroup, kBC3Cells per lane: first the x slabs (all y, z), then the y slabs (x
 // inside), then the z slabs (x, y inside), z fastest.  A cell whose every
 // index lies outside the ghost ranges of edges with a condition is left
 // as it is.  Every value read is of a cell no fill rewrites, or one whose
@@ -3174,6 +3174,7 @@ __device__ __forceinline__ double bc3_val(const lare::BCVar3 &V, const E *p, con
 // the host takes the per-dim fills for a staggered line with a periodic
 // lower and a clamped upper edge, whose lower fill reads the point n that
 // the upper fill then changes.
+constexpr int kBC3Cells = 4;  // ghost cells per lane of e_bc3
 template <class T>
 KERNEL void e_bc3(const EDev<T> *devs, const lare::BCSet3 *sets, const int *boff, int ngrid, int nblk)
 {
@@ -3185,8 +3186,6 @@ KERNEL void e_bc3(const EDev<T> *devs, const lare::BCSet3 *sets, const int *boff
     int vi = 0;
     while (vi + 1 < S.nv && S.v[vi + 1].b0 <= lb) ++vi;
     const lare::BCVar3 &V = S.v[vi];
-    const int t = (lb - V.b0) * 256 + (int)threadIdx.x;
-    if (t >= V.nsh) return;
     const int n3[3] = {G.nx, G.ny, G.nz};
     int lo[3], ext[3], wl[3], wd[3], in[3];
     for (int d = 0; d < 3; ++d) {
@@ -3197,8 +3196,14 @@ KERNEL void e_bc3(const EDev<T> *devs, const lare::BCSet3 *sets, const int *boff
         in[d] = n3[d] - V.s[d];
     }
     auto slab = [&](int d, int a) { return a < wl[d] ? lo[d] + a : n3[d] + 1 - V.s[d] + (a - wl[d]); };
-    int x, y, z;
     const int cx = wd[0] * ext[1] * ext[2], cy = in[0] * wd[1] * ext[2];
+    // kBC3Cells ghost cells per lane, 256 apart, so the workgroup's set-up
+    // (its grid, variable and rules) serves 256 kBC3Cells cells
+    const int tb = (lb - V.b0) * (256 * kBC3Cells) + (int)threadIdx.x;
+    for (int r = 0; r < kBC3Cells; ++r) {
+    const int t = tb + 256 * r;
+    if (t >= V.nsh) break;
+    int x, y, z;
     if (t < cx) {
         const int a = t / (ext[1] * ext[2]), r = t - a * (ext[1] * ext[2]);
         x = slab(0, a);
@@ -3221,13 +3226,14 @@ KERNEL void e_bc3(const EDev<T> *devs, const lare::BCSet3 *sets, const int *boff
     bool filled = false;
     for (int d = 0; d < 3; ++d)
         filled = filled || (V.k0[d] && pos[d] <= 0) || (V.k1[d] && pos[d] >= n3[d] + 1 - V.s[d]);
-    if (!filled) return;
+    if (!filled) continue;
     if (V.f64) {
         double *p = (double *)V.p;
         p[IX(x, y, z)] = bc3_val<double, 3>(V, p, G, x, y, z);
     } else {
         T *p = (T *)V.p;
         p[IX(x, y, z)] = (T)bc3_val<T, 3>(V, p, G, x, y, z);
+    }
     }
 }
 
@@ -3635,7 +3641,7 @@ Group::BTab &btable3(Group &gp, int vg, int subset)
             }
             b.nsh = (int)(nsh - inner);
             b.b0 = nb;
-            nb += (b.nsh + 255) / 256;
+            nb += (b.nsh + 256 * kBC3Cells - 1) / (256 * kBC3Cells);
         }
         boff[m] = (int)total;
         total += nb;
