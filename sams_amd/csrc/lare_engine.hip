@@ -3591,9 +3591,9 @@ Group::BTab &btable(Group &gp, int vg, int dim, int subset)
 // whose fill carries values over from the other state copy (an edge
 // without a condition), or a staggered line with a periodic lower and a
 // clamped upper edge.
-Group::BTab &btable3(Group &gp, int vg, int subset)
+Group::BTab &btable3(Group &gp, int vg, int subset, int only_dim = -1)
 {
-    const int key = vg * 2 + subset;
+    const int key = (vg * 2 + subset) * 4 + (only_dim + 1);
     auto it = gp.btabs.find({key, 3});
     if (it != gp.btabs.end()) return it->second;
     Group::BTab t;
@@ -3619,14 +3619,17 @@ Group::BTab &btable3(Group &gp, int vg, int subset)
             bool none_edge = false;
             for (int d = 0; d < 3; ++d) {
                 b.s[d] = kStagE[u][d];
-                b.k0[d] = g.bck[u][d][0];
-                b.k1[d] = g.bck[u][d][1];
+                // one dimension's fill: the other dimensions have no condition
+                const bool on = only_dim < 0 || d == only_dim;
+                b.k0[d] = on ? g.bck[u][d][0] : 0;
+                b.k1[d] = on ? g.bck[u][d][1] : 0;
                 b.v0[d] = g.bcv[u][d][0];
                 b.v1[d] = g.bcv[u][d][1];
                 for (int j = 0; j < 3; ++j) {
                     b.m0[d][j] = g.bcm[u][d][0][j];
                     b.m1[d][j] = g.bcm[u][d][1][j];
                 }
+                if (!on) continue;
                 none_edge = none_edge || !edge_writes_all(g, u, d, 0) || !edge_writes_all(g, u, d, 1);
                 if (b.s[d] && b.k0[d] == 1 && b.k1[d] == 4) ok = false;
                 if (map_reads_written(g, u, d, n3[d])) ok = false;
@@ -3662,12 +3665,16 @@ Group::BTab &btable3(Group &gp, int vg, int subset)
 template <class T>
 void ebcs(Group &gp, int vg, int only_dim = -1, int subset = 0)
 {
-    if (only_dim < 0 && gp.bc_all) {
-        Group::BTab &t = btable3(gp, vg, subset);
+    // the fill of all three dimensions, or of one, as one pass over the ghost
+    // cells (e_bc3) where that pass equals the fills one dimension and edge
+    // at a time; otherwise those fills (e_bc)
+    if (gp.bc_all) {
+        Group::BTab &t = btable3(gp, vg, subset, only_dim);
         if (t.nblk >= 0) {
             if (gp.planning || !t.any || t.nblk == 0) return;
             Prof *pf = gp.prof;
-            Prof::Rec rec{"e_bc3", 0.0, nullptr, nullptr};
+            Prof::Rec rec{only_dim < 0 ? std::string("e_bc3") : std::string("e_bc<") + char('0' + only_dim) + ">", 0.0,
+                          nullptr, nullptr};
             if (pf && pf->on) {
                 (void)hipEventCreate(&rec.a);
                 (void)hipEventCreate(&rec.b);
