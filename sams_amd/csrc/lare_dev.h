@@ -129,9 +129,6 @@ template <class T>
 struct Fld {
     T *p;
     __device__ __forceinline__ Ref<T> operator[](int i) const { return Ref<T>{gbl(p) + i}; }
-    // a store of data no kernel reads again before it leaves the cache
-    // (the streaming hint of the store)
-    __device__ __forceinline__ void stream(int i, double v) const { __builtin_nontemporal_store((T)v, gbl(p) + i); }
 };
 
 template <class T>

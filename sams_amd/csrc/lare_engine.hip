@@ -301,17 +301,17 @@ KERNEL void f_prep_all(const Src<T> S)
     double avg[6], o[3];
     prep_avg(V, avg);
     prep_out<0>(A, ix, iy, iz, avg, dt, 1.0, 1.0, 1.0, o);
-    A.cv1.stream(c, o[0]);
-    A.cv2.stream(c, o[1]);
-    A.db1.stream(c, o[2]);
+    A.cv1[c] = o[0];
+    A.cv2[c] = o[1];
+    A.db1[c] = o[2];
     prep_out<1>(A, ix, iy, iz, avg, dt, 0.0, 1.0, 1.0, o);
-    A.alpha1.stream(c, o[0]);
-    A.alpha2.stream(c, o[1]);
-    A.alpha3.stream(c, o[2]);
+    A.alpha1[c] = o[0];
+    A.alpha2[c] = o[1];
+    A.alpha3[c] = o[2];
     prep_out<2>(A, ix, iy, iz, avg, dt, 0.0, 0.0, 1.0, o);
-    A.fxv.stream(c, o[0]);
-    A.fyv.stream(c, o[1]);
-    A.fzv.stream(c, o[2]);
+    A.fxv[c] = o[0];
+    A.fyv[c] = o[1];
+    A.fzv[c] = o[2];
     };
     ECELL(cell)
 }
