@@ -257,7 +257,7 @@ typedef struct lare_tune {
     int32_t zx_k;       /* cells per lane of the z-line x-march kernels: 8 or 16 */
     int32_t zline;      /* z-sweep kernels exchanging z-neighbours between lanes: 1, or 0 */
     int32_t order;      /* workgroup order within a grid: 1 band-major, 0 plane-major */
-    int32_t ct_fuse;    /* x and z sweeps' CT flux and update in one kernel (lane-walking): 1, or 0 */
+    int32_t ct_fuse;    /* CT flux and update in one kernel (lane-walking): 2 all sweeps, 1 x and z, 0 none */
     double ms_per_step; /* the chosen configuration's measured step time (autotune) */
 } lare_tune;
 /* Times one captured step of every candidate configuration (coordinate

@@ -142,16 +142,16 @@ int ensure_solo(lare_ctx *c)
 lare_tune default_tune()
 {
     lare::Group d;
-    return lare_tune{d.walk_mode, d.march_k, d.zx_k, d.zline ? 1 : 0, d.order, d.ct_fuse ? 1 : 0, 0.0};
+    return lare_tune{d.walk_mode, d.march_k, d.zx_k, d.zline ? 1 : 0, d.order, d.ct_fuse, 0.0};
 }
 lare_tune get_tune(const lare::Group &gp)
 {
-    return lare_tune{gp.walk_mode, gp.march_k, gp.zx_k, gp.zline ? 1 : 0, gp.order, gp.ct_fuse ? 1 : 0, 0.0};
+    return lare_tune{gp.walk_mode, gp.march_k, gp.zx_k, gp.zline ? 1 : 0, gp.order, gp.ct_fuse, 0.0};
 }
 bool valid_tune(const lare_tune &t)
 {
     return t.walk >= 0 && t.walk <= 2 && (t.march_k == 8 || t.march_k == 16) && (t.zx_k == 8 || t.zx_k == 16) &&
-           (t.zline == 0 || t.zline == 1) && (t.order == 0 || t.order == 1) && (t.ct_fuse == 0 || t.ct_fuse == 1);
+           (t.zline == 0 || t.zline == 1) && (t.order == 0 || t.order == 1) && t.ct_fuse >= 0 && t.ct_fuse <= 2;
 }
 void apply_tune(lare::Group &gp, const lare_tune &t)
 {
@@ -221,16 +221,16 @@ int tune_group(lare::Group &gp, hipStream_t stream, int steps, lare_tune *best_o
     double best_ms = 0.0;
     hipError_t e = measure(best, best_ms);
     // one field at a time, keeping whatever is fastest so far
-    const int walks[2] = {1, 2}, ks[2] = {8, 16}, flags[2] = {0, 1};
+    const int walks[2] = {1, 2}, ks[2] = {8, 16}, flags[2] = {0, 1}, fuses[3] = {0, 1, 2};
     for (int field = 0; field < 6 && e == hipSuccess; ++field) {
-        for (int v = 0; v < 2 && e == hipSuccess; ++v) {
+        for (int v = 0; v < (field == 5 ? 3 : 2) && e == hipSuccess; ++v) {
             lare_tune t = best;
             if (field == 0) t.walk = walks[v];
             if (field == 1) t.march_k = ks[v];
             if (field == 2) t.zx_k = ks[v];
             if (field == 3) t.zline = flags[v];
             if (field == 4) t.order = flags[v];
-            if (field == 5) t.ct_fuse = flags[v];
+            if (field == 5) t.ct_fuse = fuses[v];
             if (t.walk == best.walk && t.march_k == best.march_k && t.zx_k == best.zx_k && t.zline == best.zline &&
                 t.order == best.order && t.ct_fuse == best.ct_fuse)
                 continue;
@@ -376,7 +376,7 @@ PoolLayout pool_layout()
     return L;
 }
 
-// Engine arrays (lare::ArrE) of one grid in storage type T: 39 T arrays and
+// Engine arrays (lare::ArrE) of one grid in storage type T: 42 T arrays and
 // 6 FP64 flux arrays carved from the pool at p.
 template <class T>
 void carve_engine(lare::ArrE<T> &a, char *&p, size_t box, const PoolLayout &L)
@@ -394,12 +394,13 @@ void carve_engine(lare::ArrE<T> &a, char *&p, size_t box, const PoolLayout &L)
     lare::Fld<T> *ts[] = {&a.rho, &a.ee, &a.ei, &a.vx, &a.vy, &a.vz, &a.bx, &a.by, &a.bz, &a.vx1, &a.vy1,
                           &a.vz1, &a.cs, &a.cs_v, &a.rho_v, &a.alpha1, &a.alpha2, &a.alpha3, &a.visc_heat,
                           &a.fxv, &a.fyv, &a.fzv, &a.cv1, &a.bx1, &a.by1, &a.bz1, &a.p_e, &a.p_i, &a.eta,
-                          &a.curlb, &a.cv2, &a.cvc1, &a.db1, &a.rho2, &a.ee2, &a.ei2, &a.vx2, &a.vy2, &a.vz2};
+                          &a.curlb, &a.cv2, &a.cvc1, &a.db1, &a.rho2, &a.ee2, &a.ei2, &a.vx2, &a.vy2, &a.vz2,
+                          &a.bxy, &a.byy, &a.bzy};
     for (auto *q : ts) *q = tk();
     double **ds[] = {&a.dm, &a.flux_x, &a.flux_y, &a.flux_z, &a.fl[0], &a.fl[1]};
     for (double **q : ds) *q = dk();
 }
-const int kEngineT = 39, kEngineD = 6;
+const int kEngineT = 42, kEngineD = 6;
 
 // The one-kernel-per-reference-kernel set's view of the FP64 engine arrays.
 void alias_set0(lare::Arr &a, const lare::ArrE<double> &e)

@@ -149,6 +149,9 @@ struct ArrE {
     // constrained-transport fluxes
     Fld<T> cv2, cvc1, db1;
     double *fl[2];
+    // B's third copy: the fused y sweep reads B from bx1..bz1 (where the x
+    // sweep left it) and writes it here, for the z sweep to read back
+    Fld<T> bxy, byy, bzy;
     // the remap kinetic-energy correction's delta_ke (FP64; only with rke)
     double *dke;
     // 1D axis deltas, index i stored at [i + 2]

@@ -55,7 +55,7 @@ struct Group {
     int zx_k = 8;       // cells per lane of the z-line x-march kernels (0: one cell per lane)
     double walk_min_cells = 3.0e5;  // launches over fewer cells use one cell per lane
     int walk_mode = 0;              // lane-walking kernels: 0 by launch size, 1 always, 2 never
-    int ct_fuse = 1;                // x and z sweeps' CT flux and update in one kernel (lane-walking only)
+    int ct_fuse = 2;                // CT flux and update in one kernel (lane-walking only): 1 the x and z sweeps, 2 all three
     bool prep_all = true;           // the three sweeps' cell volumes and widths in one kernel
     bool bc_all = true;             // a variable group's fills of all three dimensions in one kernel
     // one captured step of the whole group

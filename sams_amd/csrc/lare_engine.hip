@@ -79,7 +79,8 @@ __device__ __forceinline__ int xcd_swizzle(int b, int nblk)
 // rho2 .. vz2 the other one; with bit 1 set, bx .. bz name bx1 .. bz1 (B's
 // second copy during the remap) and bx1 .. bz1 name B's own arrays; bits 2
 // and 3 name the y or z sweep's cv1, cv2, db1 (kept in alpha1..3 or
-// fxv..fzv by f_prep_all).
+// fxv..fzv by f_prep_all); bit 4 (applied after bit 1) names B's third
+// copy bxy .. bzy as bx1 .. bz1.
 template <class T>
 __device__ __forceinline__ ArrE<T> state_view(const ArrE<T> &a, int st)
 {
@@ -97,6 +98,11 @@ __device__ __forceinline__ ArrE<T> state_view(const ArrE<T> &a, int st)
         v.bx1 = a.bx;
         v.by1 = a.by;
         v.bz1 = a.bz;
+    }
+    if (st & 16) {  // B's third copy, written by the fused y sweep
+        v.bx1 = a.bxy;
+        v.by1 = a.byy;
+        v.bz1 = a.bzy;
     }
     if (st & 1) {
         v.rho = a.rho2;
@@ -1558,7 +1564,7 @@ KERNEL OCC(3) void f_ctx_fused(const Src<T> S)
         const int xc = clx(ix);
         const double w0 = VX1[IX(xc, iy, iz)], wm = VX1[IX(xc, ym, iz)];
         const double ro3 = RHO[IX(clx(ix + 2), iy, iz)];
-        const double bxc = BX[c];
+        const double bxc = BX[IX(xc, iy, iz)];  // the warm-up position may lie below the box
         __builtin_amdgcn_sched_barrier(0);
         double fa3, da2, sa2, fm3, dm2, fb3, db2, sb2;
         col(v3, ix + 2, fa3, da2, sa2, fm3, dm2, fb3, db2, sb2);
@@ -1629,6 +1635,179 @@ KERNEL OCC(3) void f_ctx_fused(const Src<T> S)
         cvm_0 = cvm_p;
         c1a_0 = c1a_p;
         c1m_0 = c1m_p;
+    }
+}
+
+// Lanes of a z-line y-march kernel: as zx_map with the roles of x and y
+// exchanged, each wavefront covering 64 consecutive positions of the z-rows
+// of one (x, z) plane flattened z fastest (H halo positions beyond each end
+// of a row), each lane walking K cells along y after one warm-up position.
+const int kZY = 4000;  // table code kZY + 100 * H + K
+
+struct ZYPos {
+    int iy0, iy1;  // first (warm-up) and last y of the lane
+    int ix, iz;
+    bool wz;       // lane writes (inside, not a halo lane, z in range)
+};
+
+template <int H, int K, class T>
+__device__ __forceinline__ ZYPos zy_map(const Src<T> &S, const Geo &G, int g_, int b_)
+{
+    const R3 R = {S.r.x0, G.nx + S.r.dx1, S.r.y0, G.ny + S.r.dy1, S.r.z0, G.nz + S.r.dz1};
+    const int nxr = R.x1 - R.x0 + 1, nzr = R.z1 - R.z0 + 1;
+    const int L = nzr + 2 * H, P = nxr * L;
+    const int per = 4 * (64 - 2 * H);
+    const int bpp = (P + per - 1) / per;
+    const int lb = b_ - S.boff[g_];
+    const int yc = lb / bpp;
+    const int lane = (int)threadIdx.x & 63;
+    const int w = (lb - yc * bpp) * 4 + ((int)threadIdx.x >> 6);
+    int u = (64 - 2 * H) * w + lane - H;
+    const bool inside = u >= 0 && u < P;
+    u = u < 0 ? 0 : (u >= P ? P - 1 : u);
+    const int row = u / L;
+    ZYPos z;
+    z.ix = R.x0 + row;
+    z.iz = R.z0 - H + (u - row * L);
+    z.wz = inside && lane >= H && lane < 64 - H && z.iz >= R.z0 && z.iz <= R.z1;
+    if (z.iz > G.nz + 2) z.iz = G.nz + 2;
+    if (z.iz < -2) z.iz = -2;
+    z.iy0 = R.y0 + yc * K - 1;
+    z.iy1 = z.iy0 + K < R.y1 ? z.iy0 + K : R.y1;
+    return z;
+}
+
+// R3, R4 and R5 of the y sweep in one z-line y-march (H = 1) over the union
+// of the B extents: the CT fluxes of f_cty_mass (the bx flux on this lane's
+// column and on column x - 1, the bz flux, whose value at z - 1 comes from
+// the lane below), the update of f_ct_update<1> and the y mass flux on
+// f_cty_mass's range.  B is read from the copy the x sweep wrote (B view bit
+// 2) and written into the third copy (bxy..bzy, view bit 16), unchanged
+// where no flux updates it, so no lane reads a B that another lane has
+// updated.  Each flux, update and mass flux is the expression of f_cty_mass
+// and f_ct_update<1> on the same operands; positions beyond the box are
+// clamped and feed only values that are not stored.
+template <class T, int K>
+KERNEL OCC(3) void f_cty_fused(const Src<T> S)
+{
+    const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
+    const int g_ = grid_of(S.boff, S.ngrid, b_);
+    const EDev<T> &D_ = S.devs[g_];
+    const Geo G = D_.g;
+    const ArrE<T> A = state_view(D_.a, S.st);
+    const ZYPos Z = zy_map<1, K>(S, G, g_, b_);
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const int sx = G.sx;
+    const int ix = Z.ix, iz = Z.iz;
+    const double dt = lare::gbl(A.s)->dt;
+    const Fld<T> Ba = BX, Bb = BZ, W = VY1, db1 = A.db1;
+    const Fld<T> OX = A.bx1, OY = A.by1, OZ = A.bz1;
+    // columns x - 1 and x + 1, clamped into the box
+    const int oxm = (ix - 1 < -2 ? 0 : -1) * sx, oxp = (ix + 1 > nx + 2 ? 0 : 1) * sx;
+    auto cy = [&](int y) { return y > ny + 2 ? ny + 2 : (y < -2 ? -2 : y); };
+    auto at = [&](int y) { return IX(ix, cy(y), iz); };
+    // f = B / db at y with db the average with the column across (bx flux:
+    // x + 1, from the column loads; bz flux: z + 1, from the next lane)
+    struct Col {
+        double ba, bam, bb, d0, dm, dp;
+    };
+    auto col = [&](int y) {
+        const int p = at(y);
+        return Col{Ba[p], Ba[p + oxm], Bb[p], db1[p], db1[p + oxm], db1[p + oxp]};
+    };
+    auto fvals = [&](const Col &v, double &fa, double &da, double &fam, double &dam, double &fb, double &db) {
+        da = (v.d0 + v.dp) * 0.5;
+        fa = ddiv(v.ba, da);
+        dam = (v.dm + v.d0) * 0.5;
+        fam = ddiv(v.bam, dam);
+        db = (v.d0 + lane_next(v.d0)) * 0.5;
+        fb = ddiv(v.bb, db);
+    };
+    double fa[3], fam[3], fb[3], da[2], dam[2], db[2], t0, t1, t2;
+    {
+        const Col v0 = col(Z.iy0 - 1), v1 = col(Z.iy0), v2 = col(Z.iy0 + 1);
+        fvals(v0, fa[0], t0, fam[0], t1, fb[0], t2);
+        fvals(v1, fa[1], da[0], fam[1], dam[0], fb[1], db[0]);
+        fvals(v2, fa[2], da[1], fam[2], dam[1], fb[2], db[1]);
+    }
+    double cvc_m = A.cvc1[at(Z.iy0 - 1)], cvc_0 = A.cvc1[at(Z.iy0)], cv1_0 = A.cv1[at(Z.iy0)];
+    double cvcm_m = A.cvc1[at(Z.iy0 - 1) + oxm], cvcm_0 = A.cvc1[at(Z.iy0) + oxm], cv1m_0 = A.cv1[at(Z.iy0) + oxm];
+    double ro[3] = {RHO[at(Z.iy0 - 1)], RHO[at(Z.iy0)], RHO[at(Z.iy0 + 1)]};
+    const double area_m = DYAB(ix, iz);  // face area of the y mass flux
+    const bool x0 = ix >= 0 && ix <= nx, x1 = ix >= 1 && ix <= nx;
+    const bool z0 = iz >= 0 && iz <= nz, z1 = iz >= 1 && iz <= nz;
+    const bool ebx = iz >= -1, eby = ix >= -1 && iz >= -1, ebz = ix >= -1;  // B extents but y
+    const bool mf_xz = ix >= 0 && ix <= nx + 1 && iz >= 0 && iz <= nz + 1;
+    double FA_y = 0.0, FB_y = 0.0;  // fluxes at y - 1
+    for (int d = Z.iy0; d <= Z.iy1; ++d) {
+        const int c = at(d);  // the warm-up position may lie below the box
+        // every load of the position, issued before any is used
+        const Col v3 = col(d + 2);
+        const int p1 = at(d + 1);
+        const double cvc_p = A.cvc1[p1], cv1_p = A.cv1[p1], cvcm_p = A.cvc1[p1 + oxm], cv1m_p = A.cv1[p1 + oxm];
+        const double ro3 = RHO[at(d + 2)];
+        const double w0 = W[c], wxm = W[c + oxm];
+        const double bx = BX[c], by = BY[c], bz = BZ[c];
+        __builtin_amdgcn_sched_barrier(0);
+        double fa3, da2, fam3, dam2, fb3, db2;
+        fvals(v3, fa3, da2, fam3, dam2, fb3, db2);
+        const double w0z = lane_prev(w0), wxmz = lane_prev(wxm);
+        const double va = (w0 + w0z) * 0.5, vb = (w0 + wxm) * 0.5, vam = (wxm + wxmz) * 0.5;
+        const double FA = ct_flux_v(fa[0], fa[1], fa[2], fa3, da[0], da[1], va, dt, cvc_m, cvc_0, cvc_p, cv1_0, cv1_p);
+        const double FB = ct_flux_v(fb[0], fb[1], fb[2], fb3, db[0], db[1], vb, dt, cvc_m, cvc_0, cvc_p, cv1_0, cv1_p);
+        const double FAm = ct_flux_v(fam[0], fam[1], fam[2], fam3, dam[0], dam[1], vam, dt, cvcm_m, cvcm_0, cvcm_p,
+                                     cv1m_0, cv1m_p);
+        const double FBz = lane_prev(FB);
+        if (Z.wz && d > Z.iy0) {
+            // R5 on f_cty_mass's range (x 0..n+1, y -1..n+2, z 0..n+1)
+            if (mf_xz && d >= -1 && d <= ny + 2) {
+                double dmf = 0.0;
+                if (d >= 0 && d <= ny) {
+                    const double v_advect = (wxm + w0 + wxmz + w0z) * 0.25;  // face_v<1>
+                    const double o_v = v_advect * dt * area_m;
+                    dmf = mass_flux_v(ro[0], ro[1], ro[2], ro3, v_advect, o_v, cvc_m, cvc_0, cvc_p, cv1_0, cv1_p);
+                }
+                DM[c] = dmf;
+            }
+            // R4 (f_ct_update<1>), into the third copy
+            const bool y0 = d >= 0 && d <= ny, y1 = d >= 1 && d <= ny;
+            double bxo = bx, byo = by, bzo = bz;
+            if (x0 && y1 && z1) bxo = bx - FA + FA_y;
+            if (x1 && y0 && z1) {
+                byo = by + FA - FAm;
+                byo = byo + FB - FBz;
+            }
+            if (x1 && y1 && z0) bzo = bz - FB + FB_y;
+            if (ebx && d >= -1) OX[c] = bxo;
+            if (eby) OY[c] = byo;
+            if (ebz && d >= -1) OZ[c] = bzo;
+        }
+        FA_y = FA;
+        FB_y = FB;
+        ro[0] = ro[1];
+        ro[1] = ro[2];
+        ro[2] = ro3;
+        fa[0] = fa[1];
+        fa[1] = fa[2];
+        fa[2] = fa3;
+        fam[0] = fam[1];
+        fam[1] = fam[2];
+        fam[2] = fam3;
+        fb[0] = fb[1];
+        fb[1] = fb[2];
+        fb[2] = fb3;
+        da[0] = da[1];
+        da[1] = da2;
+        dam[0] = dam[1];
+        dam[1] = dam2;
+        db[0] = db[1];
+        db[1] = db2;
+        cvc_m = cvc_0;
+        cvc_0 = cvc_p;
+        cv1_0 = cv1_p;
+        cvcm_m = cvcm_0;
+        cvcm_0 = cvcm_p;
+        cv1m_0 = cv1m_p;
     }
 }
 
@@ -3350,15 +3529,17 @@ bool map_reads_written(const Grid &g, int u, int d, int n)
 
 // Variable ids of the boundary fills: 0..12 the registered variables (state
 // in the first copy), 13..18 the second copy of rho, e_e, e_i, vx, vy, vz,
-// 19..21 bx1..bz1 as the second copy of B (resistive substeps).
-const int kSecond = 13;
-inline int base_var(int v) { return v >= kSecond ? v - kSecond : v; }
+// 19..21 bx1..bz1 as the second copy of B (resistive substeps and the
+// remap), 22..24 bxy..bzy as B's third copy (the fused y sweep).
+const int kSecond = 13, kThirdB = 22;
+inline int base_var(int v) { return v >= kThirdB ? v - kThirdB + 6 : v >= kSecond ? v - kSecond : v; }
 
 template <class T>
 void *field_ptr_t(const ArrE<T> &a, int f)
 {
-    const Fld<T> fs[22] = {a.rho, a.ee, a.ei, a.vx,  a.vy,  a.vz,  a.bx,  a.by,  a.bz,  a.vx1,  a.vy1,
-                           a.vz1, {nullptr}, a.rho2, a.ee2, a.ei2, a.vx2, a.vy2, a.vz2, a.bx1, a.by1, a.bz1};
+    const Fld<T> fs[25] = {a.rho, a.ee,  a.ei,  a.vx,  a.vy,  a.vz,  a.bx,  a.by,  a.bz,
+                           a.vx1, a.vy1, a.vz1, {nullptr}, a.rho2, a.ee2, a.ei2, a.vx2, a.vy2,
+                           a.vz2, a.bx1, a.by1, a.bz1, a.bxy, a.byy, a.bzy};
     return f == 12 ? (void *)a.dm : (void *)fs[f].p;
 }
 void *field_ptr(const Grid &g, int f) { return g.fp32 ? field_ptr_t(g.e32.a, f) : field_ptr_t(g.e64.a, f); }
@@ -3386,7 +3567,11 @@ Group::Tab &table(Group &gp, int subset, const RSpec &r, int order)
         boff.push_back((int)total);
         t.units += (double)G.nx * G.ny * G.nz;
         if (nxr > 0 && nyr > 0 && nzr > 0) {
-            if (order >= kZX) {
+            if (order >= kZY) {
+                const int H = (order - kZY) / 100, K = (order - kZY) % 100;
+                const long per = 4 * (64 - 2 * H);
+                total += ((nyr + K - 1) / K) * ((nxr * (nzr + 2 * H) + per - 1) / per);
+            } else if (order >= kZX) {
                 const int H = (order - kZX) / 100, K = (order - kZX) % 100;
                 const long per = 4 * (64 - 2 * H);
                 total += ((nxr + K - 1) / K) * ((nyr * (nzr + 2 * H) + per - 1) / per);
@@ -3480,7 +3665,8 @@ void per_grid(void (*k)(const EDev<T> *, int), const char *name, Group &gp)
 // Variable groups of the boundary fills (boundary.cpp:27-96): the variables
 // filled and, for a variable just written into the other state copy, the
 // copy it was read from (-1: updated in place).
-enum { VB, VE, VRHO, VV, VV1, VDM, VALL, VERHO, VBE, VCORR_2, VALL_2, VALL_1, VBE_1, VBE_0, VALL_2B, VALL_1B, VV_2, NVG };
+enum { VB, VE, VRHO, VV, VV1, VDM, VALL, VERHO, VBE, VCORR_2, VALL_2, VALL_1, VBE_1, VBE_0, VALL_2B, VALL_1B, VV_2,
+       VALL_2C, VALL_1C, NVG };
 struct VarGroup {
     int n;
     int dst[9];
@@ -3513,6 +3699,10 @@ const VarGroup kGroups[NVG] = {
     {9, {19, 20, 21, 1, 2, 0, 3, 4, 5}, {-1, -1, -1, 14, 15, 13, 16, 17, 18}},
     // the corrector's first velocity_bcs, into the second copy
     {3, {16, 17, 18}, {3, 4, 5}},
+    // boundary_conditions after the fused y sweep, B in bxy..bzy; the state
+    // as in VALL_2 / VALL_1
+    {9, {22, 23, 24, 14, 15, 13, 16, 17, 18}, {-1, -1, -1, 1, 2, 0, 3, 4, 5}},
+    {9, {22, 23, 24, 1, 2, 0, 3, 4, 5}, {-1, -1, -1, 14, 15, 13, 16, 17, 18}},
 };
 
 void drop_btabs(Group &gp)
@@ -3852,20 +4042,25 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
     else
         elaunch_(f_cvc1<T>, "f_cvc1", 2 * w, gp, 0, rs(-1, 1, -1, 1, -1, 1), pv, gp.order);
     // fused CT (flux and update in one kernel): B moves to bx1..bz1 in the x
-    // sweep, is updated there in place by the y sweep (B view bit 2) and
-    // comes back in the z sweep
+    // sweep, from there into bxy..bzy in the y sweep (or is updated in place
+    // by the y sweep's two kernels, B view bit 1) and comes back in the z
+    // sweep (reading bxy..bzy through view bit 4 after the fused y sweep)
     const bool fuse = ct_fused(gp);
+    const bool yfuse = fuse && gp.ct_fuse >= 2;
     const int bst = fuse && DIR == 1 ? 2 : 0;
     const RSpec ext = rs(-2, 2, -2, 2, -2, 2);
-    if (fuse && DIR != 1) {
+    if (fuse && (DIR != 1 || yfuse)) {
         const int K = zxk(gp);
         // also the sweep's mass flux (rho from the current state copy: st)
         if (DIR == 0)
             elaunch_(K == 16 ? f_ctx_fused<T, 16> : f_ctx_fused<T, 8>, "f_ct<0>", 11 * w + 8, gp, 0, ext, st | pv,
                      kZX + 100 + K);
+        else if (DIR == 1)
+            elaunch_(K == 16 ? f_cty_fused<T, 16> : f_cty_fused<T, 8>, "f_ct<1>", 11 * w + 8, gp, 0, ext,
+                     2 | 16 | st | pv, kZY + 100 + K);
         else
-            elaunch_(K == 16 ? f_ctz_fused<T, 16> : f_ctz_fused<T, 8>, "f_ct<2>", 11 * w + 8, gp, 0, ext, st | pv,
-                     kZX + 200 + K);
+            elaunch_(K == 16 ? f_ctz_fused<T, 16> : f_ctz_fused<T, 8>, "f_ct<2>", 11 * w + 8, gp, 0, ext,
+                     st | pv | (yfuse ? 16 : 0), kZX + 200 + K);
     } else {
         if (DIR == 2 && gp.zline && use_walk(gp)) {
             elaunch_(f_ct_zline<T>, "f_ct_fluxes<2>", 7 * w + 2 * 8, gp, 0, all0, bst | pv, kZLine);
@@ -3890,7 +4085,7 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
     }
     // mass flux: its reference range, widened to the ghost faces along DIR
     // (the fused CT kernels of the x and z sweeps compute it themselves)
-    const bool mass_done = (fuse && DIR != 1) || (DIR == 1 && gp.march_k > 0 && use_walk(gp));
+    const bool mass_done = (fuse && (DIR != 1 || yfuse)) || (DIR == 1 && gp.march_k > 0 && use_walk(gp));
     if (!mass_done)
         ELS((f_mass_flux<T, DIR>), 4 * w + 8, gp, 0,
             rs(DIR == 0 ? -1 : 0, DIR == 0 ? 2 : 1, DIR == 1 ? -1 : 0, DIR == 1 ? 2 : 1, DIR == 2 ? -1 : 0,
@@ -3932,7 +4127,9 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
         ELS((f_wshift<T, DIR, 1>), w + 8, gp, 0, sh, pv);
     }
     if (gp.prof) gp.prof->dir = -1;
-    if (fuse && DIR != 2)
+    if (fuse && DIR == 1 && yfuse)
+        ebcs<T>(gp, st ? VALL_1C : VALL_2C);
+    else if (fuse && DIR != 2)
         ebcs<T>(gp, st ? VALL_1B : VALL_2B);
     else
         ebcs<T>(gp, st ? VALL_1 : VALL_2);
