@@ -376,7 +376,7 @@ PoolLayout pool_layout()
     return L;
 }
 
-// Engine arrays (lare::ArrE) of one grid in storage type T: 42 T arrays and
+// Engine arrays (lare::ArrE) of one grid in storage type T: 43 T arrays and
 // 6 FP64 flux arrays carved from the pool at p.
 template <class T>
 void carve_engine(lare::ArrE<T> &a, char *&p, size_t box, const PoolLayout &L)
@@ -395,12 +395,12 @@ void carve_engine(lare::ArrE<T> &a, char *&p, size_t box, const PoolLayout &L)
                           &a.vz1, &a.cs, &a.cs_v, &a.rho_v, &a.alpha1, &a.alpha2, &a.alpha3, &a.visc_heat,
                           &a.fxv, &a.fyv, &a.fzv, &a.cv1, &a.bx1, &a.by1, &a.bz1, &a.p_e, &a.p_i, &a.eta,
                           &a.curlb, &a.cv2, &a.cvc1, &a.db1, &a.rho2, &a.ee2, &a.ei2, &a.vx2, &a.vy2, &a.vz2,
-                          &a.bxy, &a.byy, &a.bzy};
+                          &a.bxy, &a.byy, &a.bzy, &a.vol};
     for (auto *q : ts) *q = tk();
     double **ds[] = {&a.dm, &a.flux_x, &a.flux_y, &a.flux_z, &a.fl[0], &a.fl[1]};
     for (double **q : ds) *q = dk();
 }
-const int kEngineT = 42, kEngineD = 6;
+const int kEngineT = 43, kEngineD = 6;
 
 // The one-kernel-per-reference-kernel set's view of the FP64 engine arrays.
 void alias_set0(lare::Arr &a, const lare::ArrE<double> &e)
@@ -626,7 +626,13 @@ extern "C" int lare_ctx_create(const lare_desc *d, lare_ctx **out)
                                    g.stream);
         }
         if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
-        if (e != hipSuccess) rc = fail(-3, std::string("metric factor upload: ") + hipGetErrorString(e));
+        if (e != hipSuccess) {
+            rc = fail(-3, std::string("metric factor upload: ") + hipGetErrorString(e));
+            break;
+        }
+        std::string err;
+        const int vrc = lare::grid_fill_volume(g, err);
+        if (vrc) rc = fail(vrc, err);
     } while (0);
     if (rc) {
         (void)hipFree(g.pool);

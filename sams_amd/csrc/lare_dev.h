@@ -152,6 +152,10 @@ struct ArrE {
     // B's third copy: the fused y sweep reads B from bx1..bz1 (where the x
     // sweep left it) and writes it here, for the z sweep to read back
     Fld<T> bxy, byy, bzy;
+    // the cell volume CV(ix, iy, iz) over the whole box, filled once when the
+    // grid is created (the z sweep's cv2: its passes make the Lagrangian
+    // volume after the sweep the cell volume itself)
+    Fld<T> vol;
     // the remap kinetic-energy correction's delta_ke (FP64; only with rke)
     double *dke;
     // 1D axis deltas, index i stored at [i + 2]

@@ -107,6 +107,10 @@ int energy_scratch_address(double **out, std::string &err);
 void group_energy_prepare(Group &gp);
 int group_energy_noalloc(Group &gp, double *scratch_dev, double *out, std::string &err);
 
+// Fill the engine's constant cell-volume array of grid g (after its axes are
+// on the device; waits for the fill).
+int grid_fill_volume(Grid &g, std::string &err);
+
 // q[i] = a[i] / d[i] for host arrays of n values, computed on the device by
 // the kernels' division (mode 0) or shared-divisor quotient (mode 1); mode 2
 // gives q[i] = sqrt(a[i]) as the kernels compute it (d unused).

@@ -78,17 +78,17 @@ __device__ __forceinline__ int xcd_swizzle(int b, int nblk)
 // copy holding the current state (the second copy when bit 0 of st is set),
 // rho2 .. vz2 the other one; with bit 1 set, bx .. bz name bx1 .. bz1 (B's
 // second copy during the remap) and bx1 .. bz1 name B's own arrays; bits 2
-// and 3 name the y or z sweep's cv1, cv2, db1 (kept in alpha1..3 or
-// fxv..fzv by f_prep_all); bit 4 (applied after bit 1) names B's third
-// copy bxy .. bzy as bx1 .. bz1.
+// and 3 name the y sweep's cv1, cv2, db1 (alpha1..3, from f_prep_all) or
+// the z sweep's (alpha2, the cell volume, fzv); bit 4 (applied after bit 1)
+// names B's third copy bxy .. bzy as bx1 .. bz1.
 template <class T>
 __device__ __forceinline__ ArrE<T> state_view(const ArrE<T> &a, int st)
 {
     ArrE<T> v = a;
     if (st & 12) {  // prep set of the y (4) or z (8) sweep
         const bool y = (st & 12) == 4;
-        v.cv1 = y ? a.alpha1 : a.fxv;
-        v.cv2 = y ? a.alpha2 : a.fyv;
+        v.cv1 = y ? a.alpha1 : a.alpha2;
+        v.cv2 = y ? a.alpha2 : a.vol;
         v.db1 = y ? a.alpha3 : a.fzv;
     }
     if (st & 2) {
@@ -286,9 +286,15 @@ KERNEL void f_prep(const Src<T> S, double xpass, double ypass, double zpass)
 // the same v1 face averages (the only v1 a sweep writes back is its own
 // component, which later sweeps' preps weight by a pass of 0: xpass in y and
 // z, ypass in z), with the passes of the x, y and z sweeps (1,1,1), (0,1,1),
-// (0,0,1).  The y and z sweeps' cv1, cv2, db1 go into alpha1..3 and
-// fxv..fzv, which nothing reads between the corrector and the next step's
-// viscosity; their kernels see them through the prep-set bits of the view.
+// (0,0,1).  The y sweep's cv1, cv2, db1 go into alpha1..3 and the z sweep's
+// db1 into fzv, which nothing reads between the corrector and the next
+// step's viscosity; their kernels see them through the prep-set bits of the
+// view.  With the z passes the x and y divergences are signed zeros, so the
+// z sweep's cv2 is vol * (1 + 0) = vol, the cell volume, and its cv1,
+// vol * (1 + dvzdz * dt), is the y sweep's cv2 with the same operands
+// (zpass = 1 multiplies exactly, and a signed zero added to dvzdz leaves
+// 1 + dv unchanged): neither is stored, the view names the volume array and
+// alpha2 for them.
 template <class T>
 KERNEL void f_prep_all(const Src<T> S)
 {
@@ -315,8 +321,6 @@ KERNEL void f_prep_all(const Src<T> S)
     A.alpha2[c] = o[1];
     A.alpha3[c] = o[2];
     prep_out<2>(A, ix, iy, iz, avg, dt, 0.0, 0.0, 1.0, o);
-    A.fxv[c] = o[0];
-    A.fyv[c] = o[1];
     A.fzv[c] = o[2];
     };
     ECELL(cell)
@@ -4035,7 +4039,7 @@ void sweep_t(Group &gp, double xpass, double ypass, double zpass, int st)
     if (!gp.prep_all)
         EL((f_prep<T, DIR>), 6 * w, gp, 0, rs(-1, 2, -1, 2, -1, 2), xpass, ypass, zpass);
     else if (DIR == 0)
-        elaunch_(f_prep_all<T>, "f_prep_all", 12 * w, gp, 0, rs(-1, 2, -1, 2, -1, 2), 0, gp.order);
+        elaunch_(f_prep_all<T>, "f_prep_all", 10 * w, gp, 0, rs(-1, 2, -1, 2, -1, 2), 0, gp.order);
     if (use_zx(gp, 0))
         elaunch_(zxk(gp) == 16 ? f_cvc1_zx<T, 16> : f_cvc1_zx<T, 8>, "f_cvc1", 2 * w, gp, 0, rs(-1, 1, -1, 1, -1, 1),
                  pv, kZX + 100 + zxk(gp));
@@ -4384,6 +4388,39 @@ int group_energy(Group &gp, std::vector<double> &out, std::string &err)
     if (res) (void)hipFree(res);
     if (e != hipSuccess) {
         err = std::string("energy diagnostic: ") + hipGetErrorString(e);
+        return -3;
+    }
+    return 0;
+}
+
+// The cell volume CV over the whole box of one grid (ix, iy, iz in -2..n+2).
+template <class T>
+KERNEL void k_fill_volume(const EDev<T> D)
+{
+    const Geo G = D.g;
+    const ArrE<T> &A = D.a;
+    const int ly = G.ny + 5, lz = G.nz + 5;
+    const long n = (long)(G.nx + 5) * ly * lz;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int ix = (int)(t / ((long)ly * lz)), r = (int)(t - (long)ix * ly * lz);
+    const int iy = r / lz, iz = r - iy * lz;
+    A.vol[IX(ix - 2, iy - 2, iz - 2)] = CV(ix - 2, iy - 2, iz - 2);
+}
+
+int grid_fill_volume(Grid &g, std::string &err)
+{
+    const Geo &G = g.fp32 ? g.e32.g : g.e64.g;
+    const long n = (long)(G.nx + 5) * (G.ny + 5) * (G.nz + 5);
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (g.fp32)
+        hipLaunchKernelGGL(k_fill_volume<float>, dim3(nb), dim3(256), 0, g.stream, g.e32);
+    else
+        hipLaunchKernelGGL(k_fill_volume<double>, dim3(nb), dim3(256), 0, g.stream, g.e64);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+    if (e != hipSuccess) {
+        err = std::string("cell-volume fill: ") + hipGetErrorString(e);
         return -3;
     }
     return 0;

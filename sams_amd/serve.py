@@ -50,7 +50,7 @@ class Request:
 
 
 def device_bytes(p):
-    """HBM held by one grid of problem p: 42 state/work arrays in the storage
+    """HBM held by one grid of problem p: 43 state/work arrays in the storage
     type and 6 FP64 flux arrays over the padded box (7 with the kinetic-energy
     correction), each rounded up to 256 bytes, plus axes and scalars (the
     pool of lare_ctx_create with its default layout), the field-transfer
@@ -65,7 +65,7 @@ def device_bytes(p):
         return (nbytes + 255) // 256 * 256
 
     stage = arr((int(p.nx) + 5) * (int(p.ny) + 5) * (int(p.nz) + 5) * 8)
-    return 42 * arr(box * t) + (7 if p.rke else 6) * arr(box * 8) + 6 * lmax * 8 + 256 + stage + 65536
+    return 43 * arr(box * t) + (7 if p.rke else 6) * arr(box * 8) + 6 * lmax * 8 + 256 + stage + 65536
 
 
 def poisson_requests(qps, duration, seed, nmin=32, nmax=192, smin=50, smax=500):
