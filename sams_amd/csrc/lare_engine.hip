@@ -1988,6 +1988,100 @@ KERNEL void g_vertex(const Src<T> S)
     ECELL(cell)
 }
 
+// L1 and L2 in one z-line x-march (one halo lane each end) over the
+// centred extent (-1..n+2)^3: at each x a lane computes the sound speed of
+// g_cs on its own cell column (y) and on column y+1, carries both to the
+// next x, and forms the vertex density and vertex sound speed of g_vertex
+// at vertex (x-1, y, z) from the two x planes (the values at z+1 from the
+// lane above).  It stores p_e / p_i on its own cell as g_cs does and resets
+// the CFL minimum; the sound speed is rounded to the storage type as g_cs's
+// store rounds it, and never stored.  Each value is the expression of g_cs
+// and g_vertex on the same operands; positions beyond the box are clamped
+// and feed only values that are not stored.
+template <class T, int K>
+KERNEL OCC(4) void g_csv_zx(const Src<T> S)
+{
+    const int b_ = xcd_swizzle((int)blockIdx.x, S.nblk);
+    const int g_ = grid_of(S.boff, S.ngrid, b_);
+    const EDev<T> &D_ = S.devs[g_];
+    const Geo G = D_.g;
+    const ArrE<T> A = state_view(D_.a, S.st);
+    const Phys P = D_.p;
+    const double gas_gamma = P.gas_gamma;
+    const bool store_pe = P.resistive, store_pi = P.keep_pi;
+    const ZXPos Z = zx_map<1, K>(S, G, g_, b_);
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const int sx = G.sx, sy = G.sy;
+    const int iy = Z.iy, iz = Z.iz;  // iy >= -1 over this range, so y-1 stays in the box
+    const int yn = iy + 1 > ny + 2 ? ny + 2 : iy + 1, zn = iz + 1 > nz + 2 ? nz + 2 : iz + 1;
+    const int dn = (yn - iy) * sy;  // column y+1
+    const bool vyz = iy <= ny + 1 && iz <= nz + 1;  // vertex range along y and z
+    // CV(x, y', z') = DXB(x) * (DYB(y') * DZB(z')) for y' in y, y+1 and z' in z, z+1
+    const double dy0 = DYB(iy), dy1 = DYB(yn), dz0 = DZB(iz), dz1 = DZB(zn);
+    const double a00 = dy0 * dz0, a10 = dy1 * dz0, a01 = dy0 * dz1, a11 = dy1 * dz1;
+    const double dx_row = x_row(A.dxb, Z.ix0, nx + 2);  // lane l: DXB(ix0 + l)
+    auto cs_of = [&](double bx, double bxm, double by, double bym, double bz, double bzm, double r, double ee,
+                     double ei, double &pe, double &pi) {
+        const double bx1 = 0.5 * (bx + bxm), by1 = 0.5 * (by + bym), bz1 = 0.5 * (bz + bzm);
+        pe = (gas_gamma - 1.0) * r * ee;
+        pi = (gas_gamma - 1.0) * r * ei;
+        const double p = pe + pi;
+        const double rmin = pmax(r, DBL_EPSILON);
+        const double b2 = bx1 * bx1 + by1 * by1 + bz1 * bz1;
+        return (double)(T)sqrt(ddiv(gas_gamma * p + b2, rmin));
+    };
+    // plane x-1: bx of both columns, rho and cs of both columns and their
+    // values at z+1
+    const int xm0 = Z.ix0 - 1 < -2 ? -2 : Z.ix0 - 1;
+    double bxm_o = BX[IX(xm0, iy, iz)], bxm_n = BX[IX(xm0, yn, iz)];
+    double rm_o = 0.0, rm_n = 0.0, rm_oz = 0.0, rm_nz = 0.0, sm_o = 0.0, sm_n = 0.0, sm_oz = 0.0, sm_nz = 0.0;
+    for (int ix = Z.ix0; ix <= Z.ix1; ++ix) {
+        const int c = IX(ix, iy, iz);  // ix >= -2 over this range
+        // every load of the position, issued before any is used
+        const double bxo = BX[c], bxn = BX[c + dn];
+        const double bym = BY[c - sy], byo = BY[c], byn = BY[c + dn];
+        const double bzo = BZ[c], bzn = BZ[c + dn];
+        const double ro = RHO[c], rn = RHO[c + dn];
+        const double eo = EE[c], en = EE[c + dn];
+        const double io = EI[c], in = EI[c + dn];
+        __builtin_amdgcn_sched_barrier(0);
+        const double bzo_m = lane_prev(bzo), bzn_m = lane_prev(bzn);
+        double pe, pi, pe_n, pi_n;
+        const double so = cs_of(bxo, bxm_o, byo, bym, bzo, bzo_m, ro, eo, io, pe, pi);
+        const double sn = cs_of(bxn, bxm_n, byn, byo, bzn, bzn_m, rn, en, in, pe_n, pi_n);
+        const double ro_z = lane_next(ro), rn_z = lane_next(rn), so_z = lane_next(so), sn_z = lane_next(sn);
+        if (Z.wz && ix > Z.ix0) {
+            if (store_pe) A.p_e[c] = pe;
+            if (store_pi) A.p_i[c] = pi;
+            if (ix == S.r.x0 && iy == S.r.y0 && iz == S.r.z0) lare::gbl(A.s)->dtmin_bits = 0x7FEFFFFFFFFFFFFFull;
+            if (vyz && ix >= 0 && ix <= nx + 2) {
+                // vertex (x-1, y, z): corners c000 .. c111 of g_vertex
+                const double dxm = x_at(dx_row, ix - 1 - Z.ix0), dx0 = x_at(dx_row, ix - Z.ix0);
+                const double c000 = dxm * a00, c100 = dx0 * a00, c010 = dxm * a10, c110 = dx0 * a10,
+                             c001 = dxm * a01, c101 = dx0 * a01, c011 = dxm * a11, c111 = dx0 * a11;
+                const double sum_rho_cv = rm_o * c000 + ro * c100 + rm_n * c010 + rn * c110 + rm_oz * c001 +
+                                          ro_z * c101 + rm_nz * c011 + rn_z * c111;
+                const double sum_cv = c000 + c100 + c010 + c110 + c001 + c101 + c011 + c111;
+                A.rho_v[c - sx] = ddiv(sum_rho_cv, sum_cv);
+                const double cv_v = 0.125 * sum_cv;
+                const double sum = sm_o * c000 + so * c100 + sm_n * c010 + sn * c110 + sm_oz * c001 + so_z * c101 +
+                                   sm_nz * c011 + sn_z * c111;
+                A.cs_v[c - sx] = ddiv(0.125 * sum, cv_v);
+            }
+        }
+        bxm_o = bxo;
+        bxm_n = bxn;
+        rm_o = ro;
+        rm_n = rn;
+        rm_oz = ro_z;
+        rm_nz = rn_z;
+        sm_o = so;
+        sm_n = sn;
+        sm_oz = so_z;
+        sm_nz = sn_z;
+    }
+}
+
 // edge_viscosity (lagran.cpp:56-93) between c1 and c1 + s.
 // The same from values: the velocity at the four points c0..c3 of the edge
 // line (v[k][0..2] = vx, vy, vz) and rho_v at c1, c2.
@@ -3949,8 +4043,14 @@ void start_a_t(Group &gp)
     const double w = (double)sizeof(T);
     const RSpec C = rs(-1, 2, -1, 2, -1, 2);
     const RSpec V = rs(-1, 1, -1, 1, -1, 1);
-    EL(g_cs<T>, 7 * w, gp, 0, C);
-    EL(g_vertex<T>, 4 * w, gp, 0, V);
+    if (use_zx(gp, 0)) {
+        // g_cs and g_vertex in one z-line x-march over the centred extent
+        const int K = zxk(gp);
+        elaunch_(K == 16 ? g_csv_zx<T, 16> : g_csv_zx<T, 8>, "g_csv", 8 * w, gp, 0, C, 0, kZX + 100 + K);
+    } else {
+        EL(g_cs<T>, 7 * w, gp, 0, C);
+        EL(g_vertex<T>, 4 * w, gp, 0, V);
+    }
     EL(g_alpha<T>, 8 * w, gp, 0, rs(-1, 1, -1, 2, 0, 2));
     if (use_zx(gp, 0))
         elaunch_(gp.zx_k >= 16 ? g_visc_dt_zx<T, 16> : g_visc_dt_zx<T, 8>, "g_visc_dt", 16 * w, gp, 0,
