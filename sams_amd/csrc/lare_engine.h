@@ -58,6 +58,7 @@ struct Group {
     int ct_fuse = 2;                // CT flux and update in one kernel (lane-walking only): 1 the x and z sweeps, 2 all three
     bool prep_all = true;           // the three sweeps' cell volumes and widths in one kernel
     bool bc_all = true;             // a variable group's fills of all three dimensions in one kernel
+    bool csv_fused = true;          // sound speed and vertex values in one kernel (lane-walking only)
     // one captured step of the whole group
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;

@@ -4043,7 +4043,7 @@ void start_a_t(Group &gp)
     const double w = (double)sizeof(T);
     const RSpec C = rs(-1, 2, -1, 2, -1, 2);
     const RSpec V = rs(-1, 1, -1, 1, -1, 1);
-    if (use_zx(gp, 0)) {
+    if (gp.csv_fused && use_zx(gp, 0)) {
         // g_cs and g_vertex in one z-line x-march over the centred extent
         const int K = zxk(gp);
         elaunch_(K == 16 ? g_csv_zx<T, 16> : g_csv_zx<T, 8>, "g_csv", 8 * w, gp, 0, C, 0, kZX + 100 + K);
@@ -4277,6 +4277,7 @@ int group_init(Group &gp, const std::vector<Grid *> &members, hipStream_t stream
     if (const char *o = std::getenv("LARE_CTFUSE")) gp.ct_fuse = std::atoi(o);
     if (const char *o = std::getenv("LARE_PREPALL")) gp.prep_all = std::atoi(o) != 0;
     if (const char *o = std::getenv("LARE_BCALL")) gp.bc_all = std::atoi(o) != 0;
+    if (const char *o = std::getenv("LARE_CSVFUSE")) gp.csv_fused = std::atoi(o) != 0;
     if (members.empty()) return 0;
     gp.fp32 = members[0]->fp32;
     for (size_t m = 0; m < members.size(); ++m) {
