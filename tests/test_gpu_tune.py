@@ -1,6 +1,7 @@
 """Launch configurations and the autotuner.  Every configuration of the fused
 kernel set (lane-walking or one-cell kernels, cells per lane, z-line
-exchange, workgroup order) must give the same bits, and autotuning must
+exchange, workgroup order, CT fusion of none, two or all three sweeps) must
+give the same bits, and autotuning must
 leave the state exactly where it was (it saves and restores each member's
 device state around the timed candidates)."""
 import pytest
@@ -15,6 +16,8 @@ CONFIGS = [
     dict(walk=1, march_k=16, zx_k=16, zline=0, order=0, ct_fuse=1),
     dict(walk=1, march_k=8, zx_k=16, zline=1, order=0, ct_fuse=0),
     dict(walk=2, march_k=16, zx_k=8, zline=1, order=1, ct_fuse=1),
+    dict(walk=1, march_k=16, zx_k=8, zline=1, order=1, ct_fuse=2),
+    dict(walk=1, march_k=8, zx_k=16, zline=0, order=0, ct_fuse=2),
 ]
 
 
@@ -77,6 +80,8 @@ def test_invalid_configuration_is_an_error():
     g = L.Lare.from_problem(L.problem("ot", n=8))
     with pytest.raises(RuntimeError, match="invalid configuration"):
         g.set_tune(march_k=5)
+    with pytest.raises(RuntimeError, match="invalid configuration"):
+        g.set_tune(ct_fuse=3)
     g.set_kernel_set(0)
     with pytest.raises(RuntimeError, match="kernel set 1"):
         g.autotune()
