@@ -1685,10 +1685,10 @@ __device__ __forceinline__ ZYPos zy_map(const Src<T> &S, const Geo &G, int g_, i
 // of the B extents: the CT fluxes of f_cty_mass (the bx flux on this lane's
 // column and on column x - 1, the bz flux, whose value at z - 1 comes from
 // the lane below), the update of f_ct_update<1> and the y mass flux on
-// f_cty_mass's range.  B is read from the copy the x sweep wrote (B view bit
-// 2) and written into the third copy (bxy..bzy, view bit 16), unchanged
-// where no flux updates it, so no lane reads a B that another lane has
-// updated.  Each flux, update and mass flux is the expression of f_cty_mass
+// f_cty_mass's range.  B is read from the copy the x sweep wrote (view bit
+// 1, st value 2) and written into the third copy (bxy..bzy, view bit 4, st
+// value 16), unchanged where no flux updates it, so no lane reads a B that
+// another lane has updated.  Each flux, update and mass flux is the expression of f_cty_mass
 // and f_ct_update<1> on the same operands; positions beyond the box are
 // clamped and feed only values that are not stored.
 template <class T, int K>
