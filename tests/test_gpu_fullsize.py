@@ -63,14 +63,20 @@ def test_benchmark_workload_200_steps_matches_reference():
     _run_against_digests(c, dev)
 
 
-def test_benchmark_workload_other_launch_configurations_match_reference():
-    """The other launch configurations the autotuner can choose (8 cells per
-    lane in the marches, plane-major workgroup order, the z sweep without
-    the lane exchange, CT flux and update as separate kernels) give the
+@pytest.mark.parametrize("tune", [
+    # 8 cells per lane in the marches, plane-major workgroup order, the z
+    # sweep without the lane exchange, CT flux and update as separate kernels
+    dict(walk=1, march_k=8, zx_k=8, zline=0, order=0, ct_fuse=0),
+    # the configuration the autotuner picks for this workload (16 cells per
+    # lane everywhere, the CT of all three sweeps fused)
+    dict(walk=1, march_k=16, zx_k=16, zline=1, order=1, ct_fuse=2),
+])
+def test_benchmark_workload_other_launch_configurations_match_reference(tune):
+    """The other launch configurations the autotuner can choose give the
     same bits for the first 50 steps."""
     c, p = _full_case("bw_res_256_200")
     dev = L.Lare.from_problem(p)
-    dev.set_tune(walk=1, march_k=8, zx_k=8, zline=0, order=0, ct_fuse=0)
+    dev.set_tune(**tune)
     _run_against_digests(c, dev, upto=50)
 
 
